@@ -1,0 +1,164 @@
+/*
+ * inundation_hip.h - C ABI of the MI355X shallow-water step backend.
+ *
+ * libinundation_hip.so replaces the reference's GPU-resident step pipeline,
+ * i.e. the Python backend class SWESolverGpuResidentBatch
+ * (inundation/vulkan/gpu_resident_batch.py:140-762) and the GLSL kernels it
+ * dispatches through Kompute (inundation/vulkan/shaders/resident.py,
+ * flux.py, cfl.py). The reference has no C ABI of its own: its boundary is the
+ * backend-class protocol that SWEWorkflow drives (workflow.py:783-826). Each
+ * entry point below names the reference method or kernel it stands in for.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative INU_E* code;
+ *     inu_last_error() gives the message of the last failure on a context.
+ *   - Plain pointers and sizes only. Host arrays are copied; the caller keeps
+ *     ownership. Device memory belongs to the context and is freed by
+ *     inu_destroy().
+ *   - One HIP stream per context; calls on one context are not thread-safe.
+ *     Calls that return data synchronise that stream.
+ *   - Arrays are in the solver's cell order (Hilbert) and in the edge order
+ *     sorted by the block of the left cell (reference blocks.py:28-30).
+ */
+#ifndef INUNDATION_HIP_H
+#define INUNDATION_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define INU_ABI_VERSION 1
+
+/* Error codes. INU_EINVAL maps to ValueError on the Python side, the others to
+ * RuntimeError (reference raises: gpu_resident_batch.py:641-655, 720-724). */
+#define INU_OK 0
+#define INU_EINVAL -1 /* bad argument */
+#define INU_EHIP -2   /* HIP runtime error */
+#define INU_ESTATE -3 /* call order (no mesh yet, ...) */
+#define INU_ERANGE -5 /* mesh exceeds a size limit of the kernels */
+#define INU_ENODEV -6 /* no HIP device */
+
+/* inu_set_params flags: the solver attributes the workflow sets after
+ * construction (reference gpu_resident_batch.py:143-158, workflow.py:816-826). */
+#define INU_FLAG_DETERMINISTIC 1u /* fixed-order CSR gather (the only mode implemented) */
+#define INU_FLAG_COMPENSATED 2u   /* depth as a compensated float32 pair */
+#define INU_FLAG_ACTIVE_SET 4u    /* skip blocks beyond the two-ring of water */
+#define INU_FLAG_TRACK_CLAMP 8u   /* accumulate the positivity-clamp mass */
+
+/* Fields of inu_download / inu_upload. */
+#define INU_FIELD_H 0
+#define INU_FIELD_HU 1
+#define INU_FIELD_HV 2
+#define INU_FIELD_H_LO 3
+#define INU_FIELD_H1 4
+#define INU_FIELD_HU1 5
+#define INU_FIELD_HV1 6
+#define INU_FIELD_H1_LO 7
+#define INU_FIELD_CLAMP 8
+#define INU_FIELD_BLOCK_WET 9     /* uint32 per 256-cell block */
+#define INU_FIELD_BLOCK_ACTIVE 10 /* uint32 per block */
+#define INU_FIELD_BLOCK_FORCED 11 /* uint32 per block */
+
+/* The 48-byte resident control block: the integer clock and step planning
+ * (reference RESIDENT_CTRL_DTYPE, gpu_resident_batch.py:65-77; the CTRL block
+ * of STEP_CONTROL_GLSL, resident.py:823-833). */
+typedef struct inu_ctrl {
+  uint64_t now_ticks;
+  uint64_t stop_ticks;
+  uint64_t steps;
+  uint32_t dt_ticks;
+  uint32_t dt_max_ticks;
+  uint32_t dry_dt_ticks;
+  float last_raw;
+  float cfl_safety;
+  uint32_t pad;
+} inu_ctrl;
+
+typedef struct inu_ctx inu_ctx;
+
+/* ABI version of the loaded library (INU_ABI_VERSION). No reference counterpart. */
+int inu_version(void);
+
+/* Context on HIP device `device`; replaces kp.Manager() in SWESolver.__init__
+ * (reference solver.py:186). */
+int inu_create(int device, inu_ctx** out);
+void inu_destroy(inu_ctx* ctx);
+const char* inu_last_error(const inu_ctx* ctx);
+
+/* Mesh upload; replaces _build_tensors (reference solver.py:219-296 and
+ * gpu_resident_batch.py:160-282). Edges must be sorted by the block of their
+ * left cell; edge_dz holds 2E interleaved (dzL, dzR) words (solver.py:233);
+ * cell_edge_idx holds (edge << 1) | is_right in that sorted edge numbering
+ * (blocks.py:33-44). edge_cell_r is a cell or -1 (wall); open edges are not
+ * implemented. Resets the state, parameters and sources. */
+int inu_set_mesh(inu_ctx* ctx, uint32_t n_cells, uint32_t n_edges, const int32_t* edge_cell_l,
+                 const int32_t* edge_cell_r, const float* edge_nx, const float* edge_ny,
+                 const float* edge_len, const float* edge_dz, const float* cell_area,
+                 const uint32_t* cell_edge_ptr, const uint32_t* cell_edge_idx);
+
+/* out6 = {N, E, slots, blocks, max edges per block, stage grid size}. No
+ * reference counterpart (diagnostics). */
+int inu_mesh_info(inu_ctx* ctx, uint32_t* out6);
+
+/* g, dry_tol, CFL number and INU_FLAG_* (reference constructor arguments,
+ * solver.py:149-172, and the solver attributes above). */
+int inu_set_params(inu_ctx* ctx, float g, float dry_tol, float cfl, uint32_t flags);
+
+/* Depth, momenta and the compensated low word (NULL = zeros); replaces
+ * reset() (solver.py:390-421, gpu_resident_batch.py:436-445). */
+int inu_set_state(inu_ctx* ctx, const float* h, const float* hu, const float* hv, const float* h_lo);
+
+/* Per-cell Manning roughness (binding 16, solver.py:252). */
+int inu_set_manning(inu_ctx* ctx, const float* n);
+
+/* Constant per-run sources as (cell, dh/dt) pairs; zero rates are dropped;
+ * replaces _set_sources (gpu_resident_batch.py:529-560). */
+int inu_set_sources(inu_ctx* ctx, uint32_t n, const uint32_t* cell, const float* rate);
+
+/* Control block up/down; replaces _sync_ctrl_to_device/_read_ctrl
+ * (gpu_resident_batch.py:483-487). */
+int inu_ctrl_write(inu_ctx* ctx, const inu_ctrl* ctrl);
+int inu_ctrl_read(inu_ctx* ctx, inu_ctrl* ctrl);
+
+/* The dt words: [0] this step's dt bits, [1] next-step CFL bound bits
+ * (binding 17, resident.py:6-11). */
+int inu_dt_words(inu_ctx* ctx, uint32_t* out4);
+
+/* Start of a run: block marks, forced dilation, CFL bound of the current
+ * state, step_control(START); replaces _record_prologue
+ * (gpu_resident_batch.py:567-591). */
+int inu_prologue(inu_ctx* ctx);
+
+/* New output window: step_control(RETARGET); replaces _record_retarget
+ * (gpu_resident_batch.py:593-599). */
+int inu_retarget(inu_ctx* ctx);
+
+/* Run n recorded steps and wait; use_graph replays a cached hipGraph of n
+ * steps (recorded on first use). Replaces steps_seq.eval() of
+ * _record_steps (gpu_resident_batch.py:601-607, 717). */
+int inu_steps(inu_ctx* ctx, uint32_t n, int use_graph);
+
+/* n steps launched one by one with HIP events around the two stage kernels:
+ * ms_out3 = {predictor kernel ms, corrector kernel ms, whole span ms}. No
+ * reference counterpart (measurement). */
+int inu_steps_timed(inu_ctx* ctx, uint32_t n, float* ms_out3);
+
+/* Field readback/upload (INU_FIELD_*); replaces download_h/hu/hv,
+ * download_h_residual, download_block_flags, download_clamp_mass
+ * (solver.py:473-497, gpu_resident_batch.py:469-481). */
+int inu_download(inu_ctx* ctx, int which, void* out);
+int inu_upload(inu_ctx* ctx, int which, const void* in);
+
+/* Arithmetic probe on n (a, b) pairs: out5 per pair = {two-sum error of a + b,
+ * a / b, sqrt|a|, a^(4/3), film drag of depth a over dt b}. Generalises the
+ * EFT probe (reference EFT_PROBE_GLSL, resident.py:889-907, and
+ * _check_error_free_arithmetic, solver.py:449-467). */
+int inu_probe(inu_ctx* ctx, uint32_t n, const float* pairs, float* out5, float g, float dry_tol);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INUNDATION_HIP_H */

@@ -1,0 +1,1237 @@
+// MI355X (gfx950) shallow-water step backend: kernels, device runtime, C ABI.
+//
+// One context owns the device copy of one mesh, its state, one HIP stream and
+// the hipGraphs of recorded step blocks. A Heun step is five kernels in stream
+// order (no host round trip):
+//
+//   k_stage<0>      predictor: per active 256-cell block, every edge touching
+//                   the block is evaluated once into LDS, then each cell sums
+//                   its edges in CSR order and updates (h1, hu1, hv1, h1_lo);
+//   k_stage<1>      corrector: same on the predictor state, Heun average,
+//                   the block's sources, wet mark, CFL min for the next step;
+//   k_source_idle   sources of blocks the active set skips this step;
+//   k_source_bound  the next step's dt bound from each wetted source cell;
+//   k_dilate        active = wet over each block's two-ring, plus the list of
+//                   blocks the next step visits (wave-ballot compaction);
+//   k_step_control  the uint64 tick clock and the next dt.
+//
+// The fused stage kernel replaces the reference's flux kernel + gather update
+// (inundation/vulkan/shaders/resident.py:187-287 and :401-516): the per-edge
+// words live in LDS instead of a 6-word-per-edge HBM buffer. An edge whose
+// two sides are dry contributes zero words instead of being skipped; the sums
+// start at +0.0, so adding +0.0 leaves them bit-identical to skipping.
+//
+// The C ABI (include/inundation_hip.h) returns 0 or a negative INU_E* code
+// and keeps a per-context error string.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "swe_math.h"
+#include "../../include/inundation_hip.h"
+
+using inu::Phys;
+
+namespace {
+
+constexpr uint32_t kBlockCells = 256;  // ACTIVE_BLOCK_CELLS; one workgroup per block
+constexpr uint32_t kXcds = 8;          // MI355X: 8 accelerator complex dies
+constexpr float kTicksPerSecond = 1048576.0f;
+constexpr float kDtSentinel = 1e30f;
+constexpr uint32_t kNoBound = 0x7f800000u;  // +inf bits: larger than any bound
+
+// Control words shared by the step kernels. The first 48 bytes are the
+// reference's resident control block (gpu_resident_batch.py:65-77).
+struct Ctl {
+  inu_ctrl ctrl;
+  uint32_t dtbuf[4];  // [0] this step's dt bits (0 = no-op), [1] next-step bound
+  uint32_t cnt_cur;   // blocks in `list` for the current step
+  uint32_t cnt_next;  // blocks appended by k_dilate for the next step
+  uint32_t swap_on;   // 1 when k_dilate runs every step (active set on)
+  uint32_t pad;
+};
+static_assert(sizeof(inu_ctrl) == 48, "control block must be 48 bytes");
+
+// Device pointers and sizes, passed by value to every kernel.
+struct Args {
+  Phys p;
+  uint32_t N, NB, max_local, n_src, track_clamp;
+  // cells
+  float *h, *hu, *hv, *hlo, *h1, *hu1, *hv1, *h1lo, *area, *nman, *clamp, *src;
+  const uint32_t* cptr;
+  const uint16_t* lidx;
+  const int32_t* snb;
+  const float* slen;
+  // edges (sorted by the block of the left cell)
+  const int2* ecell;
+  const float2* enrm;
+  const float* elen;
+  const float2* edz;
+  const uint32_t *eo, *hp, *hx;
+  // blocks
+  uint32_t *wet, *active, *forced, *has_src, *list;
+  const uint32_t *bptr, *bidx;
+  // sources
+  const uint32_t* src_cell;
+  const uint32_t* src_blocks;
+  uint32_t n_src_blocks;
+  Ctl* ctl;
+};
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = (uint32_t)__shfl_xor((int)v, off, 64);
+    v = (o < v) ? o : v;
+  }
+  return v;
+}
+
+// Min of a per-thread float bound (positive or kNoBound) over the workgroup,
+// one atomicMin per wave. Positive floats order like their bit patterns.
+__device__ __forceinline__ void block_atomic_min(uint32_t* dst, uint32_t bits) {
+  const uint32_t m = wave_min_u32(bits);
+  if ((threadIdx.x & 63u) == 0u && m != kNoBound) atomicMin(dst, m);
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernels
+// ---------------------------------------------------------------------------
+
+// Stage input of a cell outside the block being processed: the corrector reads
+// the predictor state only where that cell's block was active this step
+// (reference resident.py:209-226).
+template <int ST>
+__device__ __forceinline__ void load_outside(const Args& a, int c, float& h, float& hu, float& hv) {
+  if (ST == 1 && a.active[(uint32_t)c >> 8] != 0u) {
+    h = a.h1[c];
+    hu = a.hu1[c];
+    hv = a.hv1[c];
+  } else {
+    h = a.h[c];
+    hu = a.hu[c];
+    hv = a.hv[c];
+  }
+}
+
+struct Lds {
+  float* s_h;
+  float* s_hu;
+  float* s_hv;
+  float* w[6];
+};
+
+__device__ __forceinline__ Lds carve(float* sm, uint32_t max_local) {
+  Lds l;
+  l.s_h = sm;
+  l.s_hu = sm + kBlockCells;
+  l.s_hv = sm + 2 * kBlockCells;
+  float* w = sm + 3 * kBlockCells;
+  for (int k = 0; k < 6; ++k) l.w[k] = w + k * max_local;
+  return l;
+}
+
+// Every edge with a side in block b: its words into LDS (zeros when both
+// sides are dry). Owned edges (left cell in b) are a contiguous range; halo
+// edges (right cell in b, left cell elsewhere) come from the block's list.
+template <int ST>
+__device__ __forceinline__ void block_edges(const Args& a, uint32_t b, const Lds& l) {
+  const uint32_t c0 = b << 8;
+  const uint32_t e0 = a.eo[b];
+  const uint32_t n_own = a.eo[b + 1] - e0;
+  const uint32_t x0 = a.hp[b];
+  const uint32_t n_loc = n_own + (a.hp[b + 1] - x0);
+  for (uint32_t j = threadIdx.x; j < n_loc; j += blockDim.x) {
+    const uint32_t e = (j < n_own) ? e0 + j : a.hx[x0 + (j - n_own)];
+    const int2 c = a.ecell[e];
+    const float2 nrm = a.enrm[e];
+    float hL, huL, hvL, hR, huR, hvR;
+    if ((uint32_t)c.x - c0 < kBlockCells) {
+      const uint32_t q = (uint32_t)c.x - c0;
+      hL = l.s_h[q];
+      huL = l.s_hu[q];
+      hvL = l.s_hv[q];
+    } else {
+      load_outside<ST>(a, c.x, hL, huL, hvL);
+    }
+    const bool wall = c.y < 0;
+    if (wall) {
+      hR = hL;
+      inu::wall_ghost(huL, hvL, nrm.x, nrm.y, huR, hvR);
+    } else if ((uint32_t)c.y - c0 < kBlockCells) {
+      const uint32_t q = (uint32_t)c.y - c0;
+      hR = l.s_h[q];
+      huR = l.s_hu[q];
+      hvR = l.s_hv[q];
+    } else {
+      load_outside<ST>(a, c.y, hR, huR, hvR);
+    }
+    float w[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (!(hL < a.p.dry && hR < a.p.dry)) {
+      const float2 dz = a.edz[e];
+      inu::edge_words(a.p, wall, hL, huL, hvL, hR, huR, hvR, dz.x, dz.y, nrm.x, nrm.y, a.elen[e], w);
+    }
+    l.w[0][j] = w[0];
+    l.w[1][j] = w[1];
+    l.w[2][j] = w[2];
+    l.w[3][j] = w[3];
+    l.w[4][j] = w[4];
+    if (ST == 1) l.w[5][j] = w[5];
+  }
+}
+
+// One active block of one Heun stage (reference resident.py:425-512, gather +
+// compensated variants), with the corrector also applying this block's
+// sources (resident.py:691-708).
+template <int ST, bool COMP>
+__device__ void stage_block(const Args& a, uint32_t b, float dt, const Lds& l) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t c0 = b << 8;
+  const uint32_t nc = min(kBlockCells, a.N - c0);
+  __syncthreads();  // the previous block's LDS reads are done
+  if (tid < nc) {
+    const uint32_t i = c0 + tid;
+    l.s_h[tid] = (ST == 0) ? a.h[i] : a.h1[i];
+    l.s_hu[tid] = (ST == 0) ? a.hu[i] : a.hu1[i];
+    l.s_hv[tid] = (ST == 0) ? a.hv[i] : a.hv1[i];
+  }
+  __syncthreads();
+  block_edges<ST>(a, b, l);
+  __syncthreads();
+
+  const Phys& p = a.p;
+  bool wet = false;
+  uint32_t bound = kNoBound;
+  if (tid < nc) {
+    const uint32_t i = c0 + tid;
+    float sdh = 0.0f, sdhu = 0.0f, sdhv = 0.0f, lam = 0.0f;
+    const uint32_t k1 = a.cptr[i + 1];
+    for (uint32_t k = a.cptr[i]; k < k1; ++k) {
+      const uint32_t s = a.lidx[k];
+      const uint32_t j = s >> 1;
+      if (s & 1u) {
+        sdh -= l.w[0][j];
+        sdhu -= l.w[3][j];
+        sdhv -= l.w[4][j];
+      } else {
+        sdh += l.w[0][j];
+        sdhu += l.w[1][j];
+        sdhv += l.w[2][j];
+      }
+      if (ST == 1) lam += l.w[5][j];
+    }
+    const float A = a.area[i];
+    const float dti = dt / A;
+    const float hi = l.s_h[tid];
+    const float hui = l.s_hu[tid];
+    const float hvi = l.s_hv[tid];
+    float h_new, lo_new = 0.0f, h_raw;
+    if (COMP) {
+      const float loi = (ST == 0) ? a.hlo[i] : a.h1lo[i];
+      inu::add_compensated(hi, loi, -dti * sdh, h_new, lo_new);
+      float s_raw, e_raw;
+      inu::two_sum(hi, -dti * sdh, s_raw, e_raw);
+      h_raw = s_raw + (e_raw + loi);
+    } else {
+      h_raw = hi - dti * sdh;
+      h_new = inu::gmax(h_raw, 0.0f);
+    }
+    float hu_new = hui - dti * sdhu;
+    float hv_new = hvi - dti * sdhv;
+    if (a.track_clamp) a.clamp[i] += 0.5f * A * (h_new - h_raw);
+    if (h_new < p.dry) {
+      hu_new = 0.0f;
+      hv_new = 0.0f;
+    } else {
+      const float drag = inu::film_drag(p, h_new, dt);
+      hu_new *= drag;
+      hv_new *= drag;
+      const float n = a.nman[i];
+      const float u = hu_new / h_new;
+      const float v = hv_new / h_new;
+      const float spd = sqrtf(u * u + v * v);
+      const float h43 = inu::pow43(h_new);
+      const float Sf = p.g * n * n * spd / h43;
+      const float den = 1.0f + dt * Sf;
+      hu_new = hu_new / den;
+      hv_new = hv_new / den;
+    }
+    if (ST == 0) {
+      a.h1[i] = h_new;
+      if (COMP) a.h1lo[i] = lo_new;
+      a.hu1[i] = hu_new;
+      a.hv1[i] = hv_new;
+    } else {
+      float hf, lof = 0.0f;
+      if (COMP) {
+        float s, e;
+        inu::two_sum(a.h[i], h_new, s, e);
+        e += a.hlo[i] + lo_new;
+        inu::two_sum(0.5f * s, 0.5f * e, hf, lof);
+      } else {
+        hf = 0.5f * (a.h[i] + h_new);
+      }
+      float huf = 0.5f * (a.hu[i] + hu_new);
+      float hvf = 0.5f * (a.hv[i] + hv_new);
+      if (hf < p.dry) {
+        huf = 0.0f;
+        hvf = 0.0f;
+      }
+      wet = hf >= p.dry;
+      if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
+      // Sources of this block (reference resident.py:695-708), after the
+      // corrector; they change depth only.
+      if (a.n_src != 0u) {
+        const float rate = a.src[i];
+        if (rate != 0.0f) {
+          const float inc = dt * rate;
+          if (COMP) {
+            float hs, ls;
+            inu::add_compensated(hf, lof, inc, hs, ls);
+            hf = hs;
+            lof = ls;
+          } else {
+            hf = inu::gmax(hf + inc, 0.0f);
+          }
+          wet = wet || hf >= p.dry;
+        }
+      }
+      a.h[i] = hf;
+      if (COMP) a.hlo[i] = lof;
+      a.hu[i] = huf;
+      a.hv[i] = hvf;
+    }
+  }
+  if (ST == 1) {
+    const int any_wet = __syncthreads_or(wet ? 1 : 0);
+    if (tid == 0) a.wet[b] = any_wet ? 1u : 0u;
+    block_atomic_min(&a.ctl->dtbuf[1], bound);
+  }
+}
+
+template <int ST, bool COMP>
+__global__ __launch_bounds__(256) void k_stage(const Args a) {
+  extern __shared__ float sm[];
+  const uint32_t dtw = a.ctl->dtbuf[0];
+  if (dtw == 0u) return;  // no-op step: the window's stop was reached
+  const float dt = __uint_as_float(dtw);
+  const Lds l = carve(sm, a.max_local);
+  // XCD-aware split: workgroup w runs on XCD w % 8, so each XCD walks one
+  // contiguous (spatially compact) slice of the block list and its L2 keeps
+  // the halo cells its neighbours read.
+  const uint32_t cnt = a.ctl->cnt_cur;
+  const uint32_t xcd = blockIdx.x % kXcds;
+  const uint32_t per = gridDim.x / kXcds;
+  const uint32_t lo = (uint32_t)(((uint64_t)cnt * xcd) / kXcds);
+  const uint32_t hi = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
+  for (uint32_t t = lo + blockIdx.x / kXcds; t < hi; t += per) {
+    const uint32_t b = a.list[t];
+    if (a.active[b] == 0u) continue;  // listed only for its sources
+    stage_block<ST, COMP>(a, b, dt, l);
+  }
+}
+
+// Sources of blocks the active set skipped this step (reference
+// resident.py:691-708): nothing else reads or writes those cells now.
+template <bool COMP>
+__global__ __launch_bounds__(256) void k_source_idle(const Args a) {
+  const uint32_t dtw = a.ctl->dtbuf[0];
+  if (dtw == 0u) return;
+  const float dt = __uint_as_float(dtw);
+  for (uint32_t t = blockIdx.x; t < a.n_src_blocks; t += gridDim.x) {
+    const uint32_t b = a.src_blocks[t];
+    if (a.active[b] != 0u) continue;  // applied by the corrector
+    const uint32_t i = (b << 8) + threadIdx.x;
+    bool wet = false;
+    if (i < a.N) {
+      const float rate = a.src[i];
+      if (rate != 0.0f) {
+        const float inc = dt * rate;
+        float hn;
+        if (COMP) {
+          float ln;
+          inu::add_compensated(a.h[i], a.hlo[i], inc, hn, ln);
+          a.hlo[i] = ln;
+        } else {
+          hn = inu::gmax(a.h[i] + inc, 0.0f);
+        }
+        a.h[i] = hn;
+        wet = hn >= a.p.dry;
+      }
+    }
+    if (__syncthreads_or(wet ? 1 : 0) && threadIdx.x == 0) a.wet[b] = 1u;
+  }
+}
+
+// Next-step dt bound of each source cell from its post-source depth (reference
+// resident.py:709-728). Neighbour depths are read after every source of the
+// step has been applied.
+__global__ __launch_bounds__(256) void k_source_bound(const Args a) {
+  if (a.ctl->dtbuf[0] == 0u) return;
+  const Phys& p = a.p;
+  uint32_t bound = kNoBound;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < a.n_src) {
+    const uint32_t i = a.src_cell[k];
+    const float hn = a.h[i];
+    if (hn >= p.dry) {
+      const float hui = a.hu[i];
+      const float hvi = a.hv[i];
+      const float u = sqrtf(hui * hui + hvi * hvi) / hn;
+      const float c = sqrtf(p.g * hn);
+      float lam = 0.0f;
+      const uint32_t s1 = a.cptr[i + 1];
+      for (uint32_t s = a.cptr[i]; s < s1; ++s) {
+        const int nb = a.snb[s];
+        const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
+        lam += (u + (dry_side ? 2.0f : 1.0f) * c) * a.slen[s];
+      }
+      const float A = a.area[i];
+      if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
+    }
+  }
+  block_atomic_min(&a.ctl->dtbuf[1], bound);
+}
+
+// ---------------------------------------------------------------------------
+// Active set and clock
+// ---------------------------------------------------------------------------
+
+// Start of a run: wet = a cell is wet or moving; or every block active with
+// the active set off (reference resident.py:738-773). Also resets dt slot 0
+// (cfl.py:130-166) and the list being built.
+__global__ void k_block_mark(const Args a, int all_active) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) {
+    a.ctl->dtbuf[0] = __float_as_uint(kDtSentinel);
+    a.ctl->cnt_next = 0u;
+  }
+  if (b >= a.NB) return;
+  if (all_active) {
+    a.wet[b] = 0u;
+    a.active[b] = 1u;
+    return;
+  }
+  const uint32_t first = b << 8;
+  const uint32_t last = min(first + kBlockCells, a.N);
+  uint32_t live = 0u;
+  for (uint32_t i = first; i < last; ++i) {
+    if (a.h[i] >= a.p.dry || a.hu[i] != 0.0f || a.hv[i] != 0.0f) {
+      live = 1u;
+      break;
+    }
+  }
+  a.wet[b] = live;
+}
+
+// active = (wet | forced) over the block's two-ring (reference
+// resident.py:778-805); blocks to visit next step (active, or holding a
+// source) are appended with one atomic per wave, in block order within it.
+__global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int keep_flags) {
+  if (!force && a.ctl->dtbuf[0] == 0u) return;  // no-op step changed nothing
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  bool take = false;
+  if (b < a.NB) {
+    uint32_t v;
+    if (keep_flags) {
+      v = a.active[b];
+    } else {
+      v = a.wet[b] | a.forced[b];
+      for (uint32_t k = a.bptr[b]; v == 0u && k < a.bptr[b + 1]; ++k) {
+        const uint32_t n = a.bidx[k];
+        v = a.wet[n] | a.forced[n];
+      }
+      a.active[b] = v;
+    }
+    take = (v != 0u) || (a.has_src[b] != 0u);
+  }
+  const uint64_t mask = __ballot(take ? 1 : 0);
+  if (mask == 0ull) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&a.ctl->cnt_next, (uint32_t)__popcll(mask));
+  base = (uint32_t)__shfl((int)base, (int)leader, 64);
+  if (take) base += (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  if (take) a.list[base] = b;
+}
+
+// Prologue CFL bound of the current state, every cell (reference
+// resident.py:528-610): per-edge wave bound into LDS, per-cell CSR sum,
+// atomicMin into dt slot 0.
+__global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
+  extern __shared__ float sm[];
+  const Lds l = carve(sm, a.max_local);
+  const Phys& p = a.p;
+  for (uint32_t b = blockIdx.x; b < a.NB; b += gridDim.x) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t c0 = b << 8;
+    const uint32_t nc = min(kBlockCells, a.N - c0);
+    __syncthreads();
+    if (tid < nc) {
+      l.s_h[tid] = a.h[c0 + tid];
+      l.s_hu[tid] = a.hu[c0 + tid];
+      l.s_hv[tid] = a.hv[c0 + tid];
+    }
+    __syncthreads();
+    const uint32_t e0 = a.eo[b];
+    const uint32_t n_own = a.eo[b + 1] - e0;
+    const uint32_t x0 = a.hp[b];
+    const uint32_t n_loc = n_own + (a.hp[b + 1] - x0);
+    for (uint32_t j = tid; j < n_loc; j += blockDim.x) {
+      const uint32_t e = (j < n_own) ? e0 + j : a.hx[x0 + (j - n_own)];
+      const int2 c = a.ecell[e];
+      const float2 nrm = a.enrm[e];
+      float hL, huL, hvL, hR, huR, hvR;
+      if ((uint32_t)c.x - c0 < kBlockCells) {
+        hL = l.s_h[c.x - c0];
+        huL = l.s_hu[c.x - c0];
+        hvL = l.s_hv[c.x - c0];
+      } else {
+        hL = a.h[c.x];
+        huL = a.hu[c.x];
+        hvL = a.hv[c.x];
+      }
+      if (c.y < 0) {
+        hR = hL;
+        inu::wall_ghost(huL, hvL, nrm.x, nrm.y, huR, hvR);
+      } else if ((uint32_t)c.y - c0 < kBlockCells) {
+        hR = l.s_h[c.y - c0];
+        huR = l.s_hu[c.y - c0];
+        hvR = l.s_hv[c.y - c0];
+      } else {
+        hR = a.h[c.y];
+        huR = a.hu[c.y];
+        hvR = a.hv[c.y];
+      }
+      float contrib = 0.0f;
+      if (hL >= p.dry || hR >= p.dry) {
+        const float2 dz = a.edz[e];
+        contrib = inu::wave_bound(p, hL, huL, hvL, dz.x, hR, huR, hvR, dz.y, nrm.x, nrm.y) * a.elen[e];
+      }
+      l.w[5][j] = contrib;
+    }
+    __syncthreads();
+    uint32_t bound = kNoBound;
+    if (tid < nc) {
+      const uint32_t i = c0 + tid;
+      float lam = 0.0f;
+      for (uint32_t k = a.cptr[i]; k < a.cptr[i + 1]; ++k) lam += l.w[5][a.lidx[k] >> 1];
+      const float A = a.area[i];
+      if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
+    }
+    block_atomic_min(&a.ctl->dtbuf[0], bound);
+  }
+}
+
+// The integer clock (reference resident.py:842-882). Mode 0 START plans from
+// the prologue bound in slot 0, mode 1 STEP advances past the step just taken
+// and plans from slot 1, mode 2 RETARGET re-plans from the last bound. START
+// and every real STEP with dilation also publish the block list it built.
+__global__ void k_step_control(const Args a, int mode) {
+  Ctl* t = a.ctl;
+  inu_ctrl& c = t->ctrl;
+  float raw;
+  bool publish = false;
+  if (mode == 0) {
+    raw = __uint_as_float(t->dtbuf[0]);
+    c.last_raw = raw;
+    publish = true;
+  } else if (mode == 1) {
+    const uint32_t done = c.dt_ticks;
+    c.now_ticks += (uint64_t)done;
+    if (done > 0u) {
+      c.steps += 1ull;
+      raw = __uint_as_float(t->dtbuf[1]);
+      c.last_raw = raw;
+      publish = t->swap_on != 0u;
+    } else {
+      raw = c.last_raw;
+    }
+  } else {
+    raw = c.last_raw;
+  }
+  if (publish) {
+    t->cnt_cur = t->cnt_next;
+    t->cnt_next = 0u;
+  }
+  const uint64_t remaining = (c.stop_ticks > c.now_ticks) ? c.stop_ticks - c.now_ticks : 0ull;
+  uint32_t ticks;
+  if (remaining == 0ull || raw < 1e-10f) {
+    ticks = 0u;
+  } else if (raw >= 1e10f) {
+    ticks = c.dry_dt_ticks;
+  } else {
+    const float cap = (float)c.dt_max_ticks / kTicksPerSecond;
+    ticks = (uint32_t)(inu::gmin(raw * c.cfl_safety, cap) * kTicksPerSecond);
+  }
+  ticks = min(ticks, c.dt_max_ticks);
+  if ((uint64_t)ticks > remaining) ticks = (uint32_t)remaining;
+  c.dt_ticks = ticks;
+  t->dtbuf[0] = __float_as_uint((float)ticks / kTicksPerSecond);
+  t->dtbuf[1] = __float_as_uint(kDtSentinel);
+}
+
+// Arithmetic probes: v[0..n) pairs (a, b) -> two-sum error, a / b, sqrt(a),
+// a^(4/3), film drag of a at dt b (reference solver.py:449-467 checks the
+// two-sum error term survives compilation).
+__global__ void k_probe(const float* in, float* out, uint32_t n, Phys p) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float x = in[2 * k], y = in[2 * k + 1];
+  float s, e;
+  inu::two_sum(x, y, s, e);
+  out[5 * k + 0] = e;
+  out[5 * k + 1] = x / y;
+  out[5 * k + 2] = sqrtf(fabsf(x));
+  out[5 * k + 3] = (x > 0.0f) ? inu::pow43(x) : 0.0f;
+  out[5 * k + 4] = (x > 0.0f) ? inu::film_drag(p, x, y) : 0.0f;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host runtime
+// ===========================================================================
+
+struct inu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int cus = 256;
+  // mesh
+  uint32_t N = 0, E = 0, S = 0, NB = 0, max_local = 0, n_halo = 0;
+  bool has_mesh = false;
+  // params
+  Phys p{};
+  uint32_t flags = INU_FLAG_COMPENSATED | INU_FLAG_ACTIVE_SET;
+  // sources
+  uint32_t n_src = 0, n_src_blocks = 0;
+  // device memory
+  std::vector<void*> allocs;
+  Args a{};
+  Ctl* ctl = nullptr;
+  // launch shape
+  uint32_t grid_stage = 0;
+  size_t lds_stage0 = 0, lds_stage1 = 0;
+  // graphs of recorded step blocks, by step count
+  std::map<uint32_t, hipGraphExec_t> graphs;
+  std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+int fail(inu_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                  \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess)                                                                  \
+      return fail((ctx), INU_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+  } while (0)
+
+template <typename T>
+int dalloc(inu_ctx* c, T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  void* q = nullptr;
+  HIPCHK(c, hipMalloc(&q, n * sizeof(T)));
+  c->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return 0;
+}
+
+template <typename T>
+int upload(inu_ctx* c, T* dst, const T* src, size_t n) {
+  if (n == 0) return 0;
+  HIPCHK(c, hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+void free_all(inu_ctx* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  for (void* q : c->allocs) (void)hipFree(q);
+  c->allocs.clear();
+  c->a = Args{};
+  c->ctl = nullptr;
+  c->has_mesh = false;
+}
+
+Phys make_phys(float g, float dry, float cfl) {
+  Phys p;
+  p.g = g;
+  p.dry = dry;
+  p.cfl = cfl;
+  volatile float fd = 4.0f;  // FILM_DEPTHS
+  p.e_film = fd * dry;
+  p.e_film2 = p.e_film * p.e_film;
+  p.half_g = 0.5f * g;
+  p.sqrt_g_e = std::sqrt(g / p.e_film);
+  return p;
+}
+
+int check_ready(inu_ctx* c) {
+  if (!c) return INU_EINVAL;
+  if (!c->has_mesh) return fail(c, INU_ESTATE, "no mesh: call inu_set_mesh first");
+  return 0;
+}
+
+// Kernel launches of one step, in stream order. With `ev` set, events are
+// recorded before the predictor, between the two stages and after the
+// corrector (no host wait).
+int launch_step(inu_ctx* c, const hipEvent_t* ev) {
+  const Args& a = c->a;
+  const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
+  if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  if (comp)
+    hipLaunchKernelGGL((k_stage<0, true>), dim3(c->grid_stage), dim3(256), c->lds_stage0, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_stage<0, false>), dim3(c->grid_stage), dim3(256), c->lds_stage0, c->stream, a);
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  if (comp)
+    hipLaunchKernelGGL((k_stage<1, true>), dim3(c->grid_stage), dim3(256), c->lds_stage1, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_stage<1, false>), dim3(c->grid_stage), dim3(256), c->lds_stage1, c->stream, a);
+  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  if (c->n_src) {
+    const uint32_t gb = std::min<uint32_t>(c->n_src_blocks, 8u * (uint32_t)c->cus);
+    if (comp)
+      hipLaunchKernelGGL((k_source_idle<true>), dim3(std::max(gb, 1u)), dim3(256), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_source_idle<false>), dim3(std::max(gb, 1u)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_source_bound, dim3((c->n_src + 255) / 256), dim3(256), 0, c->stream, a);
+  }
+  if (c->flags & INU_FLAG_ACTIVE_SET)
+    hipLaunchKernelGGL(k_dilate, dim3((c->NB + 255) / 256), dim3(256), 0, c->stream, a, 0, 0);
+  hipLaunchKernelGGL(k_step_control, dim3(1), dim3(1), 0, c->stream, a, 1);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+int inu_version(void) { return INU_ABI_VERSION; }
+
+int inu_create(int device, inu_ctx** out) {
+  if (!out) return INU_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return INU_ENODEV;
+  if (device < 0 || device >= n) return INU_EINVAL;
+  inu_ctx* c = new inu_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return INU_EHIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
+  c->p = make_phys(9.81f, 1e-4f, 0.8f);
+  *out = c;
+  return 0;
+}
+
+void inu_destroy(inu_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* inu_last_error(const inu_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* edge_cell_l,
+                 const int32_t* edge_cell_r, const float* edge_nx, const float* edge_ny,
+                 const float* edge_len, const float* edge_dz, const float* cell_area,
+                 const uint32_t* cell_edge_ptr, const uint32_t* cell_edge_idx) {
+  if (!c) return INU_EINVAL;
+  if (n_cells == 0 || n_edges == 0 || !edge_cell_l || !edge_cell_r || !edge_nx || !edge_ny ||
+      !edge_len || !edge_dz || !cell_area || !cell_edge_ptr || !cell_edge_idx)
+    return fail(c, INU_EINVAL, "inu_set_mesh: empty mesh or null array");
+  HIPCHK(c, hipSetDevice(c->device));
+  const uint32_t N = n_cells, E = n_edges, NB = (N + kBlockCells - 1) / kBlockCells;
+  const uint32_t S = cell_edge_ptr[N];
+  if (cell_edge_ptr[0] != 0) return fail(c, INU_EINVAL, "cell_edge_ptr[0] must be 0");
+  // Validate the edge order and the signed CSR.
+  std::vector<uint32_t> eo(NB + 1, 0);
+  for (uint32_t e = 0; e < E; ++e) {
+    const int32_t l = edge_cell_l[e], r = edge_cell_r[e];
+    if (l < 0 || (uint32_t)l >= N) return fail(c, INU_EINVAL, "edge_cell_l out of range");
+    if (r < -1 || r >= (int64_t)N) return fail(c, INU_EINVAL, "edge_cell_r out of range (open edges are not supported)");
+    if (e > 0 && (uint32_t)l / kBlockCells < (uint32_t)edge_cell_l[e - 1] / kBlockCells)
+      return fail(c, INU_EINVAL, "edges must be sorted by the block of their left cell");
+    eo[(uint32_t)l / kBlockCells + 1]++;
+  }
+  for (uint32_t b = 0; b < NB; ++b) eo[b + 1] += eo[b];
+  // Halo edges: right cell in b, left cell in another block; ascending ids.
+  std::vector<uint32_t> hp(NB + 1, 0);
+  for (uint32_t e = 0; e < E; ++e) {
+    const int32_t r = edge_cell_r[e];
+    if (r >= 0 && (uint32_t)r / kBlockCells != (uint32_t)edge_cell_l[e] / kBlockCells) hp[(uint32_t)r / kBlockCells + 1]++;
+  }
+  for (uint32_t b = 0; b < NB; ++b) hp[b + 1] += hp[b];
+  std::vector<uint32_t> hx(hp[NB]);
+  {
+    std::vector<uint32_t> fill(hp.begin(), hp.end() - 1);
+    for (uint32_t e = 0; e < E; ++e) {
+      const int32_t r = edge_cell_r[e];
+      if (r >= 0 && (uint32_t)r / kBlockCells != (uint32_t)edge_cell_l[e] / kBlockCells) hx[fill[(uint32_t)r / kBlockCells]++] = e;
+    }
+  }
+  uint32_t max_local = 1;
+  for (uint32_t b = 0; b < NB; ++b)
+    max_local = std::max(max_local, (eo[b + 1] - eo[b]) + (hp[b + 1] - hp[b]));
+  if (max_local >= 32768u) return fail(c, INU_ERANGE, "a block touches more than 32767 edges");
+  // Local CSR: each slot's edge as its position in its block's LDS edge list.
+  std::vector<uint16_t> lidx(S);
+  std::vector<int32_t> snb(S);
+  std::vector<float> slen(S);
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t b = i / kBlockCells;
+    if (cell_edge_ptr[i + 1] < cell_edge_ptr[i] || cell_edge_ptr[i + 1] > S)
+      return fail(c, INU_EINVAL, "cell_edge_ptr must be non-decreasing and end at the slot count");
+    for (uint32_t k = cell_edge_ptr[i]; k < cell_edge_ptr[i + 1]; ++k) {
+      const uint32_t s = cell_edge_idx[k], e = s >> 1, right = s & 1u;
+      if (e >= E) return fail(c, INU_EINVAL, "cell_edge_idx names an edge out of range");
+      const int32_t own = right ? edge_cell_r[e] : edge_cell_l[e];
+      if (own != (int32_t)i) return fail(c, INU_EINVAL, "cell_edge_idx entry does not belong to its cell");
+      uint32_t j;
+      if ((uint32_t)edge_cell_l[e] / kBlockCells == b) {
+        j = e - eo[b];
+      } else {
+        const uint32_t* lo = hx.data() + hp[b];
+        const uint32_t* hi = hx.data() + hp[b + 1];
+        const uint32_t* f = std::lower_bound(lo, hi, e);
+        if (f == hi || *f != e) return fail(c, INU_EINVAL, "internal: halo edge missing");
+        j = (eo[b + 1] - eo[b]) + (uint32_t)(f - lo);
+      }
+      lidx[k] = (uint16_t)((j << 1) | right);
+      snb[k] = right ? edge_cell_l[e] : edge_cell_r[e];
+      slen[k] = edge_len[e];
+    }
+  }
+  // Two-ring block graph (reference blocks.py:63-118).
+  std::vector<std::vector<uint32_t>> adj(NB);
+  for (uint32_t e = 0; e < E; ++e) {
+    const int32_t r = edge_cell_r[e];
+    if (r < 0) continue;
+    const uint32_t bl = (uint32_t)edge_cell_l[e] / kBlockCells, br = (uint32_t)r / kBlockCells;
+    if (bl != br) {
+      adj[bl].push_back(br);
+      adj[br].push_back(bl);
+    }
+  }
+  for (auto& v : adj) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  std::vector<uint32_t> bptr(NB + 1, 0), bidx;
+  for (uint32_t b = 0; b < NB; ++b) {
+    std::vector<uint32_t> ring(adj[b]);
+    for (uint32_t n : adj[b]) ring.insert(ring.end(), adj[n].begin(), adj[n].end());
+    std::sort(ring.begin(), ring.end());
+    ring.erase(std::unique(ring.begin(), ring.end()), ring.end());
+    for (uint32_t n : ring)
+      if (n != b) bidx.push_back(n);
+    bptr[b + 1] = (uint32_t)bidx.size();
+  }
+
+  free_all(c);
+  c->N = N;
+  c->E = E;
+  c->S = S;
+  c->NB = NB;
+  c->max_local = max_local;
+  c->n_halo = hp[NB];
+  c->n_src = 0;
+  c->n_src_blocks = 0;
+  Args& a = c->a;
+  const size_t NP = (size_t)NB * kBlockCells;
+  int rc = 0;
+  rc |= dalloc(c, &a.h, NP); rc |= dalloc(c, &a.hu, NP); rc |= dalloc(c, &a.hv, NP); rc |= dalloc(c, &a.hlo, NP);
+  rc |= dalloc(c, &a.h1, NP); rc |= dalloc(c, &a.hu1, NP); rc |= dalloc(c, &a.hv1, NP); rc |= dalloc(c, &a.h1lo, NP);
+  rc |= dalloc(c, &a.area, NP); rc |= dalloc(c, &a.nman, NP); rc |= dalloc(c, &a.clamp, NP); rc |= dalloc(c, &a.src, NP);
+  uint32_t* cptr; uint16_t* lid; int32_t* sn; float* sl;
+  rc |= dalloc(c, &cptr, (size_t)N + 1); rc |= dalloc(c, &lid, S); rc |= dalloc(c, &sn, S); rc |= dalloc(c, &sl, S);
+  int2* ecell; float2* enrm; float* elen; float2* edz;
+  rc |= dalloc(c, &ecell, E); rc |= dalloc(c, &enrm, E); rc |= dalloc(c, &elen, E); rc |= dalloc(c, &edz, E);
+  uint32_t *deo, *dhp, *dhx, *dbptr, *dbidx;
+  rc |= dalloc(c, &deo, NB + 1); rc |= dalloc(c, &dhp, NB + 1); rc |= dalloc(c, &dhx, hx.size());
+  rc |= dalloc(c, &dbptr, NB + 1); rc |= dalloc(c, &dbidx, bidx.size());
+  rc |= dalloc(c, &a.wet, NB); rc |= dalloc(c, &a.active, NB); rc |= dalloc(c, &a.forced, NB);
+  rc |= dalloc(c, &a.has_src, NB); rc |= dalloc(c, &a.list, NB);
+  uint32_t* src_cell; uint32_t* src_blocks;
+  rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1);
+  rc |= dalloc(c, &c->ctl, 1);
+  if (rc) return rc;
+  a.cptr = cptr; a.lidx = lid; a.snb = sn; a.slen = sl;
+  a.ecell = ecell; a.enrm = enrm; a.elen = elen; a.edz = edz;
+  a.eo = deo; a.hp = dhp; a.hx = dhx; a.bptr = dbptr; a.bidx = dbidx;
+  a.src_cell = src_cell; a.src_blocks = src_blocks;
+  a.ctl = c->ctl;
+  a.N = N; a.NB = NB; a.max_local = max_local;
+  // Host-side repack of the edge record into vector-loadable SoA.
+  std::vector<int2> h_ecell(E);
+  std::vector<float2> h_nrm(E), h_dz(E);
+  for (uint32_t e = 0; e < E; ++e) {
+    h_ecell[e] = make_int2(edge_cell_l[e], edge_cell_r[e]);
+    h_nrm[e] = make_float2(edge_nx[e], edge_ny[e]);
+    h_dz[e] = make_float2(edge_dz[2 * e], edge_dz[2 * e + 1]);
+  }
+  HIPCHK(c, hipMemsetAsync(a.h, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hu, 0, NP * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.hv, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hlo, 0, NP * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.h1, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hu1, 0, NP * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.hv1, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.h1lo, 0, NP * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.area, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.clamp, 0, NP * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.src, 0, NP * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.wet, 0, NB * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.active, 0, NB * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.forced, 0, NB * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.has_src, 0, NB * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->ctl, 0, sizeof(Ctl), c->stream));
+  std::vector<float> nm(NP, 0.03f);
+  if ((rc = upload(c, a.nman, nm.data(), NP))) return rc;
+  if ((rc = upload(c, a.area, cell_area, N))) return rc;
+  if ((rc = upload(c, cptr, cell_edge_ptr, (size_t)N + 1))) return rc;
+  if ((rc = upload(c, lid, lidx.data(), S))) return rc;
+  if ((rc = upload(c, sn, snb.data(), S))) return rc;
+  if ((rc = upload(c, sl, slen.data(), S))) return rc;
+  if ((rc = upload(c, ecell, h_ecell.data(), E))) return rc;
+  if ((rc = upload(c, enrm, h_nrm.data(), E))) return rc;
+  if ((rc = upload(c, elen, edge_len, E))) return rc;
+  if ((rc = upload(c, edz, h_dz.data(), E))) return rc;
+  if ((rc = upload(c, deo, eo.data(), NB + 1))) return rc;
+  if ((rc = upload(c, dhp, hp.data(), NB + 1))) return rc;
+  if ((rc = upload(c, dhx, hx.data(), hx.size()))) return rc;
+  if ((rc = upload(c, dbptr, bptr.data(), NB + 1))) return rc;
+  if ((rc = upload(c, dbidx, bidx.data(), bidx.size()))) return rc;
+  c->lds_stage0 = (3 * kBlockCells + 6 * (size_t)max_local) * sizeof(float);
+  c->lds_stage1 = c->lds_stage0;
+  if (c->lds_stage1 > 160 * 1024) return fail(c, INU_ERANGE, "a block's edge list does not fit in LDS");
+  int occ = 0;
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_stage<1, true>, 256, c->lds_stage1));
+  occ = std::max(occ, 1);
+  uint32_t g = (uint32_t)(occ * c->cus);
+  g = std::max<uint32_t>(kXcds, std::min<uint32_t>(g, ((NB + kXcds - 1) / kXcds) * kXcds));
+  c->grid_stage = (g / kXcds) * kXcds;
+  c->a.p = c->p;
+  c->has_mesh = true;
+  return 0;
+}
+
+int inu_mesh_info(inu_ctx* c, uint32_t* out6) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!out6) return fail(c, INU_EINVAL, "null output");
+  out6[0] = c->N; out6[1] = c->E; out6[2] = c->S; out6[3] = c->NB; out6[4] = c->max_local;
+  out6[5] = c->grid_stage;
+  return 0;
+}
+
+int inu_set_params(inu_ctx* c, float g, float dry_tol, float cfl, uint32_t flags) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!(g > 0.0f) || !(dry_tol > 0.0f) || !(cfl > 0.0f) || !std::isfinite(g) || !std::isfinite(dry_tol) || !std::isfinite(cfl))
+    return fail(c, INU_EINVAL, "g, dry_tol and cfl must be finite and positive");
+  if (!(flags & INU_FLAG_DETERMINISTIC))
+    return fail(c, INU_EINVAL, "only deterministic (fixed-order gather) accumulation is implemented");
+  if ((flags ^ c->flags) & ~INU_FLAG_TRACK_CLAMP) {
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+  }
+  c->p = make_phys(g, dry_tol, cfl);
+  c->flags = flags;
+  c->a.p = c->p;
+  c->a.track_clamp = (flags & INU_FLAG_TRACK_CLAMP) ? 1u : 0u;
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  uint32_t swap = (flags & INU_FLAG_ACTIVE_SET) ? 1u : 0u;
+  HIPCHK(c, hipMemcpyAsync(&c->ctl->swap_on, &swap, 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inu_set_state(inu_ctx* c, const float* h, const float* hu, const float* hv, const float* h_lo) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!h || !hu || !hv) return fail(c, INU_EINVAL, "inu_set_state: null array");
+  if ((rc = upload(c, c->a.h, h, c->N))) return rc;
+  if ((rc = upload(c, c->a.hu, hu, c->N))) return rc;
+  if ((rc = upload(c, c->a.hv, hv, c->N))) return rc;
+  if (h_lo) {
+    if ((rc = upload(c, c->a.hlo, h_lo, c->N))) return rc;
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->a.hlo, 0, (size_t)c->N * 4, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int inu_set_manning(inu_ctx* c, const float* n) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!n) return fail(c, INU_EINVAL, "inu_set_manning: null array");
+  return upload(c, c->a.nman, n, c->N);
+}
+
+int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* rate) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (n && (!cell || !rate)) return fail(c, INU_EINVAL, "inu_set_sources: null array");
+  std::vector<float> dense((size_t)c->NB * kBlockCells, 0.0f);
+  std::vector<uint32_t> has(c->NB, 0), cells;
+  cells.reserve(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    if (cell[k] >= c->N) return fail(c, INU_EINVAL, "source cell out of range");
+    if (!std::isfinite(rate[k])) return fail(c, INU_EINVAL, "source rate must be finite");
+    if (rate[k] == 0.0f) continue;
+    if (dense[cell[k]] != 0.0f) return fail(c, INU_EINVAL, "a source cell is listed twice");
+    dense[cell[k]] = rate[k];
+    cells.push_back(cell[k]);
+    has[cell[k] / kBlockCells] = 1;
+  }
+  std::sort(cells.begin(), cells.end());
+  std::vector<uint32_t> blocks;
+  for (uint32_t b = 0; b < c->NB; ++b)
+    if (has[b]) blocks.push_back(b);
+  // Replace the source lists (their sizes change).
+  for (void* q : {(void*)c->a.src_cell, (void*)c->a.src_blocks}) {
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
+    if (it != c->allocs.end()) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      (void)hipFree(q);
+      c->allocs.erase(it);
+    }
+  }
+  c->a.src_cell = nullptr;
+  c->a.src_blocks = nullptr;
+  uint32_t *dcell = nullptr, *dblk = nullptr;
+  int rc2 = dalloc(c, &dcell, cells.size());
+  if (rc2) return rc2;
+  rc2 = dalloc(c, &dblk, blocks.size());
+  if (rc2) return rc2;
+  if ((rc = upload(c, dcell, cells.data(), cells.size()))) return rc;
+  if ((rc = upload(c, dblk, blocks.data(), blocks.size()))) return rc;
+  if ((rc = upload(c, c->a.src, dense.data(), dense.size()))) return rc;
+  if ((rc = upload(c, c->a.has_src, has.data(), has.size()))) return rc;
+  c->a.src_cell = dcell;
+  c->a.src_blocks = dblk;
+  c->n_src = (uint32_t)cells.size();
+  c->n_src_blocks = (uint32_t)blocks.size();
+  c->a.n_src = c->n_src;
+  c->a.n_src_blocks = c->n_src_blocks;
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  return 0;
+}
+
+int inu_ctrl_write(inu_ctx* c, const inu_ctrl* ctrl) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!ctrl) return fail(c, INU_EINVAL, "null ctrl");
+  HIPCHK(c, hipMemcpyAsync(&c->ctl->ctrl, ctrl, sizeof(inu_ctrl), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inu_ctrl_read(inu_ctx* c, inu_ctrl* ctrl) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!ctrl) return fail(c, INU_EINVAL, "null ctrl");
+  HIPCHK(c, hipMemcpyAsync(ctrl, &c->ctl->ctrl, sizeof(inu_ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inu_dt_words(inu_ctx* c, uint32_t* out4) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!out4) return fail(c, INU_EINVAL, "null output");
+  HIPCHK(c, hipMemcpyAsync(out4, c->ctl->dtbuf, 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inu_prologue(inu_ctx* c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  const Args& a = c->a;
+  const bool act = (c->flags & INU_FLAG_ACTIVE_SET) != 0;
+  const uint32_t gb = (c->NB + 255) / 256;
+  hipLaunchKernelGGL(k_block_mark, dim3(gb), dim3(256), 0, c->stream, a, act ? 0 : 1);
+  hipLaunchKernelGGL(k_dilate, dim3(gb), dim3(256), 0, c->stream, a, 1, act ? 0 : 1);
+  const uint32_t gp = std::min<uint32_t>(c->NB, c->grid_stage);
+  hipLaunchKernelGGL(k_prologue_cfl, dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
+  hipLaunchKernelGGL(k_step_control, dim3(1), dim3(1), 0, c->stream, a, 0);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inu_retarget(inu_ctx* c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_step_control, dim3(1), dim3(1), 0, c->stream, c->a, 2);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inu_steps(inu_ctx* c, uint32_t n, int use_graph) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  if (!use_graph) {
+    for (uint32_t k = 0; k < n; ++k)
+      if ((rc = launch_step(c, nullptr))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+  }
+  auto it = c->graphs.find(n);
+  if (it == c->graphs.end()) {
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t k = 0; k < n; ++k) {
+      if ((rc = launch_step(c, nullptr))) {
+        hipGraph_t dummy = nullptr;
+        (void)hipStreamEndCapture(c->stream, &dummy);
+        if (dummy) (void)hipGraphDestroy(dummy);
+        return rc;
+      }
+    }
+    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(c, INU_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    it = c->graphs.emplace(n, ex).first;
+  }
+  HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inu_steps_timed(inu_ctx* c, uint32_t n, float* ms_out3) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!ms_out3) return fail(c, INU_EINVAL, "null output");
+  if (n == 0 || n > 100000u) return fail(c, INU_EINVAL, "inu_steps_timed: n must be in [1, 100000]");
+  const size_t need = 3 * (size_t)n;
+  while (c->events.size() < need) {
+    hipEvent_t e;
+    HIPCHK(c, hipEventCreate(&e));
+    c->events.push_back(e);
+  }
+  for (uint32_t k = 0; k < n; ++k)
+    if ((rc = launch_step(c, &c->events[3 * (size_t)k]))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double t0 = 0.0, t1 = 0.0;
+  for (uint32_t k = 0; k < n; ++k) {
+    float a0 = 0.0f, a1 = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&a0, c->events[3 * (size_t)k], c->events[3 * (size_t)k + 1]));
+    HIPCHK(c, hipEventElapsedTime(&a1, c->events[3 * (size_t)k + 1], c->events[3 * (size_t)k + 2]));
+    t0 += a0;
+    t1 += a1;
+  }
+  float tot = 0.0f;
+  HIPCHK(c, hipEventElapsedTime(&tot, c->events[0], c->events[3 * (size_t)(n - 1) + 2]));
+  ms_out3[0] = (float)t0;
+  ms_out3[1] = (float)t1;
+  ms_out3[2] = tot;
+  return 0;
+}
+
+int inu_download(inu_ctx* c, int which, void* out) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!out) return fail(c, INU_EINVAL, "null output");
+  const void* src = nullptr;
+  size_t bytes = (size_t)c->N * 4;
+  switch (which) {
+    case INU_FIELD_H: src = c->a.h; break;
+    case INU_FIELD_HU: src = c->a.hu; break;
+    case INU_FIELD_HV: src = c->a.hv; break;
+    case INU_FIELD_H_LO: src = c->a.hlo; break;
+    case INU_FIELD_H1: src = c->a.h1; break;
+    case INU_FIELD_HU1: src = c->a.hu1; break;
+    case INU_FIELD_HV1: src = c->a.hv1; break;
+    case INU_FIELD_H1_LO: src = c->a.h1lo; break;
+    case INU_FIELD_CLAMP: src = c->a.clamp; break;
+    case INU_FIELD_BLOCK_WET: src = c->a.wet; bytes = (size_t)c->NB * 4; break;
+    case INU_FIELD_BLOCK_ACTIVE: src = c->a.active; bytes = (size_t)c->NB * 4; break;
+    case INU_FIELD_BLOCK_FORCED: src = c->a.forced; bytes = (size_t)c->NB * 4; break;
+    default: return fail(c, INU_EINVAL, "unknown field");
+  }
+  HIPCHK(c, hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inu_upload(inu_ctx* c, int which, const void* in) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!in) return fail(c, INU_EINVAL, "null input");
+  void* dst = nullptr;
+  switch (which) {
+    case INU_FIELD_H: dst = c->a.h; break;
+    case INU_FIELD_HU: dst = c->a.hu; break;
+    case INU_FIELD_HV: dst = c->a.hv; break;
+    case INU_FIELD_H_LO: dst = c->a.hlo; break;
+    case INU_FIELD_H1: dst = c->a.h1; break;
+    case INU_FIELD_HU1: dst = c->a.hu1; break;
+    case INU_FIELD_HV1: dst = c->a.hv1; break;
+    case INU_FIELD_H1_LO: dst = c->a.h1lo; break;
+    case INU_FIELD_CLAMP: dst = c->a.clamp; break;
+    default: return fail(c, INU_EINVAL, "field cannot be uploaded");
+  }
+  return upload(c, static_cast<float*>(dst), static_cast<const float*>(in), c->N);
+}
+
+int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out5, float g, float dry_tol) {
+  if (!c) return INU_EINVAL;
+  if (!pairs || !out5) return fail(c, INU_EINVAL, "null array");
+  if (n == 0) return 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  float *din = nullptr, *dout = nullptr;
+  HIPCHK(c, hipMalloc(&din, (size_t)n * 8));
+  hipError_t e = hipMalloc(&dout, (size_t)n * 20);
+  if (e != hipSuccess) {
+    (void)hipFree(din);
+    return fail(c, INU_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  e = hipMemcpy(din, pairs, (size_t)n * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_probe, dim3((n + 255) / 256), dim3(256), 0, c->stream, din, dout, n, make_phys(g, dry_tol, 0.8f));
+    e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpy(out5, dout, (size_t)n * 20, hipMemcpyDeviceToHost);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("probe: ") + hipGetErrorString(e));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+// Device arithmetic of one shallow-water edge and cell, FP32, for gfx950.
+//
+// Every function is a sequence of IEEE-754 single-precision operations
+// (+, -, *, / and sqrt, all correctly rounded; the library is compiled with
+// -ffp-contract=off so no multiply-add is fused behind our back). The order of
+// operations inside each expression follows the reference GLSL, so results
+// depend only on the inputs:
+//   safe_vel         inundation/vulkan/shaders/flux.py:63-66
+//   hllc_speeds      inundation/vulkan/shaders/flux.py:75-95
+//   hllc             inundation/vulkan/shaders/flux.py:98-140
+//   face_states      inundation/vulkan/shaders/flux.py:146-169
+//   wave_bound       inundation/vulkan/shaders/flux.py:172-182
+//   film_drag        inundation/vulkan/shaders/flux.py:46-51
+//   two_sum / add_compensated   inundation/vulkan/shaders/flux.py:301-319
+// GLSL min/max/clamp are spelled out as comparisons (GLSL defines
+// min(x, y) = y < x ? y : x), so signed zeros come out the same everywhere.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace inu {
+
+// Scalar parameters of a run, with the float products the reference forms in
+// every call precomputed once (same single rounding, so the same bits).
+struct Phys {
+  float g;         // gravity
+  float dry;       // dry_tol
+  float cfl;       // CFL number
+  float e_film;    // FILM_DEPTHS * dry_tol
+  float e_film2;   // e_film * e_film
+  float half_g;    // 0.5f * g   (== g * 0.5f)
+  float sqrt_g_e;  // sqrt(g / e_film)
+};
+
+__device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }
+__device__ __forceinline__ float gmax(float x, float y) { return (x < y) ? y : x; }
+__device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+
+__device__ __forceinline__ float safe_vel(const Phys& p, float h, float hq) {
+  return (h > p.dry) ? (2.0f * h * hq) / (h * h + gmax(h * h, p.e_film2)) : 0.0f;
+}
+
+__device__ __forceinline__ void hllc_speeds(const Phys& p, float hL, float unL, float hR, float unR,
+                                            float& SL, float& SR) {
+  const float cL = sqrtf(p.g * gmax(hL, 0.0f));
+  const float cR = sqrtf(p.g * gmax(hR, 0.0f));
+  const float sqL = sqrtf(gmax(hL, 0.0f));
+  const float sqR = sqrtf(gmax(hR, 0.0f));
+  const float den = sqL + sqR;
+  const float u_roe = (den > 1e-12f) ? (sqL * unL + sqR * unR) / den : 0.0f;
+  const float c_roe = sqrtf(p.half_g * (hL + hR));
+  float sl = gmin(unL - cL, u_roe - c_roe);
+  float sr = gmax(unR + cR, u_roe + c_roe);
+  const float wetL = gclamp(hL / p.dry - 1.0f, 0.0f, 1.0f);
+  const float wetR = gclamp(hR / p.dry - 1.0f, 0.0f, 1.0f);
+  const float dryLeft = (1.0f - wetL) * wetR;
+  const float dryRight = (1.0f - wetR) * wetL;
+  const float roe = 1.0f - dryLeft - dryRight;
+  SL = roe * sl + dryLeft * (unR - 2.0f * cR) + dryRight * (unL - cL);
+  SR = roe * sr + dryLeft * (unR + cR) + dryRight * (unL + 2.0f * cL);
+}
+
+__device__ __forceinline__ void hllc(const Phys& p, float hL, float hqnL, float hqtL, float hR,
+                                     float hqnR, float hqtR, float& Fh, float& Fhn, float& Fht,
+                                     float& Smax) {
+  const float unL = safe_vel(p, hL, hqnL);
+  const float utL = safe_vel(p, hL, hqtL);
+  const float unR = safe_vel(p, hR, hqnR);
+  const float utR = safe_vel(p, hR, hqtR);
+  float SL, SR;
+  hllc_speeds(p, hL, unL, hR, unR, SL, SR);
+  Smax = gmax(fabsf(SL), fabsf(SR));
+
+  const float num_sm = SL * hR * (unR - SR) - SR * hL * (unL - SL);
+  const float den_sm = hR * (unR - SR) - hL * (unL - SL);
+  const float SM = (fabsf(den_sm) > 1e-12f) ? num_sm / den_sm : 0.0f;
+
+  const float FL_h = hL * unL;
+  const float FL_hn = hL * unL * unL + p.half_g * hL * hL;
+  const float FL_ht = hL * unL * utL;
+  const float FR_h = hR * unR;
+  const float FR_hn = hR * unR * unR + p.half_g * hR * hR;
+  const float FR_ht = hR * unR * utR;
+
+  // Region of the fan that holds the face (Toro 2001, eq. 10.26); only the
+  // star state that is used is formed.
+  if (SL >= 0.0f) {
+    Fh = FL_h; Fhn = FL_hn; Fht = FL_ht;
+  } else if (SM >= 0.0f) {
+    const float dSL = SL - SM;
+    const float kL = hL * (SL - unL) / ((fabsf(dSL) > 1e-14f) ? dSL : 1e-14f);
+    Fh = FL_h + SL * (kL - hL);
+    Fhn = FL_hn + SL * (kL * SM - hL * unL);
+    Fht = FL_ht + SL * (kL * utL - hL * utL);
+  } else if (SR >= 0.0f) {
+    const float dSR = SR - SM;
+    const float kR = hR * (SR - unR) / ((fabsf(dSR) > 1e-14f) ? dSR : 1e-14f);
+    Fh = FR_h + SR * (kR - hR);
+    Fhn = FR_hn + SR * (kR * SM - hR * unR);
+    Fht = FR_ht + SR * (kR * utR - hR * utR);
+  } else {
+    Fh = FR_h; Fhn = FR_hn; Fht = FR_ht;
+  }
+}
+
+__device__ __forceinline__ void face_states(const Phys& p, float hL, float huL, float hvL, float dzL,
+                                            float hR, float huR, float hvR, float dzR, float nx,
+                                            float ny, float& hLs, float& hqnL, float& hqtL,
+                                            float& hRs, float& hqnR, float& hqtR) {
+  hLs = gmax(0.0f, hL - dzL);
+  hRs = gmax(0.0f, hR - dzR);
+  const float uL = safe_vel(p, hL, huL);
+  const float vL = safe_vel(p, hL, hvL);
+  const float uR = safe_vel(p, hR, huR);
+  const float vR = safe_vel(p, hR, hvR);
+  const float huLs = hLs * uL;
+  const float hvLs = hLs * vL;
+  const float huRs = hRs * uR;
+  const float hvRs = hRs * vR;
+  hqnL = huLs * nx + hvLs * ny;
+  hqtL = -huLs * ny + hvLs * nx;
+  hqnR = huRs * nx + hvRs * ny;
+  hqtR = -huRs * ny + hvRs * nx;
+}
+
+__device__ __forceinline__ float wave_bound(const Phys& p, float hL, float huL, float hvL, float dzL,
+                                            float hR, float huR, float hvR, float dzR, float nx,
+                                            float ny) {
+  float hLs, hqnL, hqtL, hRs, hqnR, hqtR;
+  face_states(p, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx, ny, hLs, hqnL, hqtL, hRs, hqnR, hqtR);
+  float SL, SR;
+  hllc_speeds(p, hLs, safe_vel(p, hLs, hqnL), hRs, safe_vel(p, hRs, hqnR), SL, SR);
+  return gmax(fabsf(SL), fabsf(SR));
+}
+
+// Reflective wall: the ghost mirrors the inner state with the normal momentum
+// negated (reference resident.py:126-129).
+__device__ __forceinline__ void wall_ghost(float huL, float hvL, float nx, float ny, float& huR,
+                                           float& hvR) {
+  huR = huL - 2.0f * (huL * nx + hvL * ny) * nx;
+  hvR = hvL - 2.0f * (huL * nx + hvL * ny) * ny;
+}
+
+// The per-edge words of one live edge: mass flux, left momentum (x, y), right
+// momentum (x, y), all times the edge length, and the CFL term Smax * len
+// (reference resident.py:165-184 with flux.py:98-169, 271-272).
+__device__ __forceinline__ void edge_words(const Phys& p, bool wall, float hL, float huL, float hvL,
+                                           float hR, float huR, float hvR, float dzL, float dzR,
+                                           float nx, float ny, float len, float w[6]) {
+  float hLs, hqnL, hqtL, hRs, hqnR, hqtR;
+  face_states(p, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx, ny, hLs, hqnL, hqtL, hRs, hqnR, hqtR);
+  float Fh, Fhn, Fht, Smax;
+  hllc(p, hLs, hqnL, hqtL, hRs, hqnR, hqtR, Fh, Fhn, Fht, Smax);
+  if (wall) Fh = 0.0f;  // a wall passes no mass
+  const float Fhu = Fhn * nx - Fht * ny;
+  const float Fhv = Fhn * ny + Fht * nx;
+  const float SL_g = p.half_g * (hL * hL - hLs * hLs);
+  const float SR_g = p.half_g * (hR * hR - hRs * hRs);
+  w[0] = Fh * len;
+  w[1] = Fhu * len + SL_g * nx * len;
+  w[2] = Fhv * len + SL_g * ny * len;
+  w[3] = Fhu * len + SR_g * nx * len;
+  w[4] = Fhv * len + SR_g * ny * len;
+  w[5] = Smax * len;
+}
+
+__device__ __forceinline__ float film_drag(const Phys& p, float h, float dt) {
+  if (h >= p.e_film) return 1.0f;
+  const float rate = (p.e_film2 - h * h) / (2.0f * h * h) * p.sqrt_g_e;
+  return 1.0f / (1.0f + dt * rate);
+}
+
+// Knuth two-sum: s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  const float t = a + b;
+  const float bb = t - a;
+  e = (a - (t - bb)) + (b - bb);
+  s = t;
+}
+
+// (hi + lo) + inc as a renormalised pair, clamped at zero.
+__device__ __forceinline__ void add_compensated(float hi, float lo, float inc, float& h_new,
+                                                float& lo_new) {
+  float s, e;
+  two_sum(hi, inc, s, e);
+  e += lo;
+  two_sum(s, e, h_new, lo_new);
+  if (h_new < 0.0f || (h_new == 0.0f && lo_new < 0.0f)) {
+    h_new = 0.0f;
+    lo_new = 0.0f;
+  }
+}
+
+// x^(1/3) for positive normal x, from exact exponent splitting and two Halley
+// steps on the mantissa in [1, 8). Only IEEE operations, so the result is the
+// same on every conforming device; within 2 ulp of the true cube root.
+__device__ __forceinline__ float cbrt_ieee(float x) {
+  const uint32_t u = __float_as_uint(x);
+  const int e = (int)((u >> 23) & 0xffu) - 127;
+  const int k = (e >= 0) ? e / 3 : -((2 - e) / 3);
+  const int r = e - 3 * k;  // 0, 1 or 2
+  const float m = __uint_as_float((u & 0x7fffffu) | ((uint32_t)(127 + r) << 23));
+  float y = 0.80169147f + m * (0.24778609f - m * 0.0127262f);  // cbrt on [1, 8), 3.7 %
+  float t = y * y * y;
+  y = y * (t + 2.0f * m) / (2.0f * t + m);
+  t = y * y * y;
+  y = y * (t + 2.0f * m) / (2.0f * t + m);
+  return y * __uint_as_float((uint32_t)(127 + k) << 23);
+}
+
+// h^(4/3) as h * cbrt(h): the Manning friction depth factor.
+__device__ __forceinline__ float pow43(float h) { return h * cbrt_ieee(h); }
+
+}  // namespace inu

@@ -1,0 +1,282 @@
+"""ctypes binding of ``libinundation_hip.so`` (declared in ``include/inundation_hip.h``).
+
+The library is loaded from the package directory only; a missing library is a
+hard error (there is no CPU fallback on the product path).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+from numpy.typing import NDArray
+
+LIB_PATH = Path(__file__).resolve().parent.parent / "libinundation_hip.so"
+
+OK = 0
+EINVAL = -1
+EHIP = -2
+ESTATE = -3
+ERANGE = -5
+ENODEV = -6
+
+FLAG_DETERMINISTIC = 1
+FLAG_COMPENSATED = 2
+FLAG_ACTIVE_SET = 4
+FLAG_TRACK_CLAMP = 8
+
+FIELD_H = 0
+FIELD_HU = 1
+FIELD_HV = 2
+FIELD_H_LO = 3
+FIELD_H1 = 4
+FIELD_HU1 = 5
+FIELD_HV1 = 6
+FIELD_H1_LO = 7
+FIELD_CLAMP = 8
+FIELD_BLOCK_WET = 9
+FIELD_BLOCK_ACTIVE = 10
+FIELD_BLOCK_FORCED = 11
+
+# Every entry point the header declares, for the export check.
+EXPORTS = (
+    "inu_version",
+    "inu_create",
+    "inu_destroy",
+    "inu_last_error",
+    "inu_set_mesh",
+    "inu_mesh_info",
+    "inu_set_params",
+    "inu_set_state",
+    "inu_set_manning",
+    "inu_set_sources",
+    "inu_ctrl_write",
+    "inu_ctrl_read",
+    "inu_dt_words",
+    "inu_prologue",
+    "inu_retarget",
+    "inu_steps",
+    "inu_steps_timed",
+    "inu_download",
+    "inu_upload",
+    "inu_probe",
+)
+
+
+class InuCtrl(C.Structure):
+    """``inu_ctrl``: the 48-byte resident control block (gpu_resident_batch.py:65-77)."""
+
+    _fields_ = [
+        ("now_ticks", C.c_uint64),
+        ("stop_ticks", C.c_uint64),
+        ("steps", C.c_uint64),
+        ("dt_ticks", C.c_uint32),
+        ("dt_max_ticks", C.c_uint32),
+        ("dry_dt_ticks", C.c_uint32),
+        ("last_raw", C.c_float),
+        ("cfl_safety", C.c_float),
+        ("pad", C.c_uint32),
+    ]
+
+
+class HipError(RuntimeError):
+    """A HIP runtime failure inside the library."""
+
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_I = C.c_int
+_F = C.c_float
+_lib: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Load the in-tree library once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m inundation_amd.build` "
+            "(the HIP backend has no CPU fallback)"
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    sig = {
+        "inu_version": (_I, []),
+        "inu_create": (_I, [_I, C.POINTER(_P)]),
+        "inu_destroy": (None, [_P]),
+        "inu_last_error": (C.c_char_p, [_P]),
+        "inu_set_mesh": (_I, [_P, _U32, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+        "inu_mesh_info": (_I, [_P, _P]),
+        "inu_set_params": (_I, [_P, _F, _F, _F, _U32]),
+        "inu_set_state": (_I, [_P, _P, _P, _P, _P]),
+        "inu_set_manning": (_I, [_P, _P]),
+        "inu_set_sources": (_I, [_P, _U32, _P, _P]),
+        "inu_ctrl_write": (_I, [_P, C.POINTER(InuCtrl)]),
+        "inu_ctrl_read": (_I, [_P, C.POINTER(InuCtrl)]),
+        "inu_dt_words": (_I, [_P, _P]),
+        "inu_prologue": (_I, [_P]),
+        "inu_retarget": (_I, [_P]),
+        "inu_steps": (_I, [_P, _U32, _I]),
+        "inu_steps_timed": (_I, [_P, _U32, _P]),
+        "inu_download": (_I, [_P, _I, _P]),
+        "inu_upload": (_I, [_P, _I, _P]),
+        "inu_probe": (_I, [_P, _U32, _P, _P, _F, _F]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a: NDArray) -> int:
+    """Address of a C-contiguous array's data."""
+    if not a.flags.c_contiguous:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data
+
+
+class Context:
+    """Owns one ``inu_ctx``: one device copy of a mesh, its state and stream."""
+
+    def __init__(self, device: int = 0) -> None:
+        self._lib = load()
+        h = _P()
+        rc = self._lib.inu_create(device, C.byref(h))
+        if rc != OK:
+            if rc == ENODEV:
+                raise RuntimeError("no HIP device available for the hip_resident backend")
+            raise RuntimeError(f"inu_create failed with code {rc}")
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.inu_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int) -> None:
+        """Map a negative return code to ValueError (bad argument) or RuntimeError."""
+        if rc == OK:
+            return
+        msg = self._lib.inu_last_error(self._h).decode(errors="replace")
+        if rc == EINVAL:
+            raise ValueError(msg)
+        if rc == EHIP:
+            raise HipError(msg)
+        raise RuntimeError(f"{msg} (code {rc})")
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def call(self, name: str, *args: object) -> None:
+        self.check(getattr(self._lib, name)(self._h, *args))
+
+    def set_mesh(
+        self,
+        cell_l: NDArray[np.int32],
+        cell_r: NDArray[np.int32],
+        nx: NDArray[np.float32],
+        ny: NDArray[np.float32],
+        length: NDArray[np.float32],
+        dz2: NDArray[np.float32],
+        area: NDArray[np.float32],
+        cptr: NDArray[np.uint32],
+        cidx: NDArray[np.uint32],
+    ) -> None:
+        arrs = [
+            np.ascontiguousarray(cell_l, dtype=np.int32),
+            np.ascontiguousarray(cell_r, dtype=np.int32),
+            np.ascontiguousarray(nx, dtype=np.float32),
+            np.ascontiguousarray(ny, dtype=np.float32),
+            np.ascontiguousarray(length, dtype=np.float32),
+            np.ascontiguousarray(dz2, dtype=np.float32),
+            np.ascontiguousarray(area, dtype=np.float32),
+            np.ascontiguousarray(cptr, dtype=np.uint32),
+            np.ascontiguousarray(cidx, dtype=np.uint32),
+        ]
+        n_cells = arrs[6].size
+        n_edges = arrs[0].size
+        self.call("inu_set_mesh", n_cells, n_edges, *[ptr(a) for a in arrs])
+
+    def mesh_info(self) -> dict[str, int]:
+        out = np.zeros(6, dtype=np.uint32)
+        self.call("inu_mesh_info", ptr(out))
+        keys = ("N", "E", "S", "NB", "max_local", "grid_stage")
+        return {k: int(v) for k, v in zip(keys, out.tolist())}
+
+    def set_params(self, g: float, dry_tol: float, cfl: float, flags: int) -> None:
+        self.call("inu_set_params", g, dry_tol, cfl, flags)
+
+    def set_state(
+        self,
+        h: NDArray[np.float32],
+        hu: NDArray[np.float32],
+        hv: NDArray[np.float32],
+        h_lo: NDArray[np.float32] | None = None,
+    ) -> None:
+        a = [np.ascontiguousarray(x, dtype=np.float32) for x in (h, hu, hv)]
+        lo = None if h_lo is None else np.ascontiguousarray(h_lo, dtype=np.float32)
+        self.call("inu_set_state", *[ptr(x) for x in a], None if lo is None else ptr(lo))
+
+    def set_manning(self, n: NDArray[np.float32]) -> None:
+        a = np.ascontiguousarray(n, dtype=np.float32)
+        self.call("inu_set_manning", ptr(a))
+
+    def set_sources(self, cells: NDArray[np.uint32], rates: NDArray[np.float32]) -> None:
+        c = np.ascontiguousarray(cells, dtype=np.uint32)
+        r = np.ascontiguousarray(rates, dtype=np.float32)
+        if c.size != r.size:
+            raise ValueError("cells and rates differ in length")
+        self.call("inu_set_sources", c.size, ptr(c) if c.size else None, ptr(r) if r.size else None)
+
+    def ctrl_write(self, ctrl: InuCtrl) -> None:
+        self.call("inu_ctrl_write", C.byref(ctrl))
+
+    def ctrl_read(self) -> InuCtrl:
+        c = InuCtrl()
+        self.call("inu_ctrl_read", C.byref(c))
+        return c
+
+    def dt_words(self) -> NDArray[np.uint32]:
+        out = np.zeros(4, dtype=np.uint32)
+        self.call("inu_dt_words", ptr(out))
+        return out
+
+    def prologue(self) -> None:
+        self.call("inu_prologue")
+
+    def retarget(self) -> None:
+        self.call("inu_retarget")
+
+    def steps(self, n: int, use_graph: bool = True) -> None:
+        self.call("inu_steps", n, 1 if use_graph else 0)
+
+    def steps_timed(self, n: int) -> tuple[float, float, float]:
+        out = np.zeros(3, dtype=np.float32)
+        self.call("inu_steps_timed", n, ptr(out))
+        return float(out[0]), float(out[1]), float(out[2])
+
+    def download(self, which: int, n: int, dtype: type = np.float32) -> NDArray:
+        out = np.empty(n, dtype=dtype)
+        self.call("inu_download", which, ptr(out))
+        return out
+
+    def upload(self, which: int, values: NDArray[np.float32]) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        self.call("inu_upload", which, ptr(a))
+
+    def probe(self, pairs: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:
+        p = np.ascontiguousarray(pairs, dtype=np.float32).reshape(-1, 2)
+        out = np.empty((p.shape[0], 5), dtype=np.float32)
+        self.call("inu_probe", p.shape[0], ptr(p), ptr(out), g, dry_tol)
+        return out

@@ -1,0 +1,666 @@
+// FP32 CPU restatement of the reference's GPU-resident shallow-water step.
+//
+// TEST INFRASTRUCTURE: the oracle the HIP backend is checked against, and the
+// timed CPU baseline of the benchmark. It is never part of the product path.
+//
+// It follows the structure of the reference kernels literally, kernel by
+// kernel, so a bitwise match with the fused HIP kernels also checks that the
+// fusion changed nothing:
+//   flux_resident_gather     inundation/vulkan/shaders/resident.py:187-287
+//                            (workgroup skip over erange :233-239, predictor
+//                            rule :209-226, wall ghost :126-129, dry skip
+//                            :140-145, per-edge words :165-184)
+//   update_resident_gather[_comp]  resident.py:401-516 (sum :312-344, depth
+//                            :359-373, store :375-398)
+//   source_resident          resident.py:665-733
+//   block_mark / dilate      resident.py:738-805
+//   step_control             resident.py:813-885
+//   cfl_edge_gather / cfl_gather / dt_reset   resident.py:528-610, cfl.py:130-166
+//   record order             inundation/vulkan/gpu_resident_batch.py:489-513, 567-599
+//   HLLC, reconstruction, film drag, two-sum   inundation/vulkan/shaders/flux.py:43-185, 301-320
+//
+// One statement differs from the reference and is a choice, not a port: the
+// source kernel's dt bound reads each neighbour's depth after *every* source
+// of the step has been applied (the reference reads it while other source
+// threads may still be writing it, so its result there depends on thread
+// timing). Friction's h^(4/3) is h * cbrt(h) with the IEEE-only cube root of
+// the backend, restated here.
+//
+// Build: see oracle/build.py (g++ -O2 -fopenmp -ffp-contract=off
+// -fno-fast-math -fno-math-errno). FP32 arithmetic, no contraction.
+
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr uint32_t kBlk = 256;
+constexpr uint32_t kWg = 256;
+constexpr float kTicks = 1048576.0f;
+constexpr float kSentinel = 1e30f;
+constexpr uint32_t kWords = 6;  // EF_WORDS
+constexpr uint32_t kCfl = 5;    // EF_CFL
+
+inline uint32_t fbits(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+inline float bitsf(uint32_t u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+// GLSL min(x, y) = y < x ? y : x; max(x, y) = x < y ? y : x.
+inline float gmin(float x, float y) { return (y < x) ? y : x; }
+inline float gmax(float x, float y) { return (x < y) ? y : x; }
+inline float gclamp(float x, float a, float b) { return gmin(gmax(x, a), b); }
+
+struct Ctrl {
+  uint64_t now_ticks, stop_ticks, steps;
+  uint32_t dt_ticks, dt_max_ticks, dry_dt_ticks;
+  float last_raw, cfl_safety;
+  uint32_t pad;
+};
+static_assert(sizeof(Ctrl) == 48, "ctrl layout");
+
+struct Pc {
+  float g, dry, cfl;
+};
+
+// ---- flux.py:63-182 -------------------------------------------------------
+
+inline float safe_vel(const Pc& pc, float h_, float hq) {
+  float e_ = 4.0f * pc.dry;  // FILM_DEPTHS * dry_tol
+  return (h_ > pc.dry) ? 2.0f * h_ * hq / (h_ * h_ + gmax(h_ * h_, e_ * e_)) : 0.0f;
+}
+
+inline void hllc_speeds(const Pc& pc, float hL, float unL, float hR, float unR, float& SL, float& SR) {
+  float cL = std::sqrt(pc.g * gmax(hL, 0.0f));
+  float cR = std::sqrt(pc.g * gmax(hR, 0.0f));
+  float sqL = std::sqrt(gmax(hL, 0.0f));
+  float sqR = std::sqrt(gmax(hR, 0.0f));
+  float den = sqL + sqR;
+  float u_roe = (den > 1e-12f) ? (sqL * unL + sqR * unR) / den : 0.0f;
+  float c_roe = std::sqrt(pc.g * 0.5f * (hL + hR));
+  SL = gmin(unL - cL, u_roe - c_roe);
+  SR = gmax(unR + cR, u_roe + c_roe);
+  float wetL = gclamp(hL / pc.dry - 1.0f, 0.0f, 1.0f);
+  float wetR = gclamp(hR / pc.dry - 1.0f, 0.0f, 1.0f);
+  float dryLeft = (1.0f - wetL) * wetR;
+  float dryRight = (1.0f - wetR) * wetL;
+  float roe = 1.0f - dryLeft - dryRight;
+  SL = roe * SL + dryLeft * (unR - 2.0f * cR) + dryRight * (unL - cL);
+  SR = roe * SR + dryLeft * (unR + cR) + dryRight * (unL + 2.0f * cL);
+}
+
+inline void hllc(const Pc& pc, float hL, float hqnL, float hqtL, float hR, float hqnR, float hqtR,
+                 float& Fh, float& Fhn, float& Fht, float& Smax) {
+  float unL = safe_vel(pc, hL, hqnL);
+  float utL = safe_vel(pc, hL, hqtL);
+  float unR = safe_vel(pc, hR, hqnR);
+  float utR = safe_vel(pc, hR, hqtR);
+  float SL, SR;
+  hllc_speeds(pc, hL, unL, hR, unR, SL, SR);
+  Smax = gmax(std::fabs(SL), std::fabs(SR));
+  float num_sm = SL * hR * (unR - SR) - SR * hL * (unL - SL);
+  float den_sm = hR * (unR - SR) - hL * (unL - SL);
+  float SM = (std::fabs(den_sm) > 1e-12f) ? num_sm / den_sm : 0.0f;
+  float FL_h = hL * unL;
+  float FL_hn = hL * unL * unL + 0.5f * pc.g * hL * hL;
+  float FL_ht = hL * unL * utL;
+  float FR_h = hR * unR;
+  float FR_hn = hR * unR * unR + 0.5f * pc.g * hR * hR;
+  float FR_ht = hR * unR * utR;
+  float dSL = SL - SM;
+  float dSR = SR - SM;
+  float kL = hL * (SL - unL) / ((std::fabs(dSL) > 1e-14f) ? dSL : 1e-14f);
+  float kR = hR * (SR - unR) / ((std::fabs(dSR) > 1e-14f) ? dSR : 1e-14f);
+  float FsL_h = FL_h + SL * (kL - hL);
+  float FsL_hn = FL_hn + SL * (kL * SM - hL * unL);
+  float FsL_ht = FL_ht + SL * (kL * utL - hL * utL);
+  float FsR_h = FR_h + SR * (kR - hR);
+  float FsR_hn = FR_hn + SR * (kR * SM - hR * unR);
+  float FsR_ht = FR_ht + SR * (kR * utR - hR * utR);
+  if (SL >= 0.0f) { Fh = FL_h; Fhn = FL_hn; Fht = FL_ht; }
+  else if (SM >= 0.0f) { Fh = FsL_h; Fhn = FsL_hn; Fht = FsL_ht; }
+  else if (SR >= 0.0f) { Fh = FsR_h; Fhn = FsR_hn; Fht = FsR_ht; }
+  else { Fh = FR_h; Fhn = FR_hn; Fht = FR_ht; }
+}
+
+inline void face_states(const Pc& pc, float hL, float huL, float hvL, float dzL, float hR, float huR,
+                        float hvR, float dzR, float nx_, float ny_, float& hLs, float& hqnL,
+                        float& hqtL, float& hRs, float& hqnR, float& hqtR) {
+  hLs = gmax(0.0f, hL - dzL);
+  hRs = gmax(0.0f, hR - dzR);
+  float uL = safe_vel(pc, hL, huL);
+  float vL = safe_vel(pc, hL, hvL);
+  float uR = safe_vel(pc, hR, huR);
+  float vR = safe_vel(pc, hR, hvR);
+  float huLs = hLs * uL;
+  float hvLs = hLs * vL;
+  float huRs = hRs * uR;
+  float hvRs = hRs * vR;
+  hqnL = huLs * nx_ + hvLs * ny_;
+  hqtL = -huLs * ny_ + hvLs * nx_;
+  hqnR = huRs * nx_ + hvRs * ny_;
+  hqtR = -huRs * ny_ + hvRs * nx_;
+}
+
+inline float wave_bound(const Pc& pc, float hL, float huL, float hvL, float dzL, float hR, float huR,
+                        float hvR, float dzR, float nx_, float ny_) {
+  float hLs, hqnL, hqtL, hRs, hqnR, hqtR;
+  face_states(pc, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx_, ny_, hLs, hqnL, hqtL, hRs, hqnR, hqtR);
+  float SL, SR;
+  hllc_speeds(pc, hLs, safe_vel(pc, hLs, hqnL), hRs, safe_vel(pc, hRs, hqnR), SL, SR);
+  return gmax(std::fabs(SL), std::fabs(SR));
+}
+
+inline float film_drag(const Pc& pc, float h_, float dt_) {
+  float e_ = 4.0f * pc.dry;
+  if (h_ >= e_) return 1.0f;
+  float rate = (e_ * e_ - h_ * h_) / (2.0f * h_ * h_) * std::sqrt(pc.g / e_);
+  return 1.0f / (1.0f + dt_ * rate);
+}
+
+// ---- flux.py:301-319 -------------------------------------------------------
+
+inline void two_sum(float a, float b, float& s, float& e) {
+  float t = a + b;
+  float bb = t - a;
+  float err = (a - (t - bb)) + (b - bb);
+  s = t;
+  e = err;
+}
+
+inline void add_compensated(float hi, float loi, float inc, float& h_new, float& lo_new) {
+  float s, e;
+  two_sum(hi, inc, s, e);
+  e += loi;
+  two_sum(s, e, h_new, lo_new);
+  if (h_new < 0.0f || (h_new == 0.0f && lo_new < 0.0f)) {
+    h_new = 0.0f;
+    lo_new = 0.0f;
+  }
+}
+
+// h^(4/3) = h * cbrt(h): exact exponent split, cubic start on [1, 8), two
+// Halley steps (the backend's IEEE-only cube root, restated).
+inline float cbrt_ieee(float x) {
+  uint32_t u = fbits(x);
+  int e = (int)((u >> 23) & 0xffu) - 127;
+  int k = (e >= 0) ? e / 3 : -((2 - e) / 3);
+  int r = e - 3 * k;
+  float m = bitsf((u & 0x7fffffu) | ((uint32_t)(127 + r) << 23));
+  float y = 0.80169147f + m * (0.24778609f - m * 0.0127262f);
+  float t = y * y * y;
+  y = y * (t + 2.0f * m) / (2.0f * t + m);
+  t = y * y * y;
+  y = y * (t + 2.0f * m) / (2.0f * t + m);
+  return y * bitsf((uint32_t)(127 + k) << 23);
+}
+inline float pow43(float h) { return h * cbrt_ieee(h); }
+
+}  // namespace
+
+struct cref {
+  uint32_t N = 0, E = 0, NB = 0, NWG = 0;
+  std::vector<int32_t> ecL, ecR;
+  std::vector<float> enx, eny, elen, edz, area;
+  std::vector<uint32_t> cptr, cidx, erange, bptr, bidx;
+  // state
+  std::vector<float> h, hu, hv, h1, hu1, hv1, hlo, h1lo, nman, clamp, ef;
+  std::vector<uint8_t> touched;
+  std::vector<uint32_t> blk;  // 3 words per block: wet, active, forced
+  std::vector<uint32_t> src_cell;
+  std::vector<float> src_rate;
+  uint32_t dtbuf[4] = {0, 0, 0, 0};
+  Ctrl ctrl{};
+  Pc pc{9.81f, 1e-4f, 0.8f};
+  bool compensated = true, active_set = true, track_clamp = false;
+};
+
+namespace {
+
+inline bool block_active(const cref& r, uint32_t b) { return r.blk[3 * b + 1] != 0u; }
+
+inline void load_state(const cref& r, int stage, int c, float& h_, float& hu_, float& hv_) {
+  if (stage == 1 && block_active(r, (uint32_t)c >> 8)) {
+    h_ = r.h1[c]; hu_ = r.hu1[c]; hv_ = r.hv1[c];
+  } else {
+    h_ = r.h[c]; hu_ = r.hu[c]; hv_ = r.hv[c];
+  }
+}
+
+void flux(cref& r, int stage) {
+  if (r.dtbuf[0] == 0u) return;
+  const Pc pc = r.pc;
+#pragma omp parallel for schedule(static)
+  for (int64_t wg = 0; wg < (int64_t)r.NWG; ++wg) {
+    bool live = false;
+    for (uint32_t b = r.erange[2 * wg]; b <= r.erange[2 * wg + 1]; ++b)
+      if (block_active(r, b)) { live = true; break; }
+    if (!live) continue;
+    const uint32_t e1 = std::min<uint32_t>((uint32_t)(wg + 1) * kWg, r.E);
+    for (uint32_t eid = (uint32_t)wg * kWg; eid < e1; ++eid) {
+      int cL = r.ecL[eid], cR = r.ecR[eid];
+      float nx_ = r.enx[eid], ny_ = r.eny[eid], len = r.elen[eid];
+      float hL, huL, hvL, hR, huR, hvR;
+      load_state(r, stage, cL, hL, huL, hvL);
+      if (cR == -1) {
+        hR = hL;
+        huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
+        hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
+      } else {
+        load_state(r, stage, cR, hR, huR, hvR);
+      }
+      if (hL < pc.dry && hR < pc.dry) continue;
+      float dzL = r.edz[2 * eid], dzR = r.edz[2 * eid + 1];
+      float hLs, hqnL, hqtL, hRs, hqnR, hqtR;
+      face_states(pc, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx_, ny_, hLs, hqnL, hqtL, hRs, hqnR, hqtR);
+      float Fh, Fhn, Fht, Smax;
+      hllc(pc, hLs, hqnL, hqtL, hRs, hqnR, hqtR, Fh, Fhn, Fht, Smax);
+      if (cR < 0) Fh = 0.0f;  // boundary_mass_flux of a wall
+      float Fhu = Fhn * nx_ - Fht * ny_;
+      float Fhv = Fhn * ny_ + Fht * nx_;
+      float SL_g = 0.5f * pc.g * (hL * hL - hLs * hLs);
+      float SR_g = 0.5f * pc.g * (hR * hR - hRs * hRs);
+      __atomic_store_n(&r.touched[cL], (uint8_t)1, __ATOMIC_RELAXED);
+      if (cR >= 0) __atomic_store_n(&r.touched[cR], (uint8_t)1, __ATOMIC_RELAXED);
+      float* o = &r.ef[(size_t)kWords * eid];
+      o[0] = Fh * len;
+      o[1] = Fhu * len + SL_g * nx_ * len;
+      o[2] = Fhv * len + SL_g * ny_ * len;
+      o[3] = Fhu * len + SR_g * nx_ * len;
+      o[4] = Fhv * len + SR_g * ny_ * len;
+      if (stage == 1) o[kCfl] = Smax * len;
+    }
+  }
+}
+
+void update(cref& r, int stage) {
+  if (r.dtbuf[0] == 0u) return;
+  const Pc pc = r.pc;
+  const float dt = bitsf(r.dtbuf[0]);
+  if (stage == 0)
+    for (uint32_t b = 0; b < r.NB; ++b)
+      if (block_active(r, b)) r.blk[3 * b + 0] = 0u;
+  uint32_t best = 0xffffffffu;
+#pragma omp parallel for schedule(static) reduction(min : best)
+  for (int64_t ii = 0; ii < (int64_t)r.N; ++ii) {
+    const uint32_t i = (uint32_t)ii;
+    const uint32_t b = i >> 8;
+    if (!block_active(r, b)) continue;
+    float A = r.area[i];
+    float dti = dt / A;
+    float hi, hui, hvi;
+    if (stage == 0) { hi = r.h[i]; hui = r.hu[i]; hvi = r.hv[i]; }
+    else { hi = r.h1[i]; hui = r.hu1[i]; hvi = r.hv1[i]; }
+    // _UPDATE_SUM_GATHER
+    float sdh = 0.0f, sdhu = 0.0f, sdhv = 0.0f, lam = 0.0f;
+    uint32_t kend = r.cptr[i];
+    if (r.touched[i] != 0u) {
+      r.touched[i] = 0u;
+      kend = r.cptr[i + 1];
+    }
+    bool own_dry = hi < pc.dry;
+    for (uint32_t k = r.cptr[i]; k < kend; ++k) {
+      uint32_t s = r.cidx[k];
+      uint32_t e = s >> 1;
+      bool right = (s & 1u) != 0u;
+      if (own_dry) {
+        int nb = right ? r.ecL[e] : r.ecR[e];
+        if (nb == -1) continue;
+        if (nb >= 0) {
+          bool pred = stage == 1 && block_active(r, (uint32_t)nb >> 8);
+          if ((pred ? r.h1[nb] : r.h[nb]) < pc.dry) continue;
+        }
+      }
+      const float* o = &r.ef[(size_t)kWords * e];
+      if (right) { sdh -= o[0]; sdhu -= o[3]; sdhv -= o[4]; }
+      else { sdh += o[0]; sdhu += o[1]; sdhv += o[2]; }
+      if (stage == 1) lam += o[kCfl];
+    }
+    float h_new, lo_new, h_raw;
+    if (r.compensated) {
+      float loi = (stage == 0) ? r.hlo[i] : r.h1lo[i];
+      add_compensated(hi, loi, -dti * sdh, h_new, lo_new);
+      float s_raw, e_raw;
+      two_sum(hi, -dti * sdh, s_raw, e_raw);
+      h_raw = s_raw + (e_raw + loi);
+    } else {
+      h_raw = hi - dti * sdh;
+      h_new = gmax(h_raw, 0.0f);
+      lo_new = 0.0f;
+    }
+    float hu_new = hui - dti * sdhu;
+    float hv_new = hvi - dti * sdhv;
+    if (r.track_clamp) r.clamp[i] += 0.5f * A * (h_new - h_raw);
+    if (h_new < pc.dry) { hu_new = 0.0f; hv_new = 0.0f; }
+    if (h_new >= pc.dry) {
+      float drag = film_drag(pc, h_new, dt);
+      hu_new *= drag;
+      hv_new *= drag;
+    }
+    if (h_new >= pc.dry) {
+      float n = r.nman[i];
+      float u_ = hu_new / h_new;
+      float v_ = hv_new / h_new;
+      float spd = std::sqrt(u_ * u_ + v_ * v_);
+      float h43 = pow43(h_new);
+      float Sf = pc.g * n * n * spd / h43;
+      float den = 1.0f + dt * Sf;
+      hu_new = hu_new / den;
+      hv_new = hv_new / den;
+    }
+    float hf = 0.0f;
+    if (r.compensated) {
+      if (stage == 0) {
+        r.h1[i] = h_new;
+        r.h1lo[i] = lo_new;
+      } else {
+        float s, e;
+        two_sum(r.h[i], h_new, s, e);
+        e += r.hlo[i] + lo_new;
+        float lof;
+        two_sum(0.5f * s, 0.5f * e, hf, lof);
+        r.h[i] = hf;
+        r.hlo[i] = lof;
+      }
+    } else {
+      if (stage == 0) r.h1[i] = h_new;
+      else {
+        hf = 0.5f * (r.h[i] + h_new);
+        r.h[i] = hf;
+      }
+    }
+    if (stage == 0) {
+      r.hu1[i] = hu_new;
+      r.hv1[i] = hv_new;
+    } else {
+      float huf = 0.5f * (r.hu[i] + hu_new);
+      float hvf = 0.5f * (r.hv[i] + hv_new);
+      if (hf < pc.dry) { huf = 0.0f; hvf = 0.0f; }
+      r.hu[i] = huf;
+      r.hv[i] = hvf;
+      if (hf >= pc.dry) __atomic_store_n(&r.blk[3 * b + 0], 1u, __ATOMIC_RELAXED);
+      if (lam > 1e-30f && A > 1e-10f) best = std::min(best, fbits(pc.cfl * A / lam));
+    }
+  }
+  if (stage == 1) r.dtbuf[1] = std::min(r.dtbuf[1], best);
+}
+
+void source(cref& r) {
+  if (r.dtbuf[0] == 0u) return;
+  const Pc pc = r.pc;
+  const float dt = bitsf(r.dtbuf[0]);
+  const int64_t n = (int64_t)r.src_cell.size();
+  // Every source first ...
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < n; ++k) {
+    uint32_t i = r.src_cell[k];
+    float inc = dt * r.src_rate[k];
+    float h_new;
+    if (r.compensated) {
+      float lo_new;
+      add_compensated(r.h[i], r.hlo[i], inc, h_new, lo_new);
+      r.hlo[i] = lo_new;
+    } else {
+      h_new = gmax(r.h[i] + inc, 0.0f);
+    }
+    r.h[i] = h_new;
+    if (h_new >= pc.dry) __atomic_store_n(&r.blk[3 * (i >> 8) + 0], 1u, __ATOMIC_RELAXED);
+  }
+  // ... then the post-source dt bound of each wet source cell.
+  uint32_t best = 0xffffffffu;
+#pragma omp parallel for schedule(static) reduction(min : best)
+  for (int64_t k = 0; k < n; ++k) {
+    uint32_t i = r.src_cell[k];
+    float h_new = r.h[i];
+    if (!(h_new >= pc.dry)) continue;
+    float u = std::sqrt(r.hu[i] * r.hu[i] + r.hv[i] * r.hv[i]) / h_new;
+    float c = std::sqrt(pc.g * h_new);
+    float lam = 0.0f;
+    for (uint32_t kk = r.cptr[i]; kk < r.cptr[i + 1]; ++kk) {
+      uint32_t s = r.cidx[kk];
+      uint32_t e = s >> 1;
+      int nb = ((s & 1u) != 0u) ? r.ecL[e] : r.ecR[e];
+      bool dry_side = nb >= 0 && r.h[nb] < pc.dry;
+      lam += (u + (dry_side ? 2.0f : 1.0f) * c) * r.elen[e];
+    }
+    if (lam > 1e-30f && r.area[i] > 1e-10f) best = std::min(best, fbits(pc.cfl * r.area[i] / lam));
+  }
+  r.dtbuf[1] = std::min(r.dtbuf[1], best);
+}
+
+void block_mark(cref& r, bool all_active) {
+  for (uint32_t b = 0; b < r.NB; ++b) {
+    if (all_active) {
+      r.blk[3 * b + 0] = 0u;
+      r.blk[3 * b + 1] = 1u;
+      continue;
+    }
+    uint32_t first = b << 8, last = std::min(first + kBlk, r.N), live = 0u;
+    for (uint32_t i = first; i < last; ++i)
+      if (r.h[i] >= r.pc.dry || r.hu[i] != 0.0f || r.hv[i] != 0.0f) { live = 1u; break; }
+    r.blk[3 * b + 0] = live;
+  }
+}
+
+void dilate(cref& r, bool force) {
+  if (!force && r.dtbuf[0] == 0u) return;
+  std::vector<uint32_t> act(r.NB);
+  for (uint32_t b = 0; b < r.NB; ++b) {
+    uint32_t v = r.blk[3 * b + 0] | r.blk[3 * b + 2];
+    for (uint32_t k = r.bptr[b]; v == 0u && k < r.bptr[b + 1]; ++k)
+      v = r.blk[3 * r.bidx[k] + 0] | r.blk[3 * r.bidx[k] + 2];
+    act[b] = v;
+  }
+  for (uint32_t b = 0; b < r.NB; ++b) r.blk[3 * b + 1] = act[b];
+}
+
+void cfl_prologue(cref& r) {
+  const Pc pc = r.pc;
+#pragma omp parallel for schedule(static)
+  for (int64_t ee = 0; ee < (int64_t)r.E; ++ee) {
+    uint32_t eid = (uint32_t)ee;
+    int cL = r.ecL[eid], cR = r.ecR[eid];
+    float nx_ = r.enx[eid], ny_ = r.eny[eid];
+    float hL = r.h[cL], huL = r.hu[cL], hvL = r.hv[cL];
+    float hR, huR, hvR;
+    if (cR == -1) {
+      hR = hL;
+      huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
+      hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
+    } else {
+      hR = r.h[cR]; huR = r.hu[cR]; hvR = r.hv[cR];
+    }
+    float contrib = 0.0f;
+    if (hL >= pc.dry || hR >= pc.dry)
+      contrib = wave_bound(pc, hL, huL, hvL, r.edz[2 * eid], hR, huR, hvR, r.edz[2 * eid + 1], nx_, ny_) * r.elen[eid];
+    r.ef[(size_t)kWords * eid + kCfl] = contrib;
+  }
+  uint32_t best = 0xffffffffu;
+#pragma omp parallel for schedule(static) reduction(min : best)
+  for (int64_t ii = 0; ii < (int64_t)r.N; ++ii) {
+    uint32_t i = (uint32_t)ii;
+    float lam = 0.0f;
+    for (uint32_t k = r.cptr[i]; k < r.cptr[i + 1]; ++k) lam += r.ef[(size_t)kWords * (r.cidx[k] >> 1) + kCfl];
+    if (lam > 1e-30f && r.area[i] > 1e-10f) best = std::min(best, fbits(pc.cfl * r.area[i] / lam));
+  }
+  r.dtbuf[0] = std::min(r.dtbuf[0], best);
+}
+
+void step_control(cref& r, int mode) {
+  Ctrl& c = r.ctrl;
+  float raw;
+  if (mode == 0) {
+    raw = bitsf(r.dtbuf[0]);
+    c.last_raw = raw;
+  } else if (mode == 1) {
+    uint32_t done = c.dt_ticks;
+    c.now_ticks += (uint64_t)done;
+    if (done > 0u) {
+      c.steps += 1u;
+      raw = bitsf(r.dtbuf[1]);
+      c.last_raw = raw;
+    } else {
+      raw = c.last_raw;
+    }
+  } else {
+    raw = c.last_raw;
+  }
+  uint64_t remaining = (c.stop_ticks > c.now_ticks) ? c.stop_ticks - c.now_ticks : 0u;
+  uint32_t ticks;
+  if (remaining == 0u || raw < 1e-10f) ticks = 0u;
+  else if (raw >= 1e10f) ticks = c.dry_dt_ticks;
+  else {
+    float cap = (float)c.dt_max_ticks / kTicks;
+    ticks = (uint32_t)(gmin(raw * c.cfl_safety, cap) * kTicks);
+  }
+  ticks = std::min(ticks, c.dt_max_ticks);
+  if ((uint64_t)ticks > remaining) ticks = (uint32_t)remaining;
+  c.dt_ticks = ticks;
+  r.dtbuf[0] = fbits((float)ticks / kTicks);
+  r.dtbuf[1] = fbits(kSentinel);
+}
+
+}  // namespace
+
+extern "C" {
+
+cref* cref_create(uint32_t N, uint32_t E, const int32_t* ecL, const int32_t* ecR, const float* nx,
+                  const float* ny, const float* len, const float* dz2, const float* area,
+                  const uint32_t* cptr, const uint32_t* cidx, const uint32_t* erange,
+                  const uint32_t* bptr, const uint32_t* bidx) {
+  cref* r = new cref();
+  r->N = N;
+  r->E = E;
+  r->NB = (N + kBlk - 1) / kBlk;
+  r->NWG = (E + kWg - 1) / kWg;
+  r->ecL.assign(ecL, ecL + E);
+  r->ecR.assign(ecR, ecR + E);
+  r->enx.assign(nx, nx + E);
+  r->eny.assign(ny, ny + E);
+  r->elen.assign(len, len + E);
+  r->edz.assign(dz2, dz2 + 2 * (size_t)E);
+  r->area.assign(area, area + N);
+  r->cptr.assign(cptr, cptr + N + 1);
+  r->cidx.assign(cidx, cidx + cptr[N]);
+  r->erange.assign(erange, erange + 2 * (size_t)r->NWG);
+  r->bptr.assign(bptr, bptr + r->NB + 1);
+  r->bidx.assign(bidx, bidx + bptr[r->NB]);
+  for (auto* v : {&r->h, &r->hu, &r->hv, &r->h1, &r->hu1, &r->hv1, &r->hlo, &r->h1lo, &r->clamp}) v->assign(N, 0.0f);
+  r->nman.assign(N, 0.03f);
+  r->ef.assign((size_t)kWords * E, 0.0f);
+  r->touched.assign(N, 0);
+  r->blk.assign(3 * (size_t)r->NB, 0u);
+  return r;
+}
+
+void cref_destroy(cref* r) { delete r; }
+
+void cref_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+int cref_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void cref_set_params(cref* r, float g, float dry, float cfl, uint32_t flags) {
+  r->pc = Pc{g, dry, cfl};
+  r->compensated = (flags & 2u) != 0;
+  r->active_set = (flags & 4u) != 0;
+  r->track_clamp = (flags & 8u) != 0;
+}
+
+void cref_set_state(cref* r, const float* h, const float* hu, const float* hv, const float* hlo) {
+  r->h.assign(h, h + r->N);
+  r->hu.assign(hu, hu + r->N);
+  r->hv.assign(hv, hv + r->N);
+  if (hlo) r->hlo.assign(hlo, hlo + r->N);
+  else r->hlo.assign(r->N, 0.0f);
+  r->h1lo.assign(r->N, 0.0f);
+}
+
+void cref_set_manning(cref* r, const float* n) { r->nman.assign(n, n + r->N); }
+
+void cref_set_sources(cref* r, uint32_t n, const uint32_t* cell, const float* rate) {
+  r->src_cell.assign(cell, cell + n);
+  r->src_rate.assign(rate, rate + n);
+}
+
+void cref_ctrl_write(cref* r, const void* ctrl48) { memcpy(&r->ctrl, ctrl48, 48); }
+void cref_ctrl_read(const cref* r, void* ctrl48) { memcpy(ctrl48, &r->ctrl, 48); }
+void cref_dt_words(const cref* r, uint32_t* out4) { memcpy(out4, r->dtbuf, 16); }
+
+// gpu_resident_batch.py:567-591
+void cref_prologue(cref* r) {
+  block_mark(*r, !r->active_set);
+  if (r->active_set) dilate(*r, true);
+  r->dtbuf[0] = fbits(kSentinel);
+  cfl_prologue(*r);
+  step_control(*r, 0);
+}
+
+void cref_retarget(cref* r) { step_control(*r, 2); }
+
+// gpu_resident_batch.py:489-513
+void cref_steps(cref* r, uint32_t n) {
+  for (uint32_t k = 0; k < n; ++k) {
+    for (int stage = 0; stage < 2; ++stage) {
+      flux(*r, stage);
+      update(*r, stage);
+    }
+    if (!r->src_cell.empty()) source(*r);
+    if (r->active_set) dilate(*r, false);
+    step_control(*r, 1);
+  }
+}
+
+// Fields as the backend numbers them: h hu hv h_lo h1 hu1 hv1 h1_lo clamp,
+// then per-block wet, active, forced.
+void cref_download(const cref* r, int which, void* out) {
+  const std::vector<float>* f[] = {&r->h, &r->hu, &r->hv, &r->hlo, &r->h1, &r->hu1, &r->hv1, &r->h1lo, &r->clamp};
+  if (which >= 0 && which <= 8) {
+    memcpy(out, f[which]->data(), 4 * (size_t)r->N);
+  } else if (which >= 9 && which <= 11) {
+    uint32_t* o = static_cast<uint32_t*>(out);
+    for (uint32_t b = 0; b < r->NB; ++b) o[b] = r->blk[3 * b + (which - 9)];
+  }
+}
+
+// Same probe as the backend's inu_probe.
+void cref_probe(uint32_t n, const float* pairs, float* out5, float g, float dry) {
+  Pc pc{g, dry, 0.8f};
+  for (uint32_t k = 0; k < n; ++k) {
+    float x = pairs[2 * k], y = pairs[2 * k + 1];
+    float s, e;
+    two_sum(x, y, s, e);
+    out5[5 * k + 0] = e;
+    out5[5 * k + 1] = x / y;
+    out5[5 * k + 2] = std::sqrt(std::fabs(x));
+    out5[5 * k + 3] = (x > 0.0f) ? pow43(x) : 0.0f;
+    out5[5 * k + 4] = (x > 0.0f) ? film_drag(pc, x, y) : 0.0f;
+  }
+}
+
+}  // extern "C"

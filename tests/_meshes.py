@@ -1,0 +1,82 @@
+"""Test meshes and initial states shared by the test modules.
+
+The valley follows the reference's test problem (``tests/unit/_valley.py:19-71``):
+quads of DX metres whose bed falls 0.002 in +x with a Gaussian valley along
+the centre line, Hilbert-ordered. The four step-parity cases follow
+``tests/unit/test_gpu_step_parity.py:81-115``.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from inundation_amd.mesh.geometry import MeshGeometry, build_geometry, hilbert_reorder
+from inundation_amd.mesh.synth import quad_grid
+
+MANNING = 0.03
+DRY_TOL = 1e-3
+DT = 2.0**-10
+
+
+def valley_grid(nx: int = 256, ny: int = 128, dx: float = 2.0, datum: float = 0.0) -> MeshGeometry:
+    verts, faces = quad_grid(nx, ny, dx)
+    verts = verts.astype(np.float32)
+    cx = (np.arange(nx * ny) % nx + 0.5) * dx
+    cy = (np.arange(nx * ny) // nx + 0.5) * dx
+    zb = (datum - 0.002 * cx - 0.3 * np.exp(-(((cy - ny * dx / 2) / (0.2 * ny * dx)) ** 2))).astype(np.float32)
+    geom = build_geometry(verts, faces, zb_from_file=zb)
+    return hilbert_reorder(geom, verbose=False)[0]
+
+
+def _extent(g: MeshGeometry) -> tuple[float, float]:
+    v = g.cell_vertices
+    return float(v[:, 0].max()), float(v[:, 1].max())
+
+
+def dam(g: MeshGeometry) -> np.ndarray:
+    lx, ly = _extent(g)
+    x, y = g.centroid[:, 0], g.centroid[:, 1]
+    inside = (x < lx * 0.25) & (np.abs(y - ly / 2) < ly * 0.15)
+    return np.where(inside, 1.0, 0.0).astype(np.float32)
+
+
+def sheet(g: MeshGeometry) -> np.ndarray:
+    lx, _ = _extent(g)
+    return np.where(g.centroid[:, 0] < lx * 0.3, 0.02, 0.0).astype(np.float32)
+
+
+def wet_lake(g: MeshGeometry) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    lx, _ = _extent(g)
+    x = g.centroid[:, 0].astype(np.float64)
+    h = float(g.zb.max()) + 0.5 - g.zb.astype(np.float64)
+    hu = 0.1 * np.sin(2 * np.pi * x / lx) * h
+    return h, hu, np.zeros_like(h)
+
+
+def dam_break(g: MeshGeometry) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    h = dam(g).astype(np.float64)
+    return h, np.zeros_like(h), np.zeros_like(h)
+
+
+def draining(g: MeshGeometry) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    h = sheet(g).astype(np.float64)
+    return h, np.where(h > 0.0, 0.05, 0.0), np.zeros_like(h)
+
+
+def dry_momentum(g: MeshGeometry) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    lx, _ = _extent(g)
+    h = dam(g).astype(np.float64)
+    hu = np.where(g.centroid[:, 0] > lx * 0.8, 0.1, 0.0)
+    return h, hu, -hu
+
+
+CASES = {
+    "wet_lake": wet_lake,
+    "dam_break": dam_break,
+    "draining": draining,
+    "dry_momentum": dry_momentum,
+}
+
+
+def f32_state(h, hu, hv):  # noqa: ANN001, ANN201
+    return tuple(np.asarray(a, dtype=np.float32) for a in (h, hu, hv))

@@ -1,0 +1,174 @@
+"""HIP backend against the oracles (needs an MI355X).
+
+- bitwise against the FP32 oracle (``oracle/cpu_ref.cpp``): state, low words,
+  block flags, clock and dt words after fixed-dt and CFL-driven runs, with and
+  without the active set, compensation and sources;
+- within the reference's FP32 tolerances of the FP64 oracle
+  (``tests/unit/test_gpu_step_parity.py:31-35, 161-163``): 2e-6 m depth and
+  2e-5 momentum after 8 steps of dt = 2^-10; 5e-6 / 6e-7 after 64 steps on a
+  bed 300 m above the datum;
+- the arithmetic probe bit for bit against the CPU (two-sum error term, IEEE
+  division and sqrt, the friction depth power, film drag).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from inundation_amd.hip import lib as L
+from inundation_amd.hip.solver import device_mesh_arrays
+from inundation_amd.mesh.blocks import num_blocks
+from oracle import cpu_ref, ref64
+from tests import _drive as D
+from tests._meshes import CASES, DRY_TOL, DT, MANNING, f32_state, valley_grid
+from tests.conftest import gpu_context
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():  # noqa: ANN201
+    c = gpu_context()
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def grid():  # noqa: ANN201
+    return valley_grid(64, 32, 2.0)
+
+
+@pytest.fixture(scope="module")
+def mesh(grid):  # noqa: ANN001, ANN201
+    return device_mesh_arrays(grid)
+
+
+def _both(ctx, mesh):  # noqa: ANN001, ANN202
+    ctx.set_mesh(mesh["cell_l"], mesh["cell_r"], mesh["nx"], mesh["ny"], mesh["len"], mesh["dz2"],
+                 mesh["area"], mesh["cptr"], mesh["cidx"])
+    return ctx, cpu_ref.CpuRef(mesh)
+
+
+def test_probe_is_bitwise_the_cpu(ctx) -> None:  # noqa: ANN001
+    rng = np.random.default_rng(5)
+    n = 1 << 18
+    a = np.concatenate([
+        rng.uniform(0.0, 2.0, n // 4),
+        10.0 ** rng.uniform(-6, 3, n // 4),
+        rng.normal(0, 100, n // 4),
+        1.0 + rng.uniform(-1e-6, 1e-6, n // 4),
+    ]).astype(np.float32)
+    b = np.concatenate([
+        10.0 ** rng.uniform(-9, -1, n // 2),
+        rng.normal(0, 1, n // 2),
+    ]).astype(np.float32)
+    b[b == 0] = 1.0
+    pairs = np.column_stack([a, b]).astype(np.float32)
+    got = ctx.probe(pairs, 9.81, 1e-3)
+    want = cpu_ref.probe(pairs, 9.81, 1e-3)
+    for k, name in enumerate(["two_sum err", "div", "sqrt", "pow43", "film"]):
+        both_nan = np.isnan(got[:, k]) & np.isnan(want[:, k])
+        d = np.flatnonzero((got[:, k].view(np.uint32) != want[:, k].view(np.uint32)) & ~both_nan)
+        assert d.size == 0, f"{name}: {d.size} differ, e.g. {pairs[d[:3]].tolist()} {got[d[:3], k]} {want[d[:3], k]}"
+    # The two-sum error term survives: the reference's EFT probe (solver.py:453-461).
+    e = ctx.probe(np.array([[1.0, 1e-9]], np.float32), 9.81, 1e-3)[0, 0]
+    assert e == np.float32(1e-9)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("flags", [D.FLAGS_DEFAULT, L.FLAG_DETERMINISTIC | L.FLAG_ACTIVE_SET,
+                                   L.FLAG_DETERMINISTIC | L.FLAG_COMPENSATED],
+                         ids=["comp+active", "plain+active", "comp+all"])
+def test_fixed_dt_steps_are_bitwise_the_fp32_oracle(ctx, mesh, grid, case, flags) -> None:  # noqa: ANN001
+    gpu, ref = _both(ctx, mesh)
+    h, hu, hv = f32_state(*CASES[case](grid))
+    nb = num_blocks(grid.N, 256)
+    for be in (gpu, ref):
+        D.setup(be, grid.N, h, hu, hv, MANNING, dry_tol=DRY_TOL, flags=flags)
+        D.start(be, 8 * DT, DT, DT)
+    D.assert_same(D.snapshot(gpu, grid.N, nb), D.snapshot(ref, grid.N, nb), "prologue")
+    gpu.steps(3, use_graph=False)
+    ref.steps(3)
+    D.assert_same(D.snapshot(gpu, grid.N, nb), D.snapshot(ref, grid.N, nb), "3 steps")
+    gpu.steps(7, use_graph=True)  # past the stop: the last two are no-ops
+    ref.steps(7)
+    got, want = D.snapshot(gpu, grid.N, nb), D.snapshot(ref, grid.N, nb)
+    D.assert_same(got, want, "10 steps")
+    assert int(got["ctrl"][1]) == 8 and int(got["ctrl"][0]) == 8 * 1024
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fixed_dt_steps_match_fp64_within_reference_tolerance(ctx, mesh, grid, case) -> None:  # noqa: ANN001
+    gpu, _ = _both(ctx, mesh)
+    h, hu, hv = f32_state(*CASES[case](grid))
+    D.setup(gpu, grid.N, h, hu, hv, MANNING, dry_tol=DRY_TOL)
+    D.start(gpu, 8 * DT, DT, DT)
+    gpu.steps(8, use_graph=True)
+    got = D.snapshot(gpu, grid.N, num_blocks(grid.N, 256))
+    m64 = ref64.Mesh64.from_device_arrays(mesh)
+    s0 = ref64.State64(h.astype(np.float64), hu.astype(np.float64), hv.astype(np.float64))
+    want = ref64.run_dt_sequence(m64, s0, np.full(grid.N, MANNING), [DT] * 8, 9.81, DRY_TOL)
+    assert np.abs(got["h"] - want.h).max() <= 2e-6
+    assert np.abs(got["hu"] - want.hu).max() <= 2e-5
+    assert np.abs(got["hv"] - want.hv).max() <= 2e-5
+
+
+@pytest.mark.parametrize("case", ["dam_break", "draining"])
+def test_cfl_driven_run_is_bitwise_the_fp32_oracle(ctx, mesh, grid, case) -> None:  # noqa: ANN001
+    """Every step picks its own dt on device; windows retarget the clock."""
+    gpu, ref = _both(ctx, mesh)
+    h, hu, hv = f32_state(*CASES[case](grid))
+    nb = num_blocks(grid.N, 256)
+    for be in (gpu, ref):
+        D.setup(be, grid.N, h, hu, hv, MANNING, dry_tol=DRY_TOL)
+        D.start(be, 2.0, 1.0, 0.01)
+    for k in range(6):
+        gpu.steps(25, use_graph=True)
+        ref.steps(25)
+        D.assert_same(D.snapshot(gpu, grid.N, nb), D.snapshot(ref, grid.N, nb), f"block {k}")
+    # A second window: move the stop and re-plan.
+    for be in (gpu, ref):
+        c = be.ctrl_read()
+        c.stop_ticks = int(c.stop_ticks) + 2 * 2**20
+        be.ctrl_write(c)
+        be.retarget()
+    gpu.steps(50, use_graph=True)
+    ref.steps(50)
+    D.assert_same(D.snapshot(gpu, grid.N, nb), D.snapshot(ref, grid.N, nb), "after retarget")
+
+
+def test_sources_are_bitwise_the_fp32_oracle(ctx, mesh, grid) -> None:  # noqa: ANN001
+    """A point source into a dry cell and rain on a patch of cells."""
+    gpu, ref = _both(ctx, mesh)
+    n = grid.N
+    nb = num_blocks(n, 256)
+    z = np.zeros(n, np.float32)
+    x, y = grid.centroid[:, 0], grid.centroid[:, 1]
+    point = int(np.argmin((x - 100.0) ** 2 + (y - 32.0) ** 2))
+    patch = np.flatnonzero((x > 20.0) & (x < 60.0)).astype(np.uint32)
+    cells = np.union1d(patch, [point]).astype(np.uint32)
+    rates = np.where(cells == point, 0.5 / grid.area[point], 1.4e-5).astype(np.float32)
+    for be in (gpu, ref):
+        D.setup(be, n, z, z, z, MANNING, dry_tol=DRY_TOL, src_cells=cells, src_rates=rates)
+        D.start(be, 4.0, 0.5, 0.05)
+    for k in range(4):
+        gpu.steps(40, use_graph=True)
+        ref.steps(40)
+        D.assert_same(D.snapshot(gpu, n, nb), D.snapshot(ref, n, nb), f"block {k}")
+    assert D.snapshot(gpu, n, nb)["h"].max() > DRY_TOL
+
+
+def test_active_set_on_equals_off(ctx, mesh, grid) -> None:  # noqa: ANN001
+    """Skipping blocks is exact (reference test_gpu_active_set.py:77-91)."""
+    gpu, _ = _both(ctx, mesh)
+    h, hu, hv = f32_state(*CASES["dam_break"](grid))
+    nb = num_blocks(grid.N, 256)
+    out = []
+    for flags in (D.FLAGS_DEFAULT, L.FLAG_DETERMINISTIC | L.FLAG_COMPENSATED):
+        D.setup(gpu, grid.N, h, hu, hv, MANNING, dry_tol=DRY_TOL, flags=flags)
+        D.start(gpu, 3.0, 1.0, 0.01)
+        gpu.steps(150, use_graph=True)
+        s = D.snapshot(gpu, grid.N, nb)
+        out.append({k: s[k] for k in ("h", "hu", "hv", "h_lo", "ctrl")})
+    D.assert_same(out[0], out[1], "active on vs off")
