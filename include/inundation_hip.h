@@ -131,6 +131,13 @@ int inu_dt_words(inu_ctx* ctx, uint32_t* out4);
  * (gpu_resident_batch.py:567-591). */
 int inu_prologue(inu_ctx* ctx);
 
+/* Work the active set let through, summed over the real steps since the last
+ * reset: out4 = {active cells, active owned edges, active CSR slots, steps}
+ * (a block's cells and the edges whose left cell lies in it, as the
+ * algorithmic-bytes rule counts them). reset != 0 zeroes the sums. No
+ * reference counterpart (measurement). */
+int inu_activity(inu_ctx* ctx, uint64_t* out4, int reset);
+
 /* New output window: step_control(RETARGET); replaces _record_retarget
  * (gpu_resident_batch.py:593-599). */
 int inu_retarget(inu_ctx* ctx);

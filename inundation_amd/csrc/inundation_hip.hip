@@ -56,6 +56,13 @@ struct Ctl {
   uint32_t cnt_next;  // blocks appended by k_dilate for the next step
   uint32_t swap_on;   // 1 when k_dilate runs every step (active set on)
   uint32_t pad;
+  // Work of the active set, for accounting: {cells, owned edges, CSR slots}
+  // of the active blocks of the next step, of the current step, and summed
+  // over the real steps taken.
+  unsigned long long act_next[3];
+  unsigned long long act_cur[3];
+  unsigned long long act_total[3];
+  unsigned long long act_steps;
 };
 static_assert(sizeof(inu_ctrl) == 48, "control block must be 48 bytes");
 
@@ -98,11 +105,21 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
   return v;
 }
 
-// Min of a per-thread float bound (positive or kNoBound) over the workgroup,
-// one atomicMin per wave. Positive floats order like their bit patterns.
+// Min of a per-thread float bound (positive or kNoBound) over the workgroup
+// (blockDim.x <= 256), then one atomicMin: positive floats order like their
+// bit patterns. Every thread of the workgroup must call it.
 __device__ __forceinline__ void block_atomic_min(uint32_t* dst, uint32_t bits) {
+  __shared__ uint32_t wave_min[4];
   const uint32_t m = wave_min_u32(bits);
-  if ((threadIdx.x & 63u) == 0u && m != kNoBound) atomicMin(dst, m);
+  const uint32_t w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63u) == 0u) wave_min[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t v = wave_min[0];
+    for (uint32_t k = 1; k < (blockDim.x + 63u) / 64u; ++k) v = (wave_min[k] < v) ? wave_min[k] : v;
+    if (v != kNoBound) atomicMin(dst, v);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -195,7 +212,7 @@ __device__ __forceinline__ void block_edges(const Args& a, uint32_t b, const Lds
 // compensated variants), with the corrector also applying this block's
 // sources (resident.py:691-708).
 template <int ST, bool COMP>
-__device__ void stage_block(const Args& a, uint32_t b, float dt, const Lds& l) {
+__device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& l) {
   const uint32_t tid = threadIdx.x;
   const uint32_t c0 = b << 8;
   const uint32_t nc = min(kBlockCells, a.N - c0);
@@ -316,8 +333,8 @@ __device__ void stage_block(const Args& a, uint32_t b, float dt, const Lds& l) {
   if (ST == 1) {
     const int any_wet = __syncthreads_or(wet ? 1 : 0);
     if (tid == 0) a.wet[b] = any_wet ? 1u : 0u;
-    block_atomic_min(&a.ctl->dtbuf[1], bound);
   }
+  return bound;
 }
 
 template <int ST, bool COMP>
@@ -335,11 +352,14 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
   const uint32_t per = gridDim.x / kXcds;
   const uint32_t lo = (uint32_t)(((uint64_t)cnt * xcd) / kXcds);
   const uint32_t hi = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
+  uint32_t bound = kNoBound;  // this thread's smallest next-step bound
   for (uint32_t t = lo + blockIdx.x / kXcds; t < hi; t += per) {
     const uint32_t b = a.list[t];
     if (a.active[b] == 0u) continue;  // listed only for its sources
-    stage_block<ST, COMP>(a, b, dt, l);
+    const uint32_t bb = stage_block<ST, COMP>(a, b, dt, l);
+    bound = (bb < bound) ? bb : bound;
   }
+  if (ST == 1) block_atomic_min(&a.ctl->dtbuf[1], bound);
 }
 
 // Sources of blocks the active set skipped this step (reference
@@ -381,24 +401,25 @@ __global__ __launch_bounds__(256) void k_source_bound(const Args a) {
   if (a.ctl->dtbuf[0] == 0u) return;
   const Phys& p = a.p;
   uint32_t bound = kNoBound;
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < a.n_src) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.n_src; k += gridDim.x * blockDim.x) {
     const uint32_t i = a.src_cell[k];
     const float hn = a.h[i];
-    if (hn >= p.dry) {
-      const float hui = a.hu[i];
-      const float hvi = a.hv[i];
-      const float u = sqrtf(hui * hui + hvi * hvi) / hn;
-      const float c = sqrtf(p.g * hn);
-      float lam = 0.0f;
-      const uint32_t s1 = a.cptr[i + 1];
-      for (uint32_t s = a.cptr[i]; s < s1; ++s) {
-        const int nb = a.snb[s];
-        const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
-        lam += (u + (dry_side ? 2.0f : 1.0f) * c) * a.slen[s];
-      }
-      const float A = a.area[i];
-      if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
+    if (!(hn >= p.dry)) continue;
+    const float hui = a.hu[i];
+    const float hvi = a.hv[i];
+    const float u = sqrtf(hui * hui + hvi * hvi) / hn;
+    const float c = sqrtf(p.g * hn);
+    float lam = 0.0f;
+    const uint32_t s1 = a.cptr[i + 1];
+    for (uint32_t s = a.cptr[i]; s < s1; ++s) {
+      const int nb = a.snb[s];
+      const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
+      lam += (u + (dry_side ? 2.0f : 1.0f) * c) * a.slen[s];
+    }
+    const float A = a.area[i];
+    if (lam > 1e-30f && A > 1e-10f) {
+      const uint32_t bb = __float_as_uint(p.cfl * A / lam);
+      bound = (bb < bound) ? bb : bound;
     }
   }
   block_atomic_min(&a.ctl->dtbuf[1], bound);
@@ -416,6 +437,7 @@ __global__ void k_block_mark(const Args a, int all_active) {
   if (b == 0) {
     a.ctl->dtbuf[0] = __float_as_uint(kDtSentinel);
     a.ctl->cnt_next = 0u;
+    for (int k = 0; k < 3; ++k) a.ctl->act_next[k] = 0ull;
   }
   if (b >= a.NB) return;
   if (all_active) {
@@ -442,6 +464,7 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
   if (!force && a.ctl->dtbuf[0] == 0u) return;  // no-op step changed nothing
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   bool take = false;
+  uint32_t w_cells = 0, w_edges = 0, w_slots = 0;
   if (b < a.NB) {
     uint32_t v;
     if (keep_flags) {
@@ -455,6 +478,35 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
       a.active[b] = v;
     }
     take = (v != 0u) || (a.has_src[b] != 0u);
+    if (v != 0u) {
+      const uint32_t c0 = b << 8;
+      const uint32_t c1 = min(c0 + kBlockCells, a.N);
+      w_cells = c1 - c0;
+      w_edges = a.eo[b + 1] - a.eo[b];
+      w_slots = a.cptr[c1] - a.cptr[c0];
+    }
+  }
+  // Active-set work of the next step: one atomic per workgroup and counter.
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    w_cells += (uint32_t)__shfl_xor((int)w_cells, off, 64);
+    w_edges += (uint32_t)__shfl_xor((int)w_edges, off, 64);
+    w_slots += (uint32_t)__shfl_xor((int)w_slots, off, 64);
+  }
+  {
+    __shared__ uint32_t part[4][3];
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0u) {
+      part[w][0] = w_cells;
+      part[w][1] = w_edges;
+      part[w][2] = w_slots;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      unsigned long long tot = 0;
+      for (uint32_t k = 0; k < blockDim.x / 64u; ++k) tot += part[k][threadIdx.x];
+      if (tot) atomicAdd(&a.ctl->act_next[threadIdx.x], tot);
+    }
   }
   const uint64_t mask = __ballot(take ? 1 : 0);
   if (mask == 0ull) return;
@@ -474,6 +526,7 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
   extern __shared__ float sm[];
   const Lds l = carve(sm, a.max_local);
   const Phys& p = a.p;
+  uint32_t best = kNoBound;
   for (uint32_t b = blockIdx.x; b < a.NB; b += gridDim.x) {
     const uint32_t tid = threadIdx.x;
     const uint32_t c0 = b << 8;
@@ -531,8 +584,9 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
       const float A = a.area[i];
       if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
     }
-    block_atomic_min(&a.ctl->dtbuf[0], bound);
+    best = (bound < best) ? bound : best;
   }
+  block_atomic_min(&a.ctl->dtbuf[0], best);
 }
 
 // The integer clock (reference resident.py:842-882). Mode 0 START plans from
@@ -556,6 +610,8 @@ __global__ void k_step_control(const Args a, int mode) {
       raw = __uint_as_float(t->dtbuf[1]);
       c.last_raw = raw;
       publish = t->swap_on != 0u;
+      for (int k = 0; k < 3; ++k) t->act_total[k] += t->act_cur[k];
+      t->act_steps += 1ull;
     } else {
       raw = c.last_raw;
     }
@@ -565,6 +621,10 @@ __global__ void k_step_control(const Args a, int mode) {
   if (publish) {
     t->cnt_cur = t->cnt_next;
     t->cnt_next = 0u;
+    for (int k = 0; k < 3; ++k) {
+      t->act_cur[k] = t->act_next[k];
+      t->act_next[k] = 0ull;
+    }
   }
   const uint64_t remaining = (c.stop_ticks > c.now_ticks) ? c.stop_ticks - c.now_ticks : 0ull;
   uint32_t ticks;
@@ -715,7 +775,8 @@ int launch_step(inu_ctx* c, const hipEvent_t* ev) {
       hipLaunchKernelGGL((k_source_idle<true>), dim3(std::max(gb, 1u)), dim3(256), 0, c->stream, a);
     else
       hipLaunchKernelGGL((k_source_idle<false>), dim3(std::max(gb, 1u)), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(k_source_bound, dim3((c->n_src + 255) / 256), dim3(256), 0, c->stream, a);
+    const uint32_t gs = std::min<uint32_t>((c->n_src + 255) / 256, 8u * (uint32_t)c->cus);
+    hipLaunchKernelGGL(k_source_bound, dim3(std::max(gs, 1u)), dim3(256), 0, c->stream, a);
   }
   if (c->flags & INU_FLAG_ACTIVE_SET)
     hipLaunchKernelGGL(k_dilate, dim3((c->NB + 255) / 256), dim3(256), 0, c->stream, a, 0, 0);
@@ -1072,6 +1133,23 @@ int inu_dt_words(inu_ctx* c, uint32_t* out4) {
   if (!out4) return fail(c, INU_EINVAL, "null output");
   HIPCHK(c, hipMemcpyAsync(out4, c->ctl->dtbuf, 16, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inu_activity(inu_ctx* c, uint64_t* out4, int reset) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!out4) return fail(c, INU_EINVAL, "null output");
+  Ctl h;
+  HIPCHK(c, hipMemcpyAsync(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 3; ++k) out4[k] = h.act_total[k];
+  out4[3] = h.act_steps;
+  if (reset) {
+    unsigned long long z[4] = {0ull, 0ull, 0ull, 0ull};
+    HIPCHK(c, hipMemcpyAsync(&c->ctl->act_total[0], z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   return 0;
 }
 

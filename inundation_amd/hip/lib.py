@@ -54,6 +54,7 @@ EXPORTS = (
     "inu_ctrl_write",
     "inu_ctrl_read",
     "inu_dt_words",
+    "inu_activity",
     "inu_prologue",
     "inu_retarget",
     "inu_steps",
@@ -116,6 +117,7 @@ def load() -> C.CDLL:
         "inu_ctrl_write": (_I, [_P, C.POINTER(InuCtrl)]),
         "inu_ctrl_read": (_I, [_P, C.POINTER(InuCtrl)]),
         "inu_dt_words": (_I, [_P, _P]),
+        "inu_activity": (_I, [_P, _P, _I]),
         "inu_prologue": (_I, [_P]),
         "inu_retarget": (_I, [_P]),
         "inu_steps": (_I, [_P, _U32, _I]),
@@ -251,6 +253,12 @@ class Context:
         out = np.zeros(4, dtype=np.uint32)
         self.call("inu_dt_words", ptr(out))
         return out
+
+    def activity(self, reset: bool = False) -> dict[str, int]:
+        """Active cells, owned edges and slots summed over the real steps, and the step count."""
+        out = np.zeros(4, dtype=np.uint64)
+        self.call("inu_activity", ptr(out), 1 if reset else 0)
+        return {"cells": int(out[0]), "edges": int(out[1]), "slots": int(out[2]), "steps": int(out[3])}
 
     def prologue(self) -> None:
         self.call("inu_prologue")

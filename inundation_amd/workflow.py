@@ -1,0 +1,471 @@
+"""Application API: ``SWEWorkflow`` with the ``hip_resident`` backend.
+
+Mirrors the reference's ``inundation/workflow.py``: the frozen
+``WorkflowConfig`` (``:135-227``), ``PointSource`` (``:60-71``),
+``SimulationPhase`` (``:123-132``), ``WorkflowResult`` (``:230-255``), the
+area-proportional point-source split (``:317-358, 599-625``), initial-state
+loading (``:450-596``) and ``SWEWorkflow.prepare()/run()`` (``:658-969``).
+
+``solver_impl`` accepts the reference's names; ``"hip_resident"`` (the
+default here) is the MI355X backend. The Vulkan backends need Kompute, which
+this package does not ship, and ``prepare()`` says so. Open boundaries and
+mesh-file loading are not implemented by this package: ``prepare()`` takes
+the geometry (``MeshGeometry`` + permutation) or a geometry-cache artifact
+written by ``inundation_amd.mesh.cache``.
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+import time
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import TYPE_CHECKING, Any, Literal, cast
+
+import numpy as np
+
+from .tuning import DRY_TOL_DEFAULT, GRAVITY_G, MOMENTUM_SNAPSHOT_BACKENDS
+
+if TYPE_CHECKING:
+    from collections.abc import Mapping, Sequence
+
+    from numpy.typing import NDArray
+
+    from .mesh.geometry import MeshGeometry
+
+SOLVER_IMPLS = ("baseline", "fixed_dt_batch_barrier", "gpu_resident_batch", "hip_resident")
+INITIAL_STATE_METADATA_KEYS = frozenset({"saved_order", "mesh_source", "t_end_s", "dt_max", "cfl_interval"})
+
+
+@dataclass(frozen=True)
+class PointSource:
+    """Volumetric source over a circular area (discharge in m^3/s, centre in mesh coordinates)."""
+
+    discharge_m3s: float
+    center_xy: tuple[float, float]
+    radius_m: float
+
+
+@dataclass(frozen=True)
+class SimulationPhase:
+    """One phase in a multi-phase hydrograph schedule."""
+
+    duration_s: float
+    sources: Sequence[PointSource]
+    boundaries: Mapping[str, Any] = field(default_factory=dict)
+
+
+@dataclass(frozen=True)
+class WorkflowConfig:
+    """Runtime configuration for ``SWEWorkflow`` (reference ``workflow.py:135-227``)."""
+
+    mesh_source: str | None
+    manning_n: float
+    output_interval_s: float
+    dt_max: float
+    cfl_interval: int
+    gravity: float = GRAVITY_G
+    dry_tol: float = DRY_TOL_DEFAULT
+    cfl: float | None = None
+    workgroup_size: int = 256
+    dt_init: float = 1e-2
+    use_hilbert_reorder: bool = True
+    progress: bool = True
+    area_tol: float = 1e-6
+    initial_state_source: dict[str, object] | str | os.PathLike[str] | None = None
+    initial_state_order: str = "solver"
+    initial_state_id_col: str = "cell_id"
+    track_clamp_mass: bool = False
+    geometry_cache_dir: str | os.PathLike[str] | None = None
+    geometry_cache_source: str | os.PathLike[str] | None = None
+    capture_momentum_snapshots: bool = False
+    open_boundaries: Sequence[Any] = ()
+    steps_per_graph: int | None = None
+    deterministic: bool | None = None
+    compensated: bool | None = None
+    output_every_steps: int | None = None
+    solver_impl: Literal["baseline", "fixed_dt_batch_barrier", "gpu_resident_batch", "hip_resident"] = (
+        "hip_resident"
+    )
+    device: int = 0
+
+    def __post_init__(self) -> None:
+        if self.manning_n <= 0:
+            raise ValueError(f"manning_n must be positive, got {self.manning_n}")
+        if self.output_interval_s <= 0:
+            raise ValueError(f"output_interval_s must be positive, got {self.output_interval_s}")
+        if self.dt_max <= 0:
+            raise ValueError(f"dt_max must be positive, got {self.dt_max}")
+        if self.cfl_interval <= 0:
+            raise ValueError(f"cfl_interval must be positive, got {self.cfl_interval}")
+        if self.steps_per_graph is not None and self.steps_per_graph <= 0:
+            raise ValueError(f"steps_per_graph must be positive, got {self.steps_per_graph}")
+        if self.output_every_steps is not None and self.output_every_steps <= 0:
+            raise ValueError(f"output_every_steps must be positive, got {self.output_every_steps}")
+        if self.solver_impl not in SOLVER_IMPLS:
+            raise ValueError(f"unsupported solver_impl: {self.solver_impl}")
+        if self.capture_momentum_snapshots and self.solver_impl not in MOMENTUM_SNAPSHOT_BACKENDS:
+            raise ValueError(
+                f"capture_momentum_snapshots is not supported by solver_impl "
+                f"{self.solver_impl!r}; use one of {sorted(MOMENTUM_SNAPSHOT_BACKENDS)}"
+            )
+        if self.open_boundaries:
+            raise ValueError("open_boundaries are not implemented by the hip_resident backend")
+
+
+@dataclass
+class WorkflowResult:
+    """Primary outputs (reference ``workflow.py:230-255``)."""
+
+    snapshots: list[NDArray[np.float32]]
+    snap_times: list[float]
+    h_final: NDArray[np.float32]
+    volume_final_m3: float
+    volume_injected_m3: float
+    wall_seconds: float
+    steps_total: int
+    hu_final: NDArray[np.float32] | None = None
+    hv_final: NDArray[np.float32] | None = None
+    hu_snapshots: list[NDArray[np.float32]] | None = None
+    hv_snapshots: list[NDArray[np.float32]] | None = None
+    clamp_mass_m3: float | None = None
+    boundary_volumes_m3: dict[str, NDArray[np.float64]] = field(default_factory=dict)
+
+
+def _point_in_polygon(point: NDArray[np.float64], polygon: NDArray[np.float64]) -> bool:
+    """Inside or on the boundary of a polygon, even-odd rule (reference ``workflow.py:294-314``)."""
+    x, y = float(point[0]), float(point[1])
+    scale = max(1.0, float(np.abs(polygon).max()), abs(x), abs(y))
+    tol = 1e-10 * scale
+    inside = False
+    for a, b in zip(polygon, np.roll(polygon, -1, axis=0)):
+        ax, ay, bx, by = float(a[0]), float(a[1]), float(b[0]), float(b[1])
+        cross = (x - ax) * (by - ay) - (y - ay) * (bx - ax)
+        if (
+            abs(cross) <= tol
+            and min(ax, bx) - tol <= x <= max(ax, bx) + tol
+            and min(ay, by) - tol <= y <= max(ay, by) + tol
+        ):
+            return True
+        if (ay > y) != (by > y) and x < ax + (y - ay) * (bx - ax) / (by - ay):
+            inside = not inside
+    return inside
+
+
+def select_cells_within_radius(
+    centroid: NDArray[np.float32],
+    center_xy: tuple[float, float],
+    radius_m: float,
+    valid_mask: NDArray[np.bool_] | None = None,
+    cell_vertices: NDArray[np.float32] | None = None,
+    cell_vertex_ptr: NDArray[np.int32] | None = None,
+    cell_bbox: NDArray[np.float32] | None = None,
+) -> NDArray[np.bool_]:
+    """Cells whose centroid lies within the radius, plus the cell containing the centre.
+
+    Reference ``workflow.py:317-358``.
+    """
+    center = np.asarray(center_xy, dtype=np.float64)
+    if radius_m < 0.0:
+        return np.zeros(centroid.shape[0], dtype=bool)
+    mask = np.hypot(centroid[:, 0] - center[0], centroid[:, 1] - center[1]) <= float(radius_m)
+    if valid_mask is not None:
+        mask &= valid_mask
+    if cell_vertices is None or cell_vertex_ptr is None or cell_bbox is None or not centroid.shape[0]:
+        return mask
+    bb = np.asarray(cell_bbox, dtype=np.float64)
+    cand = (bb[:, 0] <= center[0]) & (bb[:, 1] >= center[0]) & (bb[:, 2] <= center[1]) & (bb[:, 3] >= center[1])
+    if valid_mask is not None:
+        cand &= valid_mask
+    for c in np.flatnonzero(cand):
+        poly = np.asarray(cell_vertices[int(cell_vertex_ptr[c]) : int(cell_vertex_ptr[c + 1])], dtype=np.float64)
+        if _point_in_polygon(center, poly):
+            mask[c] = True
+    return mask
+
+
+def build_phase_source_rate(
+    geom: MeshGeometry, phase: SimulationPhase, valid_cell: NDArray[np.bool_]
+) -> NDArray[np.float32]:
+    """Per-cell source [m^3/s] of one phase, split by cell area (reference ``workflow.py:599-625``)."""
+    src = np.zeros(geom.N, dtype=np.float32)
+    ox, oy = geom.origin.tolist()
+    for s in phase.sources:
+        cx, cy = s.center_xy
+        mask = select_cells_within_radius(
+            geom.centroid,
+            (cx - ox, cy - oy),
+            s.radius_m,
+            valid_mask=valid_cell,
+            cell_vertices=geom.cell_vertices,
+            cell_vertex_ptr=geom.cell_vertex_ptr,
+            cell_bbox=geom.cell_bbox,
+        )
+        if not mask.any():
+            continue
+        a = geom.area[mask]
+        total = float(a.sum())
+        if total <= 0.0:
+            continue
+        src[mask] += np.float32(s.discharge_m3s) * (a / total)
+    return src
+
+
+def _as_1d_float32(name: str, values: object, n: int) -> NDArray[np.float32]:
+    arr = np.asarray(values, dtype=np.float32).reshape(-1)
+    if arr.size != n:
+        raise ValueError(f"{name}: expected {n} values, got {arr.size}")
+    if not np.isfinite(arr).all():
+        raise ValueError(f"{name}: contains non-finite values")
+    return arr
+
+
+def load_initial_state(
+    source: dict[str, object] | str | os.PathLike[str] | None,
+    n_cells: int,
+    zb: NDArray[np.float32],
+    *,
+    order: str,
+    id_col: str,
+    perm: NDArray[np.int32],
+    dry_tol: float,
+) -> tuple[NDArray[np.float32], NDArray[np.float32], NDArray[np.float32], NDArray[np.float32] | None] | None:
+    """Optional per-cell initial state (h or wse, hu/hv or u/v, optional n_mann).
+
+    Reference ``workflow.py:450-596`` for dict, ``.npz`` and ``.npy`` sources.
+    """
+    if source is None:
+        return None
+    if isinstance(source, dict):
+        state = {k: np.asarray(v) for k, v in source.items()}
+    else:
+        path = Path(os.fspath(source))
+        if not path.exists():
+            raise FileNotFoundError(f"Initial state file not found: {path}")
+        suffix = path.suffix.lower()
+        if suffix == ".npz":
+            with np.load(path) as npz:
+                state = {k: npz[k] for k in npz.files}
+        elif suffix == ".npy":
+            arr = np.load(path)
+            if arr.ndim != 2 or arr.shape[1] not in (3, 4):
+                raise ValueError("For .npy input, expected shape (N,3)=[h,hu,hv] or (N,4)=[h,hu,hv,n_mann]")
+            state = {"h": arr[:, 0], "hu": arr[:, 1], "hv": arr[:, 2]}
+            if arr.shape[1] == 4:
+                state["n_mann"] = arr[:, 3]
+        else:
+            raise ValueError(f"Unsupported initial state format: {suffix}. Use .npz or .npy")
+    valid = {"h", "wse", "hu", "hv", "u", "v", "n_mann", id_col}
+    state = {k: v for k, v in state.items() if k not in INITIAL_STATE_METADATA_KEYS}
+    bad = sorted(k for k in state if k not in valid)
+    if bad:
+        raise ValueError(f"initial state contains unsupported fields: {bad}. Supported fields: {sorted(valid)}")
+    if id_col in state:
+        ids_f = np.asarray(state[id_col]).reshape(-1).astype(np.float64)
+        if not np.isfinite(ids_f).all() or not np.equal(ids_f, np.floor(ids_f)).all():
+            raise ValueError(f"{id_col}: values must be finite integers")
+        ids = ids_f.astype(np.int64)
+        if ids.size != n_cells or np.unique(ids).size != n_cells or ids.min() < 0 or ids.max() >= n_cells:
+            raise ValueError(f"{id_col}: must be a permutation of [0, {n_cells - 1}]")
+        aligned = {}
+        for k, v in state.items():
+            if k == id_col:
+                continue
+            src = np.asarray(v).reshape(-1)
+            if src.size != n_cells:
+                raise ValueError(f"{k}: expected {n_cells} rows, got {src.size}")
+            dst = np.empty(n_cells, dtype=src.dtype)
+            dst[ids] = src
+            aligned[k] = dst
+        state = aligned
+    if "h" in state:
+        h = _as_1d_float32("h", state["h"], n_cells)
+    elif "wse" in state:
+        h = _as_1d_float32("wse", state["wse"], n_cells) - zb.astype(np.float32)
+    else:
+        raise ValueError("Initial state must provide 'h' or 'wse'")
+    if "hu" in state and "hv" in state:
+        hu = _as_1d_float32("hu", state["hu"], n_cells)
+        hv = _as_1d_float32("hv", state["hv"], n_cells)
+    elif "u" in state and "v" in state:
+        hu = h * _as_1d_float32("u", state["u"], n_cells)
+        hv = h * _as_1d_float32("v", state["v"], n_cells)
+    else:
+        raise ValueError("Initial state must provide (hu,hv) or (u,v)")
+    n_mann = _as_1d_float32("n_mann", state["n_mann"], n_cells) if "n_mann" in state else None
+    if order not in ("solver", "original"):
+        raise ValueError("initial_state_order must be 'solver' or 'original'")
+    if order == "original":
+        h, hu, hv = h[perm], hu[perm], hv[perm]
+        if n_mann is not None:
+            n_mann = n_mann[perm]
+    h = np.maximum(h, 0.0).astype(np.float32)
+    wet = h > np.float32(dry_tol)
+    hu = np.where(wet, hu, 0.0).astype(np.float32)
+    hv = np.where(wet, hv, 0.0).astype(np.float32)
+    if n_mann is not None:
+        n_mann = np.maximum(n_mann.astype(np.float32), np.float32(1e-4))
+    return h, hu, hv, n_mann
+
+
+class SWEWorkflow:
+    """Prepare once, run phase schedules (reference ``workflow.py:628-969``)."""
+
+    def __init__(self, config: WorkflowConfig) -> None:
+        self.config = config
+        self.geom: MeshGeometry | None = None
+        self.perm: NDArray[np.int32] | None = None
+        self.solver: Any = None
+
+    def prepare(self, geometry: MeshGeometry | None = None, perm: NDArray[np.int32] | None = None) -> None:
+        """Geometry (given, or from ``geometry_cache_source``), initial state, solver."""
+        cfg = self.config
+        if cfg.solver_impl != "hip_resident":
+            raise RuntimeError(
+                f"solver_impl {cfg.solver_impl!r} is a Vulkan/Kompute backend of the reference; "
+                "this package provides 'hip_resident'"
+            )
+        if geometry is None:
+            if cfg.geometry_cache_source is None:
+                raise ValueError(
+                    "pass the geometry to prepare() or set geometry_cache_source "
+                    "(mesh-file loading is not part of this package)"
+                )
+            from .mesh.cache import load_geometry_cache_artifact
+
+            geometry, perm, meta = load_geometry_cache_artifact(cfg.geometry_cache_source)
+            if meta.use_hilbert_reorder != cfg.use_hilbert_reorder:
+                raise ValueError(
+                    f"geometry cache artifact {cfg.geometry_cache_source} was built with "
+                    f"use_hilbert_reorder={meta.use_hilbert_reorder}, but "
+                    f"WorkflowConfig.use_hilbert_reorder is {cfg.use_hilbert_reorder}"
+                )
+        geom = geometry
+        if perm is None:
+            perm = np.arange(geom.N, dtype=np.int32)
+        h0 = np.zeros(geom.N, dtype=np.float32)
+        hu0 = np.zeros(geom.N, dtype=np.float32)
+        hv0 = np.zeros(geom.N, dtype=np.float32)
+        n0 = geom.manning_n if geom.manning_n is not None else np.full(geom.N, cfg.manning_n, dtype=np.float32)
+        loaded = load_initial_state(
+            cfg.initial_state_source,
+            geom.N,
+            geom.zb,
+            order=cfg.initial_state_order,
+            id_col=cfg.initial_state_id_col,
+            perm=perm,
+            dry_tol=cfg.dry_tol,
+        )
+        if loaded is not None:
+            h0, hu0, hv0, n_loaded = loaded
+            if n_loaded is not None:
+                n0 = n_loaded
+        from .hip.solver import SWESolverHipResident
+
+        solver = SWESolverHipResident(
+            geom,
+            None,
+            h0,
+            hu0,
+            hv0,
+            n0,
+            g=cfg.gravity,
+            dry_tol=cfg.dry_tol,
+            cfl=cfg.cfl,
+            workgroup_size=cfg.workgroup_size,
+            device=cfg.device,
+        )
+        if cfg.track_clamp_mass:
+            solver.track_clamp = True
+        if cfg.steps_per_graph is not None:
+            solver.steps_per_graph = cfg.steps_per_graph
+        if cfg.deterministic is not None:
+            solver.deterministic = cfg.deterministic
+        if cfg.compensated is not None:
+            solver.compensated = cfg.compensated
+        self.geom = geom
+        self.perm = perm
+        self.solver = solver
+
+    def run(self, phases: Sequence[SimulationPhase]) -> WorkflowResult:
+        """Execute the phases in order and return the outputs (reference ``workflow.py:828-969``)."""
+        if self.geom is None or self.solver is None:
+            raise RuntimeError("Workflow not prepared. Call prepare() first.")
+        if not phases:
+            raise ValueError("At least one phase is required.")
+        cfg = self.config
+        area = self.geom.area
+        valid = area > cfg.area_tol
+        snaps: list[NDArray[np.float32]] = []
+        times: list[float] = []
+        hu_s: list[NDArray[np.float32]] | None = [] if cfg.capture_momentum_snapshots else None
+        hv_s: list[NDArray[np.float32]] | None = [] if cfg.capture_momentum_snapshots else None
+        t = 0.0
+        wall0 = time.perf_counter()
+        for k, phase in enumerate(phases):
+            if phase.duration_s <= 0.0:
+                raise ValueError(f"Phase {k} has non-positive duration.")
+            if phase.boundaries:
+                raise ValueError("open boundaries are not implemented by the hip_resident backend")
+            t_end = t + phase.duration_s
+            src = build_phase_source_rate(self.geom, phase, valid)
+            res = self.solver.run(
+                t_end=t_end,
+                output_interval_s=cfg.output_interval_s,
+                dt_max=cfg.dt_max,
+                dt_init=cfg.dt_init,
+                progress=cfg.progress,
+                source_rate=src,
+                resume=k > 0,
+                t_start=t,
+                cfl_interval=cfg.cfl_interval,
+                output_every_steps=None,
+                capture_momentum_snapshots=cfg.capture_momentum_snapshots,
+            )
+            if len(res) == 4:
+                ps, pt, phu, phv = res
+                if hu_s is not None and hv_s is not None:
+                    hu_s.extend(phu)
+                    hv_s.extend(phv)
+            else:
+                ps, pt = res
+            snaps.extend(ps)
+            times.extend(pt)
+            t = t_end
+        wall = time.perf_counter() - wall0
+        h_final = self.solver.download_h()
+        h_precise = cast("NDArray[np.float64]", self.solver.download_h_precise())
+        vol = float((h_precise * area.astype(np.float64)).sum())
+        injected = float(sum(p.duration_s * sum(s.discharge_m3s for s in p.sources) for p in phases))
+        clamp = None
+        if cfg.track_clamp_mass:
+            clamp = float(np.asarray(self.solver.download_clamp_mass(), dtype=np.float64).sum())
+        if cfg.progress:
+            sys.stdout.write(f"[workflow] {self.solver.steps_total} steps in {wall:.2f}s\n")
+        return WorkflowResult(
+            snapshots=snaps,
+            snap_times=times,
+            h_final=h_final,
+            volume_final_m3=vol,
+            volume_injected_m3=injected,
+            wall_seconds=wall,
+            steps_total=int(self.solver.steps_total),
+            hu_final=self.solver.download_hu(),
+            hv_final=self.solver.download_hv(),
+            hu_snapshots=hu_s,
+            hv_snapshots=hv_s,
+            clamp_mass_m3=clamp,
+        )
+
+
+__all__ = [
+    "PointSource",
+    "SWEWorkflow",
+    "SimulationPhase",
+    "WorkflowConfig",
+    "WorkflowResult",
+    "build_phase_source_rate",
+    "load_initial_state",
+    "select_cells_within_radius",
+]

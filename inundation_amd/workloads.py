@@ -1,0 +1,132 @@
+"""The benchmark workloads, built from seeds (seed = 1000 + workload number).
+
+- ``lake_at_rest``: 64 x 64 quads of 10 m over 12 Gaussian bumps (some dry
+  islands), still surface at 1 m, n = 0.03, CFL 0.5, 1000 steps;
+- ``rainfall``: the 16 003 008-cell mixed triangle/quad lattice over 8 km of
+  fBm terrain, 50 mm/h uniform rain from a dry start, per-cell n
+  log-uniform in [0.015, 0.1], 2000 steps (``scale`` shrinks the lattice for
+  tests);
+- ``inflow``: a Lloyd-relaxed Voronoi mesh over 2 km of fBm terrain with a
+  carved channel and one point source with a triangular hydrograph (peak
+  20 m^3/s at 30 min over 1 h), n = 0.035.
+
+Terrain is evaluated at the mean of each cell's vertices (the polygon centroid
+for the lattice quads).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+from numpy.typing import NDArray
+
+from .mesh.geometry import MeshGeometry, build_geometry, hilbert_reorder
+from .mesh.synth import fbm_terrain, gaussian_bumps, lattice_tri_quad, quad_grid, voronoi_mesh
+from .workflow import PointSource, SimulationPhase, build_phase_source_rate
+
+
+@dataclass
+class Workload:
+    name: str
+    geom: MeshGeometry
+    h0: NDArray[np.float32]
+    hu0: NDArray[np.float32]
+    hv0: NDArray[np.float32]
+    manning: NDArray[np.float32]
+    source_rate: NDArray[np.float32] | None  # m^3/s per cell (constant)
+    dry_tol: float
+    cfl: float
+    dt_max: float
+    dt_init: float
+    steps: int
+    seed: int
+
+    def source_pairs(self) -> tuple[NDArray[np.uint32], NDArray[np.float32]]:
+        """(cell, dh/dt) pairs as the resident backend forms them (gpu_resident_batch.py:666-669)."""
+        if self.source_rate is None:
+            return np.empty(0, np.uint32), np.empty(0, np.float32)
+        dh = np.asarray(self.source_rate, dtype=np.float32) * (1.0 / self.geom.area).astype(np.float32)
+        cells = np.flatnonzero(dh).astype(np.uint32)
+        return cells, dh[cells].astype(np.float32)
+
+
+def _vertex_means(verts: NDArray[np.float64], faces: NDArray[np.int32], offsets: NDArray[np.int32]) -> tuple:
+    deg = np.diff(offsets)
+    cell = np.repeat(np.arange(deg.size), deg)
+    cx = np.bincount(cell, verts[faces, 0], minlength=deg.size) / deg
+    cy = np.bincount(cell, verts[faces, 1], minlength=deg.size) / deg
+    return cx, cy
+
+
+def lake_at_rest(seed: int = 1001) -> Workload:
+    rng = np.random.default_rng(seed)
+    nx = ny = 64
+    dx = 10.0
+    verts, faces = quad_grid(nx, ny, dx)
+    cx = (np.arange(nx * ny) % nx + 0.5) * dx
+    cy = (np.arange(nx * ny) // nx + 0.5) * dx
+    zb = gaussian_bumps(cx, cy, nx * dx, rng)
+    geom, _ = hilbert_reorder(build_geometry(verts, faces, zb_from_file=zb), verbose=False)
+    h0 = np.maximum(np.float32(1.0) - geom.zb, np.float32(0.0)).astype(np.float32)
+    z = np.zeros(geom.N, np.float32)
+    return Workload(
+        name="lake-at-rest 4096 quads (64x64 x 10 m, 12 Gaussian bumps), 1000 steps",
+        geom=geom, h0=h0, hu0=z, hv0=z.copy(), manning=np.full(geom.N, 0.03, np.float32),
+        source_rate=None, dry_tol=1e-4, cfl=0.5, dt_max=1.0, dt_init=0.01, steps=1000, seed=seed,
+    )
+
+
+def rainfall(scale: int = 1, seed: int = 1003, rain_mm_h: float = 50.0) -> Workload:
+    """The 16 M-cell rainfall flood; ``scale`` k uses a 3024/k lattice over 8000/k m."""
+    rng = np.random.default_rng(seed)
+    nq = 3024 // scale
+    length = 8000.0 / scale
+    verts, faces, offsets = lattice_tri_quad(nq, length, rng)
+    cx, cy = _vertex_means(verts, faces, offsets)
+    zb = fbm_terrain(cx, cy, length, rng)
+    del cx, cy
+    n_cells = offsets.size - 1
+    manning = np.exp(rng.uniform(np.log(0.015), np.log(0.1), n_cells)).astype(np.float32)
+    geom = build_geometry(verts, faces, offsets, zb_from_file=zb, manning_n_from_file=manning)
+    del verts, faces, offsets, zb
+    geom, _ = hilbert_reorder(geom, verbose=False)
+    # Uniform rain as one domain-covering point source: the area-proportional
+    # split gives every cell the same intensity (reference workflow.py:617-624).
+    total_area = float(np.float64(geom.area.astype(np.float64).sum()))
+    q = rain_mm_h / 1000.0 / 3600.0 * total_area
+    phase = SimulationPhase(1800.0, [PointSource(q, (length / 2, length / 2), 2.0 * length)])
+    src = build_phase_source_rate(geom, phase, geom.area > 1e-6)
+    z = np.zeros(geom.N, np.float32)
+    return Workload(
+        name=f"rainfall {geom.N} mixed tri/quad cells over {length:g} m fBm, {rain_mm_h:g} mm/h from dry",
+        geom=geom, h0=z, hu0=z.copy(), hv0=z.copy(), manning=geom.manning_n, source_rate=src,
+        dry_tol=1e-4, cfl=0.8, dt_max=0.5, dt_init=0.5, steps=2000, seed=seed,
+    )
+
+
+def inflow(n_points: int = 1_000_000, seed: int = 1002, length: float = 2000.0) -> tuple[Workload, list[SimulationPhase]]:
+    """Voronoi inflow flood and its 60 one-minute hydrograph phases."""
+    rng = np.random.default_rng(seed)
+    verts, faces, offsets = voronoi_mesh(n_points, length, rng)
+    cx, cy = _vertex_means(verts, faces, offsets)
+    zb = fbm_terrain(cx, cy, length, rng)
+    geom, _ = hilbert_reorder(build_geometry(verts, faces, offsets, zb_from_file=zb), verbose=False)
+    # The source sits in the channel near the upstream edge.
+    x0 = 0.05 * length
+    y0 = 0.5 * length + 0.1 * length * np.sin(2.0 * np.pi * x0 / length)
+    phases = []
+    for k in range(60):
+        t_mid = (k + 0.5) * 60.0
+        q = 20.0 * (t_mid / 1800.0 if t_mid <= 1800.0 else (3600.0 - t_mid) / 1800.0)
+        phases.append(SimulationPhase(60.0, [PointSource(q, (x0, y0), 0.0)]))
+    z = np.zeros(geom.N, np.float32)
+    wl = Workload(
+        name=f"inflow {geom.N} Voronoi cells over {length:g} m, 20 m^3/s peak hydrograph",
+        geom=geom, h0=z, hu0=z.copy(), hv0=z.copy(), manning=np.full(geom.N, 0.035, np.float32),
+        source_rate=None, dry_tol=1e-4, cfl=0.8, dt_max=1.0, dt_init=0.1, steps=0, seed=seed,
+    )
+    return wl, phases
+
+
+__all__ = ["Workload", "inflow", "lake_at_rest", "rainfall"]
