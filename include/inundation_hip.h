@@ -46,6 +46,9 @@ extern "C" {
 #define INU_FLAG_COMPENSATED 2u   /* depth as a compensated float32 pair */
 #define INU_FLAG_ACTIVE_SET 4u    /* skip blocks beyond the two-ring of water */
 #define INU_FLAG_TRACK_CLAMP 8u   /* accumulate the positivity-clamp mass */
+#define INU_FLAG_EXACT_ORDER 16u  /* the reference's operation order: one IEEE division or
+                                     square root per occurrence, no fused multiply-add;
+                                     without it the kernels share reciprocals and use FMA */
 
 /* Fields of inu_download / inu_upload. */
 #define INU_FIELD_H 0
@@ -158,11 +161,12 @@ int inu_steps_timed(inu_ctx* ctx, uint32_t n, float* ms_out3);
 int inu_download(inu_ctx* ctx, int which, void* out);
 int inu_upload(inu_ctx* ctx, int which, const void* in);
 
-/* Arithmetic probe on n (a, b) pairs: out5 per pair = {two-sum error of a + b,
- * a / b, sqrt|a|, a^(4/3), film drag of depth a over dt b}. Generalises the
- * EFT probe (reference EFT_PROBE_GLSL, resident.py:889-907, and
- * _check_error_free_arithmetic, solver.py:449-467). */
-int inu_probe(inu_ctx* ctx, uint32_t n, const float* pairs, float* out5, float g, float dry_tol);
+/* Arithmetic probe on n (a, b) pairs: out7 per pair = {two-sum error of a + b,
+ * a / b, sqrt|a|, a^(4/3), film drag of depth a over dt b, a^(-1/3), film drag
+ * in the fast arithmetic}. Generalises the EFT probe (reference
+ * EFT_PROBE_GLSL, resident.py:889-907, and _check_error_free_arithmetic,
+ * solver.py:449-467). */
+int inu_probe(inu_ctx* ctx, uint32_t n, const float* pairs, float* out7, float g, float dry_tol);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,7 @@ constexpr uint32_t kXcds = 8;          // MI355X: 8 accelerator complex dies
 constexpr float kTicksPerSecond = 1048576.0f;
 constexpr float kDtSentinel = 1e30f;
 constexpr uint32_t kNoBound = 0x7f800000u;  // +inf bits: larger than any bound
+constexpr uint32_t kProbeOut = 7;           // outputs per inu_probe pair
 
 // Control words shared by the step kernels. The first 48 bytes are the
 // reference's resident control block (gpu_resident_batch.py:65-77).
@@ -67,21 +68,29 @@ struct Ctl {
 static_assert(sizeof(inu_ctrl) == 48, "control block must be 48 bytes");
 
 // Device pointers and sizes, passed by value to every kernel.
+// Where block b's data lives: owned edges [e0, e0 + n_own) in the sorted edge
+// arrays, halo edges hx[x0, x0 + n_halo), CSR slots [s0, s0 + n_slots).
+struct BlockInfo {
+  uint32_t e0, n_own, x0, n_halo, s0, n_slots, pad0, pad1;
+};
+
 struct Args {
   Phys p;
-  uint32_t N, NB, max_local, n_src, track_clamp;
+  uint32_t N, NB, max_local, max_slots, n_src, track_clamp;
   // cells
   float *h, *hu, *hv, *hlo, *h1, *hu1, *hv1, *h1lo, *area, *nman, *clamp, *src;
   const uint32_t* cptr;
   const uint16_t* lidx;
   const int32_t* snb;
   const float* slen;
+  const uint8_t* cell_inside;  // 1 when every neighbour of the cell is in its block or a wall
   // edges (sorted by the block of the left cell)
   const int2* ecell;
   const float2* enrm;
   const float* elen;
   const float2* edz;
-  const uint32_t *eo, *hp, *hx;
+  const BlockInfo* binfo;
+  const uint32_t* hx;
   // blocks
   uint32_t *wet, *active, *forced, *has_src, *list;
   const uint32_t *bptr, *bidx;
@@ -129,113 +138,201 @@ __device__ __forceinline__ void block_atomic_min(uint32_t* dst, uint32_t bits) {
 // Stage input of a cell outside the block being processed: the corrector reads
 // the predictor state only where that cell's block was active this step
 // (reference resident.py:209-226).
-template <int ST>
-__device__ __forceinline__ void load_outside(const Args& a, int c, float& h, float& hu, float& hv) {
+// LO: also read the compensated low word (the fast order's face depths use it).
+template <int ST, bool LO>
+__device__ __forceinline__ void load_outside(const Args& a, int c, float& h, float& hu, float& hv, float& lo) {
   if (ST == 1 && a.active[(uint32_t)c >> 8] != 0u) {
     h = a.h1[c];
     hu = a.hu1[c];
     hv = a.hv1[c];
+    lo = LO ? a.h1lo[c] : 0.0f;
   } else {
     h = a.h[c];
     hu = a.hu[c];
     hv = a.hv[c];
+    lo = LO ? a.hlo[c] : 0.0f;
   }
 }
 
+// LDS of one block: stage input of its cells, its CSR slots (local edge
+// numbers) and the words of every edge touching it.
 struct Lds {
   float* s_h;
   float* s_hu;
   float* s_hv;
+  float* s_lo;
+  uint16_t* s_slot;
   float* w[6];
+  uint32_t* s_side;  // per local edge: (left local cell << 16) | right local cell
+  float* s_len;      // per local edge: length
 };
 
-__device__ __forceinline__ Lds carve(float* sm, uint32_t max_local) {
+// Local cell numbers of s_side: a cell of the block, a wall, or elsewhere.
+constexpr uint32_t kSideOutside = 0xffffu;
+constexpr uint32_t kSideWall = 0xfffeu;
+
+constexpr uint32_t kLdsCellWords = 4 * kBlockCells;
+constexpr uint32_t kEdgesAhead = 2;  // edge records each thread loads before the barrier
+constexpr uint32_t kSlotsAhead = 4;  // CSR slots each thread loads before the barrier
+
+__device__ __forceinline__ Lds carve(float* sm, uint32_t max_local, uint32_t max_slots) {
   Lds l;
   l.s_h = sm;
   l.s_hu = sm + kBlockCells;
   l.s_hv = sm + 2 * kBlockCells;
-  float* w = sm + 3 * kBlockCells;
+  l.s_lo = sm + 3 * kBlockCells;
+  l.s_slot = reinterpret_cast<uint16_t*>(sm + kLdsCellWords);
+  float* w = sm + kLdsCellWords + (max_slots + 1) / 2;
   for (int k = 0; k < 6; ++k) l.w[k] = w + k * max_local;
+  l.s_side = reinterpret_cast<uint32_t*>(w + 6 * max_local);
+  l.s_len = w + 7 * max_local;
   return l;
 }
 
-// Every edge with a side in block b: its words into LDS (zeros when both
-// sides are dry). Owned edges (left cell in b) are a contiguous range; halo
-// edges (right cell in b, left cell elsewhere) come from the block's list.
-template <int ST>
-__device__ __forceinline__ void block_edges(const Args& a, uint32_t b, const Lds& l) {
-  const uint32_t c0 = b << 8;
-  const uint32_t e0 = a.eo[b];
-  const uint32_t n_own = a.eo[b + 1] - e0;
-  const uint32_t x0 = a.hp[b];
-  const uint32_t n_loc = n_own + (a.hp[b + 1] - x0);
-  for (uint32_t j = threadIdx.x; j < n_loc; j += blockDim.x) {
-    const uint32_t e = (j < n_own) ? e0 + j : a.hx[x0 + (j - n_own)];
-    const int2 c = a.ecell[e];
-    const float2 nrm = a.enrm[e];
-    float hL, huL, hvL, hR, huR, hvR;
-    if ((uint32_t)c.x - c0 < kBlockCells) {
-      const uint32_t q = (uint32_t)c.x - c0;
-      hL = l.s_h[q];
-      huL = l.s_hu[q];
-      hvL = l.s_hv[q];
-    } else {
-      load_outside<ST>(a, c.x, hL, huL, hvL);
-    }
-    const bool wall = c.y < 0;
-    if (wall) {
-      hR = hL;
-      inu::wall_ghost(huL, hvL, nrm.x, nrm.y, huR, hvR);
-    } else if ((uint32_t)c.y - c0 < kBlockCells) {
-      const uint32_t q = (uint32_t)c.y - c0;
-      hR = l.s_h[q];
-      huR = l.s_hu[q];
-      hvR = l.s_hv[q];
-    } else {
-      load_outside<ST>(a, c.y, hR, huR, hvR);
-    }
-    float w[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (!(hL < a.p.dry && hR < a.p.dry)) {
-      const float2 dz = a.edz[e];
-      inu::edge_words(a.p, wall, hL, huL, hvL, hR, huR, hvR, dz.x, dz.y, nrm.x, nrm.y, a.elen[e], w);
-    }
-    l.w[0][j] = w[0];
-    l.w[1][j] = w[1];
-    l.w[2][j] = w[2];
-    l.w[3][j] = w[3];
-    l.w[4][j] = w[4];
-    if (ST == 1) l.w[5][j] = w[5];
+// One edge record, as loaded from the SoA edge arrays.
+struct EdgeRec {
+  int2 c;
+  float2 n;
+  float len;
+  float2 dz;
+};
+
+__device__ __forceinline__ EdgeRec load_edge(const Args& a, const BlockInfo& bi, uint32_t j) {
+  const uint32_t e = (j < bi.n_own) ? bi.e0 + j : a.hx[bi.x0 + (j - bi.n_own)];
+  EdgeRec r;
+  r.c = a.ecell[e];
+  r.n = a.enrm[e];
+  r.len = a.elen[e];
+  r.dz = a.edz[e];
+  return r;
+}
+
+// Words of local edge j into LDS (zeros when both sides are dry).
+template <int ST, bool FM, bool LO>
+__device__ __forceinline__ void edge_eval(const Args& a, uint32_t c0, const Lds& l, const EdgeRec& r, uint32_t j) {
+  if (ST == 1 && a.n_src != 0u) {
+    const uint32_t ql = ((uint32_t)r.c.x - c0 < kBlockCells) ? (uint32_t)r.c.x - c0 : kSideOutside;
+    const uint32_t qr = (r.c.y < 0) ? kSideWall
+                        : ((uint32_t)r.c.y - c0 < kBlockCells) ? (uint32_t)r.c.y - c0 : kSideOutside;
+    l.s_side[j] = (ql << 16) | qr;
+    l.s_len[j] = r.len;
   }
+  float hL, huL, hvL, hR, huR, hvR, loL, loR;
+  if ((uint32_t)r.c.x - c0 < kBlockCells) {
+    const uint32_t q = (uint32_t)r.c.x - c0;
+    hL = l.s_h[q];
+    huL = l.s_hu[q];
+    hvL = l.s_hv[q];
+    loL = LO ? l.s_lo[q] : 0.0f;
+  } else {
+    load_outside<ST, LO>(a, r.c.x, hL, huL, hvL, loL);
+  }
+  const bool wall = r.c.y < 0;
+  if (wall) {
+    hR = hL;
+    loR = loL;
+    if (FM) inu::fast::wall_ghost(huL, hvL, r.n.x, r.n.y, huR, hvR);
+    else inu::wall_ghost(huL, hvL, r.n.x, r.n.y, huR, hvR);
+  } else if ((uint32_t)r.c.y - c0 < kBlockCells) {
+    const uint32_t q = (uint32_t)r.c.y - c0;
+    hR = l.s_h[q];
+    huR = l.s_hu[q];
+    hvR = l.s_hv[q];
+    loR = LO ? l.s_lo[q] : 0.0f;
+  } else {
+    load_outside<ST, LO>(a, r.c.y, hR, huR, hvR, loR);
+  }
+  float w[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (!(hL < a.p.dry && hR < a.p.dry)) {
+    if (FM)
+      inu::fast::edge_words(a.p, wall, hL, huL, hvL, hR, huR, hvR, r.dz.x, r.dz.y, r.n.x, r.n.y, r.len, loL,
+                            loR, w);
+    else
+      inu::edge_words(a.p, wall, hL, huL, hvL, hR, huR, hvR, r.dz.x, r.dz.y, r.n.x, r.n.y, r.len, w);
+  }
+  l.w[0][j] = w[0];
+  l.w[1][j] = w[1];
+  l.w[2][j] = w[2];
+  l.w[3][j] = w[3];
+  l.w[4][j] = w[4];
+  if (ST == 1) l.w[5][j] = w[5];
 }
 
 // One active block of one Heun stage (reference resident.py:425-512, gather +
 // compensated variants), with the corrector also applying this block's
-// sources (resident.py:691-708).
-template <int ST, bool COMP>
+// sources (resident.py:691-708). Three phases: (A) every load that needs no
+// other load (cell data, CSR slots, the thread's first edge records) is
+// issued at once; (B) each edge touching the block is evaluated once into
+// LDS, reading cells outside the block from global memory; (C) each cell sums
+// its edges in CSR order from LDS and updates.
+template <int ST, bool COMP, bool FM>
 __device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& l) {
+  constexpr bool LO = COMP && FM;  // the fast order's face depths read the low words
+  const Phys& p = a.p;
   const uint32_t tid = threadIdx.x;
   const uint32_t c0 = b << 8;
   const uint32_t nc = min(kBlockCells, a.N - c0);
+  const BlockInfo bi = a.binfo[b];
+  const uint32_t n_loc = bi.n_own + bi.n_halo;
+
+  // ---- (A) independent loads -------------------------------------------
+  EdgeRec er[kEdgesAhead];
+#pragma unroll
+  for (uint32_t m = 0; m < kEdgesAhead; ++m)
+    if (tid + m * kBlockCells < n_loc) er[m] = load_edge(a, bi, tid + m * kBlockCells);
+  uint32_t sl[kSlotsAhead];
+#pragma unroll
+  for (uint32_t m = 0; m < kSlotsAhead; ++m)
+    if (tid + m * kBlockCells < bi.n_slots) sl[m] = a.lidx[bi.s0 + tid + m * kBlockCells];
+  const uint32_t i = c0 + tid;
+  float hi = 0.0f, hui = 0.0f, hvi = 0.0f, loi = 0.0f, A = 1.0f, nm = 0.0f;
+  float h0 = 0.0f, hu0 = 0.0f, hv0 = 0.0f, lo0 = 0.0f, rate = 0.0f;
+  uint32_t k0 = 0, k1 = 0;
+  if (tid < nc) {
+    hi = (ST == 0) ? a.h[i] : a.h1[i];
+    hui = (ST == 0) ? a.hu[i] : a.hu1[i];
+    hvi = (ST == 0) ? a.hv[i] : a.hv1[i];
+    if (COMP) loi = (ST == 0) ? a.hlo[i] : a.h1lo[i];
+    k0 = a.cptr[i] - bi.s0;
+    k1 = a.cptr[i + 1] - bi.s0;
+    A = a.area[i];
+    nm = a.nman[i];
+    if (ST == 1) {
+      h0 = a.h[i];
+      hu0 = a.hu[i];
+      hv0 = a.hv[i];
+      if (COMP) lo0 = a.hlo[i];
+      if (a.n_src != 0u) rate = a.src[i];
+    }
+  }
   __syncthreads();  // the previous block's LDS reads are done
   if (tid < nc) {
-    const uint32_t i = c0 + tid;
-    l.s_h[tid] = (ST == 0) ? a.h[i] : a.h1[i];
-    l.s_hu[tid] = (ST == 0) ? a.hu[i] : a.hu1[i];
-    l.s_hv[tid] = (ST == 0) ? a.hv[i] : a.hv1[i];
+    l.s_h[tid] = hi;
+    l.s_hu[tid] = hui;
+    l.s_hv[tid] = hvi;
+    l.s_lo[tid] = loi;
   }
-  __syncthreads();
-  block_edges<ST>(a, b, l);
+#pragma unroll
+  for (uint32_t m = 0; m < kSlotsAhead; ++m)
+    if (tid + m * kBlockCells < bi.n_slots) l.s_slot[tid + m * kBlockCells] = (uint16_t)sl[m];
+  for (uint32_t k = tid + kSlotsAhead * kBlockCells; k < bi.n_slots; k += kBlockCells) l.s_slot[k] = a.lidx[bi.s0 + k];
   __syncthreads();
 
-  const Phys& p = a.p;
+  // ---- (B) edges -----------------------------------------------------------
+#pragma unroll
+  for (uint32_t m = 0; m < kEdgesAhead; ++m)
+    if (tid + m * kBlockCells < n_loc) edge_eval<ST, FM, LO>(a, c0, l, er[m], tid + m * kBlockCells);
+  for (uint32_t j = tid + kEdgesAhead * kBlockCells; j < n_loc; j += kBlockCells)
+    edge_eval<ST, FM, LO>(a, c0, l, load_edge(a, bi, j), j);
+  __syncthreads();
+
+  // ---- (C) cells -----------------------------------------------------------
   bool wet = false;
   uint32_t bound = kNoBound;
   if (tid < nc) {
-    const uint32_t i = c0 + tid;
     float sdh = 0.0f, sdhu = 0.0f, sdhv = 0.0f, lam = 0.0f;
-    const uint32_t k1 = a.cptr[i + 1];
-    for (uint32_t k = a.cptr[i]; k < k1; ++k) {
-      const uint32_t s = a.lidx[k];
+    for (uint32_t k = k0; k < k1; ++k) {
+      const uint32_t s = l.s_slot[k];
       const uint32_t j = s >> 1;
       if (s & 1u) {
         sdh -= l.w[0][j];
@@ -248,38 +345,34 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& 
       }
       if (ST == 1) lam += l.w[5][j];
     }
-    const float A = a.area[i];
     const float dti = dt / A;
-    const float hi = l.s_h[tid];
-    const float hui = l.s_hu[tid];
-    const float hvi = l.s_hv[tid];
     float h_new, lo_new = 0.0f, h_raw;
     if (COMP) {
-      const float loi = (ST == 0) ? a.hlo[i] : a.h1lo[i];
       inu::add_compensated(hi, loi, -dti * sdh, h_new, lo_new);
       float s_raw, e_raw;
       inu::two_sum(hi, -dti * sdh, s_raw, e_raw);
       h_raw = s_raw + (e_raw + loi);
     } else {
-      h_raw = hi - dti * sdh;
+      h_raw = FM ? fmaf(-dti, sdh, hi) : hi - dti * sdh;
       h_new = inu::gmax(h_raw, 0.0f);
     }
-    float hu_new = hui - dti * sdhu;
-    float hv_new = hvi - dti * sdhv;
+    float hu_new = FM ? fmaf(-dti, sdhu, hui) : hui - dti * sdhu;
+    float hv_new = FM ? fmaf(-dti, sdhv, hvi) : hvi - dti * sdhv;
     if (a.track_clamp) a.clamp[i] += 0.5f * A * (h_new - h_raw);
     if (h_new < p.dry) {
       hu_new = 0.0f;
       hv_new = 0.0f;
+    } else if (FM) {
+      inu::fast::friction(p, h_new, nm, dt, hu_new, hv_new);
     } else {
       const float drag = inu::film_drag(p, h_new, dt);
       hu_new *= drag;
       hv_new *= drag;
-      const float n = a.nman[i];
       const float u = hu_new / h_new;
       const float v = hv_new / h_new;
       const float spd = sqrtf(u * u + v * v);
       const float h43 = inu::pow43(h_new);
-      const float Sf = p.g * n * n * spd / h43;
+      const float Sf = p.g * nm * nm * spd / h43;
       const float den = 1.0f + dt * Sf;
       hu_new = hu_new / den;
       hv_new = hv_new / den;
@@ -293,14 +386,14 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& 
       float hf, lof = 0.0f;
       if (COMP) {
         float s, e;
-        inu::two_sum(a.h[i], h_new, s, e);
-        e += a.hlo[i] + lo_new;
+        inu::two_sum(h0, h_new, s, e);
+        e += lo0 + lo_new;
         inu::two_sum(0.5f * s, 0.5f * e, hf, lof);
       } else {
-        hf = 0.5f * (a.h[i] + h_new);
+        hf = 0.5f * (h0 + h_new);
       }
-      float huf = 0.5f * (a.hu[i] + hu_new);
-      float hvf = 0.5f * (a.hv[i] + hv_new);
+      float huf = 0.5f * (hu0 + hu_new);
+      float hvf = 0.5f * (hv0 + hv_new);
       if (hf < p.dry) {
         huf = 0.0f;
         hvf = 0.0f;
@@ -309,41 +402,72 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& 
       if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
       // Sources of this block (reference resident.py:695-708), after the
       // corrector; they change depth only.
-      if (a.n_src != 0u) {
-        const float rate = a.src[i];
-        if (rate != 0.0f) {
-          const float inc = dt * rate;
-          if (COMP) {
-            float hs, ls;
-            inu::add_compensated(hf, lof, inc, hs, ls);
-            hf = hs;
-            lof = ls;
-          } else {
-            hf = inu::gmax(hf + inc, 0.0f);
-          }
-          wet = wet || hf >= p.dry;
+      if (rate != 0.0f) {
+        const float inc = dt * rate;
+        if (COMP) {
+          float hs, ls;
+          inu::add_compensated(hf, lof, inc, hs, ls);
+          hf = hs;
+          lof = ls;
+        } else {
+          hf = inu::gmax(hf + inc, 0.0f);
         }
+        wet = wet || hf >= p.dry;
       }
       a.h[i] = hf;
       if (COMP) a.hlo[i] = lof;
       a.hu[i] = huf;
       a.hv[i] = hvf;
+      hi = hf;  // post-source depth, for the source dt bound below
+      hui = huf;
+      hvi = hvf;
     }
   }
   if (ST == 1) {
     const int any_wet = __syncthreads_or(wet ? 1 : 0);
     if (tid == 0) a.wet[b] = any_wet ? 1u : 0u;
+    if (a.n_src != 0u) {
+      // Next-step dt bound of the block's wetted source cells whose
+      // neighbours all lie in the block (resident.py:709-728), from the
+      // post-source depths; the others are left to k_source_bound.
+      if (tid < nc) l.s_h[tid] = hi;
+      __syncthreads();
+      if (tid < nc && rate != 0.0f && hi >= p.dry) {
+        const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hi : sqrtf(hui * hui + hvi * hvi) / hi;
+        const float c = FM ? p.sqrt_g * sqrtf(hi) : sqrtf(p.g * hi);
+        float lam = 0.0f;
+        bool inside = true;
+        for (uint32_t k = k0; k < k1; ++k) {
+          const uint32_t s = l.s_slot[k];
+          const uint32_t j = s >> 1;
+          const uint32_t pk = l.s_side[j];
+          const uint32_t q = (s & 1u) ? (pk >> 16) : (pk & 0xffffu);
+          if (q == kSideOutside) {
+            inside = false;
+            break;
+          }
+          const bool dry_side = q != kSideWall && l.s_h[q] < p.dry;
+          const float kf = dry_side ? 2.0f : 1.0f;
+          if (FM) lam = fmaf(fmaf(kf, c, u), l.s_len[j], lam);
+          else lam += (u + kf * c) * l.s_len[j];
+        }
+        if (inside && lam > 1e-30f && A > 1e-10f) {
+          const uint32_t bb = __float_as_uint(p.cfl * A / lam);
+          bound = (bb < bound) ? bb : bound;
+        }
+      }
+    }
   }
   return bound;
 }
 
-template <int ST, bool COMP>
+template <int ST, bool COMP, bool FM>
 __global__ __launch_bounds__(256) void k_stage(const Args a) {
   extern __shared__ float sm[];
   const uint32_t dtw = a.ctl->dtbuf[0];
   if (dtw == 0u) return;  // no-op step: the window's stop was reached
   const float dt = __uint_as_float(dtw);
-  const Lds l = carve(sm, a.max_local);
+  const Lds l = carve(sm, a.max_local, a.max_slots);
   // XCD-aware split: workgroup w runs on XCD w % 8, so each XCD walks one
   // contiguous (spatially compact) slice of the block list and its L2 keeps
   // the halo cells its neighbours read.
@@ -356,7 +480,7 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
   for (uint32_t t = lo + blockIdx.x / kXcds; t < hi; t += per) {
     const uint32_t b = a.list[t];
     if (a.active[b] == 0u) continue;  // listed only for its sources
-    const uint32_t bb = stage_block<ST, COMP>(a, b, dt, l);
+    const uint32_t bb = stage_block<ST, COMP, FM>(a, b, dt, l);
     bound = (bb < bound) ? bb : bound;
   }
   if (ST == 1) block_atomic_min(&a.ctl->dtbuf[1], bound);
@@ -397,24 +521,36 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
 // Next-step dt bound of each source cell from its post-source depth (reference
 // resident.py:709-728). Neighbour depths are read after every source of the
 // step has been applied.
+// Next-step dt bound of each wetted source cell from its post-source depth
+// (reference resident.py:709-728), for the sources the corrector did not
+// bound: every source of a block the active set skipped, and sources with a
+// neighbour in another block. Neighbour depths are read after every source of
+// the step has been applied.
+template <bool FM>
 __global__ __launch_bounds__(256) void k_source_bound(const Args a) {
   if (a.ctl->dtbuf[0] == 0u) return;
   const Phys& p = a.p;
   uint32_t bound = kNoBound;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.n_src; k += gridDim.x * blockDim.x) {
-    const uint32_t i = a.src_cell[k];
+  for (uint32_t t = blockIdx.x; t < a.n_src_blocks; t += gridDim.x) {
+    const uint32_t b = a.src_blocks[t];
+    const bool act = a.active[b] != 0u;
+    const uint32_t i = (b << 8) + threadIdx.x;
+    if (i >= a.N || a.src[i] == 0.0f) continue;
+    if (act && a.cell_inside[i] != 0u) continue;  // bounded by the corrector
     const float hn = a.h[i];
     if (!(hn >= p.dry)) continue;
     const float hui = a.hu[i];
     const float hvi = a.hv[i];
-    const float u = sqrtf(hui * hui + hvi * hvi) / hn;
-    const float c = sqrtf(p.g * hn);
+    const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
+    const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
     float lam = 0.0f;
     const uint32_t s1 = a.cptr[i + 1];
     for (uint32_t s = a.cptr[i]; s < s1; ++s) {
       const int nb = a.snb[s];
       const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
-      lam += (u + (dry_side ? 2.0f : 1.0f) * c) * a.slen[s];
+      const float k = dry_side ? 2.0f : 1.0f;
+      if (FM) lam = fmaf(fmaf(k, c, u), a.slen[s], lam);
+      else lam += (u + k * c) * a.slen[s];
     }
     const float A = a.area[i];
     if (lam > 1e-30f && A > 1e-10f) {
@@ -482,7 +618,7 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
       const uint32_t c0 = b << 8;
       const uint32_t c1 = min(c0 + kBlockCells, a.N);
       w_cells = c1 - c0;
-      w_edges = a.eo[b + 1] - a.eo[b];
+      w_edges = a.binfo[b].n_own;
       w_slots = a.cptr[c1] - a.cptr[c0];
     }
   }
@@ -522,9 +658,10 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
 // Prologue CFL bound of the current state, every cell (reference
 // resident.py:528-610): per-edge wave bound into LDS, per-cell CSR sum,
 // atomicMin into dt slot 0.
+template <bool FM, bool LO>
 __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
   extern __shared__ float sm[];
-  const Lds l = carve(sm, a.max_local);
+  const Lds l = carve(sm, a.max_local, a.max_slots);
   const Phys& p = a.p;
   uint32_t best = kNoBound;
   for (uint32_t b = blockIdx.x; b < a.NB; b += gridDim.x) {
@@ -536,42 +673,52 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
       l.s_h[tid] = a.h[c0 + tid];
       l.s_hu[tid] = a.hu[c0 + tid];
       l.s_hv[tid] = a.hv[c0 + tid];
+      l.s_lo[tid] = LO ? a.hlo[c0 + tid] : 0.0f;
     }
     __syncthreads();
-    const uint32_t e0 = a.eo[b];
-    const uint32_t n_own = a.eo[b + 1] - e0;
-    const uint32_t x0 = a.hp[b];
-    const uint32_t n_loc = n_own + (a.hp[b + 1] - x0);
+    const BlockInfo bi = a.binfo[b];
+    const uint32_t e0 = bi.e0;
+    const uint32_t n_own = bi.n_own;
+    const uint32_t x0 = bi.x0;
+    const uint32_t n_loc = n_own + bi.n_halo;
     for (uint32_t j = tid; j < n_loc; j += blockDim.x) {
       const uint32_t e = (j < n_own) ? e0 + j : a.hx[x0 + (j - n_own)];
       const int2 c = a.ecell[e];
       const float2 nrm = a.enrm[e];
-      float hL, huL, hvL, hR, huR, hvR;
+      float hL, huL, hvL, hR, huR, hvR, loL, loR;
       if ((uint32_t)c.x - c0 < kBlockCells) {
         hL = l.s_h[c.x - c0];
         huL = l.s_hu[c.x - c0];
         hvL = l.s_hv[c.x - c0];
+        loL = l.s_lo[c.x - c0];
       } else {
         hL = a.h[c.x];
         huL = a.hu[c.x];
         hvL = a.hv[c.x];
+        loL = LO ? a.hlo[c.x] : 0.0f;
       }
       if (c.y < 0) {
         hR = hL;
-        inu::wall_ghost(huL, hvL, nrm.x, nrm.y, huR, hvR);
+        loR = loL;
+        if (FM) inu::fast::wall_ghost(huL, hvL, nrm.x, nrm.y, huR, hvR);
+        else inu::wall_ghost(huL, hvL, nrm.x, nrm.y, huR, hvR);
       } else if ((uint32_t)c.y - c0 < kBlockCells) {
         hR = l.s_h[c.y - c0];
         huR = l.s_hu[c.y - c0];
         hvR = l.s_hv[c.y - c0];
+        loR = l.s_lo[c.y - c0];
       } else {
         hR = a.h[c.y];
         huR = a.hu[c.y];
         hvR = a.hv[c.y];
+        loR = LO ? a.hlo[c.y] : 0.0f;
       }
       float contrib = 0.0f;
       if (hL >= p.dry || hR >= p.dry) {
         const float2 dz = a.edz[e];
-        contrib = inu::wave_bound(p, hL, huL, hvL, dz.x, hR, huR, hvR, dz.y, nrm.x, nrm.y) * a.elen[e];
+        contrib = (FM ? inu::fast::wave_bound(p, hL, huL, hvL, dz.x, hR, huR, hvR, dz.y, nrm.x, nrm.y, loL, loR)
+                      : inu::wave_bound(p, hL, huL, hvL, dz.x, hR, huR, hvR, dz.y, nrm.x, nrm.y)) *
+                  a.elen[e];
       }
       l.w[5][j] = contrib;
     }
@@ -643,20 +790,23 @@ __global__ void k_step_control(const Args a, int mode) {
   t->dtbuf[1] = __float_as_uint(kDtSentinel);
 }
 
-// Arithmetic probes: v[0..n) pairs (a, b) -> two-sum error, a / b, sqrt(a),
-// a^(4/3), film drag of a at dt b (reference solver.py:449-467 checks the
-// two-sum error term survives compilation).
+// Arithmetic probes: pairs (a, b) -> two-sum error, a / b, sqrt|a|, a^(4/3),
+// film drag of depth a at dt b, a^(-1/3), fast film drag (reference
+// solver.py:449-467 checks that the two-sum error term survives compilation).
 __global__ void k_probe(const float* in, float* out, uint32_t n, Phys p) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const float x = in[2 * k], y = in[2 * k + 1];
   float s, e;
   inu::two_sum(x, y, s, e);
-  out[5 * k + 0] = e;
-  out[5 * k + 1] = x / y;
-  out[5 * k + 2] = sqrtf(fabsf(x));
-  out[5 * k + 3] = (x > 0.0f) ? inu::pow43(x) : 0.0f;
-  out[5 * k + 4] = (x > 0.0f) ? inu::film_drag(p, x, y) : 0.0f;
+  float* o = out + (size_t)kProbeOut * k;
+  o[0] = e;
+  o[1] = x / y;
+  o[2] = sqrtf(fabsf(x));
+  o[3] = (x > 0.0f) ? inu::pow43(x) : 0.0f;
+  o[4] = (x > 0.0f) ? inu::film_drag(p, x, y) : 0.0f;
+  o[5] = (x > 0.0f) ? inu::fast::rcbrt(x) : 0.0f;
+  o[6] = (x > 0.0f) ? inu::fast::film_drag(p, x, y) : 0.0f;
 }
 
 }  // namespace
@@ -743,6 +893,8 @@ Phys make_phys(float g, float dry, float cfl) {
   p.e_film2 = p.e_film * p.e_film;
   p.half_g = 0.5f * g;
   p.sqrt_g_e = std::sqrt(g / p.e_film);
+  p.inv_dry = 1.0f / dry;
+  p.sqrt_g = std::sqrt(g);
   return p;
 }
 
@@ -755,19 +907,24 @@ int check_ready(inu_ctx* c) {
 // Kernel launches of one step, in stream order. With `ev` set, events are
 // recorded before the predictor, between the two stages and after the
 // corrector (no host wait).
+template <int ST>
+void launch_stage(inu_ctx* c, bool comp, bool fm) {
+  const size_t lds = ST == 0 ? c->lds_stage0 : c->lds_stage1;
+  const dim3 g(c->grid_stage), b(256);
+  if (comp && fm) hipLaunchKernelGGL((k_stage<ST, true, true>), g, b, lds, c->stream, c->a);
+  else if (comp) hipLaunchKernelGGL((k_stage<ST, true, false>), g, b, lds, c->stream, c->a);
+  else if (fm) hipLaunchKernelGGL((k_stage<ST, false, true>), g, b, lds, c->stream, c->a);
+  else hipLaunchKernelGGL((k_stage<ST, false, false>), g, b, lds, c->stream, c->a);
+}
+
 int launch_step(inu_ctx* c, const hipEvent_t* ev) {
   const Args& a = c->a;
   const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
+  const bool fm = (c->flags & INU_FLAG_EXACT_ORDER) == 0;
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
-  if (comp)
-    hipLaunchKernelGGL((k_stage<0, true>), dim3(c->grid_stage), dim3(256), c->lds_stage0, c->stream, a);
-  else
-    hipLaunchKernelGGL((k_stage<0, false>), dim3(c->grid_stage), dim3(256), c->lds_stage0, c->stream, a);
+  launch_stage<0>(c, comp, fm);
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-  if (comp)
-    hipLaunchKernelGGL((k_stage<1, true>), dim3(c->grid_stage), dim3(256), c->lds_stage1, c->stream, a);
-  else
-    hipLaunchKernelGGL((k_stage<1, false>), dim3(c->grid_stage), dim3(256), c->lds_stage1, c->stream, a);
+  launch_stage<1>(c, comp, fm);
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   if (c->n_src) {
     const uint32_t gb = std::min<uint32_t>(c->n_src_blocks, 8u * (uint32_t)c->cus);
@@ -775,8 +932,9 @@ int launch_step(inu_ctx* c, const hipEvent_t* ev) {
       hipLaunchKernelGGL((k_source_idle<true>), dim3(std::max(gb, 1u)), dim3(256), 0, c->stream, a);
     else
       hipLaunchKernelGGL((k_source_idle<false>), dim3(std::max(gb, 1u)), dim3(256), 0, c->stream, a);
-    const uint32_t gs = std::min<uint32_t>((c->n_src + 255) / 256, 8u * (uint32_t)c->cus);
-    hipLaunchKernelGGL(k_source_bound, dim3(std::max(gs, 1u)), dim3(256), 0, c->stream, a);
+    const uint32_t gs = std::max(1u, std::min<uint32_t>(c->n_src_blocks, 8u * (uint32_t)c->cus));
+    if (fm) hipLaunchKernelGGL((k_source_bound<true>), dim3(gs), dim3(256), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_source_bound<false>), dim3(gs), dim3(256), 0, c->stream, a);
   }
   if (c->flags & INU_FLAG_ACTIVE_SET)
     hipLaunchKernelGGL(k_dilate, dim3((c->NB + 255) / 256), dim3(256), 0, c->stream, a, 0, 0);
@@ -864,14 +1022,21 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
       if (r >= 0 && (uint32_t)r / kBlockCells != (uint32_t)edge_cell_l[e] / kBlockCells) hx[fill[(uint32_t)r / kBlockCells]++] = e;
     }
   }
-  uint32_t max_local = 1;
-  for (uint32_t b = 0; b < NB; ++b)
-    max_local = std::max(max_local, (eo[b + 1] - eo[b]) + (hp[b + 1] - hp[b]));
+  uint32_t max_local = 1, max_slots = 1;
+  std::vector<BlockInfo> binfo(NB);
+  for (uint32_t b = 0; b < NB; ++b) {
+    const uint32_t cb = b * kBlockCells, ce = std::min(cb + kBlockCells, N);
+    binfo[b] = BlockInfo{eo[b], eo[b + 1] - eo[b], hp[b], hp[b + 1] - hp[b], cell_edge_ptr[cb],
+                         cell_edge_ptr[ce] - cell_edge_ptr[cb], 0u, 0u};
+    max_local = std::max(max_local, binfo[b].n_own + binfo[b].n_halo);
+    max_slots = std::max(max_slots, binfo[b].n_slots);
+  }
   if (max_local >= 32768u) return fail(c, INU_ERANGE, "a block touches more than 32767 edges");
   // Local CSR: each slot's edge as its position in its block's LDS edge list.
   std::vector<uint16_t> lidx(S);
   std::vector<int32_t> snb(S);
   std::vector<float> slen(S);
+  std::vector<uint8_t> inside(N, 1);
   for (uint32_t i = 0; i < N; ++i) {
     const uint32_t b = i / kBlockCells;
     if (cell_edge_ptr[i + 1] < cell_edge_ptr[i] || cell_edge_ptr[i + 1] > S)
@@ -894,6 +1059,7 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
       lidx[k] = (uint16_t)((j << 1) | right);
       snb[k] = right ? edge_cell_l[e] : edge_cell_r[e];
       slen[k] = edge_len[e];
+      if (snb[k] >= 0 && (uint32_t)snb[k] / kBlockCells != b) inside[i] = 0;
     }
   }
   // Two-ring block graph (reference blocks.py:63-118).
@@ -937,12 +1103,14 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   rc |= dalloc(c, &a.h, NP); rc |= dalloc(c, &a.hu, NP); rc |= dalloc(c, &a.hv, NP); rc |= dalloc(c, &a.hlo, NP);
   rc |= dalloc(c, &a.h1, NP); rc |= dalloc(c, &a.hu1, NP); rc |= dalloc(c, &a.hv1, NP); rc |= dalloc(c, &a.h1lo, NP);
   rc |= dalloc(c, &a.area, NP); rc |= dalloc(c, &a.nman, NP); rc |= dalloc(c, &a.clamp, NP); rc |= dalloc(c, &a.src, NP);
-  uint32_t* cptr; uint16_t* lid; int32_t* sn; float* sl;
+  uint32_t* cptr; uint16_t* lid; int32_t* sn; float* sl; uint8_t* cin;
+  rc |= dalloc(c, &cin, N);
   rc |= dalloc(c, &cptr, (size_t)N + 1); rc |= dalloc(c, &lid, S); rc |= dalloc(c, &sn, S); rc |= dalloc(c, &sl, S);
   int2* ecell; float2* enrm; float* elen; float2* edz;
   rc |= dalloc(c, &ecell, E); rc |= dalloc(c, &enrm, E); rc |= dalloc(c, &elen, E); rc |= dalloc(c, &edz, E);
-  uint32_t *deo, *dhp, *dhx, *dbptr, *dbidx;
-  rc |= dalloc(c, &deo, NB + 1); rc |= dalloc(c, &dhp, NB + 1); rc |= dalloc(c, &dhx, hx.size());
+  uint32_t *dhx, *dbptr, *dbidx;
+  BlockInfo* dbinfo;
+  rc |= dalloc(c, &dbinfo, NB); rc |= dalloc(c, &dhx, hx.size());
   rc |= dalloc(c, &dbptr, NB + 1); rc |= dalloc(c, &dbidx, bidx.size());
   rc |= dalloc(c, &a.wet, NB); rc |= dalloc(c, &a.active, NB); rc |= dalloc(c, &a.forced, NB);
   rc |= dalloc(c, &a.has_src, NB); rc |= dalloc(c, &a.list, NB);
@@ -950,12 +1118,12 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1);
   rc |= dalloc(c, &c->ctl, 1);
   if (rc) return rc;
-  a.cptr = cptr; a.lidx = lid; a.snb = sn; a.slen = sl;
+  a.cptr = cptr; a.lidx = lid; a.snb = sn; a.slen = sl; a.cell_inside = cin;
   a.ecell = ecell; a.enrm = enrm; a.elen = elen; a.edz = edz;
-  a.eo = deo; a.hp = dhp; a.hx = dhx; a.bptr = dbptr; a.bidx = dbidx;
+  a.binfo = dbinfo; a.hx = dhx; a.bptr = dbptr; a.bidx = dbidx;
   a.src_cell = src_cell; a.src_blocks = src_blocks;
   a.ctl = c->ctl;
-  a.N = N; a.NB = NB; a.max_local = max_local;
+  a.N = N; a.NB = NB; a.max_local = max_local; a.max_slots = max_slots;
   // Host-side repack of the edge record into vector-loadable SoA.
   std::vector<int2> h_ecell(E);
   std::vector<float2> h_nrm(E), h_dz(E);
@@ -980,20 +1148,20 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   if ((rc = upload(c, lid, lidx.data(), S))) return rc;
   if ((rc = upload(c, sn, snb.data(), S))) return rc;
   if ((rc = upload(c, sl, slen.data(), S))) return rc;
+  if ((rc = upload(c, cin, inside.data(), N))) return rc;
   if ((rc = upload(c, ecell, h_ecell.data(), E))) return rc;
   if ((rc = upload(c, enrm, h_nrm.data(), E))) return rc;
   if ((rc = upload(c, elen, edge_len, E))) return rc;
   if ((rc = upload(c, edz, h_dz.data(), E))) return rc;
-  if ((rc = upload(c, deo, eo.data(), NB + 1))) return rc;
-  if ((rc = upload(c, dhp, hp.data(), NB + 1))) return rc;
+  if ((rc = upload(c, dbinfo, binfo.data(), NB))) return rc;
   if ((rc = upload(c, dhx, hx.data(), hx.size()))) return rc;
   if ((rc = upload(c, dbptr, bptr.data(), NB + 1))) return rc;
   if ((rc = upload(c, dbidx, bidx.data(), bidx.size()))) return rc;
-  c->lds_stage0 = (3 * kBlockCells + 6 * (size_t)max_local) * sizeof(float);
+  c->lds_stage0 = (kLdsCellWords + (max_slots + 1) / 2 + 8 * (size_t)max_local) * sizeof(float);
   c->lds_stage1 = c->lds_stage0;
   if (c->lds_stage1 > 160 * 1024) return fail(c, INU_ERANGE, "a block's edge list does not fit in LDS");
   int occ = 0;
-  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_stage<1, true>, 256, c->lds_stage1));
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_stage<1, true, true>, 256, c->lds_stage1));
   occ = std::max(occ, 1);
   uint32_t g = (uint32_t)(occ * c->cus);
   g = std::max<uint32_t>(kXcds, std::min<uint32_t>(g, ((NB + kXcds - 1) / kXcds) * kXcds));
@@ -1162,7 +1330,12 @@ int inu_prologue(inu_ctx* c) {
   hipLaunchKernelGGL(k_block_mark, dim3(gb), dim3(256), 0, c->stream, a, act ? 0 : 1);
   hipLaunchKernelGGL(k_dilate, dim3(gb), dim3(256), 0, c->stream, a, 1, act ? 0 : 1);
   const uint32_t gp = std::min<uint32_t>(c->NB, c->grid_stage);
-  hipLaunchKernelGGL(k_prologue_cfl, dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
+  if (c->flags & INU_FLAG_EXACT_ORDER)
+    hipLaunchKernelGGL((k_prologue_cfl<false, false>), dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
+  else if (c->flags & INU_FLAG_COMPENSATED)
+    hipLaunchKernelGGL((k_prologue_cfl<true, true>), dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_prologue_cfl<true, false>), dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
   hipLaunchKernelGGL(k_step_control, dim3(1), dim3(1), 0, c->stream, a, 0);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1288,14 +1461,14 @@ int inu_upload(inu_ctx* c, int which, const void* in) {
   return upload(c, static_cast<float*>(dst), static_cast<const float*>(in), c->N);
 }
 
-int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out5, float g, float dry_tol) {
+int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out7, float g, float dry_tol) {
   if (!c) return INU_EINVAL;
-  if (!pairs || !out5) return fail(c, INU_EINVAL, "null array");
+  if (!pairs || !out7) return fail(c, INU_EINVAL, "null array");
   if (n == 0) return 0;
   HIPCHK(c, hipSetDevice(c->device));
   float *din = nullptr, *dout = nullptr;
   HIPCHK(c, hipMalloc(&din, (size_t)n * 8));
-  hipError_t e = hipMalloc(&dout, (size_t)n * 20);
+  hipError_t e = hipMalloc(&dout, (size_t)n * 4 * kProbeOut);
   if (e != hipSuccess) {
     (void)hipFree(din);
     return fail(c, INU_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -1305,7 +1478,7 @@ int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out5, float g, 
     hipLaunchKernelGGL(k_probe, dim3((n + 255) / 256), dim3(256), 0, c->stream, din, dout, n, make_phys(g, dry_tol, 0.8f));
     e = hipStreamSynchronize(c->stream);
   }
-  if (e == hipSuccess) e = hipMemcpy(out5, dout, (size_t)n * 20, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(out7, dout, (size_t)n * 4 * kProbeOut, hipMemcpyDeviceToHost);
   (void)hipFree(din);
   (void)hipFree(dout);
   if (e != hipSuccess) return fail(c, INU_EHIP, std::string("probe: ") + hipGetErrorString(e));

@@ -25,6 +25,7 @@ FLAG_DETERMINISTIC = 1
 FLAG_COMPENSATED = 2
 FLAG_ACTIVE_SET = 4
 FLAG_TRACK_CLAMP = 8
+FLAG_EXACT_ORDER = 16
 
 FIELD_H = 0
 FIELD_HU = 1
@@ -285,6 +286,6 @@ class Context:
 
     def probe(self, pairs: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:
         p = np.ascontiguousarray(pairs, dtype=np.float32).reshape(-1, 2)
-        out = np.empty((p.shape[0], 5), dtype=np.float32)
+        out = np.empty((p.shape[0], 7), dtype=np.float32)
         self.call("inu_probe", p.shape[0], ptr(p), ptr(out), g, dry_tol)
         return out

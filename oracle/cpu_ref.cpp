@@ -19,6 +19,11 @@
 //   record order             inundation/vulkan/gpu_resident_batch.py:489-513, 567-599
 //   HLLC, reconstruction, film drag, two-sum   inundation/vulkan/shaders/flux.py:43-185, 301-320
 //
+// Two arithmetic orders, as in the backend (flag 16 selects the first): the
+// reference's literal order (one IEEE division or sqrt per occurrence, no
+// fused multiply-add), and the backend's default fast order (namespace fastm:
+// shared reciprocals, explicit fma), algebraically the same scheme.
+//
 // One statement differs from the reference and is a choice, not a port: the
 // source kernel's dt bound reads each neighbour's depth after *every* source
 // of the step has been applied (the reference reads it while other source
@@ -210,6 +215,153 @@ inline float cbrt_ieee(float x) {
 }
 inline float pow43(float h) { return h * cbrt_ieee(h); }
 
+// ---- the backend's fast arithmetic, restated -------------------------------
+// Same scheme, algebraically unchanged: one division per side for the
+// desingularised velocities, c = sqrt(g) sqrt(h), h * (1 / dry_tol), one star
+// state chosen before it is formed, explicit fused multiply-adds, and
+// h^(-4/3) from a division-free inverse cube root.
+namespace fastm {
+
+struct K {
+  float g, dry, cfl, e, e2, half_g, sqrt_g_e, inv_dry, sqrt_g;
+};
+
+inline K consts(const Pc& pc) {
+  K k;
+  k.g = pc.g;
+  k.dry = pc.dry;
+  k.cfl = pc.cfl;
+  k.e = 4.0f * pc.dry;
+  k.e2 = k.e * k.e;
+  k.half_g = 0.5f * pc.g;
+  k.sqrt_g_e = std::sqrt(pc.g / k.e);
+  k.inv_dry = 1.0f / pc.dry;
+  k.sqrt_g = std::sqrt(pc.g);
+  return k;
+}
+
+inline float vel_factor(const K& k, float h) {
+  if (!(h > k.dry)) return 0.0f;
+  float hh = h * h;
+  return (2.0f * h) / (hh + gmax(hh, k.e2));
+}
+
+inline void speeds(const K& k, float hL, float unL, float hR, float unR, float& SL, float& SR) {
+  float sqL = std::sqrt(gmax(hL, 0.0f));
+  float sqR = std::sqrt(gmax(hR, 0.0f));
+  float cL = k.sqrt_g * sqL;
+  float cR = k.sqrt_g * sqR;
+  float den = sqL + sqR;
+  float u_roe = (den > 1e-12f) ? std::fma(sqL, unL, sqR * unR) / den : 0.0f;
+  float c_roe = std::sqrt(k.half_g * (hL + hR));
+  float sl = gmin(unL - cL, u_roe - c_roe);
+  float sr = gmax(unR + cR, u_roe + c_roe);
+  float wetL = gclamp(std::fma(hL, k.inv_dry, -1.0f), 0.0f, 1.0f);
+  float wetR = gclamp(std::fma(hR, k.inv_dry, -1.0f), 0.0f, 1.0f);
+  float dryLeft = (1.0f - wetL) * wetR;
+  float dryRight = (1.0f - wetR) * wetL;
+  float roe = 1.0f - dryLeft - dryRight;
+  SL = std::fma(roe, sl, std::fma(dryLeft, unR - 2.0f * cR, dryRight * (unL - cL)));
+  SR = std::fma(roe, sr, std::fma(dryLeft, unR + cR, dryRight * (unL + 2.0f * cL)));
+}
+
+inline void faces(const K& k, float hL, float huL, float hvL, float dzL, float hR, float huR, float hvR,
+                  float dzR, float nx, float ny, float loL, float loR, float& hLs, float& qnL, float& qtL,
+                  float& hRs, float& qnR, float& qtR) {
+  // At most one side steps up to the face. Its face depth is the other side's
+  // depth plus the surface difference, formed from the full compensated
+  // depths (h + lo), so level water gives both sides the same bits.
+  hLs = (dzL == 0.0f) ? hL : gmax(0.0f, hR + (((hL - hR) - dzL) + (loL - loR)));
+  hRs = (dzR == 0.0f) ? hR : gmax(0.0f, hL + (((hR - hL) - dzR) + (loR - loL)));
+  float rL = vel_factor(k, hL) * hLs;
+  float rR = vel_factor(k, hR) * hRs;
+  float huLs = huL * rL, hvLs = hvL * rL, huRs = huR * rR, hvRs = hvR * rR;
+  qnL = std::fma(huLs, nx, hvLs * ny);
+  qtL = std::fma(hvLs, nx, -(huLs * ny));
+  qnR = std::fma(huRs, nx, hvRs * ny);
+  qtR = std::fma(hvRs, nx, -(huRs * ny));
+}
+
+inline void hllc(const K& k, float hL, float unL, float utL, float hR, float unR, float utR, float& Fh,
+                 float& Fhn, float& Fht, float& Smax) {
+  float SL, SR;
+  speeds(k, hL, unL, hR, unR, SL, SR);
+  Smax = gmax(std::fabs(SL), std::fabs(SR));
+  // Toro's contact speed with the numerator regrouped as
+  // SL SR (hL - hR) + (SL hR unR - SR hL unL): exactly zero for still water.
+  float num_sm = std::fma(SL * SR, hL - hR, std::fma(SL * hR, unR, -(SR * hL * unL)));
+  float den_sm = std::fma(hR, unR - SR, -(hL * (unL - SL)));
+  float SM = (std::fabs(den_sm) > 1e-12f) ? num_sm / den_sm : 0.0f;
+  float FL_h = hL * unL, FL_hn = std::fma(FL_h, unL, k.half_g * hL * hL), FL_ht = FL_h * utL;
+  float FR_h = hR * unR, FR_hn = std::fma(FR_h, unR, k.half_g * hR * hR), FR_ht = FR_h * utR;
+  bool left = SM >= 0.0f;
+  float S = left ? SL : SR, hS = left ? hL : hR, unS = left ? unL : unR, utS = left ? utL : utR;
+  float FS_h = left ? FL_h : FR_h, FS_hn = left ? FL_hn : FR_hn, FS_ht = left ? FL_ht : FR_ht;
+  // Star depth minus the side depth, hS (SM - unS) / (S - SM): zero when the
+  // contact does not move relative to the side.
+  float dS = S - SM;
+  float ksh = (std::fabs(dS) > 1e-14f) ? hS * (SM - unS) / dS : hS * (S - unS) / 1e-14f - hS;
+  float kS = hS + ksh;
+  if (SL >= 0.0f) { Fh = FL_h; Fhn = FL_hn; Fht = FL_ht; }
+  else if (!left && !(SR >= 0.0f)) { Fh = FR_h; Fhn = FR_hn; Fht = FR_ht; }
+  else {
+    Fh = std::fma(S, ksh, FS_h);
+    Fhn = std::fma(S, std::fma(kS, SM, -(hS * unS)), FS_hn);
+    Fht = std::fma(S, ksh * utS, FS_ht);
+  }
+}
+
+inline void wall(float huL, float hvL, float nx, float ny, float& huR, float& hvR) {
+  float q2 = 2.0f * std::fma(huL, nx, hvL * ny);
+  huR = std::fma(-q2, nx, huL);
+  hvR = std::fma(-q2, ny, hvL);
+}
+
+inline float wave_bound(const K& k, float hL, float huL, float hvL, float dzL, float hR, float huR,
+                        float hvR, float dzR, float nx, float ny, float loL, float loR) {
+  float hLs, qnL, qtL, hRs, qnR, qtR;
+  faces(k, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx, ny, loL, loR, hLs, qnL, qtL, hRs, qnR, qtR);
+  float SL, SR;
+  speeds(k, hLs, qnL * vel_factor(k, hLs), hRs, qnR * vel_factor(k, hRs), SL, SR);
+  return gmax(std::fabs(SL), std::fabs(SR));
+}
+
+inline float film(const K& k, float h, float dt) {
+  if (h >= k.e) return 1.0f;
+  float hh = h * h;
+  float rate = (k.e2 - hh) / (2.0f * hh) * k.sqrt_g_e;
+  return 1.0f / std::fma(dt, rate, 1.0f);
+}
+
+inline float rcbrt(float x) {
+  uint32_t u = fbits(x);
+  int e = (int)((u >> 23) & 0xffu) - 127;
+  int kk = (e >= 0) ? e / 3 : -((2 - e) / 3);
+  int r = e - 3 * kk;
+  float m = bitsf((u & 0x7fffffu) | ((uint32_t)(127 + r) << 23));
+  float y = std::fma(std::fma(std::fma(-0.00191212f, m, 0.03603631f), m, -0.25165805f), m, 1.17793f);
+  float m3 = m * 0.33333334f;
+  for (int it = 0; it < 3; ++it) y = std::fma(-m3, y * y * y, 1.3333334f) * y;
+  return y * bitsf((uint32_t)(127 - kk) << 23);
+}
+
+inline void friction(const K& k, float h, float n, float dt, float& hu, float& hv) {
+  float drag = film(k, h, dt);
+  hu *= drag;
+  hv *= drag;
+  float ih = 1.0f / h;
+  float u = hu * ih, v = hv * ih;
+  float spd = std::sqrt(std::fma(u, u, v * v));
+  float r = rcbrt(h);
+  float r2 = r * r;
+  float Sf = k.g * n * n * spd * (r2 * r2);
+  float id = 1.0f / std::fma(dt, Sf, 1.0f);
+  hu *= id;
+  hv *= id;
+}
+
+}  // namespace fastm
+
 }  // namespace
 
 struct cref {
@@ -226,12 +378,18 @@ struct cref {
   uint32_t dtbuf[4] = {0, 0, 0, 0};
   Ctrl ctrl{};
   Pc pc{9.81f, 1e-4f, 0.8f};
-  bool compensated = true, active_set = true, track_clamp = false;
+  bool compensated = true, active_set = true, track_clamp = false, fast = true;
 };
 
 namespace {
 
 inline bool block_active(const cref& r, uint32_t b) { return r.blk[3 * b + 1] != 0u; }
+
+// Low word of the stage input depth (compensated runs; zero otherwise).
+inline float load_lo(const cref& r, int stage, int c) {
+  if (!r.compensated) return 0.0f;
+  return (stage == 1 && block_active(r, (uint32_t)c >> 8)) ? r.h1lo[c] : r.hlo[c];
+}
 
 inline void load_state(const cref& r, int stage, int c, float& h_, float& hu_, float& hv_) {
   if (stage == 1 && block_active(r, (uint32_t)c >> 8)) {
@@ -244,6 +402,7 @@ inline void load_state(const cref& r, int stage, int c, float& h_, float& hu_, f
 void flux(cref& r, int stage) {
   if (r.dtbuf[0] == 0u) return;
   const Pc pc = r.pc;
+  const fastm::K kk = fastm::consts(pc);
 #pragma omp parallel for schedule(static)
   for (int64_t wg = 0; wg < (int64_t)r.NWG; ++wg) {
     bool live = false;
@@ -256,15 +415,46 @@ void flux(cref& r, int stage) {
       float nx_ = r.enx[eid], ny_ = r.eny[eid], len = r.elen[eid];
       float hL, huL, hvL, hR, huR, hvR;
       load_state(r, stage, cL, hL, huL, hvL);
+      const float loL = load_lo(r, stage, cL);
+      const float loR = (cR >= 0) ? load_lo(r, stage, cR) : loL;
       if (cR == -1) {
         hR = hL;
-        huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
-        hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
+        if (r.fast) {
+          fastm::wall(huL, hvL, nx_, ny_, huR, hvR);
+        } else {
+          huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
+          hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
+        }
       } else {
         load_state(r, stage, cR, hR, huR, hvR);
       }
       if (hL < pc.dry && hR < pc.dry) continue;
       float dzL = r.edz[2 * eid], dzR = r.edz[2 * eid + 1];
+      if (r.fast) {
+        const fastm::K& k = kk;
+        float hLs, qnL, qtL, hRs, qnR, qtR;
+        fastm::faces(k, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx_, ny_, loL, loR, hLs, qnL, qtL, hRs, qnR, qtR);
+        float rL = fastm::vel_factor(k, hLs), rR = fastm::vel_factor(k, hRs);
+        float Fh, Fhn, Fht, Smax;
+        fastm::hllc(k, hLs, qnL * rL, qtL * rL, hRs, qnR * rR, qtR * rR, Fh, Fhn, Fht, Smax);
+        if (cR < 0) Fh = 0.0f;
+        // Normal momentum flux plus the hydrostatic correction, grouped as
+        // (Fhn - P(face depth)) + P(cell depth): still water gives exactly the
+        // cell's own pressure on every edge.
+        float nL = (Fhn - k.half_g * hLs * hLs) + k.half_g * hL * hL;
+        float nR = (Fhn - k.half_g * hRs * hRs) + k.half_g * hR * hR;
+        float tx = Fht * nx_, ty = Fht * ny_;
+        __atomic_store_n(&r.touched[cL], (uint8_t)1, __ATOMIC_RELAXED);
+        if (cR >= 0) __atomic_store_n(&r.touched[cR], (uint8_t)1, __ATOMIC_RELAXED);
+        float* o = &r.ef[(size_t)kWords * eid];
+        o[0] = Fh * len;
+        o[1] = std::fma(nL, nx_, -ty) * len;
+        o[2] = std::fma(nL, ny_, tx) * len;
+        o[3] = std::fma(nR, nx_, -ty) * len;
+        o[4] = std::fma(nR, ny_, tx) * len;
+        if (stage == 1) o[kCfl] = Smax * len;
+        continue;
+      }
       float hLs, hqnL, hqtL, hRs, hqnR, hqtR;
       face_states(pc, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx_, ny_, hLs, hqnL, hqtL, hRs, hqnR, hqtR);
       float Fh, Fhn, Fht, Smax;
@@ -290,6 +480,7 @@ void flux(cref& r, int stage) {
 void update(cref& r, int stage) {
   if (r.dtbuf[0] == 0u) return;
   const Pc pc = r.pc;
+  const fastm::K kk = fastm::consts(pc);
   const float dt = bitsf(r.dtbuf[0]);
   if (stage == 0)
     for (uint32_t b = 0; b < r.NB; ++b)
@@ -338,20 +529,22 @@ void update(cref& r, int stage) {
       two_sum(hi, -dti * sdh, s_raw, e_raw);
       h_raw = s_raw + (e_raw + loi);
     } else {
-      h_raw = hi - dti * sdh;
+      h_raw = r.fast ? std::fma(-dti, sdh, hi) : hi - dti * sdh;
       h_new = gmax(h_raw, 0.0f);
       lo_new = 0.0f;
     }
-    float hu_new = hui - dti * sdhu;
-    float hv_new = hvi - dti * sdhv;
+    float hu_new = r.fast ? std::fma(-dti, sdhu, hui) : hui - dti * sdhu;
+    float hv_new = r.fast ? std::fma(-dti, sdhv, hvi) : hvi - dti * sdhv;
     if (r.track_clamp) r.clamp[i] += 0.5f * A * (h_new - h_raw);
     if (h_new < pc.dry) { hu_new = 0.0f; hv_new = 0.0f; }
-    if (h_new >= pc.dry) {
+    if (r.fast && h_new >= pc.dry) {
+      fastm::friction(kk, h_new, r.nman[i], dt, hu_new, hv_new);
+    } else if (h_new >= pc.dry) {
       float drag = film_drag(pc, h_new, dt);
       hu_new *= drag;
       hv_new *= drag;
     }
-    if (h_new >= pc.dry) {
+    if (!r.fast && h_new >= pc.dry) {
       float n = r.nman[i];
       float u_ = hu_new / h_new;
       float v_ = hv_new / h_new;
@@ -427,15 +620,19 @@ void source(cref& r) {
     uint32_t i = r.src_cell[k];
     float h_new = r.h[i];
     if (!(h_new >= pc.dry)) continue;
-    float u = std::sqrt(r.hu[i] * r.hu[i] + r.hv[i] * r.hv[i]) / h_new;
-    float c = std::sqrt(pc.g * h_new);
+    const bool F = r.fast;
+    float u = F ? std::sqrt(std::fma(r.hu[i], r.hu[i], r.hv[i] * r.hv[i])) / h_new
+                : std::sqrt(r.hu[i] * r.hu[i] + r.hv[i] * r.hv[i]) / h_new;
+    float c = F ? std::sqrt(pc.g) * std::sqrt(h_new) : std::sqrt(pc.g * h_new);
     float lam = 0.0f;
     for (uint32_t kk = r.cptr[i]; kk < r.cptr[i + 1]; ++kk) {
       uint32_t s = r.cidx[kk];
       uint32_t e = s >> 1;
       int nb = ((s & 1u) != 0u) ? r.ecL[e] : r.ecR[e];
       bool dry_side = nb >= 0 && r.h[nb] < pc.dry;
-      lam += (u + (dry_side ? 2.0f : 1.0f) * c) * r.elen[e];
+      float kf = dry_side ? 2.0f : 1.0f;
+      if (F) lam = std::fma(std::fma(kf, c, u), r.elen[e], lam);
+      else lam += (u + kf * c) * r.elen[e];
     }
     if (lam > 1e-30f && r.area[i] > 1e-10f) best = std::min(best, fbits(pc.cfl * r.area[i] / lam));
   }
@@ -470,6 +667,7 @@ void dilate(cref& r, bool force) {
 
 void cfl_prologue(cref& r) {
   const Pc pc = r.pc;
+  const fastm::K kk = fastm::consts(pc);
 #pragma omp parallel for schedule(static)
   for (int64_t ee = 0; ee < (int64_t)r.E; ++ee) {
     uint32_t eid = (uint32_t)ee;
@@ -479,14 +677,22 @@ void cfl_prologue(cref& r) {
     float hR, huR, hvR;
     if (cR == -1) {
       hR = hL;
-      huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
-      hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
+      if (r.fast) {
+        fastm::wall(huL, hvL, nx_, ny_, huR, hvR);
+      } else {
+        huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
+        hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
+      }
     } else {
       hR = r.h[cR]; huR = r.hu[cR]; hvR = r.hv[cR];
     }
     float contrib = 0.0f;
     if (hL >= pc.dry || hR >= pc.dry)
-      contrib = wave_bound(pc, hL, huL, hvL, r.edz[2 * eid], hR, huR, hvR, r.edz[2 * eid + 1], nx_, ny_) * r.elen[eid];
+      contrib = (r.fast ? fastm::wave_bound(kk, hL, huL, hvL, r.edz[2 * eid], hR, huR, hvR,
+                                            r.edz[2 * eid + 1], nx_, ny_, load_lo(r, 0, cL),
+                                            cR >= 0 ? load_lo(r, 0, cR) : load_lo(r, 0, cL))
+                        : wave_bound(pc, hL, huL, hvL, r.edz[2 * eid], hR, huR, hvR, r.edz[2 * eid + 1], nx_, ny_)) *
+                r.elen[eid];
     r.ef[(size_t)kWords * eid + kCfl] = contrib;
   }
   uint32_t best = 0xffffffffu;
@@ -590,6 +796,7 @@ void cref_set_params(cref* r, float g, float dry, float cfl, uint32_t flags) {
   r->compensated = (flags & 2u) != 0;
   r->active_set = (flags & 4u) != 0;
   r->track_clamp = (flags & 8u) != 0;
+  r->fast = (flags & 16u) == 0;
 }
 
 void cref_set_state(cref* r, const float* h, const float* hu, const float* hv, const float* hlo) {
@@ -649,17 +856,21 @@ void cref_download(const cref* r, int which, void* out) {
 }
 
 // Same probe as the backend's inu_probe.
-void cref_probe(uint32_t n, const float* pairs, float* out5, float g, float dry) {
+void cref_probe(uint32_t n, const float* pairs, float* out7, float g, float dry) {
   Pc pc{g, dry, 0.8f};
-  for (uint32_t k = 0; k < n; ++k) {
-    float x = pairs[2 * k], y = pairs[2 * k + 1];
+  const fastm::K k = fastm::consts(pc);
+  for (uint32_t j = 0; j < n; ++j) {
+    float x = pairs[2 * j], y = pairs[2 * j + 1];
     float s, e;
     two_sum(x, y, s, e);
-    out5[5 * k + 0] = e;
-    out5[5 * k + 1] = x / y;
-    out5[5 * k + 2] = std::sqrt(std::fabs(x));
-    out5[5 * k + 3] = (x > 0.0f) ? pow43(x) : 0.0f;
-    out5[5 * k + 4] = (x > 0.0f) ? film_drag(pc, x, y) : 0.0f;
+    float* o = out7 + 7 * (size_t)j;
+    o[0] = e;
+    o[1] = x / y;
+    o[2] = std::sqrt(std::fabs(x));
+    o[3] = (x > 0.0f) ? pow43(x) : 0.0f;
+    o[4] = (x > 0.0f) ? film_drag(pc, x, y) : 0.0f;
+    o[5] = (x > 0.0f) ? fastm::rcbrt(x) : 0.0f;
+    o[6] = (x > 0.0f) ? fastm::film(k, x, y) : 0.0f;
   }
 }
 

@@ -79,7 +79,7 @@ def max_threads() -> int:
 
 def probe(pairs: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:
     p = np.ascontiguousarray(pairs, dtype=np.float32).reshape(-1, 2)
-    out = np.empty((p.shape[0], 5), dtype=np.float32)
+    out = np.empty((p.shape[0], 7), dtype=np.float32)
     load().cref_probe(p.shape[0], _p(p), _p(out), g, dry_tol)
     return out
 

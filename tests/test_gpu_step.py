@@ -67,7 +67,7 @@ def test_probe_is_bitwise_the_cpu(ctx) -> None:  # noqa: ANN001
     pairs = np.column_stack([a, b]).astype(np.float32)
     got = ctx.probe(pairs, 9.81, 1e-3)
     want = cpu_ref.probe(pairs, 9.81, 1e-3)
-    for k, name in enumerate(["two_sum err", "div", "sqrt", "pow43", "film"]):
+    for k, name in enumerate(["two_sum err", "div", "sqrt", "pow43", "film", "rcbrt", "fast film"]):
         both_nan = np.isnan(got[:, k]) & np.isnan(want[:, k])
         d = np.flatnonzero((got[:, k].view(np.uint32) != want[:, k].view(np.uint32)) & ~both_nan)
         assert d.size == 0, f"{name}: {d.size} differ, e.g. {pairs[d[:3]].tolist()} {got[d[:3], k]} {want[d[:3], k]}"
@@ -76,10 +76,18 @@ def test_probe_is_bitwise_the_cpu(ctx) -> None:  # noqa: ANN001
     assert e == np.float32(1e-9)
 
 
+EXACT = L.FLAG_EXACT_ORDER
+FLAG_SETS = {
+    "comp+active": D.FLAGS_DEFAULT,
+    "plain+active": L.FLAG_DETERMINISTIC | L.FLAG_ACTIVE_SET,
+    "comp+all": L.FLAG_DETERMINISTIC | L.FLAG_COMPENSATED,
+    "exact+comp+active": D.FLAGS_DEFAULT | EXACT,
+    "exact+plain+all": L.FLAG_DETERMINISTIC | EXACT,
+}
+
+
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("flags", [D.FLAGS_DEFAULT, L.FLAG_DETERMINISTIC | L.FLAG_ACTIVE_SET,
-                                   L.FLAG_DETERMINISTIC | L.FLAG_COMPENSATED],
-                         ids=["comp+active", "plain+active", "comp+all"])
+@pytest.mark.parametrize("flags", list(FLAG_SETS.values()), ids=list(FLAG_SETS))
 def test_fixed_dt_steps_are_bitwise_the_fp32_oracle(ctx, mesh, grid, case, flags) -> None:  # noqa: ANN001
     gpu, ref = _both(ctx, mesh)
     h, hu, hv = f32_state(*CASES[case](grid))
@@ -99,10 +107,11 @@ def test_fixed_dt_steps_are_bitwise_the_fp32_oracle(ctx, mesh, grid, case, flags
 
 
 @pytest.mark.parametrize("case", list(CASES))
-def test_fixed_dt_steps_match_fp64_within_reference_tolerance(ctx, mesh, grid, case) -> None:  # noqa: ANN001
+@pytest.mark.parametrize("order", [0, EXACT], ids=["fast", "exact"])
+def test_fixed_dt_steps_match_fp64_within_reference_tolerance(ctx, mesh, grid, case, order) -> None:  # noqa: ANN001
     gpu, _ = _both(ctx, mesh)
     h, hu, hv = f32_state(*CASES[case](grid))
-    D.setup(gpu, grid.N, h, hu, hv, MANNING, dry_tol=DRY_TOL)
+    D.setup(gpu, grid.N, h, hu, hv, MANNING, dry_tol=DRY_TOL, flags=D.FLAGS_DEFAULT | order)
     D.start(gpu, 8 * DT, DT, DT)
     gpu.steps(8, use_graph=True)
     got = D.snapshot(gpu, grid.N, num_blocks(grid.N, 256))
@@ -115,13 +124,14 @@ def test_fixed_dt_steps_match_fp64_within_reference_tolerance(ctx, mesh, grid, c
 
 
 @pytest.mark.parametrize("case", ["dam_break", "draining"])
-def test_cfl_driven_run_is_bitwise_the_fp32_oracle(ctx, mesh, grid, case) -> None:  # noqa: ANN001
+@pytest.mark.parametrize("order", [0, EXACT], ids=["fast", "exact"])
+def test_cfl_driven_run_is_bitwise_the_fp32_oracle(ctx, mesh, grid, case, order) -> None:  # noqa: ANN001
     """Every step picks its own dt on device; windows retarget the clock."""
     gpu, ref = _both(ctx, mesh)
     h, hu, hv = f32_state(*CASES[case](grid))
     nb = num_blocks(grid.N, 256)
     for be in (gpu, ref):
-        D.setup(be, grid.N, h, hu, hv, MANNING, dry_tol=DRY_TOL)
+        D.setup(be, grid.N, h, hu, hv, MANNING, dry_tol=DRY_TOL, flags=D.FLAGS_DEFAULT | order)
         D.start(be, 2.0, 1.0, 0.01)
     for k in range(6):
         gpu.steps(25, use_graph=True)
@@ -138,7 +148,8 @@ def test_cfl_driven_run_is_bitwise_the_fp32_oracle(ctx, mesh, grid, case) -> Non
     D.assert_same(D.snapshot(gpu, grid.N, nb), D.snapshot(ref, grid.N, nb), "after retarget")
 
 
-def test_sources_are_bitwise_the_fp32_oracle(ctx, mesh, grid) -> None:  # noqa: ANN001
+@pytest.mark.parametrize("order", [0, EXACT], ids=["fast", "exact"])
+def test_sources_are_bitwise_the_fp32_oracle(ctx, mesh, grid, order) -> None:  # noqa: ANN001
     """A point source into a dry cell and rain on a patch of cells."""
     gpu, ref = _both(ctx, mesh)
     n = grid.N
@@ -150,7 +161,8 @@ def test_sources_are_bitwise_the_fp32_oracle(ctx, mesh, grid) -> None:  # noqa: 
     cells = np.union1d(patch, [point]).astype(np.uint32)
     rates = np.where(cells == point, 0.5 / grid.area[point], 1.4e-5).astype(np.float32)
     for be in (gpu, ref):
-        D.setup(be, n, z, z, z, MANNING, dry_tol=DRY_TOL, src_cells=cells, src_rates=rates)
+        D.setup(be, n, z, z, z, MANNING, dry_tol=DRY_TOL, src_cells=cells, src_rates=rates,
+                flags=D.FLAGS_DEFAULT | order)
         D.start(be, 4.0, 0.5, 0.05)
     for k in range(4):
         gpu.steps(40, use_graph=True)
