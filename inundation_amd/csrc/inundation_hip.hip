@@ -97,7 +97,8 @@ struct Args {
   // sources
   const uint32_t* src_cell;
   const uint32_t* src_blocks;
-  uint32_t n_src_blocks;
+  const uint32_t* src_bnd;  // source cells with a neighbour in another block
+  uint32_t n_src_blocks, n_src_bnd;
   Ctl* ctl;
 };
 
@@ -127,7 +128,8 @@ __device__ __forceinline__ void block_atomic_min(uint32_t* dst, uint32_t bits) {
   if (threadIdx.x == 0) {
     uint32_t v = wave_min[0];
     for (uint32_t k = 1; k < (blockDim.x + 63u) / 64u; ++k) v = (wave_min[k] < v) ? wave_min[k] : v;
-    if (v != kNoBound) atomicMin(dst, v);
+    // Most workgroups do not hold the minimum: read before the atomic.
+    if (v != kNoBound && v < __atomic_load_n(dst, __ATOMIC_RELAXED)) atomicMin(dst, v);
   }
 }
 
@@ -487,56 +489,78 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
 }
 
 // Sources of blocks the active set skipped this step (reference
-// resident.py:691-708): nothing else reads or writes those cells now.
-template <bool COMP>
+// resident.py:691-708): nothing else reads or writes those cells now. The
+// block's post-source depths then give the dt bound of its wetted source cells
+// whose neighbours all lie in the block (resident.py:709-728).
+template <bool COMP, bool FM>
 __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
+  __shared__ float s_h[kBlockCells];
   const uint32_t dtw = a.ctl->dtbuf[0];
   if (dtw == 0u) return;
   const float dt = __uint_as_float(dtw);
+  const Phys& p = a.p;
+  uint32_t bound = kNoBound;
   for (uint32_t t = blockIdx.x; t < a.n_src_blocks; t += gridDim.x) {
     const uint32_t b = a.src_blocks[t];
     if (a.active[b] != 0u) continue;  // applied by the corrector
-    const uint32_t i = (b << 8) + threadIdx.x;
+    const uint32_t c0 = b << 8;
+    const uint32_t i = c0 + threadIdx.x;
     bool wet = false;
+    float hn = 0.0f, rate = 0.0f;
     if (i < a.N) {
-      const float rate = a.src[i];
+      hn = a.h[i];
+      rate = a.src[i];
       if (rate != 0.0f) {
         const float inc = dt * rate;
-        float hn;
         if (COMP) {
           float ln;
-          inu::add_compensated(a.h[i], a.hlo[i], inc, hn, ln);
+          inu::add_compensated(hn, a.hlo[i], inc, hn, ln);
           a.hlo[i] = ln;
         } else {
-          hn = inu::gmax(a.h[i] + inc, 0.0f);
+          hn = inu::gmax(hn + inc, 0.0f);
         }
         a.h[i] = hn;
-        wet = hn >= a.p.dry;
+        wet = hn >= p.dry;
       }
     }
+    __syncthreads();  // the previous block's s_h reads are done
+    s_h[threadIdx.x] = hn;
     if (__syncthreads_or(wet ? 1 : 0) && threadIdx.x == 0) a.wet[b] = 1u;
+    if (wet && a.cell_inside[i] != 0u) {
+      const float hui = a.hu[i];
+      const float hvi = a.hv[i];
+      const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
+      const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
+      float lam = 0.0f;
+      const uint32_t s1 = a.cptr[i + 1];
+      for (uint32_t s = a.cptr[i]; s < s1; ++s) {
+        const int nb = a.snb[s];
+        const bool dry_side = nb >= 0 && s_h[(uint32_t)nb - c0] < p.dry;
+        const float k = dry_side ? 2.0f : 1.0f;
+        if (FM) lam = fmaf(fmaf(k, c, u), a.slen[s], lam);
+        else lam += (u + k * c) * a.slen[s];
+      }
+      const float A = a.area[i];
+      if (lam > 1e-30f && A > 1e-10f) {
+        const uint32_t bb = __float_as_uint(p.cfl * A / lam);
+        bound = (bb < bound) ? bb : bound;
+      }
+    }
   }
+  block_atomic_min(&a.ctl->dtbuf[1], bound);
 }
 
-// Next-step dt bound of each source cell from its post-source depth (reference
-// resident.py:709-728). Neighbour depths are read after every source of the
-// step has been applied.
-// Next-step dt bound of each wetted source cell from its post-source depth
-// (reference resident.py:709-728), for the sources the corrector did not
-// bound: every source of a block the active set skipped, and sources with a
-// neighbour in another block. Neighbour depths are read after every source of
-// the step has been applied.
+// Next-step dt bound of the wetted source cells with a neighbour in another
+// block (reference resident.py:709-728), one thread per such cell; the others
+// were bounded with their block. Neighbour depths are read after every source
+// of the step has been applied.
 template <bool FM>
 __global__ __launch_bounds__(256) void k_source_bound(const Args a) {
   if (a.ctl->dtbuf[0] == 0u) return;
   const Phys& p = a.p;
   uint32_t bound = kNoBound;
-  for (uint32_t t = blockIdx.x; t < a.n_src_blocks; t += gridDim.x) {
-    const uint32_t b = a.src_blocks[t];
-    const bool act = a.active[b] != 0u;
-    const uint32_t i = (b << 8) + threadIdx.x;
-    if (i >= a.N || a.src[i] == 0.0f) continue;
-    if (act && a.cell_inside[i] != 0u) continue;  // bounded by the corrector
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.n_src_bnd; k += gridDim.x * blockDim.x) {
+    const uint32_t i = a.src_bnd[k];
     const float hn = a.h[i];
     if (!(hn >= p.dry)) continue;
     const float hui = a.hu[i];
@@ -548,9 +572,9 @@ __global__ __launch_bounds__(256) void k_source_bound(const Args a) {
     for (uint32_t s = a.cptr[i]; s < s1; ++s) {
       const int nb = a.snb[s];
       const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
-      const float k = dry_side ? 2.0f : 1.0f;
-      if (FM) lam = fmaf(fmaf(k, c, u), a.slen[s], lam);
-      else lam += (u + k * c) * a.slen[s];
+      const float kf = dry_side ? 2.0f : 1.0f;
+      if (FM) lam = fmaf(fmaf(kf, c, u), a.slen[s], lam);
+      else lam += (u + kf * c) * a.slen[s];
     }
     const float A = a.area[i];
     if (lam > 1e-30f && A > 1e-10f) {
@@ -827,7 +851,7 @@ struct inu_ctx {
   Phys p{};
   uint32_t flags = INU_FLAG_COMPENSATED | INU_FLAG_ACTIVE_SET;
   // sources
-  uint32_t n_src = 0, n_src_blocks = 0;
+  uint32_t n_src = 0, n_src_blocks = 0, n_src_bnd = 0;
   // device memory
   std::vector<void*> allocs;
   Args a{};
@@ -835,6 +859,7 @@ struct inu_ctx {
   // launch shape
   uint32_t grid_stage = 0;
   size_t lds_stage0 = 0, lds_stage1 = 0;
+  std::vector<uint8_t> cell_inside;  // host copy, for the source lists
   // graphs of recorded step blocks, by step count
   std::map<uint32_t, hipGraphExec_t> graphs;
   std::vector<hipEvent_t> events;
@@ -927,14 +952,16 @@ int launch_step(inu_ctx* c, const hipEvent_t* ev) {
   launch_stage<1>(c, comp, fm);
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   if (c->n_src) {
-    const uint32_t gb = std::min<uint32_t>(c->n_src_blocks, 8u * (uint32_t)c->cus);
-    if (comp)
-      hipLaunchKernelGGL((k_source_idle<true>), dim3(std::max(gb, 1u)), dim3(256), 0, c->stream, a);
-    else
-      hipLaunchKernelGGL((k_source_idle<false>), dim3(std::max(gb, 1u)), dim3(256), 0, c->stream, a);
-    const uint32_t gs = std::max(1u, std::min<uint32_t>(c->n_src_blocks, 8u * (uint32_t)c->cus));
-    if (fm) hipLaunchKernelGGL((k_source_bound<true>), dim3(gs), dim3(256), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_source_bound<false>), dim3(gs), dim3(256), 0, c->stream, a);
+    const uint32_t gb = std::max(1u, std::min<uint32_t>(c->n_src_blocks, 8u * (uint32_t)c->cus));
+    if (comp && fm) hipLaunchKernelGGL((k_source_idle<true, true>), dim3(gb), dim3(256), 0, c->stream, a);
+    else if (comp) hipLaunchKernelGGL((k_source_idle<true, false>), dim3(gb), dim3(256), 0, c->stream, a);
+    else if (fm) hipLaunchKernelGGL((k_source_idle<false, true>), dim3(gb), dim3(256), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_source_idle<false, false>), dim3(gb), dim3(256), 0, c->stream, a);
+    if (c->n_src_bnd) {
+      const uint32_t gs = std::max(1u, std::min<uint32_t>((c->n_src_bnd + 255) / 256, 8u * (uint32_t)c->cus));
+      if (fm) hipLaunchKernelGGL((k_source_bound<true>), dim3(gs), dim3(256), 0, c->stream, a);
+      else hipLaunchKernelGGL((k_source_bound<false>), dim3(gs), dim3(256), 0, c->stream, a);
+    }
   }
   if (c->flags & INU_FLAG_ACTIVE_SET)
     hipLaunchKernelGGL(k_dilate, dim3((c->NB + 255) / 256), dim3(256), 0, c->stream, a, 0, 0);
@@ -1097,6 +1124,7 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   c->n_halo = hp[NB];
   c->n_src = 0;
   c->n_src_blocks = 0;
+  c->n_src_bnd = 0;
   Args& a = c->a;
   const size_t NP = (size_t)NB * kBlockCells;
   int rc = 0;
@@ -1115,13 +1143,14 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   rc |= dalloc(c, &a.wet, NB); rc |= dalloc(c, &a.active, NB); rc |= dalloc(c, &a.forced, NB);
   rc |= dalloc(c, &a.has_src, NB); rc |= dalloc(c, &a.list, NB);
   uint32_t* src_cell; uint32_t* src_blocks;
-  rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1);
+  uint32_t* src_bnd;
+  rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1); rc |= dalloc(c, &src_bnd, 1);
   rc |= dalloc(c, &c->ctl, 1);
   if (rc) return rc;
   a.cptr = cptr; a.lidx = lid; a.snb = sn; a.slen = sl; a.cell_inside = cin;
   a.ecell = ecell; a.enrm = enrm; a.elen = elen; a.edz = edz;
   a.binfo = dbinfo; a.hx = dhx; a.bptr = dbptr; a.bidx = dbidx;
-  a.src_cell = src_cell; a.src_blocks = src_blocks;
+  a.src_cell = src_cell; a.src_blocks = src_blocks; a.src_bnd = src_bnd; a.n_src_bnd = 0;
   a.ctl = c->ctl;
   a.N = N; a.NB = NB; a.max_local = max_local; a.max_slots = max_slots;
   // Host-side repack of the edge record into vector-loadable SoA.
@@ -1149,6 +1178,7 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   if ((rc = upload(c, sn, snb.data(), S))) return rc;
   if ((rc = upload(c, sl, slen.data(), S))) return rc;
   if ((rc = upload(c, cin, inside.data(), N))) return rc;
+  c->cell_inside.swap(inside);
   if ((rc = upload(c, ecell, h_ecell.data(), E))) return rc;
   if ((rc = upload(c, enrm, h_nrm.data(), E))) return rc;
   if ((rc = upload(c, elen, edge_len, E))) return rc;
@@ -1243,11 +1273,14 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
     has[cell[k] / kBlockCells] = 1;
   }
   std::sort(cells.begin(), cells.end());
+  std::vector<uint32_t> bnd;
+  for (uint32_t cl : cells)
+    if (!c->cell_inside[cl]) bnd.push_back(cl);
   std::vector<uint32_t> blocks;
   for (uint32_t b = 0; b < c->NB; ++b)
     if (has[b]) blocks.push_back(b);
   // Replace the source lists (their sizes change).
-  for (void* q : {(void*)c->a.src_cell, (void*)c->a.src_blocks}) {
+  for (void* q : {(void*)c->a.src_cell, (void*)c->a.src_blocks, (void*)c->a.src_bnd}) {
     auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
     if (it != c->allocs.end()) {
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1257,11 +1290,18 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   }
   c->a.src_cell = nullptr;
   c->a.src_blocks = nullptr;
-  uint32_t *dcell = nullptr, *dblk = nullptr;
+  c->a.src_bnd = nullptr;
+  uint32_t *dcell = nullptr, *dblk = nullptr, *dbnd = nullptr;
   int rc2 = dalloc(c, &dcell, cells.size());
   if (rc2) return rc2;
   rc2 = dalloc(c, &dblk, blocks.size());
   if (rc2) return rc2;
+  rc2 = dalloc(c, &dbnd, bnd.size());
+  if (rc2) return rc2;
+  if ((rc = upload(c, dbnd, bnd.data(), bnd.size()))) return rc;
+  c->a.src_bnd = dbnd;
+  c->n_src_bnd = (uint32_t)bnd.size();
+  c->a.n_src_bnd = c->n_src_bnd;
   if ((rc = upload(c, dcell, cells.data(), cells.size()))) return rc;
   if ((rc = upload(c, dblk, blocks.data(), blocks.size()))) return rc;
   if ((rc = upload(c, c->a.src, dense.data(), dense.size()))) return rc;
