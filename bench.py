@@ -70,6 +70,14 @@ def main() -> int:
     ap.add_argument("--scale", type=int, default=1, help="shrink the lattice by this factor (testing)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument(
+        "--copy-calibration",
+        type=int,
+        default=0,
+        metavar="MIB",
+        help="after the timed run, launch the library's plain copy kernel (k_stream_copy) over MIB MiB so "
+        "that HBM byte counters can be scaled against a kernel of known traffic",
+    )
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -132,6 +140,14 @@ def main() -> int:
         f"active {active_frac:.3f}; corrector {corr_gbs:.0f} GB/s; t_sim {ctrl.now_ticks / 2**20:.1f}s"
     )
 
+    # Attainable HBM rate: a plain copy of 2 GiB (read + write), for context.
+    copy_ms = ctx.stream_copy(2 << 30, 5)
+    copy_gbs = 2 * (2 << 30) / (copy_ms * 1e-3) / 1e9
+    log(f"plain copy of 2 GiB: {copy_ms:.3f} ms, {copy_gbs:.0f} GB/s read+write")
+    if args.copy_calibration:
+        ms = ctx.stream_copy(args.copy_calibration << 20, 3)
+        log(f"copy calibration: 4 launches of k_stream_copy over {args.copy_calibration} MiB, {ms:.3f} ms each")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ctx, wl, m, info, args.cpu_seconds)
@@ -172,6 +188,7 @@ def main() -> int:
             "frac": corr_gbs / HBM_PEAK_GBS,
             "traffic": None,
             "ms_per_launch": ms_corr / args.steps,
+            "attainable_copy_GBs": copy_gbs,
             "algorithmic_bytes_per_launch": corr_bytes / args.steps,
         },
         "heun_step_roofline": {

@@ -168,6 +168,14 @@ int inu_upload(inu_ctx* ctx, int which, const void* in);
  * solver.py:449-467). */
 int inu_probe(inu_ctx* ctx, uint32_t n, const float* pairs, float* out7, float g, float dry_tol);
 
+/* A plain device copy of n_bytes (a positive multiple of 16), launched
+ * `reps` times after one untimed launch; *ms_out = mean duration of one copy.
+ * Measurement support with no counterpart in the reference: a kernel of known
+ * traffic (n_bytes read, n_bytes written) that HBM byte counters are scaled
+ * against, and the attainable copy bandwidth. Allocates and frees its own
+ * buffers. */
+int inu_stream_copy(inu_ctx* ctx, uint64_t n_bytes, uint32_t reps, float* ms_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,7 +56,8 @@ struct Ctl {
   uint32_t cnt_cur;   // blocks in `list` for the current step
   uint32_t cnt_next;  // blocks appended by k_dilate for the next step
   uint32_t swap_on;   // 1 when k_dilate runs every step (active set on)
-  uint32_t pad;
+  uint32_t idle_cur;  // inactive source blocks in `idle` for the current step
+  uint32_t idle_next; // inactive source blocks appended for the next step
   // Work of the active set, for accounting: {cells, owned edges, CSR slots}
   // of the active blocks of the next step, of the current step, and summed
   // over the real steps taken.
@@ -83,7 +84,6 @@ struct Args {
   const uint16_t* lidx;
   const int32_t* snb;
   const float* slen;
-  const uint8_t* cell_inside;  // 1 when every neighbour of the cell is in its block or a wall
   // edges (sorted by the block of the left cell)
   const int2* ecell;
   const float2* enrm;
@@ -92,13 +92,13 @@ struct Args {
   const BlockInfo* binfo;
   const uint32_t* hx;
   // blocks
-  uint32_t *wet, *active, *forced, *has_src, *list;
+  uint32_t *wet, *active, *forced, *has_src, *list, *idle;
   const uint32_t *bptr, *bidx;
   // sources
   const uint32_t* src_cell;
   const uint32_t* src_blocks;
-  const uint32_t* src_bnd;  // source cells with a neighbour in another block
-  uint32_t n_src_blocks, n_src_bnd;
+  const uint2* src_pair;  // (source cell, its neighbour in another block)
+  uint32_t n_src_blocks, n_src_pairs;
   Ctl* ctl;
 };
 
@@ -429,31 +429,27 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& 
     const int any_wet = __syncthreads_or(wet ? 1 : 0);
     if (tid == 0) a.wet[b] = any_wet ? 1u : 0u;
     if (a.n_src != 0u) {
-      // Next-step dt bound of the block's wetted source cells whose
-      // neighbours all lie in the block (resident.py:709-728), from the
-      // post-source depths; the others are left to k_source_bound.
+      // Next-step dt bound of the block's wetted source cells
+      // (resident.py:709-728) from the post-source depths, with every
+      // neighbour in another block taken as wet; k_source_fix recomputes the
+      // cells where such a neighbour is dry.
       if (tid < nc) l.s_h[tid] = hi;
       __syncthreads();
       if (tid < nc && rate != 0.0f && hi >= p.dry) {
         const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hi : sqrtf(hui * hui + hvi * hvi) / hi;
         const float c = FM ? p.sqrt_g * sqrtf(hi) : sqrtf(p.g * hi);
         float lam = 0.0f;
-        bool inside = true;
         for (uint32_t k = k0; k < k1; ++k) {
           const uint32_t s = l.s_slot[k];
           const uint32_t j = s >> 1;
           const uint32_t pk = l.s_side[j];
           const uint32_t q = (s & 1u) ? (pk >> 16) : (pk & 0xffffu);
-          if (q == kSideOutside) {
-            inside = false;
-            break;
-          }
-          const bool dry_side = q != kSideWall && l.s_h[q] < p.dry;
+          const bool dry_side = q < kSideWall && l.s_h[q] < p.dry;  // walls and other blocks: wet
           const float kf = dry_side ? 2.0f : 1.0f;
           if (FM) lam = fmaf(fmaf(kf, c, u), l.s_len[j], lam);
           else lam += (u + kf * c) * l.s_len[j];
         }
-        if (inside && lam > 1e-30f && A > 1e-10f) {
+        if (lam > 1e-30f && A > 1e-10f) {
           const uint32_t bb = __float_as_uint(p.cfl * A / lam);
           bound = (bb < bound) ? bb : bound;
         }
@@ -481,7 +477,6 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
   uint32_t bound = kNoBound;  // this thread's smallest next-step bound
   for (uint32_t t = lo + blockIdx.x / kXcds; t < hi; t += per) {
     const uint32_t b = a.list[t];
-    if (a.active[b] == 0u) continue;  // listed only for its sources
     const uint32_t bb = stage_block<ST, COMP, FM>(a, b, dt, l);
     bound = (bb < bound) ? bb : bound;
   }
@@ -489,9 +484,10 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
 }
 
 // Sources of blocks the active set skipped this step (reference
-// resident.py:691-708): nothing else reads or writes those cells now. The
-// block's post-source depths then give the dt bound of its wetted source cells
-// whose neighbours all lie in the block (resident.py:709-728).
+// resident.py:691-708), listed by k_dilate in `idle`: nothing else reads or
+// writes those cells now. The block's post-source depths then give the dt
+// bound of its wetted source cells (resident.py:709-728), with every
+// neighbour in another block taken as wet, as in the corrector.
 template <bool COMP, bool FM>
 __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
   __shared__ float s_h[kBlockCells];
@@ -499,17 +495,17 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
   if (dtw == 0u) return;
   const float dt = __uint_as_float(dtw);
   const Phys& p = a.p;
+  const uint32_t cnt = a.ctl->idle_cur;
   uint32_t bound = kNoBound;
-  for (uint32_t t = blockIdx.x; t < a.n_src_blocks; t += gridDim.x) {
-    const uint32_t b = a.src_blocks[t];
-    if (a.active[b] != 0u) continue;  // applied by the corrector
+  for (uint32_t t = blockIdx.x; t < cnt; t += gridDim.x) {
+    const uint32_t b = a.idle[t];
     const uint32_t c0 = b << 8;
     const uint32_t i = c0 + threadIdx.x;
     bool wet = false;
-    float hn = 0.0f, rate = 0.0f;
+    float hn = 0.0f;
     if (i < a.N) {
       hn = a.h[i];
-      rate = a.src[i];
+      const float rate = a.src[i];
       if (rate != 0.0f) {
         const float inc = dt * rate;
         if (COMP) {
@@ -526,7 +522,7 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
     __syncthreads();  // the previous block's s_h reads are done
     s_h[threadIdx.x] = hn;
     if (__syncthreads_or(wet ? 1 : 0) && threadIdx.x == 0) a.wet[b] = 1u;
-    if (wet && a.cell_inside[i] != 0u) {
+    if (wet) {
       const float hui = a.hu[i];
       const float hvi = a.hv[i];
       const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
@@ -534,8 +530,8 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
       float lam = 0.0f;
       const uint32_t s1 = a.cptr[i + 1];
       for (uint32_t s = a.cptr[i]; s < s1; ++s) {
-        const int nb = a.snb[s];
-        const bool dry_side = nb >= 0 && s_h[(uint32_t)nb - c0] < p.dry;
+        const uint32_t q = (uint32_t)a.snb[s] - c0;  // walls and other blocks fall outside
+        const bool dry_side = q < kBlockCells && s_h[q] < p.dry;
         const float k = dry_side ? 2.0f : 1.0f;
         if (FM) lam = fmaf(fmaf(k, c, u), a.slen[s], lam);
         else lam += (u + k * c) * a.slen[s];
@@ -550,36 +546,40 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
   block_atomic_min(&a.ctl->dtbuf[1], bound);
 }
 
-// Next-step dt bound of the wetted source cells with a neighbour in another
-// block (reference resident.py:709-728), one thread per such cell; the others
-// were bounded with their block. Neighbour depths are read after every source
-// of the step has been applied.
+// Exact dt bound (reference resident.py:709-728) of the wetted source cells
+// with a neighbour in another block that is dry after every source of the
+// step: the corrector and k_source_idle counted that neighbour as wet. One
+// thread per (cell, neighbour) pair; the bound with the dry side is never
+// above the one taken before (every term and rounding is monotone), so the
+// minimum over both equals the exact one.
 template <bool FM>
-__global__ __launch_bounds__(256) void k_source_bound(const Args a) {
+__global__ __launch_bounds__(256) void k_source_fix(const Args a) {
   if (a.ctl->dtbuf[0] == 0u) return;
   const Phys& p = a.p;
   uint32_t bound = kNoBound;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.n_src_bnd; k += gridDim.x * blockDim.x) {
-    const uint32_t i = a.src_bnd[k];
-    const float hn = a.h[i];
-    if (!(hn >= p.dry)) continue;
-    const float hui = a.hu[i];
-    const float hvi = a.hv[i];
-    const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
-    const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
-    float lam = 0.0f;
-    const uint32_t s1 = a.cptr[i + 1];
-    for (uint32_t s = a.cptr[i]; s < s1; ++s) {
-      const int nb = a.snb[s];
-      const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
-      const float kf = dry_side ? 2.0f : 1.0f;
-      if (FM) lam = fmaf(fmaf(kf, c, u), a.slen[s], lam);
-      else lam += (u + kf * c) * a.slen[s];
-    }
-    const float A = a.area[i];
-    if (lam > 1e-30f && A > 1e-10f) {
-      const uint32_t bb = __float_as_uint(p.cfl * A / lam);
-      bound = (bb < bound) ? bb : bound;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < a.n_src_pairs) {
+    const uint2 pr = a.src_pair[k];
+    if (a.h[pr.y] < p.dry) {
+      const uint32_t i = pr.x;
+      const float hn = a.h[i];
+      if (hn >= p.dry) {
+        const float hui = a.hu[i];
+        const float hvi = a.hv[i];
+        const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
+        const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
+        float lam = 0.0f;
+        const uint32_t s1 = a.cptr[i + 1];
+        for (uint32_t s = a.cptr[i]; s < s1; ++s) {
+          const int nb = a.snb[s];
+          const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
+          const float kf = dry_side ? 2.0f : 1.0f;
+          if (FM) lam = fmaf(fmaf(kf, c, u), a.slen[s], lam);
+          else lam += (u + kf * c) * a.slen[s];
+        }
+        const float A = a.area[i];
+        if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
+      }
     }
   }
   block_atomic_min(&a.ctl->dtbuf[1], bound);
@@ -597,6 +597,7 @@ __global__ void k_block_mark(const Args a, int all_active) {
   if (b == 0) {
     a.ctl->dtbuf[0] = __float_as_uint(kDtSentinel);
     a.ctl->cnt_next = 0u;
+    a.ctl->idle_next = 0u;
     for (int k = 0; k < 3; ++k) a.ctl->act_next[k] = 0ull;
   }
   if (b >= a.NB) return;
@@ -617,13 +618,26 @@ __global__ void k_block_mark(const Args a, int all_active) {
   a.wet[b] = live;
 }
 
+// Appends `v` of every lane with `take` to `out`, one atomic per wave.
+__device__ __forceinline__ void wave_append(bool take, uint32_t v, uint32_t* cnt, uint32_t* out) {
+  const uint64_t mask = __ballot(take ? 1 : 0);
+  if (mask == 0ull) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
+  base = (uint32_t)__shfl((int)base, (int)leader, 64);
+  if (take) out[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = v;
+}
+
 // active = (wet | forced) over the block's two-ring (reference
-// resident.py:778-805); blocks to visit next step (active, or holding a
-// source) are appended with one atomic per wave, in block order within it.
+// resident.py:778-805). Active blocks are appended to `list`, inactive blocks
+// holding a source to `idle`, with one atomic per wave and list, in block
+// order within it.
 __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int keep_flags) {
   if (!force && a.ctl->dtbuf[0] == 0u) return;  // no-op step changed nothing
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  bool take = false;
+  bool take = false, take_idle = false;
   uint32_t w_cells = 0, w_edges = 0, w_slots = 0;
   if (b < a.NB) {
     uint32_t v;
@@ -637,7 +651,8 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
       }
       a.active[b] = v;
     }
-    take = (v != 0u) || (a.has_src[b] != 0u);
+    take = v != 0u;
+    take_idle = !take && a.has_src[b] != 0u;
     if (v != 0u) {
       const uint32_t c0 = b << 8;
       const uint32_t c1 = min(c0 + kBlockCells, a.N);
@@ -668,15 +683,8 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
       if (tot) atomicAdd(&a.ctl->act_next[threadIdx.x], tot);
     }
   }
-  const uint64_t mask = __ballot(take ? 1 : 0);
-  if (mask == 0ull) return;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(&a.ctl->cnt_next, (uint32_t)__popcll(mask));
-  base = (uint32_t)__shfl((int)base, (int)leader, 64);
-  if (take) base += (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-  if (take) a.list[base] = b;
+  wave_append(take, b, &a.ctl->cnt_next, a.list);
+  wave_append(take_idle, b, &a.ctl->idle_next, a.idle);
 }
 
 // Prologue CFL bound of the current state, every cell (reference
@@ -792,6 +800,8 @@ __global__ void k_step_control(const Args a, int mode) {
   if (publish) {
     t->cnt_cur = t->cnt_next;
     t->cnt_next = 0u;
+    t->idle_cur = t->idle_next;
+    t->idle_next = 0u;
     for (int k = 0; k < 3; ++k) {
       t->act_cur[k] = t->act_next[k];
       t->act_next[k] = 0ull;
@@ -812,6 +822,13 @@ __global__ void k_step_control(const Args a, int mode) {
   c.dt_ticks = ticks;
   t->dtbuf[0] = __float_as_uint((float)ticks / kTicksPerSecond);
   t->dtbuf[1] = __float_as_uint(kDtSentinel);
+}
+
+// Plain copy of n 16-byte words: the kernel of known traffic that HBM byte
+// counters are scaled against.
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ in, float4* __restrict__ out, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+    out[k] = in[k];
 }
 
 // Arithmetic probes: pairs (a, b) -> two-sum error, a / b, sqrt|a|, a^(4/3),
@@ -851,7 +868,7 @@ struct inu_ctx {
   Phys p{};
   uint32_t flags = INU_FLAG_COMPENSATED | INU_FLAG_ACTIVE_SET;
   // sources
-  uint32_t n_src = 0, n_src_blocks = 0, n_src_bnd = 0;
+  uint32_t n_src = 0, n_src_blocks = 0, n_src_pairs = 0;
   // device memory
   std::vector<void*> allocs;
   Args a{};
@@ -859,7 +876,8 @@ struct inu_ctx {
   // launch shape
   uint32_t grid_stage = 0;
   size_t lds_stage0 = 0, lds_stage1 = 0;
-  std::vector<uint8_t> cell_inside;  // host copy, for the source lists
+  // (cell, neighbour) for every neighbour in another block, by cell
+  std::vector<std::pair<uint32_t, uint32_t>> cross;
   // graphs of recorded step blocks, by step count
   std::map<uint32_t, hipGraphExec_t> graphs;
   std::vector<hipEvent_t> events;
@@ -957,10 +975,10 @@ int launch_step(inu_ctx* c, const hipEvent_t* ev) {
     else if (comp) hipLaunchKernelGGL((k_source_idle<true, false>), dim3(gb), dim3(256), 0, c->stream, a);
     else if (fm) hipLaunchKernelGGL((k_source_idle<false, true>), dim3(gb), dim3(256), 0, c->stream, a);
     else hipLaunchKernelGGL((k_source_idle<false, false>), dim3(gb), dim3(256), 0, c->stream, a);
-    if (c->n_src_bnd) {
-      const uint32_t gs = std::max(1u, std::min<uint32_t>((c->n_src_bnd + 255) / 256, 8u * (uint32_t)c->cus));
-      if (fm) hipLaunchKernelGGL((k_source_bound<true>), dim3(gs), dim3(256), 0, c->stream, a);
-      else hipLaunchKernelGGL((k_source_bound<false>), dim3(gs), dim3(256), 0, c->stream, a);
+    if (c->n_src_pairs) {
+      const uint32_t gs = (c->n_src_pairs + 255) / 256;
+      if (fm) hipLaunchKernelGGL((k_source_fix<true>), dim3(gs), dim3(256), 0, c->stream, a);
+      else hipLaunchKernelGGL((k_source_fix<false>), dim3(gs), dim3(256), 0, c->stream, a);
     }
   }
   if (c->flags & INU_FLAG_ACTIVE_SET)
@@ -1063,7 +1081,7 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   std::vector<uint16_t> lidx(S);
   std::vector<int32_t> snb(S);
   std::vector<float> slen(S);
-  std::vector<uint8_t> inside(N, 1);
+  std::vector<std::pair<uint32_t, uint32_t>> cross;
   for (uint32_t i = 0; i < N; ++i) {
     const uint32_t b = i / kBlockCells;
     if (cell_edge_ptr[i + 1] < cell_edge_ptr[i] || cell_edge_ptr[i + 1] > S)
@@ -1086,7 +1104,7 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
       lidx[k] = (uint16_t)((j << 1) | right);
       snb[k] = right ? edge_cell_l[e] : edge_cell_r[e];
       slen[k] = edge_len[e];
-      if (snb[k] >= 0 && (uint32_t)snb[k] / kBlockCells != b) inside[i] = 0;
+      if (snb[k] >= 0 && (uint32_t)snb[k] / kBlockCells != b) cross.emplace_back(i, (uint32_t)snb[k]);
     }
   }
   // Two-ring block graph (reference blocks.py:63-118).
@@ -1124,15 +1142,14 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   c->n_halo = hp[NB];
   c->n_src = 0;
   c->n_src_blocks = 0;
-  c->n_src_bnd = 0;
+  c->n_src_pairs = 0;
   Args& a = c->a;
   const size_t NP = (size_t)NB * kBlockCells;
   int rc = 0;
   rc |= dalloc(c, &a.h, NP); rc |= dalloc(c, &a.hu, NP); rc |= dalloc(c, &a.hv, NP); rc |= dalloc(c, &a.hlo, NP);
   rc |= dalloc(c, &a.h1, NP); rc |= dalloc(c, &a.hu1, NP); rc |= dalloc(c, &a.hv1, NP); rc |= dalloc(c, &a.h1lo, NP);
   rc |= dalloc(c, &a.area, NP); rc |= dalloc(c, &a.nman, NP); rc |= dalloc(c, &a.clamp, NP); rc |= dalloc(c, &a.src, NP);
-  uint32_t* cptr; uint16_t* lid; int32_t* sn; float* sl; uint8_t* cin;
-  rc |= dalloc(c, &cin, N);
+  uint32_t* cptr; uint16_t* lid; int32_t* sn; float* sl;
   rc |= dalloc(c, &cptr, (size_t)N + 1); rc |= dalloc(c, &lid, S); rc |= dalloc(c, &sn, S); rc |= dalloc(c, &sl, S);
   int2* ecell; float2* enrm; float* elen; float2* edz;
   rc |= dalloc(c, &ecell, E); rc |= dalloc(c, &enrm, E); rc |= dalloc(c, &elen, E); rc |= dalloc(c, &edz, E);
@@ -1141,16 +1158,16 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   rc |= dalloc(c, &dbinfo, NB); rc |= dalloc(c, &dhx, hx.size());
   rc |= dalloc(c, &dbptr, NB + 1); rc |= dalloc(c, &dbidx, bidx.size());
   rc |= dalloc(c, &a.wet, NB); rc |= dalloc(c, &a.active, NB); rc |= dalloc(c, &a.forced, NB);
-  rc |= dalloc(c, &a.has_src, NB); rc |= dalloc(c, &a.list, NB);
+  rc |= dalloc(c, &a.has_src, NB); rc |= dalloc(c, &a.list, NB); rc |= dalloc(c, &a.idle, NB);
   uint32_t* src_cell; uint32_t* src_blocks;
-  uint32_t* src_bnd;
-  rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1); rc |= dalloc(c, &src_bnd, 1);
+  uint2* src_pair;
+  rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1); rc |= dalloc(c, &src_pair, 1);
   rc |= dalloc(c, &c->ctl, 1);
   if (rc) return rc;
-  a.cptr = cptr; a.lidx = lid; a.snb = sn; a.slen = sl; a.cell_inside = cin;
+  a.cptr = cptr; a.lidx = lid; a.snb = sn; a.slen = sl;
   a.ecell = ecell; a.enrm = enrm; a.elen = elen; a.edz = edz;
   a.binfo = dbinfo; a.hx = dhx; a.bptr = dbptr; a.bidx = dbidx;
-  a.src_cell = src_cell; a.src_blocks = src_blocks; a.src_bnd = src_bnd; a.n_src_bnd = 0;
+  a.src_cell = src_cell; a.src_blocks = src_blocks; a.src_pair = src_pair; a.n_src_pairs = 0;
   a.ctl = c->ctl;
   a.N = N; a.NB = NB; a.max_local = max_local; a.max_slots = max_slots;
   // Host-side repack of the edge record into vector-loadable SoA.
@@ -1177,8 +1194,7 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   if ((rc = upload(c, lid, lidx.data(), S))) return rc;
   if ((rc = upload(c, sn, snb.data(), S))) return rc;
   if ((rc = upload(c, sl, slen.data(), S))) return rc;
-  if ((rc = upload(c, cin, inside.data(), N))) return rc;
-  c->cell_inside.swap(inside);
+  c->cross.swap(cross);
   if ((rc = upload(c, ecell, h_ecell.data(), E))) return rc;
   if ((rc = upload(c, enrm, h_nrm.data(), E))) return rc;
   if ((rc = upload(c, elen, edge_len, E))) return rc;
@@ -1273,14 +1289,14 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
     has[cell[k] / kBlockCells] = 1;
   }
   std::sort(cells.begin(), cells.end());
-  std::vector<uint32_t> bnd;
-  for (uint32_t cl : cells)
-    if (!c->cell_inside[cl]) bnd.push_back(cl);
+  std::vector<uint2> pairs;
+  for (const auto& pr : c->cross)
+    if (dense[pr.first] != 0.0f) pairs.push_back(make_uint2(pr.first, pr.second));
   std::vector<uint32_t> blocks;
   for (uint32_t b = 0; b < c->NB; ++b)
     if (has[b]) blocks.push_back(b);
   // Replace the source lists (their sizes change).
-  for (void* q : {(void*)c->a.src_cell, (void*)c->a.src_blocks, (void*)c->a.src_bnd}) {
+  for (void* q : {(void*)c->a.src_cell, (void*)c->a.src_blocks, (void*)c->a.src_pair}) {
     auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
     if (it != c->allocs.end()) {
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1290,18 +1306,19 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   }
   c->a.src_cell = nullptr;
   c->a.src_blocks = nullptr;
-  c->a.src_bnd = nullptr;
-  uint32_t *dcell = nullptr, *dblk = nullptr, *dbnd = nullptr;
+  c->a.src_pair = nullptr;
+  uint32_t *dcell = nullptr, *dblk = nullptr;
+  uint2* dpair = nullptr;
   int rc2 = dalloc(c, &dcell, cells.size());
   if (rc2) return rc2;
   rc2 = dalloc(c, &dblk, blocks.size());
   if (rc2) return rc2;
-  rc2 = dalloc(c, &dbnd, bnd.size());
+  rc2 = dalloc(c, &dpair, pairs.size());
   if (rc2) return rc2;
-  if ((rc = upload(c, dbnd, bnd.data(), bnd.size()))) return rc;
-  c->a.src_bnd = dbnd;
-  c->n_src_bnd = (uint32_t)bnd.size();
-  c->a.n_src_bnd = c->n_src_bnd;
+  if ((rc = upload(c, dpair, pairs.data(), pairs.size()))) return rc;
+  c->a.src_pair = dpair;
+  c->n_src_pairs = (uint32_t)pairs.size();
+  c->a.n_src_pairs = c->n_src_pairs;
   if ((rc = upload(c, dcell, cells.data(), cells.size()))) return rc;
   if ((rc = upload(c, dblk, blocks.data(), blocks.size()))) return rc;
   if ((rc = upload(c, c->a.src, dense.data(), dense.size()))) return rc;
@@ -1522,6 +1539,40 @@ int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out7, float g, 
   (void)hipFree(din);
   (void)hipFree(dout);
   if (e != hipSuccess) return fail(c, INU_EHIP, std::string("probe: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int inu_stream_copy(inu_ctx* c, uint64_t n_bytes, uint32_t reps, float* ms_out) {
+  if (!c) return INU_EINVAL;
+  if (!ms_out || reps == 0u || n_bytes == 0u || n_bytes % 16u != 0u)
+    return fail(c, INU_EINVAL, "inu_stream_copy: n_bytes must be a positive multiple of 16, reps positive");
+  HIPCHK(c, hipSetDevice(c->device));
+  void* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipError_t e = hipMalloc(&buf[0], n_bytes);
+  if (e == hipSuccess) e = hipMalloc(&buf[1], n_bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(buf[0], 0, n_bytes, c->stream);
+  if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+  const size_t n = n_bytes / 16u;
+  const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, (size_t)c->cus * 64u);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const float4*)buf[0], (float4*)buf[1], n);
+    e = hipEventRecord(ev[0], c->stream);
+  }
+  for (uint32_t r = 0; e == hipSuccess && r < reps; ++r)
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, (const float4*)buf[0], (float4*)buf[1], n);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipEventRecord(ev[1], c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
+  for (void* q : buf)
+    if (q) (void)hipFree(q);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("stream copy: ") + hipGetErrorString(e));
+  *ms_out = ms / (float)reps;
   return 0;
 }
 

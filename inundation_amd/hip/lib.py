@@ -63,6 +63,7 @@ EXPORTS = (
     "inu_download",
     "inu_upload",
     "inu_probe",
+    "inu_stream_copy",
 )
 
 
@@ -126,6 +127,7 @@ def load() -> C.CDLL:
         "inu_download": (_I, [_P, _I, _P]),
         "inu_upload": (_I, [_P, _I, _P]),
         "inu_probe": (_I, [_P, _U32, _P, _P, _F, _F]),
+        "inu_stream_copy": (_I, [_P, C.c_uint64, _U32, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -283,6 +285,12 @@ class Context:
     def upload(self, which: int, values: NDArray[np.float32]) -> None:
         a = np.ascontiguousarray(values, dtype=np.float32)
         self.call("inu_upload", which, ptr(a))
+
+    def stream_copy(self, n_bytes: int, reps: int = 5) -> float:
+        """Mean milliseconds of one plain device copy of ``n_bytes`` (read and written)."""
+        ms = np.zeros(1, dtype=np.float32)
+        self.call("inu_stream_copy", int(n_bytes), int(reps), ptr(ms))
+        return float(ms[0])
 
     def probe(self, pairs: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:
         p = np.ascontiguousarray(pairs, dtype=np.float32).reshape(-1, 2)

@@ -171,6 +171,31 @@ def test_sources_are_bitwise_the_fp32_oracle(ctx, mesh, grid, order) -> None:  #
     assert D.snapshot(gpu, n, nb)["h"].max() > DRY_TOL
 
 
+@pytest.mark.parametrize("order", [0, EXACT], ids=["fast", "exact"])
+@pytest.mark.parametrize("active", [True, False], ids=["active", "all"])
+def test_sparse_rain_dt_bound_is_bitwise_the_fp32_oracle(ctx, mesh, grid, order, active) -> None:  # noqa: ANN001
+    """Rain on every other cell beside a dam break, CFL-driven from a mostly dry start.
+
+    Many wetted source cells have a dry neighbour in another block, the case
+    whose dt bound is taken in two passes on the device; the clock and state
+    must still follow the oracle bit for bit.
+    """
+    gpu, ref = _both(ctx, mesh)
+    n = grid.N
+    nb = num_blocks(n, 256)
+    h, hu, hv = f32_state(*CASES["dam_break"](grid))
+    cells = np.arange(0, n, 2, dtype=np.uint32)
+    rates = np.where(cells % 6 == 0, 0.05, 0.004).astype(np.float32)
+    flags = (D.FLAGS_DEFAULT if active else L.FLAG_DETERMINISTIC | L.FLAG_COMPENSATED) | order
+    for be in (gpu, ref):
+        D.setup(be, n, h, hu, hv, MANNING, dry_tol=DRY_TOL, src_cells=cells, src_rates=rates, flags=flags)
+        D.start(be, 3.0, 0.5, 0.05)
+    for k in range(4):
+        gpu.steps(30, use_graph=True)
+        ref.steps(30)
+        D.assert_same(D.snapshot(gpu, n, nb), D.snapshot(ref, n, nb), f"block {k}")
+
+
 def test_active_set_on_equals_off(ctx, mesh, grid) -> None:  # noqa: ANN001
     """Skipping blocks is exact (reference test_gpu_active_set.py:77-91)."""
     gpu, _ = _both(ctx, mesh)
@@ -184,3 +209,9 @@ def test_active_set_on_equals_off(ctx, mesh, grid) -> None:  # noqa: ANN001
         s = D.snapshot(gpu, grid.N, nb)
         out.append({k: s[k] for k in ("h", "hu", "hv", "h_lo", "ctrl")})
     D.assert_same(out[0], out[1], "active on vs off")
+
+
+def test_stream_copy_reports_an_hbm_rate(ctx) -> None:  # noqa: ANN001
+    ms = ctx.stream_copy(256 << 20, 3)
+    gbs = 2 * (256 << 20) / (ms * 1e-3) / 1e9
+    assert 500.0 < gbs < 8000.0
