@@ -1,7 +1,7 @@
 // MI355X (gfx950) shallow-water step backend: kernels, device runtime, C ABI.
 //
 // One context owns the device copy of one mesh, its state, one HIP stream and
-// the hipGraphs of recorded step blocks. A Heun step is five kernels in stream
+// the hipGraphs of recorded step blocks. A Heun step is six kernels in stream
 // order (no host round trip):
 //
 //   k_stage<0>      predictor: per active 256-cell block, every edge touching
@@ -10,7 +10,8 @@
 //   k_stage<1>      corrector: same on the predictor state, Heun average,
 //                   the block's sources, wet mark, CFL min for the next step;
 //   k_source_idle   sources of blocks the active set skips this step;
-//   k_source_bound  the next step's dt bound from each wetted source cell;
+//   k_source_fix    the exact dt bound of wetted source cells beside a dry
+//                   cell of another block;
 //   k_dilate        active = wet over each block's two-ring, plus the list of
 //                   blocks the next step visits (wave-ballot compaction);
 //   k_step_control  the uint64 tick clock and the next dt.
@@ -69,28 +70,30 @@ struct Ctl {
 static_assert(sizeof(inu_ctrl) == 48, "control block must be 48 bytes");
 
 // Device pointers and sizes, passed by value to every kernel.
-// Where block b's data lives: owned edges [e0, e0 + n_own) in the sorted edge
-// arrays, halo edges hx[x0, x0 + n_halo), CSR slots [s0, s0 + n_slots).
+// Where block b's data lives: its local edges (those with the left cell in b,
+// then those with only the right cell in b) at [e0, e0 + n_loc) of the local
+// edge stream, the first n_own of them owned; its ghost cells (the cells of
+// other blocks that those edges touch) at gx[g0, g0 + n_ghost); its CSR slots
+// at [s0, s0 + n_slots).
 struct BlockInfo {
-  uint32_t e0, n_own, x0, n_halo, s0, n_slots, pad0, pad1;
+  uint32_t e0, n_loc, n_own, g0, n_ghost, s0, n_slots, pad;
 };
 
 struct Args {
   Phys p;
-  uint32_t N, NB, max_local, max_slots, n_src, track_clamp;
+  uint32_t N, NB, max_local, max_slots, max_ghost, n_src, track_clamp;
   // cells
   float *h, *hu, *hv, *hlo, *h1, *hu1, *hv1, *h1lo, *area, *nman, *clamp, *src;
   const uint32_t* cptr;
   const uint16_t* lidx;
   const int32_t* snb;
   const float* slen;
-  // edges (sorted by the block of the left cell)
-  const int2* ecell;
-  const float2* enrm;
-  const float* elen;
-  const float2* edz;
+  // local edge streams of the blocks: {sides word, nx, ny, length} and the
+  // signed bed step; ghost cell ids
+  const uint4* lrec;
+  const float* ldz;
   const BlockInfo* binfo;
-  const uint32_t* hx;
+  const uint32_t* gx;
   // blocks
   uint32_t *wet, *active, *forced, *has_src, *list, *idle;
   const uint32_t *bptr, *bidx;
@@ -137,120 +140,138 @@ __device__ __forceinline__ void block_atomic_min(uint32_t* dst, uint32_t bits) {
 // Stage kernels
 // ---------------------------------------------------------------------------
 
-// Stage input of a cell outside the block being processed: the corrector reads
-// the predictor state only where that cell's block was active this step
-// (reference resident.py:209-226).
-// LO: also read the compensated low word (the fast order's face depths use it).
-template <int ST, bool LO>
-__device__ __forceinline__ void load_outside(const Args& a, int c, float& h, float& hu, float& hv, float& lo) {
-  if (ST == 1 && a.active[(uint32_t)c >> 8] != 0u) {
-    h = a.h1[c];
-    hu = a.hu1[c];
-    hv = a.hv1[c];
-    lo = LO ? a.h1lo[c] : 0.0f;
-  } else {
-    h = a.h[c];
-    hu = a.hu[c];
-    hv = a.hv[c];
-    lo = LO ? a.hlo[c] : 0.0f;
-  }
-}
-
-// LDS of one block: stage input of its cells, its CSR slots (local edge
-// numbers) and the words of every edge touching it.
+// LDS of one block: stage input of its cells and of its ghost cells, its CSR
+// slots (local edge numbers) and the words of every edge touching it.
 struct Lds {
-  float* s_h;
+  float* s_h;  // kBlockCells + max_ghost entries each: the cells, then the ghosts
   float* s_hu;
   float* s_hv;
   float* s_lo;
+  float* s_vf;  // fast order: vel_factor and sqrt of each cell and ghost depth
+  float* s_sq;
   uint16_t* s_slot;
   float* w[6];
-  uint32_t* s_side;  // per local edge: (left local cell << 16) | right local cell
-  float* s_len;      // per local edge: length
+  uint32_t* s_side;  // per local edge: its sides word (corrector with sources)
+  float* s_len;      // per local edge: length (corrector with sources)
 };
 
-// Local cell numbers of s_side: a cell of the block, a wall, or elsewhere.
-constexpr uint32_t kSideOutside = 0xffffu;
-constexpr uint32_t kSideWall = 0xfffeu;
+// Local cell numbers of an edge's sides word ((left << 16) | right): a cell of
+// the block (0..255), a ghost (256 + its place in the block's ghost list) or,
+// on the right only, a wall.
+constexpr uint32_t kSideWall = 0xffffu;
+constexpr uint32_t kMaxGhosts = kSideWall - kBlockCells;
 
-constexpr uint32_t kLdsCellWords = 4 * kBlockCells;
 constexpr uint32_t kEdgesAhead = 2;  // edge records each thread loads before the barrier
 constexpr uint32_t kSlotsAhead = 4;  // CSR slots each thread loads before the barrier
 
-__device__ __forceinline__ Lds carve(float* sm, uint32_t max_local, uint32_t max_slots) {
+__host__ __device__ inline size_t lds_words(uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
+  return 6 * (size_t)(kBlockCells + max_ghost) + (max_slots + 1) / 2 + 8 * (size_t)max_local;
+}
+
+__device__ __forceinline__ Lds carve(float* sm, uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
+  const uint32_t nc = kBlockCells + max_ghost;
   Lds l;
   l.s_h = sm;
-  l.s_hu = sm + kBlockCells;
-  l.s_hv = sm + 2 * kBlockCells;
-  l.s_lo = sm + 3 * kBlockCells;
-  l.s_slot = reinterpret_cast<uint16_t*>(sm + kLdsCellWords);
-  float* w = sm + kLdsCellWords + (max_slots + 1) / 2;
+  l.s_hu = sm + nc;
+  l.s_hv = sm + 2 * nc;
+  l.s_lo = sm + 3 * nc;
+  l.s_vf = sm + 4 * nc;
+  l.s_sq = sm + 5 * nc;
+  l.s_slot = reinterpret_cast<uint16_t*>(sm + 6 * nc);
+  float* w = sm + 6 * nc + (max_slots + 1) / 2;
   for (int k = 0; k < 6; ++k) l.w[k] = w + k * max_local;
   l.s_side = reinterpret_cast<uint32_t*>(w + 6 * max_local);
   l.s_len = w + 7 * max_local;
   return l;
 }
 
-// One edge record, as loaded from the SoA edge arrays.
+// One local edge record: sides word, normal, length and the signed bed step
+// (dzL when positive, -dzR when negative: one of the two is always zero).
 struct EdgeRec {
-  int2 c;
-  float2 n;
-  float len;
-  float2 dz;
+  uint32_t side;
+  float nx, ny, len, d;
 };
 
 __device__ __forceinline__ EdgeRec load_edge(const Args& a, const BlockInfo& bi, uint32_t j) {
-  const uint32_t e = (j < bi.n_own) ? bi.e0 + j : a.hx[bi.x0 + (j - bi.n_own)];
+  const uint4 q = a.lrec[bi.e0 + j];
   EdgeRec r;
-  r.c = a.ecell[e];
-  r.n = a.enrm[e];
-  r.len = a.elen[e];
-  r.dz = a.edz[e];
+  r.side = q.x;
+  r.nx = __uint_as_float(q.y);
+  r.ny = __uint_as_float(q.z);
+  r.len = __uint_as_float(q.w);
+  r.d = a.ldz[bi.e0 + j];
   return r;
 }
 
-// Words of local edge j into LDS (zeros when both sides are dry).
+// Stage input of ghost cell g (a cell of another block). The corrector reads
+// the predictor state only where that cell's block was active this step
+// (reference resident.py:209-226); it loads that state first and the start
+// state only for a skipped block.
+template <int ST, bool LO>
+__device__ __forceinline__ void load_ghost(const Args& a, uint32_t g, float& h, float& hu, float& hv, float& lo) {
+  if (ST == 1) {
+    const uint32_t act = a.active[g >> 8];
+    h = a.h1[g];
+    hu = a.hu1[g];
+    hv = a.hv1[g];
+    lo = LO ? a.h1lo[g] : 0.0f;
+    if (act != 0u) return;
+  }
+  h = a.h[g];
+  hu = a.hu[g];
+  hv = a.hv[g];
+  lo = LO ? a.hlo[g] : 0.0f;
+}
+
+// Words of local edge j into LDS (zeros when both sides are dry); both sides
+// are read from LDS.
 template <int ST, bool FM, bool LO>
-__device__ __forceinline__ void edge_eval(const Args& a, uint32_t c0, const Lds& l, const EdgeRec& r, uint32_t j) {
+__device__ __forceinline__ void edge_eval(const Args& a, const Lds& l, const EdgeRec& r, uint32_t j) {
   if (ST == 1 && a.n_src != 0u) {
-    const uint32_t ql = ((uint32_t)r.c.x - c0 < kBlockCells) ? (uint32_t)r.c.x - c0 : kSideOutside;
-    const uint32_t qr = (r.c.y < 0) ? kSideWall
-                        : ((uint32_t)r.c.y - c0 < kBlockCells) ? (uint32_t)r.c.y - c0 : kSideOutside;
-    l.s_side[j] = (ql << 16) | qr;
+    l.s_side[j] = r.side;
     l.s_len[j] = r.len;
   }
-  float hL, huL, hvL, hR, huR, hvR, loL, loR;
-  if ((uint32_t)r.c.x - c0 < kBlockCells) {
-    const uint32_t q = (uint32_t)r.c.x - c0;
-    hL = l.s_h[q];
-    huL = l.s_hu[q];
-    hvL = l.s_hv[q];
-    loL = LO ? l.s_lo[q] : 0.0f;
-  } else {
-    load_outside<ST, LO>(a, r.c.x, hL, huL, hvL, loL);
-  }
-  const bool wall = r.c.y < 0;
-  if (wall) {
-    hR = hL;
-    loR = loL;
-    if (FM) inu::fast::wall_ghost(huL, hvL, r.n.x, r.n.y, huR, hvR);
-    else inu::wall_ghost(huL, hvL, r.n.x, r.n.y, huR, hvR);
-  } else if ((uint32_t)r.c.y - c0 < kBlockCells) {
-    const uint32_t q = (uint32_t)r.c.y - c0;
-    hR = l.s_h[q];
-    huR = l.s_hu[q];
-    hvR = l.s_hv[q];
-    loR = LO ? l.s_lo[q] : 0.0f;
-  } else {
-    load_outside<ST, LO>(a, r.c.y, hR, huR, hvR, loR);
-  }
+  const uint32_t ql = r.side >> 16;
+  const uint32_t qr = r.side & 0xffffu;
+  const bool wall = qr == kSideWall;
+  const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
+  const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
   float w[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  if (!(hL < a.p.dry && hR < a.p.dry)) {
-    if (FM)
-      inu::fast::edge_words(a.p, wall, hL, huL, hvL, hR, huR, hvR, r.dz.x, r.dz.y, r.n.x, r.n.y, r.len, loL,
-                            loR, w);
-    else
-      inu::edge_words(a.p, wall, hL, huL, hvL, hR, huR, hvR, r.dz.x, r.dz.y, r.n.x, r.n.y, r.len, w);
+  if (FM) {
+    inu::fast::Side L, R;
+    L.h = l.s_h[ql];
+    L.hu = l.s_hu[ql];
+    L.hv = l.s_hv[ql];
+    L.lo = LO ? l.s_lo[ql] : 0.0f;
+    L.vf = l.s_vf[ql];
+    L.sq = l.s_sq[ql];
+    if (wall) {
+      R = L;
+      inu::fast::wall_ghost(L.hu, L.hv, r.nx, r.ny, R.hu, R.hv);
+    } else {
+      R.h = l.s_h[qr];
+      R.hu = l.s_hu[qr];
+      R.hv = l.s_hv[qr];
+      R.lo = LO ? l.s_lo[qr] : 0.0f;
+      R.vf = l.s_vf[qr];
+      R.sq = l.s_sq[qr];
+    }
+    if (!(L.h < a.p.dry && R.h < a.p.dry)) inu::fast::edge_words(a.p, wall, L, R, dzL, dzR, r.nx, r.ny, r.len, w);
+  } else {
+    const float hL = l.s_h[ql];
+    const float huL = l.s_hu[ql];
+    const float hvL = l.s_hv[ql];
+    float hR, huR, hvR;
+    if (wall) {
+      hR = hL;
+      inu::wall_ghost(huL, hvL, r.nx, r.ny, huR, hvR);
+    } else {
+      hR = l.s_h[qr];
+      huR = l.s_hu[qr];
+      hvR = l.s_hv[qr];
+    }
+    if (!(hL < a.p.dry && hR < a.p.dry))
+      inu::edge_words(a.p, wall, hL, huL, hvL, hR, huR, hvR, dzL, dzR, r.nx, r.ny, r.len, w);
   }
   l.w[0][j] = w[0];
   l.w[1][j] = w[1];
@@ -260,22 +281,47 @@ __device__ __forceinline__ void edge_eval(const Args& a, uint32_t c0, const Lds&
   if (ST == 1) l.w[5][j] = w[5];
 }
 
+// Stage input of one cell or ghost into LDS slot q, with the fast order's
+// per-depth vel_factor and square root.
+template <bool FM>
+__device__ __forceinline__ void put_cell(const Lds& l, const Phys& p, uint32_t q, float h, float hu, float hv,
+                                         float lo) {
+  l.s_h[q] = h;
+  l.s_hu[q] = hu;
+  l.s_hv[q] = hv;
+  l.s_lo[q] = lo;
+  if (FM) {
+    l.s_vf[q] = inu::fast::vel_factor(p, h);
+    l.s_sq[q] = sqrtf(inu::gmax(h, 0.0f));
+  }
+}
+
+// The next blocks of a workgroup's walk over the list: the block after the
+// current one with its BlockInfo and this thread's ghost cell id, and the
+// list entry after that. Loaded while the current block is processed.
+struct Ahead {
+  uint32_t t;         // list position of the next block (>= end: none)
+  uint32_t b;         // next block
+  BlockInfo bi;       // its BlockInfo
+  uint32_t gid;       // its ghost cell id of this thread
+  uint32_t b2;        // block after it
+};
+
 // One active block of one Heun stage (reference resident.py:425-512, gather +
 // compensated variants), with the corrector also applying this block's
-// sources (resident.py:691-708). Three phases: (A) every load that needs no
-// other load (cell data, CSR slots, the thread's first edge records) is
-// issued at once; (B) each edge touching the block is evaluated once into
-// LDS, reading cells outside the block from global memory; (C) each cell sums
-// its edges in CSR order from LDS and updates.
+// sources (resident.py:691-708). Three phases: (A) every load of the block is
+// issued at once (cells, ghost cells, CSR slots, the thread's first edge
+// records); (B) each edge touching the block is evaluated once into LDS from
+// LDS; (C) each cell sums its edges in CSR order from LDS and updates.
 template <int ST, bool COMP, bool FM>
-__device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& l) {
+__device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, uint32_t gid, float dt,
+                                const Lds& l, Ahead& nx, uint32_t end, uint32_t per) {
   constexpr bool LO = COMP && FM;  // the fast order's face depths read the low words
   const Phys& p = a.p;
   const uint32_t tid = threadIdx.x;
   const uint32_t c0 = b << 8;
   const uint32_t nc = min(kBlockCells, a.N - c0);
-  const BlockInfo bi = a.binfo[b];
-  const uint32_t n_loc = bi.n_own + bi.n_halo;
+  const uint32_t n_loc = bi.n_loc;
 
   // ---- (A) independent loads -------------------------------------------
   EdgeRec er[kEdgesAhead];
@@ -286,6 +332,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& 
 #pragma unroll
   for (uint32_t m = 0; m < kSlotsAhead; ++m)
     if (tid + m * kBlockCells < bi.n_slots) sl[m] = a.lidx[bi.s0 + tid + m * kBlockCells];
+  float gh = 0.0f, ghu = 0.0f, ghv = 0.0f, glo = 0.0f;
+  if (tid < bi.n_ghost) load_ghost<ST, LO>(a, gid, gh, ghu, ghv, glo);
   const uint32_t i = c0 + tid;
   float hi = 0.0f, hui = 0.0f, hvi = 0.0f, loi = 0.0f, A = 1.0f, nm = 0.0f;
   float h0 = 0.0f, hu0 = 0.0f, hv0 = 0.0f, lo0 = 0.0f, rate = 0.0f;
@@ -307,12 +355,16 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& 
       if (a.n_src != 0u) rate = a.src[i];
     }
   }
+  // Walk ahead: the next block's BlockInfo and the list entry after it.
+  if (nx.t < end) nx.bi = a.binfo[nx.b];
+  if (nx.t + per < end) nx.b2 = a.list[nx.t + per];
   __syncthreads();  // the previous block's LDS reads are done
-  if (tid < nc) {
-    l.s_h[tid] = hi;
-    l.s_hu[tid] = hui;
-    l.s_hv[tid] = hvi;
-    l.s_lo[tid] = loi;
+  if (tid < nc) put_cell<FM>(l, p, tid, hi, hui, hvi, loi);
+  if (tid < bi.n_ghost) put_cell<FM>(l, p, kBlockCells + tid, gh, ghu, ghv, glo);
+  for (uint32_t g = tid + kBlockCells; g < bi.n_ghost; g += kBlockCells) {
+    float x, y, z, w;
+    load_ghost<ST, LO>(a, a.gx[bi.g0 + g], x, y, z, w);
+    put_cell<FM>(l, p, kBlockCells + g, x, y, z, w);
   }
 #pragma unroll
   for (uint32_t m = 0; m < kSlotsAhead; ++m)
@@ -323,9 +375,11 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& 
   // ---- (B) edges -----------------------------------------------------------
 #pragma unroll
   for (uint32_t m = 0; m < kEdgesAhead; ++m)
-    if (tid + m * kBlockCells < n_loc) edge_eval<ST, FM, LO>(a, c0, l, er[m], tid + m * kBlockCells);
+    if (tid + m * kBlockCells < n_loc) edge_eval<ST, FM, LO>(a, l, er[m], tid + m * kBlockCells);
   for (uint32_t j = tid + kEdgesAhead * kBlockCells; j < n_loc; j += kBlockCells)
-    edge_eval<ST, FM, LO>(a, c0, l, load_edge(a, bi, j), j);
+    edge_eval<ST, FM, LO>(a, l, load_edge(a, bi, j), j);
+  // Walk ahead: this thread's ghost cell id of the next block.
+  if (nx.t < end && tid < nx.bi.n_ghost) nx.gid = a.gx[nx.bi.g0 + tid];
   __syncthreads();
 
   // ---- (C) cells -----------------------------------------------------------
@@ -444,7 +498,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, float dt, const Lds& 
           const uint32_t j = s >> 1;
           const uint32_t pk = l.s_side[j];
           const uint32_t q = (s & 1u) ? (pk >> 16) : (pk & 0xffffu);
-          const bool dry_side = q < kSideWall && l.s_h[q] < p.dry;  // walls and other blocks: wet
+          const bool dry_side = q < kBlockCells && l.s_h[q] < p.dry;  // walls and ghosts: wet
           const float kf = dry_side ? 2.0f : 1.0f;
           if (FM) lam = fmaf(fmaf(kf, c, u), l.s_len[j], lam);
           else lam += (u + kf * c) * l.s_len[j];
@@ -465,19 +519,33 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
   const uint32_t dtw = a.ctl->dtbuf[0];
   if (dtw == 0u) return;  // no-op step: the window's stop was reached
   const float dt = __uint_as_float(dtw);
-  const Lds l = carve(sm, a.max_local, a.max_slots);
+  const Lds l = carve(sm, a.max_local, a.max_slots, a.max_ghost);
   // XCD-aware split: workgroup w runs on XCD w % 8, so each XCD walks one
   // contiguous (spatially compact) slice of the block list and its L2 keeps
-  // the halo cells its neighbours read.
+  // the ghost cells its neighbours read.
   const uint32_t cnt = a.ctl->cnt_cur;
   const uint32_t xcd = blockIdx.x % kXcds;
   const uint32_t per = gridDim.x / kXcds;
-  const uint32_t lo = (uint32_t)(((uint64_t)cnt * xcd) / kXcds);
-  const uint32_t hi = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
+  const uint32_t end = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
+  Ahead nx;
+  nx.t = (uint32_t)(((uint64_t)cnt * xcd) / kXcds) + blockIdx.x / kXcds;
+  nx.b = 0u;
+  nx.b2 = 0u;
+  nx.gid = 0u;
+  if (nx.t < end) {
+    nx.b = a.list[nx.t];
+    nx.bi = a.binfo[nx.b];
+    if (threadIdx.x < nx.bi.n_ghost) nx.gid = a.gx[nx.bi.g0 + threadIdx.x];
+    if (nx.t + per < end) nx.b2 = a.list[nx.t + per];
+  }
   uint32_t bound = kNoBound;  // this thread's smallest next-step bound
-  for (uint32_t t = lo + blockIdx.x / kXcds; t < hi; t += per) {
-    const uint32_t b = a.list[t];
-    const uint32_t bb = stage_block<ST, COMP, FM>(a, b, dt, l);
+  while (nx.t < end) {
+    const uint32_t b = nx.b;
+    const BlockInfo bi = nx.bi;
+    const uint32_t gid = nx.gid;
+    nx.t += per;
+    nx.b = nx.b2;
+    const uint32_t bb = stage_block<ST, COMP, FM>(a, b, bi, gid, dt, l, nx, end, per);
     bound = (bb < bound) ? bb : bound;
   }
   if (ST == 1) block_atomic_min(&a.ctl->dtbuf[1], bound);
@@ -693,13 +761,14 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
 template <bool FM, bool LO>
 __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
   extern __shared__ float sm[];
-  const Lds l = carve(sm, a.max_local, a.max_slots);
+  const Lds l = carve(sm, a.max_local, a.max_slots, a.max_ghost);
   const Phys& p = a.p;
+  const uint32_t tid = threadIdx.x;
   uint32_t best = kNoBound;
   for (uint32_t b = blockIdx.x; b < a.NB; b += gridDim.x) {
-    const uint32_t tid = threadIdx.x;
     const uint32_t c0 = b << 8;
     const uint32_t nc = min(kBlockCells, a.N - c0);
+    const BlockInfo bi = a.binfo[b];
     __syncthreads();
     if (tid < nc) {
       l.s_h[tid] = a.h[c0 + tid];
@@ -707,50 +776,42 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
       l.s_hv[tid] = a.hv[c0 + tid];
       l.s_lo[tid] = LO ? a.hlo[c0 + tid] : 0.0f;
     }
+    for (uint32_t g = tid; g < bi.n_ghost; g += kBlockCells) {
+      float x, y, z, w;
+      load_ghost<0, LO>(a, a.gx[bi.g0 + g], x, y, z, w);
+      l.s_h[kBlockCells + g] = x;
+      l.s_hu[kBlockCells + g] = y;
+      l.s_hv[kBlockCells + g] = z;
+      l.s_lo[kBlockCells + g] = w;
+    }
     __syncthreads();
-    const BlockInfo bi = a.binfo[b];
-    const uint32_t e0 = bi.e0;
-    const uint32_t n_own = bi.n_own;
-    const uint32_t x0 = bi.x0;
-    const uint32_t n_loc = n_own + bi.n_halo;
-    for (uint32_t j = tid; j < n_loc; j += blockDim.x) {
-      const uint32_t e = (j < n_own) ? e0 + j : a.hx[x0 + (j - n_own)];
-      const int2 c = a.ecell[e];
-      const float2 nrm = a.enrm[e];
-      float hL, huL, hvL, hR, huR, hvR, loL, loR;
-      if ((uint32_t)c.x - c0 < kBlockCells) {
-        hL = l.s_h[c.x - c0];
-        huL = l.s_hu[c.x - c0];
-        hvL = l.s_hv[c.x - c0];
-        loL = l.s_lo[c.x - c0];
-      } else {
-        hL = a.h[c.x];
-        huL = a.hu[c.x];
-        hvL = a.hv[c.x];
-        loL = LO ? a.hlo[c.x] : 0.0f;
-      }
-      if (c.y < 0) {
+    for (uint32_t j = tid; j < bi.n_loc; j += kBlockCells) {
+      const EdgeRec r = load_edge(a, bi, j);
+      const uint32_t ql = r.side >> 16;
+      const uint32_t qr = r.side & 0xffffu;
+      const float hL = l.s_h[ql];
+      const float huL = l.s_hu[ql];
+      const float hvL = l.s_hv[ql];
+      const float loL = l.s_lo[ql];
+      float hR, huR, hvR, loR;
+      if (qr == kSideWall) {
         hR = hL;
         loR = loL;
-        if (FM) inu::fast::wall_ghost(huL, hvL, nrm.x, nrm.y, huR, hvR);
-        else inu::wall_ghost(huL, hvL, nrm.x, nrm.y, huR, hvR);
-      } else if ((uint32_t)c.y - c0 < kBlockCells) {
-        hR = l.s_h[c.y - c0];
-        huR = l.s_hu[c.y - c0];
-        hvR = l.s_hv[c.y - c0];
-        loR = l.s_lo[c.y - c0];
+        if (FM) inu::fast::wall_ghost(huL, hvL, r.nx, r.ny, huR, hvR);
+        else inu::wall_ghost(huL, hvL, r.nx, r.ny, huR, hvR);
       } else {
-        hR = a.h[c.y];
-        huR = a.hu[c.y];
-        hvR = a.hv[c.y];
-        loR = LO ? a.hlo[c.y] : 0.0f;
+        hR = l.s_h[qr];
+        huR = l.s_hu[qr];
+        hvR = l.s_hv[qr];
+        loR = l.s_lo[qr];
       }
+      const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
+      const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
       float contrib = 0.0f;
       if (hL >= p.dry || hR >= p.dry) {
-        const float2 dz = a.edz[e];
-        contrib = (FM ? inu::fast::wave_bound(p, hL, huL, hvL, dz.x, hR, huR, hvR, dz.y, nrm.x, nrm.y, loL, loR)
-                      : inu::wave_bound(p, hL, huL, hvL, dz.x, hR, huR, hvR, dz.y, nrm.x, nrm.y)) *
-                  a.elen[e];
+        contrib = (FM ? inu::fast::wave_bound(p, hL, huL, hvL, dzL, hR, huR, hvR, dzR, r.nx, r.ny, loL, loR)
+                      : inu::wave_bound(p, hL, huL, hvL, dzL, hR, huR, hvR, dzR, r.nx, r.ny)) *
+                  r.len;
       }
       l.w[5][j] = contrib;
     }
@@ -1067,14 +1128,53 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
       if (r >= 0 && (uint32_t)r / kBlockCells != (uint32_t)edge_cell_l[e] / kBlockCells) hx[fill[(uint32_t)r / kBlockCells]++] = e;
     }
   }
-  uint32_t max_local = 1, max_slots = 1;
+  // Local edge stream of each block (its owned edges, then its halo edges)
+  // with both sides as local cell numbers, and its ghost cells, ascending.
+  for (uint32_t e = 0; e < E; ++e) {
+    const float dzl = edge_dz[2 * e], dzr = edge_dz[2 * e + 1];
+    if (!(dzl >= 0.0f) || !(dzr >= 0.0f) || !std::isfinite(dzl) || !std::isfinite(dzr) || (dzl != 0.0f && dzr != 0.0f))
+      return fail(c, INU_EINVAL, "edge_dz: both steps must be finite and non-negative with one of them zero "
+                                 "(the face sits on the higher bed)");
+  }
+  uint32_t max_local = 1, max_slots = 1, max_ghost = 1;
   std::vector<BlockInfo> binfo(NB);
+  std::vector<uint4> lrec((size_t)E + hp[NB]);
+  std::vector<float> ldz((size_t)E + hp[NB]);
+  std::vector<uint32_t> gxv;
+  std::vector<uint32_t> ghosts;
+  uint32_t e0 = 0;
   for (uint32_t b = 0; b < NB; ++b) {
     const uint32_t cb = b * kBlockCells, ce = std::min(cb + kBlockCells, N);
-    binfo[b] = BlockInfo{eo[b], eo[b + 1] - eo[b], hp[b], hp[b + 1] - hp[b], cell_edge_ptr[cb],
-                         cell_edge_ptr[ce] - cell_edge_ptr[cb], 0u, 0u};
-    max_local = std::max(max_local, binfo[b].n_own + binfo[b].n_halo);
+    const uint32_t n_own = eo[b + 1] - eo[b], n_loc = n_own + (hp[b + 1] - hp[b]);
+    ghosts.clear();
+    for (uint32_t e = eo[b]; e < eo[b + 1]; ++e)
+      if (edge_cell_r[e] >= 0 && (uint32_t)edge_cell_r[e] / kBlockCells != b) ghosts.push_back((uint32_t)edge_cell_r[e]);
+    for (uint32_t k = hp[b]; k < hp[b + 1]; ++k) ghosts.push_back((uint32_t)edge_cell_l[hx[k]]);
+    std::sort(ghosts.begin(), ghosts.end());
+    ghosts.erase(std::unique(ghosts.begin(), ghosts.end()), ghosts.end());
+    if (ghosts.size() > kMaxGhosts) return fail(c, INU_ERANGE, "a block borders too many cells of other blocks");
+    auto local = [&](int32_t cell) -> uint32_t {
+      if ((uint32_t)cell / kBlockCells == b) return (uint32_t)cell - cb;
+      return kBlockCells + (uint32_t)(std::lower_bound(ghosts.begin(), ghosts.end(), (uint32_t)cell) - ghosts.begin());
+    };
+    for (uint32_t j = 0; j < n_loc; ++j) {
+      const uint32_t e = (j < n_own) ? eo[b] + j : hx[hp[b] + (j - n_own)];
+      const uint32_t ql = local(edge_cell_l[e]);
+      const uint32_t qr = (edge_cell_r[e] < 0) ? kSideWall : local(edge_cell_r[e]);
+      uint32_t bx, by, bl;
+      memcpy(&bx, &edge_nx[e], 4);
+      memcpy(&by, &edge_ny[e], 4);
+      memcpy(&bl, &edge_len[e], 4);
+      lrec[e0 + j] = make_uint4((ql << 16) | qr, bx, by, bl);
+      ldz[e0 + j] = (edge_dz[2 * e] > 0.0f) ? edge_dz[2 * e] : -edge_dz[2 * e + 1];
+    }
+    binfo[b] = BlockInfo{e0, n_loc, n_own, (uint32_t)gxv.size(), (uint32_t)ghosts.size(), cell_edge_ptr[cb],
+                         cell_edge_ptr[ce] - cell_edge_ptr[cb], 0u};
+    gxv.insert(gxv.end(), ghosts.begin(), ghosts.end());
+    e0 += n_loc;
+    max_local = std::max(max_local, n_loc);
     max_slots = std::max(max_slots, binfo[b].n_slots);
+    max_ghost = std::max(max_ghost, (uint32_t)ghosts.size());
   }
   if (max_local >= 32768u) return fail(c, INU_ERANGE, "a block touches more than 32767 edges");
   // Local CSR: each slot's edge as its position in its block's LDS edge list.
@@ -1151,11 +1251,11 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   rc |= dalloc(c, &a.area, NP); rc |= dalloc(c, &a.nman, NP); rc |= dalloc(c, &a.clamp, NP); rc |= dalloc(c, &a.src, NP);
   uint32_t* cptr; uint16_t* lid; int32_t* sn; float* sl;
   rc |= dalloc(c, &cptr, (size_t)N + 1); rc |= dalloc(c, &lid, S); rc |= dalloc(c, &sn, S); rc |= dalloc(c, &sl, S);
-  int2* ecell; float2* enrm; float* elen; float2* edz;
-  rc |= dalloc(c, &ecell, E); rc |= dalloc(c, &enrm, E); rc |= dalloc(c, &elen, E); rc |= dalloc(c, &edz, E);
-  uint32_t *dhx, *dbptr, *dbidx;
+  uint4* dlrec; float* dldz;
+  rc |= dalloc(c, &dlrec, lrec.size()); rc |= dalloc(c, &dldz, ldz.size());
+  uint32_t *dgx, *dbptr, *dbidx;
   BlockInfo* dbinfo;
-  rc |= dalloc(c, &dbinfo, NB); rc |= dalloc(c, &dhx, hx.size());
+  rc |= dalloc(c, &dbinfo, NB); rc |= dalloc(c, &dgx, gxv.size());
   rc |= dalloc(c, &dbptr, NB + 1); rc |= dalloc(c, &dbidx, bidx.size());
   rc |= dalloc(c, &a.wet, NB); rc |= dalloc(c, &a.active, NB); rc |= dalloc(c, &a.forced, NB);
   rc |= dalloc(c, &a.has_src, NB); rc |= dalloc(c, &a.list, NB); rc |= dalloc(c, &a.idle, NB);
@@ -1165,19 +1265,11 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   rc |= dalloc(c, &c->ctl, 1);
   if (rc) return rc;
   a.cptr = cptr; a.lidx = lid; a.snb = sn; a.slen = sl;
-  a.ecell = ecell; a.enrm = enrm; a.elen = elen; a.edz = edz;
-  a.binfo = dbinfo; a.hx = dhx; a.bptr = dbptr; a.bidx = dbidx;
+  a.lrec = dlrec; a.ldz = dldz;
+  a.binfo = dbinfo; a.gx = dgx; a.bptr = dbptr; a.bidx = dbidx;
   a.src_cell = src_cell; a.src_blocks = src_blocks; a.src_pair = src_pair; a.n_src_pairs = 0;
   a.ctl = c->ctl;
-  a.N = N; a.NB = NB; a.max_local = max_local; a.max_slots = max_slots;
-  // Host-side repack of the edge record into vector-loadable SoA.
-  std::vector<int2> h_ecell(E);
-  std::vector<float2> h_nrm(E), h_dz(E);
-  for (uint32_t e = 0; e < E; ++e) {
-    h_ecell[e] = make_int2(edge_cell_l[e], edge_cell_r[e]);
-    h_nrm[e] = make_float2(edge_nx[e], edge_ny[e]);
-    h_dz[e] = make_float2(edge_dz[2 * e], edge_dz[2 * e + 1]);
-  }
+  a.N = N; a.NB = NB; a.max_local = max_local; a.max_slots = max_slots; a.max_ghost = max_ghost;
   HIPCHK(c, hipMemsetAsync(a.h, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hu, 0, NP * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.hv, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hlo, 0, NP * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.h1, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hu1, 0, NP * 4, c->stream));
@@ -1195,15 +1287,13 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   if ((rc = upload(c, sn, snb.data(), S))) return rc;
   if ((rc = upload(c, sl, slen.data(), S))) return rc;
   c->cross.swap(cross);
-  if ((rc = upload(c, ecell, h_ecell.data(), E))) return rc;
-  if ((rc = upload(c, enrm, h_nrm.data(), E))) return rc;
-  if ((rc = upload(c, elen, edge_len, E))) return rc;
-  if ((rc = upload(c, edz, h_dz.data(), E))) return rc;
+  if ((rc = upload(c, dlrec, lrec.data(), lrec.size()))) return rc;
+  if ((rc = upload(c, dldz, ldz.data(), ldz.size()))) return rc;
   if ((rc = upload(c, dbinfo, binfo.data(), NB))) return rc;
-  if ((rc = upload(c, dhx, hx.data(), hx.size()))) return rc;
+  if ((rc = upload(c, dgx, gxv.data(), gxv.size()))) return rc;
   if ((rc = upload(c, dbptr, bptr.data(), NB + 1))) return rc;
   if ((rc = upload(c, dbidx, bidx.data(), bidx.size()))) return rc;
-  c->lds_stage0 = (kLdsCellWords + (max_slots + 1) / 2 + 8 * (size_t)max_local) * sizeof(float);
+  c->lds_stage0 = lds_words(max_local, max_slots, max_ghost) * sizeof(float);
   c->lds_stage1 = c->lds_stage0;
   if (c->lds_stage1 > 160 * 1024) return fail(c, INU_ERANGE, "a block's edge list does not fit in LDS");
   int occ = 0;

@@ -238,10 +238,9 @@ __device__ __forceinline__ float vel_factor(const Phys& p, float h) {
   return (2.0f * h) / (hh + gmax(hh, p.e_film2));
 }
 
-__device__ __forceinline__ void hllc_speeds(const Phys& p, float hL, float unL, float hR, float unR,
-                                            float& SL, float& SR) {
-  const float sqL = sqrtf(gmax(hL, 0.0f));
-  const float sqR = sqrtf(gmax(hR, 0.0f));
+// sqL, sqR: sqrtf(gmax(hL, 0)), sqrtf(gmax(hR, 0)).
+__device__ __forceinline__ void hllc_speeds_sq(const Phys& p, float hL, float sqL, float unL, float hR, float sqR,
+                                               float unR, float& SL, float& SR) {
   const float cL = p.sqrt_g * sqL;
   const float cR = p.sqrt_g * sqR;
   const float den = sqL + sqR;
@@ -258,12 +257,18 @@ __device__ __forceinline__ void hllc_speeds(const Phys& p, float hL, float unL, 
   SR = fmaf(roe, sr, fmaf(dryLeft, unR + cR, dryRight * (unL + 2.0f * cL)));
 }
 
-// HLLC on face states whose velocities are already desingularised.
-__device__ __forceinline__ void hllc_vel(const Phys& p, float hL, float unL, float utL, float hR,
-                                         float unR, float utR, float& Fh, float& Fhn, float& Fht,
+__device__ __forceinline__ void hllc_speeds(const Phys& p, float hL, float unL, float hR, float unR,
+                                            float& SL, float& SR) {
+  fast::hllc_speeds_sq(p, hL, sqrtf(gmax(hL, 0.0f)), unL, hR, sqrtf(gmax(hR, 0.0f)), unR, SL, SR);
+}
+
+// HLLC on face states whose velocities are already desingularised; sqL, sqR
+// as for hllc_speeds_sq.
+__device__ __forceinline__ void hllc_vel(const Phys& p, float hL, float sqL, float unL, float utL, float hR,
+                                         float sqR, float unR, float utR, float& Fh, float& Fhn, float& Fht,
                                          float& Smax) {
   float SL, SR;
-  fast::hllc_speeds(p, hL, unL, hR, unR, SL, SR);
+  fast::hllc_speeds_sq(p, hL, sqL, unL, hR, sqR, unR, SL, SR);
   Smax = gmax(fabsf(SL), fabsf(SR));
   // Toro's contact speed, numerator regrouped as SL SR (hL - hR) +
   // (SL hR unR - SR hL unL): exactly zero for still water.
@@ -342,15 +347,40 @@ __device__ __forceinline__ void wall_ghost(float huL, float hvL, float nx, float
   hvR = fmaf(-q2, ny, hvL);
 }
 
-__device__ __forceinline__ void edge_words(const Phys& p, bool wall, float hL, float huL, float hvL,
-                                           float hR, float huR, float hvR, float dzL, float dzR,
-                                           float nx, float ny, float len, float loL, float loR,
-                                           float w[6]) {
-  const Face f = fast::face_states(p, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx, ny, loL, loR);
-  const float rL = vel_factor(p, f.hLs);
-  const float rR = vel_factor(p, f.hRs);
+// Cell quantities of one side of an edge: depth, momenta, compensated low
+// word, vel_factor(h) and sqrtf(gmax(h, 0)).
+struct Side {
+  float h, hu, hv, lo, vf, sq;
+};
+
+// The edge's six words. A side whose face does not step up keeps its cell
+// depth as face depth, so its vel_factor and square root are the cell's; at
+// most one side steps, and only its face depth needs new ones.
+__device__ __forceinline__ void edge_words(const Phys& p, bool wall, const Side& L, const Side& R, float dzL,
+                                           float dzR, float nx, float ny, float len, float w[6]) {
+  Face f;
+  f.hLs = (dzL == 0.0f) ? L.h : gmax(0.0f, R.h + (((L.h - R.h) - dzL) + (L.lo - R.lo)));
+  f.hRs = (dzR == 0.0f) ? R.h : gmax(0.0f, L.h + (((R.h - L.h) - dzR) + (R.lo - L.lo)));
+  const float rL0 = L.vf * f.hLs;
+  const float rR0 = R.vf * f.hRs;
+  const float huLs = L.hu * rL0, hvLs = L.hv * rL0;
+  const float huRs = R.hu * rR0, hvRs = R.hv * rR0;
+  f.qnL = fmaf(huLs, nx, hvLs * ny);
+  f.qtL = fmaf(hvLs, nx, -(huLs * ny));
+  f.qnR = fmaf(huRs, nx, hvRs * ny);
+  f.qtR = fmaf(hvRs, nx, -(huRs * ny));
+  const bool stepL = dzL != 0.0f;
+  const bool stepR = dzR != 0.0f;
+  const float hs = stepL ? f.hLs : f.hRs;
+  const float vfs = vel_factor(p, hs);
+  const float sqs = sqrtf(gmax(hs, 0.0f));
+  const float rL = stepL ? vfs : L.vf;
+  const float rR = stepR ? vfs : R.vf;
+  const float sqL = stepL ? sqs : L.sq;
+  const float sqR = stepR ? sqs : R.sq;
   float Fh, Fhn, Fht, Smax;
-  hllc_vel(p, f.hLs, f.qnL * rL, f.qtL * rL, f.hRs, f.qnR * rR, f.qtR * rR, Fh, Fhn, Fht, Smax);
+  hllc_vel(p, f.hLs, sqL, f.qnL * rL, f.qtL * rL, f.hRs, sqR, f.qnR * rR, f.qtR * rR, Fh, Fhn, Fht, Smax);
+  const float hL = L.h, hR = R.h;
   if (wall) Fh = 0.0f;
   // Normal momentum flux plus the hydrostatic correction of each side,
   // grouped as (Fhn - P(face depth)) + P(cell depth): still water gives
