@@ -140,19 +140,27 @@ __device__ __forceinline__ void block_atomic_min(uint32_t* dst, uint32_t bits) {
 // Stage kernels
 // ---------------------------------------------------------------------------
 
-// LDS of one block: stage input of its cells and of its ghost cells, its CSR
-// slots (local edge numbers) and the words of every edge touching it.
+// Element idx of a device array addressed as base + a 32-bit byte offset
+// (every array is below 4 GiB), so the address is a scalar base plus one
+// vector offset shared by the arrays read at the same index.
+template <typename T>
+__device__ __forceinline__ T ldg(const T* base, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (idx * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* base, uint32_t idx, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (idx * (uint32_t)sizeof(T))) = v;
+}
+
+// LDS of one block. Each cell and ghost cell has one 32-byte record: stage
+// input {h, hu, hv, lo} and, for the fast order, {vel_factor(h),
+// sqrt(max(h, 0)), g h^2 / 2, unused}. Each local edge has one 32-byte
+// record: its words {w0, w1, w2, w3}, {w4, w5, sides word, length}. Then the
+// block's CSR slots (local edge numbers).
 struct Lds {
-  float* s_h;  // kBlockCells + max_ghost entries each: the cells, then the ghosts
-  float* s_hu;
-  float* s_hv;
-  float* s_lo;
-  float* s_vf;  // fast order: vel_factor and sqrt of each cell and ghost depth
-  float* s_sq;
+  float4* cell;  // [kBlockCells + max_ghost][2]: the cells, then the ghosts
+  float4* edge;  // [max_local][2]
   uint16_t* s_slot;
-  float* w[6];
-  uint32_t* s_side;  // per local edge: its sides word (corrector with sources)
-  float* s_len;      // per local edge: length (corrector with sources)
 };
 
 // Local cell numbers of an edge's sides word ((left << 16) | right): a cell of
@@ -165,23 +173,14 @@ constexpr uint32_t kEdgesAhead = 2;  // edge records each thread loads before th
 constexpr uint32_t kSlotsAhead = 4;  // CSR slots each thread loads before the barrier
 
 __host__ __device__ inline size_t lds_words(uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
-  return 6 * (size_t)(kBlockCells + max_ghost) + (max_slots + 1) / 2 + 8 * (size_t)max_local;
+  return 8 * (size_t)(kBlockCells + max_ghost) + 8 * (size_t)max_local + (max_slots + 1) / 2;
 }
 
-__device__ __forceinline__ Lds carve(float* sm, uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
-  const uint32_t nc = kBlockCells + max_ghost;
+__device__ __forceinline__ Lds carve(float* sm, uint32_t max_local, uint32_t max_ghost) {
   Lds l;
-  l.s_h = sm;
-  l.s_hu = sm + nc;
-  l.s_hv = sm + 2 * nc;
-  l.s_lo = sm + 3 * nc;
-  l.s_vf = sm + 4 * nc;
-  l.s_sq = sm + 5 * nc;
-  l.s_slot = reinterpret_cast<uint16_t*>(sm + 6 * nc);
-  float* w = sm + 6 * nc + (max_slots + 1) / 2;
-  for (int k = 0; k < 6; ++k) l.w[k] = w + k * max_local;
-  l.s_side = reinterpret_cast<uint32_t*>(w + 6 * max_local);
-  l.s_len = w + 7 * max_local;
+  l.cell = reinterpret_cast<float4*>(sm);
+  l.edge = reinterpret_cast<float4*>(sm + 8 * (kBlockCells + max_ghost));
+  l.s_slot = reinterpret_cast<uint16_t*>(sm + 8 * (kBlockCells + max_ghost) + 8 * max_local);
   return l;
 }
 
@@ -193,13 +192,13 @@ struct EdgeRec {
 };
 
 __device__ __forceinline__ EdgeRec load_edge(const Args& a, const BlockInfo& bi, uint32_t j) {
-  const uint4 q = a.lrec[bi.e0 + j];
+  const uint4 q = ldg(a.lrec, bi.e0 + j);
   EdgeRec r;
   r.side = q.x;
   r.nx = __uint_as_float(q.y);
   r.ny = __uint_as_float(q.z);
   r.len = __uint_as_float(q.w);
-  r.d = a.ldz[bi.e0 + j];
+  r.d = ldg(a.ldz, bi.e0 + j);
   return r;
 }
 
@@ -208,103 +207,67 @@ __device__ __forceinline__ EdgeRec load_edge(const Args& a, const BlockInfo& bi,
 // (reference resident.py:209-226); it loads that state first and the start
 // state only for a skipped block.
 template <int ST, bool LO>
-__device__ __forceinline__ void load_ghost(const Args& a, uint32_t g, float& h, float& hu, float& hv, float& lo) {
+__device__ __forceinline__ float4 load_ghost(const Args& a, uint32_t g) {
   if (ST == 1) {
-    const uint32_t act = a.active[g >> 8];
-    h = a.h1[g];
-    hu = a.hu1[g];
-    hv = a.hv1[g];
-    lo = LO ? a.h1lo[g] : 0.0f;
-    if (act != 0u) return;
+    const uint32_t act = ldg(a.active, g >> 8);
+    const float4 s = make_float4(ldg(a.h1, g), ldg(a.hu1, g), ldg(a.hv1, g), LO ? ldg(a.h1lo, g) : 0.0f);
+    if (act != 0u) return s;
   }
-  h = a.h[g];
-  hu = a.hu[g];
-  hv = a.hv[g];
-  lo = LO ? a.hlo[g] : 0.0f;
+  return make_float4(ldg(a.h, g), ldg(a.hu, g), ldg(a.hv, g), LO ? ldg(a.hlo, g) : 0.0f);
+}
+
+// Stage input of one cell or ghost into LDS record q, with the fast order's
+// per-depth vel_factor, square root and hydrostatic pressure.
+template <bool FM>
+__device__ __forceinline__ void put_cell(const Lds& l, const Phys& p, uint32_t q, float4 s) {
+  l.cell[2 * q] = s;
+  if (FM) l.cell[2 * q + 1] = make_float4(inu::fast::vel_factor(p, s.x), sqrtf(inu::gmax(s.x, 0.0f)),
+                                          p.half_g * s.x * s.x, 0.0f);
 }
 
 // Words of local edge j into LDS (zeros when both sides are dry); both sides
 // are read from LDS.
 template <int ST, bool FM, bool LO>
 __device__ __forceinline__ void edge_eval(const Args& a, const Lds& l, const EdgeRec& r, uint32_t j) {
-  if (ST == 1 && a.n_src != 0u) {
-    l.s_side[j] = r.side;
-    l.s_len[j] = r.len;
-  }
   const uint32_t ql = r.side >> 16;
   const uint32_t qr = r.side & 0xffffu;
   const bool wall = qr == kSideWall;
   const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
   const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
   float w[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  if (FM) {
-    inu::fast::Side L, R;
-    L.h = l.s_h[ql];
-    L.hu = l.s_hu[ql];
-    L.hv = l.s_hv[ql];
-    L.lo = LO ? l.s_lo[ql] : 0.0f;
-    L.vf = l.s_vf[ql];
-    L.sq = l.s_sq[ql];
-    if (wall) {
-      R = L;
-      inu::fast::wall_ghost(L.hu, L.hv, r.nx, r.ny, R.hu, R.hv);
-    } else {
-      R.h = l.s_h[qr];
-      R.hu = l.s_hu[qr];
-      R.hv = l.s_hv[qr];
-      R.lo = LO ? l.s_lo[qr] : 0.0f;
-      R.vf = l.s_vf[qr];
-      R.sq = l.s_sq[qr];
-    }
-    if (!(L.h < a.p.dry && R.h < a.p.dry)) inu::fast::edge_words(a.p, wall, L, R, dzL, dzR, r.nx, r.ny, r.len, w);
+  const float4 sl = l.cell[2 * ql];
+  float4 sr;
+  if (wall) {
+    sr = sl;
+    if (FM) inu::fast::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
+    else inu::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
   } else {
-    const float hL = l.s_h[ql];
-    const float huL = l.s_hu[ql];
-    const float hvL = l.s_hv[ql];
-    float hR, huR, hvR;
-    if (wall) {
-      hR = hL;
-      inu::wall_ghost(huL, hvL, r.nx, r.ny, huR, hvR);
+    sr = l.cell[2 * qr];
+  }
+  if (!(sl.x < a.p.dry && sr.x < a.p.dry)) {
+    if (FM) {
+      const float4 xl = l.cell[2 * ql + 1];
+      const float4 xr = wall ? xl : l.cell[2 * qr + 1];
+      const inu::fast::Side L{sl.x, sl.y, sl.z, LO ? sl.w : 0.0f, xl.x, xl.y, xl.z};
+      const inu::fast::Side R{sr.x, sr.y, sr.z, LO ? sr.w : 0.0f, xr.x, xr.y, xr.z};
+      inu::fast::edge_words(a.p, wall, L, R, dzL, dzR, r.nx, r.ny, r.len, w);
     } else {
-      hR = l.s_h[qr];
-      huR = l.s_hu[qr];
-      hvR = l.s_hv[qr];
+      inu::edge_words(a.p, wall, sl.x, sl.y, sl.z, sr.x, sr.y, sr.z, dzL, dzR, r.nx, r.ny, r.len, w);
     }
-    if (!(hL < a.p.dry && hR < a.p.dry))
-      inu::edge_words(a.p, wall, hL, huL, hvL, hR, huR, hvR, dzL, dzR, r.nx, r.ny, r.len, w);
   }
-  l.w[0][j] = w[0];
-  l.w[1][j] = w[1];
-  l.w[2][j] = w[2];
-  l.w[3][j] = w[3];
-  l.w[4][j] = w[4];
-  if (ST == 1) l.w[5][j] = w[5];
-}
-
-// Stage input of one cell or ghost into LDS slot q, with the fast order's
-// per-depth vel_factor and square root.
-template <bool FM>
-__device__ __forceinline__ void put_cell(const Lds& l, const Phys& p, uint32_t q, float h, float hu, float hv,
-                                         float lo) {
-  l.s_h[q] = h;
-  l.s_hu[q] = hu;
-  l.s_hv[q] = hv;
-  l.s_lo[q] = lo;
-  if (FM) {
-    l.s_vf[q] = inu::fast::vel_factor(p, h);
-    l.s_sq[q] = sqrtf(inu::gmax(h, 0.0f));
-  }
+  l.edge[2 * j] = make_float4(w[0], w[1], w[2], w[3]);
+  l.edge[2 * j + 1] = make_float4(w[4], w[5], __uint_as_float(r.side), r.len);
 }
 
 // The next blocks of a workgroup's walk over the list: the block after the
 // current one with its BlockInfo and this thread's ghost cell id, and the
 // list entry after that. Loaded while the current block is processed.
 struct Ahead {
-  uint32_t t;         // list position of the next block (>= end: none)
-  uint32_t b;         // next block
-  BlockInfo bi;       // its BlockInfo
-  uint32_t gid;       // its ghost cell id of this thread
-  uint32_t b2;        // block after it
+  uint32_t t;    // list position of the next block (>= end: none)
+  uint32_t b;    // next block
+  BlockInfo bi;  // its BlockInfo
+  uint32_t gid;  // its ghost cell id of this thread
+  uint32_t b2;   // block after it
 };
 
 // One active block of one Heun stage (reference resident.py:425-512, gather +
@@ -331,45 +294,43 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   uint32_t sl[kSlotsAhead];
 #pragma unroll
   for (uint32_t m = 0; m < kSlotsAhead; ++m)
-    if (tid + m * kBlockCells < bi.n_slots) sl[m] = a.lidx[bi.s0 + tid + m * kBlockCells];
-  float gh = 0.0f, ghu = 0.0f, ghv = 0.0f, glo = 0.0f;
-  if (tid < bi.n_ghost) load_ghost<ST, LO>(a, gid, gh, ghu, ghv, glo);
+    if (tid + m * kBlockCells < bi.n_slots) sl[m] = ldg(a.lidx, bi.s0 + tid + m * kBlockCells);
+  float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (tid < bi.n_ghost) gs = load_ghost<ST, LO>(a, gid);
   const uint32_t i = c0 + tid;
   float hi = 0.0f, hui = 0.0f, hvi = 0.0f, loi = 0.0f, A = 1.0f, nm = 0.0f;
   float h0 = 0.0f, hu0 = 0.0f, hv0 = 0.0f, lo0 = 0.0f, rate = 0.0f;
   uint32_t k0 = 0, k1 = 0;
   if (tid < nc) {
-    hi = (ST == 0) ? a.h[i] : a.h1[i];
-    hui = (ST == 0) ? a.hu[i] : a.hu1[i];
-    hvi = (ST == 0) ? a.hv[i] : a.hv1[i];
-    if (COMP) loi = (ST == 0) ? a.hlo[i] : a.h1lo[i];
-    k0 = a.cptr[i] - bi.s0;
-    k1 = a.cptr[i + 1] - bi.s0;
-    A = a.area[i];
-    nm = a.nman[i];
+    hi = ldg((ST == 0) ? a.h : a.h1, i);
+    hui = ldg((ST == 0) ? a.hu : a.hu1, i);
+    hvi = ldg((ST == 0) ? a.hv : a.hv1, i);
+    if (COMP) loi = ldg((ST == 0) ? a.hlo : a.h1lo, i);
+    k0 = ldg(a.cptr, i) - bi.s0;
+    k1 = ldg(a.cptr, i + 1) - bi.s0;
+    A = ldg(a.area, i);
+    nm = ldg(a.nman, i);
     if (ST == 1) {
-      h0 = a.h[i];
-      hu0 = a.hu[i];
-      hv0 = a.hv[i];
-      if (COMP) lo0 = a.hlo[i];
-      if (a.n_src != 0u) rate = a.src[i];
+      h0 = ldg(a.h, i);
+      hu0 = ldg(a.hu, i);
+      hv0 = ldg(a.hv, i);
+      if (COMP) lo0 = ldg(a.hlo, i);
+      if (a.n_src != 0u) rate = ldg(a.src, i);
     }
   }
   // Walk ahead: the next block's BlockInfo and the list entry after it.
   if (nx.t < end) nx.bi = a.binfo[nx.b];
   if (nx.t + per < end) nx.b2 = a.list[nx.t + per];
   __syncthreads();  // the previous block's LDS reads are done
-  if (tid < nc) put_cell<FM>(l, p, tid, hi, hui, hvi, loi);
-  if (tid < bi.n_ghost) put_cell<FM>(l, p, kBlockCells + tid, gh, ghu, ghv, glo);
-  for (uint32_t g = tid + kBlockCells; g < bi.n_ghost; g += kBlockCells) {
-    float x, y, z, w;
-    load_ghost<ST, LO>(a, a.gx[bi.g0 + g], x, y, z, w);
-    put_cell<FM>(l, p, kBlockCells + g, x, y, z, w);
-  }
+  if (tid < nc) put_cell<FM>(l, p, tid, make_float4(hi, hui, hvi, loi));
+  if (tid < bi.n_ghost) put_cell<FM>(l, p, kBlockCells + tid, gs);
+  for (uint32_t g = tid + kBlockCells; g < bi.n_ghost; g += kBlockCells)
+    put_cell<FM>(l, p, kBlockCells + g, load_ghost<ST, LO>(a, ldg(a.gx, bi.g0 + g)));
 #pragma unroll
   for (uint32_t m = 0; m < kSlotsAhead; ++m)
     if (tid + m * kBlockCells < bi.n_slots) l.s_slot[tid + m * kBlockCells] = (uint16_t)sl[m];
-  for (uint32_t k = tid + kSlotsAhead * kBlockCells; k < bi.n_slots; k += kBlockCells) l.s_slot[k] = a.lidx[bi.s0 + k];
+  for (uint32_t k = tid + kSlotsAhead * kBlockCells; k < bi.n_slots; k += kBlockCells)
+    l.s_slot[k] = ldg(a.lidx, bi.s0 + k);
   __syncthreads();
 
   // ---- (B) edges -----------------------------------------------------------
@@ -379,7 +340,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   for (uint32_t j = tid + kEdgesAhead * kBlockCells; j < n_loc; j += kBlockCells)
     edge_eval<ST, FM, LO>(a, l, load_edge(a, bi, j), j);
   // Walk ahead: this thread's ghost cell id of the next block.
-  if (nx.t < end && tid < nx.bi.n_ghost) nx.gid = a.gx[nx.bi.g0 + tid];
+  if (nx.t < end && tid < nx.bi.n_ghost) nx.gid = ldg(a.gx, nx.bi.g0 + tid);
   __syncthreads();
 
   // ---- (C) cells -----------------------------------------------------------
@@ -390,16 +351,18 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     for (uint32_t k = k0; k < k1; ++k) {
       const uint32_t s = l.s_slot[k];
       const uint32_t j = s >> 1;
+      const float4 wa = l.edge[2 * j];
+      const float2 wb = *reinterpret_cast<const float2*>(&l.edge[2 * j + 1]);
       if (s & 1u) {
-        sdh -= l.w[0][j];
-        sdhu -= l.w[3][j];
-        sdhv -= l.w[4][j];
+        sdh -= wa.x;
+        sdhu -= wa.w;
+        sdhv -= wb.x;
       } else {
-        sdh += l.w[0][j];
-        sdhu += l.w[1][j];
-        sdhv += l.w[2][j];
+        sdh += wa.x;
+        sdhu += wa.y;
+        sdhv += wa.z;
       }
-      if (ST == 1) lam += l.w[5][j];
+      if (ST == 1) lam += wb.y;
     }
     const float dti = dt / A;
     float h_new, lo_new = 0.0f, h_raw;
@@ -414,7 +377,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     }
     float hu_new = FM ? fmaf(-dti, sdhu, hui) : hui - dti * sdhu;
     float hv_new = FM ? fmaf(-dti, sdhv, hvi) : hvi - dti * sdhv;
-    if (a.track_clamp) a.clamp[i] += 0.5f * A * (h_new - h_raw);
+    if (a.track_clamp) stg(a.clamp, i, ldg(a.clamp, i) + 0.5f * A * (h_new - h_raw));
     if (h_new < p.dry) {
       hu_new = 0.0f;
       hv_new = 0.0f;
@@ -434,10 +397,10 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       hv_new = hv_new / den;
     }
     if (ST == 0) {
-      a.h1[i] = h_new;
-      if (COMP) a.h1lo[i] = lo_new;
-      a.hu1[i] = hu_new;
-      a.hv1[i] = hv_new;
+      stg(a.h1, i, h_new);
+      if (COMP) stg(a.h1lo, i, lo_new);
+      stg(a.hu1, i, hu_new);
+      stg(a.hv1, i, hv_new);
     } else {
       float hf, lof = 0.0f;
       if (COMP) {
@@ -470,10 +433,10 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
         }
         wet = wet || hf >= p.dry;
       }
-      a.h[i] = hf;
-      if (COMP) a.hlo[i] = lof;
-      a.hu[i] = huf;
-      a.hv[i] = hvf;
+      stg(a.h, i, hf);
+      if (COMP) stg(a.hlo, i, lof);
+      stg(a.hu, i, huf);
+      stg(a.hv, i, hvf);
       hi = hf;  // post-source depth, for the source dt bound below
       hui = huf;
       hvi = hvf;
@@ -487,7 +450,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       // (resident.py:709-728) from the post-source depths, with every
       // neighbour in another block taken as wet; k_source_fix recomputes the
       // cells where such a neighbour is dry.
-      if (tid < nc) l.s_h[tid] = hi;
+      if (tid < nc) l.cell[2 * tid].x = hi;
       __syncthreads();
       if (tid < nc && rate != 0.0f && hi >= p.dry) {
         const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hi : sqrtf(hui * hui + hvi * hvi) / hi;
@@ -496,12 +459,13 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
         for (uint32_t k = k0; k < k1; ++k) {
           const uint32_t s = l.s_slot[k];
           const uint32_t j = s >> 1;
-          const uint32_t pk = l.s_side[j];
+          const float2 sd = reinterpret_cast<const float2*>(&l.edge[2 * j + 1])[1];  // sides, length
+          const uint32_t pk = __float_as_uint(sd.x);
           const uint32_t q = (s & 1u) ? (pk >> 16) : (pk & 0xffffu);
-          const bool dry_side = q < kBlockCells && l.s_h[q] < p.dry;  // walls and ghosts: wet
+          const bool dry_side = q < kBlockCells && l.cell[2 * q].x < p.dry;  // walls and ghosts: wet
           const float kf = dry_side ? 2.0f : 1.0f;
-          if (FM) lam = fmaf(fmaf(kf, c, u), l.s_len[j], lam);
-          else lam += (u + kf * c) * l.s_len[j];
+          if (FM) lam = fmaf(fmaf(kf, c, u), sd.y, lam);
+          else lam += (u + kf * c) * sd.y;
         }
         if (lam > 1e-30f && A > 1e-10f) {
           const uint32_t bb = __float_as_uint(p.cfl * A / lam);
@@ -519,7 +483,7 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
   const uint32_t dtw = a.ctl->dtbuf[0];
   if (dtw == 0u) return;  // no-op step: the window's stop was reached
   const float dt = __uint_as_float(dtw);
-  const Lds l = carve(sm, a.max_local, a.max_slots, a.max_ghost);
+  const Lds l = carve(sm, a.max_local, a.max_ghost);
   // XCD-aware split: workgroup w runs on XCD w % 8, so each XCD walks one
   // contiguous (spatially compact) slice of the block list and its L2 keeps
   // the ghost cells its neighbours read.
@@ -761,7 +725,7 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
 template <bool FM, bool LO>
 __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
   extern __shared__ float sm[];
-  const Lds l = carve(sm, a.max_local, a.max_slots, a.max_ghost);
+  const Lds l = carve(sm, a.max_local, a.max_ghost);
   const Phys& p = a.p;
   const uint32_t tid = threadIdx.x;
   uint32_t best = kNoBound;
@@ -770,57 +734,40 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
     const uint32_t nc = min(kBlockCells, a.N - c0);
     const BlockInfo bi = a.binfo[b];
     __syncthreads();
-    if (tid < nc) {
-      l.s_h[tid] = a.h[c0 + tid];
-      l.s_hu[tid] = a.hu[c0 + tid];
-      l.s_hv[tid] = a.hv[c0 + tid];
-      l.s_lo[tid] = LO ? a.hlo[c0 + tid] : 0.0f;
-    }
-    for (uint32_t g = tid; g < bi.n_ghost; g += kBlockCells) {
-      float x, y, z, w;
-      load_ghost<0, LO>(a, a.gx[bi.g0 + g], x, y, z, w);
-      l.s_h[kBlockCells + g] = x;
-      l.s_hu[kBlockCells + g] = y;
-      l.s_hv[kBlockCells + g] = z;
-      l.s_lo[kBlockCells + g] = w;
-    }
+    if (tid < nc)
+      l.cell[2 * tid] = make_float4(a.h[c0 + tid], a.hu[c0 + tid], a.hv[c0 + tid], LO ? a.hlo[c0 + tid] : 0.0f);
+    for (uint32_t g = tid; g < bi.n_ghost; g += kBlockCells)
+      l.cell[2 * (kBlockCells + g)] = load_ghost<0, LO>(a, a.gx[bi.g0 + g]);
     __syncthreads();
     for (uint32_t j = tid; j < bi.n_loc; j += kBlockCells) {
       const EdgeRec r = load_edge(a, bi, j);
       const uint32_t ql = r.side >> 16;
       const uint32_t qr = r.side & 0xffffu;
-      const float hL = l.s_h[ql];
-      const float huL = l.s_hu[ql];
-      const float hvL = l.s_hv[ql];
-      const float loL = l.s_lo[ql];
-      float hR, huR, hvR, loR;
+      const float4 sl = l.cell[2 * ql];
+      float4 sr;
       if (qr == kSideWall) {
-        hR = hL;
-        loR = loL;
-        if (FM) inu::fast::wall_ghost(huL, hvL, r.nx, r.ny, huR, hvR);
-        else inu::wall_ghost(huL, hvL, r.nx, r.ny, huR, hvR);
+        sr = sl;
+        if (FM) inu::fast::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
+        else inu::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
       } else {
-        hR = l.s_h[qr];
-        huR = l.s_hu[qr];
-        hvR = l.s_hv[qr];
-        loR = l.s_lo[qr];
+        sr = l.cell[2 * qr];
       }
       const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
       const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
       float contrib = 0.0f;
-      if (hL >= p.dry || hR >= p.dry) {
-        contrib = (FM ? inu::fast::wave_bound(p, hL, huL, hvL, dzL, hR, huR, hvR, dzR, r.nx, r.ny, loL, loR)
-                      : inu::wave_bound(p, hL, huL, hvL, dzL, hR, huR, hvR, dzR, r.nx, r.ny)) *
+      if (sl.x >= p.dry || sr.x >= p.dry) {
+        contrib = (FM ? inu::fast::wave_bound(p, sl.x, sl.y, sl.z, dzL, sr.x, sr.y, sr.z, dzR, r.nx, r.ny, sl.w, sr.w)
+                      : inu::wave_bound(p, sl.x, sl.y, sl.z, dzL, sr.x, sr.y, sr.z, dzR, r.nx, r.ny)) *
                   r.len;
       }
-      l.w[5][j] = contrib;
+      l.edge[2 * j].x = contrib;
     }
     __syncthreads();
     uint32_t bound = kNoBound;
     if (tid < nc) {
       const uint32_t i = c0 + tid;
       float lam = 0.0f;
-      for (uint32_t k = a.cptr[i]; k < a.cptr[i + 1]; ++k) lam += l.w[5][a.lidx[k] >> 1];
+      for (uint32_t k = a.cptr[i]; k < a.cptr[i + 1]; ++k) lam += l.edge[2 * (a.lidx[k] >> 1)].x;
       const float A = a.area[i];
       if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
     }

@@ -348,9 +348,9 @@ __device__ __forceinline__ void wall_ghost(float huL, float hvL, float nx, float
 }
 
 // Cell quantities of one side of an edge: depth, momenta, compensated low
-// word, vel_factor(h) and sqrtf(gmax(h, 0)).
+// word, vel_factor(h), sqrtf(gmax(h, 0)) and the pressure half_g * h * h.
 struct Side {
-  float h, hu, hv, lo, vf, sq;
+  float h, hu, hv, lo, vf, sq, pc;
 };
 
 // The edge's six words. A side whose face does not step up keeps its cell
@@ -380,13 +380,12 @@ __device__ __forceinline__ void edge_words(const Phys& p, bool wall, const Side&
   const float sqR = stepR ? sqs : R.sq;
   float Fh, Fhn, Fht, Smax;
   hllc_vel(p, f.hLs, sqL, f.qnL * rL, f.qtL * rL, f.hRs, sqR, f.qnR * rR, f.qtR * rR, Fh, Fhn, Fht, Smax);
-  const float hL = L.h, hR = R.h;
   if (wall) Fh = 0.0f;
   // Normal momentum flux plus the hydrostatic correction of each side,
   // grouped as (Fhn - P(face depth)) + P(cell depth): still water gives
   // exactly the cell's own pressure on every edge.
-  const float nL = (Fhn - p.half_g * f.hLs * f.hLs) + p.half_g * hL * hL;
-  const float nR = (Fhn - p.half_g * f.hRs * f.hRs) + p.half_g * hR * hR;
+  const float nL = (Fhn - p.half_g * f.hLs * f.hLs) + L.pc;
+  const float nR = (Fhn - p.half_g * f.hRs * f.hRs) + R.pc;
   const float tx = Fht * nx, ty = Fht * ny;
   w[0] = Fh * len;
   w[1] = fmaf(nL, nx, -ty) * len;
