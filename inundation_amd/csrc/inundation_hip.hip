@@ -882,7 +882,8 @@ struct inu_ctx {
   Args a{};
   Ctl* ctl = nullptr;
   // launch shape
-  uint32_t grid_stage = 0;
+  uint32_t grid_stage = 0;        // default corrector grid (reported by inu_mesh_info)
+  uint32_t grid[2][2][2] = {};    // [stage][compensated][fast order]
   size_t lds_stage0 = 0, lds_stage1 = 0;
   // (cell, neighbour) for every neighbour in another block, by cell
   std::vector<std::pair<uint32_t, uint32_t>> cross;
@@ -961,7 +962,7 @@ int check_ready(inu_ctx* c) {
 template <int ST>
 void launch_stage(inu_ctx* c, bool comp, bool fm) {
   const size_t lds = ST == 0 ? c->lds_stage0 : c->lds_stage1;
-  const dim3 g(c->grid_stage), b(256);
+  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0]), b(256);
   if (comp && fm) hipLaunchKernelGGL((k_stage<ST, true, true>), g, b, lds, c->stream, c->a);
   else if (comp) hipLaunchKernelGGL((k_stage<ST, true, false>), g, b, lds, c->stream, c->a);
   else if (fm) hipLaunchKernelGGL((k_stage<ST, false, true>), g, b, lds, c->stream, c->a);
@@ -1243,12 +1244,24 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   c->lds_stage0 = lds_words(max_local, max_slots, max_ghost) * sizeof(float);
   c->lds_stage1 = c->lds_stage0;
   if (c->lds_stage1 > 160 * 1024) return fail(c, INU_ERANGE, "a block's edge list does not fit in LDS");
-  int occ = 0;
-  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_stage<1, true, true>, 256, c->lds_stage1));
-  occ = std::max(occ, 1);
-  uint32_t g = (uint32_t)(occ * c->cus);
-  g = std::max<uint32_t>(kXcds, std::min<uint32_t>(g, ((NB + kXcds - 1) / kXcds) * kXcds));
-  c->grid_stage = (g / kXcds) * kXcds;
+  // Persistent grids: as many workgroups as fit at once, a multiple of the
+  // XCD count, per stage kernel variant.
+  const void* variants[2][2][2] = {
+      {{(const void*)k_stage<0, false, false>, (const void*)k_stage<0, false, true>},
+       {(const void*)k_stage<0, true, false>, (const void*)k_stage<0, true, true>}},
+      {{(const void*)k_stage<1, false, false>, (const void*)k_stage<1, false, true>},
+       {(const void*)k_stage<1, true, false>, (const void*)k_stage<1, true, true>}}};
+  for (int st = 0; st < 2; ++st)
+    for (int cp = 0; cp < 2; ++cp)
+      for (int fm = 0; fm < 2; ++fm) {
+        int occ = 0;
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, variants[st][cp][fm], 256,
+                                                                st ? c->lds_stage1 : c->lds_stage0));
+        uint32_t g = (uint32_t)(std::max(occ, 1) * c->cus);
+        g = std::max<uint32_t>(kXcds, std::min<uint32_t>(g, ((NB + kXcds - 1) / kXcds) * kXcds));
+        c->grid[st][cp][fm] = (g / kXcds) * kXcds;
+      }
+  c->grid_stage = c->grid[1][1][1];
   c->a.p = c->p;
   c->has_mesh = true;
   return 0;

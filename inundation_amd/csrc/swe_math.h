@@ -292,7 +292,11 @@ __device__ __forceinline__ void hllc_vel(const Phys& p, float hL, float sqL, flo
   // Star depth minus side depth, hS (SM - unS) / (S - SM): zero when the
   // contact does not move relative to the side.
   const float dS = S - SM;
-  const float ks_h = (fabsf(dS) > 1e-14f) ? hS * (SM - unS) / dS : hS * (S - unS) / 1e-14f - hS;
+  // One division for both cases: hS (SM - unS) / dS, or hS (S - unS) / 1e-14
+  // - hS when the star region is degenerate.
+  const bool dS_ok = fabsf(dS) > 1e-14f;
+  const float ks_q = (hS * (dS_ok ? SM - unS : S - unS)) / (dS_ok ? dS : 1e-14f);
+  const float ks_h = dS_ok ? ks_q : ks_q - hS;
   const float kS = hS + ks_h;
   if (SL >= 0.0f) {
     Fh = FL_h; Fhn = FL_hn; Fht = FL_ht;
