@@ -35,6 +35,16 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-steps/s & %HBM roofline: plain vs batched vs adjoint on 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per corrector launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (each scaled by the counters' reading for a 1 GiB plain copy) on
+# the same workload: profiles/k_stage_rainfall16M_traffic.md. Re-measured
+# whenever the stage kernels change.
+CORRECTOR_TRAFFIC = {
+    "bytes_per_launch": 1.732e9,
+    "traffic_over_algorithmic": 0.977,
+    "source": "profiles/k_stage_rainfall16M_traffic.md: mean of 300 launches from the dry start, "
+    "95.0 % of cell-steps visited",
+}
 
 
 def log(msg: str) -> None:
@@ -186,7 +196,9 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": corr_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": CORRECTOR_TRAFFIC["bytes_per_launch"],
+            "traffic_over_algorithmic": CORRECTOR_TRAFFIC["traffic_over_algorithmic"],
+            "traffic_source": CORRECTOR_TRAFFIC["source"],
             "ms_per_launch": ms_corr / args.steps,
             "attainable_copy_GBs": copy_gbs,
             "algorithmic_bytes_per_launch": corr_bytes / args.steps,
