@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define INU_ABI_VERSION 1
+#define INU_ABI_VERSION 2
 
 /* Error codes. INU_EINVAL maps to ValueError on the Python side, the others to
  * RuntimeError (reference raises: gpu_resident_batch.py:641-655, 720-724). */
@@ -79,6 +79,21 @@ typedef struct inu_ctrl {
   uint32_t pad;
 } inu_ctrl;
 
+/* One mesh, as the arguments of inu_set_mesh (for inu_set_batch). */
+typedef struct inu_mesh {
+  uint32_t n_cells;
+  uint32_t n_edges;
+  const int32_t* edge_cell_l;
+  const int32_t* edge_cell_r;
+  const float* nx;
+  const float* ny;
+  const float* len;
+  const float* edge_dz;
+  const float* cell_area;
+  const uint32_t* cell_edge_ptr;
+  const uint32_t* cell_edge_idx;
+} inu_mesh;
+
 typedef struct inu_ctx inu_ctx;
 
 /* ABI version of the loaded library (INU_ABI_VERSION). No reference counterpart. */
@@ -101,8 +116,27 @@ int inu_set_mesh(inu_ctx* ctx, uint32_t n_cells, uint32_t n_edges, const int32_t
                  const float* edge_len, const float* edge_dz, const float* cell_area,
                  const uint32_t* cell_edge_ptr, const uint32_t* cell_edge_idx);
 
-/* out6 = {N, E, slots, blocks, max edges per block, stage grid size}. No
- * reference counterpart (diagnostics). */
+/* Batched scenarios (BASELINE batching extension; no reference counterpart:
+ * the reference runs one scenario per solver). n_scenarios scenarios, scenario
+ * s on mesh scenario_mesh[s] of the n_meshes meshes, each with its own state,
+ * sources, control block and CFL dt, all advanced by the same launches.
+ * Every cell-indexed array of the other calls then holds the scenarios' cells
+ * back to back (scenario 0's cells first; inu_scenario_info gives the
+ * ranges); block-indexed arrays hold the scenarios' blocks back to back;
+ * inu_ctrl_write/read and inu_dt_words take one entry per scenario. A
+ * scenario's results are bit-identical to running it alone. inu_set_mesh is
+ * the one-scenario case. */
+int inu_set_batch(inu_ctx* ctx, uint32_t n_meshes, const inu_mesh* meshes, uint32_t n_scenarios,
+                  const uint32_t* scenario_mesh);
+
+/* Number of scenarios (>= 1 once a mesh is set), or a negative INU_E* code. */
+int inu_scenario_count(inu_ctx* ctx);
+
+/* out4 = {first cell, cells, first block, blocks} of scenario s. */
+int inu_scenario_info(inu_ctx* ctx, uint32_t s, uint32_t* out4);
+
+/* out6 = {N, E, slots, blocks, max edges per block, stage grid size}, summed
+ * over the scenarios. No reference counterpart (diagnostics). */
 int inu_mesh_info(inu_ctx* ctx, uint32_t* out6);
 
 /* g, dry_tol, CFL number and INU_FLAG_* (reference constructor arguments,
@@ -120,13 +154,20 @@ int inu_set_manning(inu_ctx* ctx, const float* n);
  * replaces _set_sources (gpu_resident_batch.py:529-560). */
 int inu_set_sources(inu_ctx* ctx, uint32_t n, const uint32_t* cell, const float* rate);
 
-/* Control block up/down; replaces _sync_ctrl_to_device/_read_ctrl
- * (gpu_resident_batch.py:483-487). */
+/* New rates for cells already in the source set (a zero rate turns a source
+ * off until a later call turns it on again); the cell set, its block lists
+ * and the recorded graphs stay. Replaces the per-phase source re-upload of
+ * the reference's run loop (gpu_resident_batch.py:529-560, 666-670) when only
+ * the rates change between phases. */
+int inu_set_source_rates(inu_ctx* ctx, uint32_t n, const uint32_t* cell, const float* rate);
+
+/* Control block up/down, one per scenario; replaces
+ * _sync_ctrl_to_device/_read_ctrl (gpu_resident_batch.py:483-487). */
 int inu_ctrl_write(inu_ctx* ctx, const inu_ctrl* ctrl);
 int inu_ctrl_read(inu_ctx* ctx, inu_ctrl* ctrl);
 
-/* The dt words: [0] this step's dt bits, [1] next-step CFL bound bits
- * (binding 17, resident.py:6-11). */
+/* The dt words, four per scenario: [0] this step's dt bits, [1] next-step CFL
+ * bound bits, [2], [3] zero (binding 17, resident.py:6-11). */
 int inu_dt_words(inu_ctx* ctx, uint32_t* out4);
 
 /* Start of a run: block marks, forced dilation, CFL bound of the current
@@ -175,6 +216,18 @@ int inu_probe(inu_ctx* ctx, uint32_t n, const float* pairs, float* out7, float g
  * against, and the attainable copy bandwidth. Allocates and frees its own
  * buffers. */
 int inu_stream_copy(inu_ctx* ctx, uint64_t n_bytes, uint32_t reps, float* ms_out);
+
+/* Autotuner of the stage kernels' tiles (BASELINE batching extension; no
+ * reference counterpart). Times `reps` launches of the predictor at each tile
+ * level on the current state (after inu_prologue; the state is left as it
+ * was) and keeps the fastest for both stages: out4 = {chosen level,
+ * microseconds per launch at levels 0, 1, 2}. A level is the number of edge
+ * records and CSR slots each thread loads before the first barrier: (2, 4),
+ * (3, 6), (4, 8). Results do not depend on the level. */
+int inu_autotune(inu_ctx* ctx, uint32_t reps, uint32_t* out4);
+
+/* Sets the tile level (0, 1 or 2) directly; inu_set_mesh/inu_set_batch reset it to 0. */
+int inu_set_tile(inu_ctx* ctx, uint32_t tile);
 
 #ifdef __cplusplus
 }

@@ -49,16 +49,25 @@ constexpr float kDtSentinel = 1e30f;
 constexpr uint32_t kNoBound = 0x7f800000u;  // +inf bits: larger than any bound
 constexpr uint32_t kProbeOut = 7;           // outputs per inu_probe pair
 
-// Control words shared by the step kernels. The first 48 bytes are the
-// reference's resident control block (gpu_resident_batch.py:65-77).
-struct Ctl {
+// Clock and dt words of one scenario: the reference's 48-byte resident
+// control block (gpu_resident_batch.py:65-77) and its dt_buf slots 0 and 1.
+struct Scen {
   inu_ctrl ctrl;
-  uint32_t dtbuf[4];  // [0] this step's dt bits (0 = no-op), [1] next-step bound
-  uint32_t cnt_cur;   // blocks in `list` for the current step
-  uint32_t cnt_next;  // blocks appended by k_dilate for the next step
-  uint32_t swap_on;   // 1 when k_dilate runs every step (active set on)
-  uint32_t idle_cur;  // inactive source blocks in `idle` for the current step
-  uint32_t idle_next; // inactive source blocks appended for the next step
+  uint32_t dt_cur;   // this step's dt bits (0: no-op step); the prologue bound before the first
+  uint32_t dt_next;  // next-step bound bits (atomicMin)
+  uint32_t pad[2];
+};
+static_assert(sizeof(Scen) == 64, "scenario record must be 64 bytes");
+
+// Control words shared by the step kernels and every scenario.
+struct Ctl {
+  uint32_t cnt_cur;    // blocks in `list` for the current step
+  uint32_t cnt_next;   // blocks appended by k_dilate for the next step
+  uint32_t swap_on;    // 1 when k_dilate runs every step (active set on)
+  uint32_t idle_cur;   // inactive source blocks in `idle` for the current step
+  uint32_t idle_next;  // inactive source blocks appended for the next step
+  uint32_t any_dt;     // 1 when some scenario has a nonzero dt this step
+  uint32_t pad[2];
   // Work of the active set, for accounting: {cells, owned edges, CSR slots}
   // of the active blocks of the next step, of the current step, and summed
   // over the real steps taken.
@@ -75,9 +84,13 @@ static_assert(sizeof(inu_ctrl) == 48, "control block must be 48 bytes");
 // edge stream, the first n_own of them owned; its ghost cells (the cells of
 // other blocks that those edges touch) at gx[g0, g0 + n_ghost); its CSR slots
 // at [s0, s0 + n_slots).
+// meta = (scenario << 9) | number of real cells (1..256).
 struct BlockInfo {
-  uint32_t e0, n_loc, n_own, g0, n_ghost, s0, n_slots, pad;
+  uint32_t e0, n_loc, n_own, g0, n_ghost, s0, n_slots, meta;
 };
+
+__host__ __device__ __forceinline__ uint32_t bi_cells(const BlockInfo& bi) { return bi.meta & 511u; }
+__host__ __device__ __forceinline__ uint32_t bi_scen(const BlockInfo& bi) { return bi.meta >> 9; }
 
 struct Args {
   Phys p;
@@ -103,6 +116,8 @@ struct Args {
   const uint2* src_pair;  // (source cell, its neighbour in another block)
   uint32_t n_src_blocks, n_src_pairs;
   Ctl* ctl;
+  Scen* scen;
+  uint32_t n_scen;
 };
 
 // ---------------------------------------------------------------------------
@@ -169,8 +184,14 @@ struct Lds {
 constexpr uint32_t kSideWall = 0xffffu;
 constexpr uint32_t kMaxGhosts = kSideWall - kBlockCells;
 
-constexpr uint32_t kEdgesAhead = 2;  // edge records each thread loads before the barrier
-constexpr uint32_t kSlotsAhead = 4;  // CSR slots each thread loads before the barrier
+// Tile levels of the stage kernels, chosen per mesh by inu_autotune: edge
+// records and CSR slots each thread loads before the first barrier (the rest
+// are loaded after it). Denser meshes (Voronoi cells have about three edges
+// per cell) need more to keep a block to one memory round trip; more costs
+// registers.
+constexpr int kTiles = 3;
+__host__ __device__ constexpr uint32_t edges_ahead(int tile) { return tile == 0 ? 2u : tile == 1 ? 3u : 4u; }
+__host__ __device__ constexpr uint32_t slots_ahead(int tile) { return tile == 0 ? 4u : tile == 1 ? 6u : 8u; }
 
 __host__ __device__ inline size_t lds_words(uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
   return 8 * (size_t)(kBlockCells + max_ghost) + 8 * (size_t)max_local + (max_slots + 1) / 2;
@@ -267,6 +288,7 @@ struct Ahead {
   uint32_t b;    // next block
   BlockInfo bi;  // its BlockInfo
   uint32_t gid;  // its ghost cell id of this thread
+  uint32_t dtw;  // its scenario's dt bits
   uint32_t b2;   // block after it
 };
 
@@ -276,14 +298,16 @@ struct Ahead {
 // issued at once (cells, ghost cells, CSR slots, the thread's first edge
 // records); (B) each edge touching the block is evaluated once into LDS from
 // LDS; (C) each cell sums its edges in CSR order from LDS and updates.
-template <int ST, bool COMP, bool FM>
+template <int ST, bool COMP, bool FM, int TILE>
 __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, uint32_t gid, float dt,
                                 const Lds& l, Ahead& nx, uint32_t end, uint32_t per) {
   constexpr bool LO = COMP && FM;  // the fast order's face depths read the low words
+  constexpr uint32_t kEdgesAhead = edges_ahead(TILE);
+  constexpr uint32_t kSlotsAhead = slots_ahead(TILE);
   const Phys& p = a.p;
   const uint32_t tid = threadIdx.x;
   const uint32_t c0 = b << 8;
-  const uint32_t nc = min(kBlockCells, a.N - c0);
+  const uint32_t nc = bi_cells(bi);
   const uint32_t n_loc = bi.n_loc;
 
   // ---- (A) independent loads -------------------------------------------
@@ -339,8 +363,11 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     if (tid + m * kBlockCells < n_loc) edge_eval<ST, FM, LO>(a, l, er[m], tid + m * kBlockCells);
   for (uint32_t j = tid + kEdgesAhead * kBlockCells; j < n_loc; j += kBlockCells)
     edge_eval<ST, FM, LO>(a, l, load_edge(a, bi, j), j);
-  // Walk ahead: this thread's ghost cell id of the next block.
-  if (nx.t < end && tid < nx.bi.n_ghost) nx.gid = ldg(a.gx, nx.bi.g0 + tid);
+  // Walk ahead: this thread's ghost cell id of the next block, its dt.
+  if (nx.t < end) {
+    if (tid < nx.bi.n_ghost) nx.gid = ldg(a.gx, nx.bi.g0 + tid);
+    nx.dtw = a.scen[bi_scen(nx.bi)].dt_cur;
+  }
   __syncthreads();
 
   // ---- (C) cells -----------------------------------------------------------
@@ -477,12 +504,20 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   return bound;
 }
 
-template <int ST, bool COMP, bool FM>
+// Loads of the walk-ahead state for the block at list position nx.t, all at
+// once (the start of a walk, and blocks skipped without processing).
+__device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t end, uint32_t per) {
+  if (nx.t >= end) return;
+  nx.bi = a.binfo[nx.b];
+  nx.gid = (threadIdx.x < nx.bi.n_ghost) ? a.gx[nx.bi.g0 + threadIdx.x] : 0u;
+  nx.dtw = a.scen[bi_scen(nx.bi)].dt_cur;
+  if (nx.t + per < end) nx.b2 = a.list[nx.t + per];
+}
+
+template <int ST, bool COMP, bool FM, int TILE>
 __global__ __launch_bounds__(256) void k_stage(const Args a) {
   extern __shared__ float sm[];
-  const uint32_t dtw = a.ctl->dtbuf[0];
-  if (dtw == 0u) return;  // no-op step: the window's stop was reached
-  const float dt = __uint_as_float(dtw);
+  if (a.ctl->any_dt == 0u) return;  // no-op step: every scenario is at its stop
   const Lds l = carve(sm, a.max_local, a.max_ghost);
   // XCD-aware split: workgroup w runs on XCD w % 8, so each XCD walks one
   // contiguous (spatially compact) slice of the block list and its L2 keeps
@@ -496,23 +531,35 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
   nx.b = 0u;
   nx.b2 = 0u;
   nx.gid = 0u;
-  if (nx.t < end) {
-    nx.b = a.list[nx.t];
-    nx.bi = a.binfo[nx.b];
-    if (threadIdx.x < nx.bi.n_ghost) nx.gid = a.gx[nx.bi.g0 + threadIdx.x];
-    if (nx.t + per < end) nx.b2 = a.list[nx.t + per];
-  }
-  uint32_t bound = kNoBound;  // this thread's smallest next-step bound
+  nx.dtw = 0u;
+  if (nx.t < end) nx.b = a.list[nx.t];
+  walk_ahead(a, nx, end, per);
+  // This thread's smallest next-step bound, for scenario cur_s; flushed when
+  // the walk moves to another scenario's blocks.
+  uint32_t bound = kNoBound;
+  uint32_t cur_s = (nx.t < end) ? bi_scen(nx.bi) : 0u;
   while (nx.t < end) {
     const uint32_t b = nx.b;
     const BlockInfo bi = nx.bi;
     const uint32_t gid = nx.gid;
+    const uint32_t dtw = nx.dtw;
     nx.t += per;
     nx.b = nx.b2;
-    const uint32_t bb = stage_block<ST, COMP, FM>(a, b, bi, gid, dt, l, nx, end, per);
+    if (ST == 1 && bi_scen(bi) != cur_s) {
+      // Rare (a scenario boundary in this walk): flush per thread.
+      if (bound != kNoBound && bound < __atomic_load_n(&a.scen[cur_s].dt_next, __ATOMIC_RELAXED))
+        atomicMin(&a.scen[cur_s].dt_next, bound);
+      bound = kNoBound;
+      cur_s = bi_scen(bi);
+    }
+    if (dtw == 0u) {  // this scenario is at its stop
+      walk_ahead(a, nx, end, per);
+      continue;
+    }
+    const uint32_t bb = stage_block<ST, COMP, FM, TILE>(a, b, bi, gid, __uint_as_float(dtw), l, nx, end, per);
     bound = (bb < bound) ? bb : bound;
   }
-  if (ST == 1) block_atomic_min(&a.ctl->dtbuf[1], bound);
+  if (ST == 1) block_atomic_min(&a.scen[cur_s].dt_next, bound);
 }
 
 // Sources of blocks the active set skipped this step (reference
@@ -523,19 +570,27 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
 template <bool COMP, bool FM>
 __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
   __shared__ float s_h[kBlockCells];
-  const uint32_t dtw = a.ctl->dtbuf[0];
-  if (dtw == 0u) return;
-  const float dt = __uint_as_float(dtw);
+  if (a.ctl->any_dt == 0u) return;
   const Phys& p = a.p;
   const uint32_t cnt = a.ctl->idle_cur;
   uint32_t bound = kNoBound;
+  uint32_t cur_s = ~0u;  // scenario whose bound is being accumulated
   for (uint32_t t = blockIdx.x; t < cnt; t += gridDim.x) {
     const uint32_t b = a.idle[t];
+    const BlockInfo bi = a.binfo[b];
+    if (bi_scen(bi) != cur_s) {
+      if (cur_s != ~0u) block_atomic_min(&a.scen[cur_s].dt_next, bound);
+      bound = kNoBound;
+      cur_s = bi_scen(bi);
+    }
+    const uint32_t dtw = a.scen[cur_s].dt_cur;
+    if (dtw == 0u) continue;  // this scenario is at its stop
+    const float dt = __uint_as_float(dtw);
     const uint32_t c0 = b << 8;
     const uint32_t i = c0 + threadIdx.x;
     bool wet = false;
     float hn = 0.0f;
-    if (i < a.N) {
+    if (threadIdx.x < bi_cells(bi)) {
       hn = a.h[i];
       const float rate = a.src[i];
       if (rate != 0.0f) {
@@ -575,7 +630,7 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
       }
     }
   }
-  block_atomic_min(&a.ctl->dtbuf[1], bound);
+  if (cur_s != ~0u) block_atomic_min(&a.scen[cur_s].dt_next, bound);
 }
 
 // Exact dt bound (reference resident.py:709-728) of the wetted source cells
@@ -586,16 +641,15 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
 // minimum over both equals the exact one.
 template <bool FM>
 __global__ __launch_bounds__(256) void k_source_fix(const Args a) {
-  if (a.ctl->dtbuf[0] == 0u) return;
+  if (a.ctl->any_dt == 0u) return;
   const Phys& p = a.p;
-  uint32_t bound = kNoBound;
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < a.n_src_pairs) {
     const uint2 pr = a.src_pair[k];
     if (a.h[pr.y] < p.dry) {
       const uint32_t i = pr.x;
       const float hn = a.h[i];
-      if (hn >= p.dry) {
+      if (hn >= p.dry && a.src[i] != 0.0f) {  // a wetted cell whose source is on
         const float hui = a.hu[i];
         const float hvi = a.hv[i];
         const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
@@ -610,11 +664,16 @@ __global__ __launch_bounds__(256) void k_source_fix(const Args a) {
           else lam += (u + kf * c) * a.slen[s];
         }
         const float A = a.area[i];
-        if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
+        if (lam > 1e-30f && A > 1e-10f) {
+          // Rare: one atomic per such cell, into its scenario's slot. A
+          // scenario at its stop ignores the slot (k_step_control).
+          const uint32_t bb = __float_as_uint(p.cfl * A / lam);
+          uint32_t* slot = &a.scen[bi_scen(a.binfo[i >> 8])].dt_next;
+          if (bb < __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMin(slot, bb);
+        }
       }
     }
   }
-  block_atomic_min(&a.ctl->dtbuf[1], bound);
 }
 
 // ---------------------------------------------------------------------------
@@ -626,8 +685,8 @@ __global__ __launch_bounds__(256) void k_source_fix(const Args a) {
 // (cfl.py:130-166) and the list being built.
 __global__ void k_block_mark(const Args a, int all_active) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < a.n_scen) a.scen[b].dt_cur = __float_as_uint(kDtSentinel);  // the prologue bound's slot
   if (b == 0) {
-    a.ctl->dtbuf[0] = __float_as_uint(kDtSentinel);
     a.ctl->cnt_next = 0u;
     a.ctl->idle_next = 0u;
     for (int k = 0; k < 3; ++k) a.ctl->act_next[k] = 0ull;
@@ -639,7 +698,7 @@ __global__ void k_block_mark(const Args a, int all_active) {
     return;
   }
   const uint32_t first = b << 8;
-  const uint32_t last = min(first + kBlockCells, a.N);
+  const uint32_t last = first + bi_cells(a.binfo[b]);
   uint32_t live = 0u;
   for (uint32_t i = first; i < last; ++i) {
     if (a.h[i] >= a.p.dry || a.hu[i] != 0.0f || a.hv[i] != 0.0f) {
@@ -665,15 +724,19 @@ __device__ __forceinline__ void wave_append(bool take, uint32_t v, uint32_t* cnt
 // active = (wet | forced) over the block's two-ring (reference
 // resident.py:778-805). Active blocks are appended to `list`, inactive blocks
 // holding a source to `idle`, with one atomic per wave and list, in block
-// order within it.
+// order within it. Blocks of a scenario that took a no-op step keep their
+// flags and are left off both lists.
 __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int keep_flags) {
-  if (!force && a.ctl->dtbuf[0] == 0u) return;  // no-op step changed nothing
+  if (!force && a.ctl->any_dt == 0u) return;  // no-op step changed nothing
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   bool take = false, take_idle = false;
   uint32_t w_cells = 0, w_edges = 0, w_slots = 0;
   if (b < a.NB) {
+    const BlockInfo bi = a.binfo[b];
+    // A scenario at its stop took a no-op step: its flags stand.
+    const bool frozen = !force && a.scen[bi_scen(bi)].dt_cur == 0u;
     uint32_t v;
-    if (keep_flags) {
+    if (keep_flags || frozen) {
       v = a.active[b];
     } else {
       v = a.wet[b] | a.forced[b];
@@ -683,13 +746,15 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
       }
       a.active[b] = v;
     }
-    take = v != 0u;
-    take_idle = !take && a.has_src[b] != 0u;
-    if (v != 0u) {
+    // A frozen scenario's blocks stay off the lists until a retarget
+    // rebuilds them from the kept flags.
+    take = v != 0u && !frozen;
+    take_idle = v == 0u && !frozen && a.has_src[b] != 0u;
+    if (take) {
       const uint32_t c0 = b << 8;
-      const uint32_t c1 = min(c0 + kBlockCells, a.N);
+      const uint32_t c1 = c0 + bi_cells(bi);
       w_cells = c1 - c0;
-      w_edges = a.binfo[b].n_own;
+      w_edges = bi.n_own;
       w_slots = a.cptr[c1] - a.cptr[c0];
     }
   }
@@ -719,6 +784,14 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
   wave_append(take_idle, b, &a.ctl->idle_next, a.idle);
 }
 
+// Empties the lists being built and their activity counters (before a
+// rebuild outside the step sequence).
+__global__ void k_list_reset(const Args a) {
+  a.ctl->cnt_next = 0u;
+  a.ctl->idle_next = 0u;
+  for (int k = 0; k < 3; ++k) a.ctl->act_next[k] = 0ull;
+}
+
 // Prologue CFL bound of the current state, every cell (reference
 // resident.py:528-610): per-edge wave bound into LDS, per-cell CSR sum,
 // atomicMin into dt slot 0.
@@ -729,10 +802,16 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
   const Phys& p = a.p;
   const uint32_t tid = threadIdx.x;
   uint32_t best = kNoBound;
+  uint32_t cur_s = ~0u;  // scenario whose bound is being accumulated
   for (uint32_t b = blockIdx.x; b < a.NB; b += gridDim.x) {
     const uint32_t c0 = b << 8;
-    const uint32_t nc = min(kBlockCells, a.N - c0);
     const BlockInfo bi = a.binfo[b];
+    const uint32_t nc = bi_cells(bi);
+    if (bi_scen(bi) != cur_s) {
+      if (cur_s != ~0u) block_atomic_min(&a.scen[cur_s].dt_cur, best);
+      best = kNoBound;
+      cur_s = bi_scen(bi);
+    }
     __syncthreads();
     if (tid < nc)
       l.cell[2 * tid] = make_float4(a.h[c0 + tid], a.hu[c0 + tid], a.hv[c0 + tid], LO ? a.hlo[c0 + tid] : 0.0f);
@@ -773,63 +852,81 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
     }
     best = (bound < best) ? bound : best;
   }
-  block_atomic_min(&a.ctl->dtbuf[0], best);
+  if (cur_s != ~0u) block_atomic_min(&a.scen[cur_s].dt_cur, best);
 }
 
-// The integer clock (reference resident.py:842-882). Mode 0 START plans from
-// the prologue bound in slot 0, mode 1 STEP advances past the step just taken
-// and plans from slot 1, mode 2 RETARGET re-plans from the last bound. START
-// and every real STEP with dilation also publish the block list it built.
+// The integer clock of every scenario (reference resident.py:842-882), one
+// thread per scenario. Mode 0 START plans from the prologue bound in dt_cur,
+// mode 1 STEP advances past the step just taken and plans from dt_next, mode 2
+// RETARGET re-plans from the last bound. START, RETARGET (after k_dilate
+// rebuilt the lists from the kept flags) and every STEP that some scenario
+// really took with dilation on publish the block lists k_dilate built.
 __global__ void k_step_control(const Args a, int mode) {
   Ctl* t = a.ctl;
-  inu_ctrl& c = t->ctrl;
-  float raw;
-  bool publish = false;
-  if (mode == 0) {
-    raw = __uint_as_float(t->dtbuf[0]);
-    c.last_raw = raw;
-    publish = true;
-  } else if (mode == 1) {
-    const uint32_t done = c.dt_ticks;
-    c.now_ticks += (uint64_t)done;
-    if (done > 0u) {
-      c.steps += 1ull;
-      raw = __uint_as_float(t->dtbuf[1]);
+  bool real = false, moving = false;
+  for (uint32_t s = threadIdx.x; s < a.n_scen; s += blockDim.x) {
+    Scen& sc = a.scen[s];
+    inu_ctrl& c = sc.ctrl;
+    float raw;
+    if (mode == 0) {
+      raw = __uint_as_float(sc.dt_cur);
       c.last_raw = raw;
-      publish = t->swap_on != 0u;
-      for (int k = 0; k < 3; ++k) t->act_total[k] += t->act_cur[k];
-      t->act_steps += 1ull;
+    } else if (mode == 1) {
+      const uint32_t done = c.dt_ticks;
+      c.now_ticks += (uint64_t)done;
+      if (done > 0u) {
+        c.steps += 1ull;
+        raw = __uint_as_float(sc.dt_next);
+        c.last_raw = raw;
+        real = true;
+      } else {
+        raw = c.last_raw;
+      }
     } else {
       raw = c.last_raw;
     }
-  } else {
-    raw = c.last_raw;
-  }
-  if (publish) {
-    t->cnt_cur = t->cnt_next;
-    t->cnt_next = 0u;
-    t->idle_cur = t->idle_next;
-    t->idle_next = 0u;
-    for (int k = 0; k < 3; ++k) {
-      t->act_cur[k] = t->act_next[k];
-      t->act_next[k] = 0ull;
+    const uint64_t remaining = (c.stop_ticks > c.now_ticks) ? c.stop_ticks - c.now_ticks : 0ull;
+    uint32_t ticks;
+    if (remaining == 0ull || raw < 1e-10f) {
+      ticks = 0u;
+    } else if (raw >= 1e10f) {
+      ticks = c.dry_dt_ticks;
+    } else {
+      const float cap = (float)c.dt_max_ticks / kTicksPerSecond;
+      ticks = (uint32_t)(inu::gmin(raw * c.cfl_safety, cap) * kTicksPerSecond);
     }
+    ticks = min(ticks, c.dt_max_ticks);
+    if ((uint64_t)ticks > remaining) ticks = (uint32_t)remaining;
+    c.dt_ticks = ticks;
+    sc.dt_cur = __float_as_uint((float)ticks / kTicksPerSecond);
+    sc.dt_next = __float_as_uint(kDtSentinel);
+    moving = moving || ticks > 0u;
   }
-  const uint64_t remaining = (c.stop_ticks > c.now_ticks) ? c.stop_ticks - c.now_ticks : 0ull;
-  uint32_t ticks;
-  if (remaining == 0ull || raw < 1e-10f) {
-    ticks = 0u;
-  } else if (raw >= 1e10f) {
-    ticks = c.dry_dt_ticks;
-  } else {
-    const float cap = (float)c.dt_max_ticks / kTicksPerSecond;
-    ticks = (uint32_t)(inu::gmin(raw * c.cfl_safety, cap) * kTicksPerSecond);
+  const int any_real = __syncthreads_or(real ? 1 : 0);
+  const int any_moving = __syncthreads_or(moving ? 1 : 0);
+  if (threadIdx.x == 0) {
+    if (mode == 1 && any_real) {
+      for (int k = 0; k < 3; ++k) t->act_total[k] += t->act_cur[k];
+      t->act_steps += 1ull;
+    }
+    if (mode == 0 || mode == 2 || (mode == 1 && any_real && t->swap_on != 0u)) {
+      t->cnt_cur = t->cnt_next;
+      t->cnt_next = 0u;
+      t->idle_cur = t->idle_next;
+      t->idle_next = 0u;
+      for (int k = 0; k < 3; ++k) {
+        t->act_cur[k] = t->act_next[k];
+        t->act_next[k] = 0ull;
+      }
+    }
+    t->any_dt = any_moving ? 1u : 0u;
   }
-  ticks = min(ticks, c.dt_max_ticks);
-  if ((uint64_t)ticks > remaining) ticks = (uint32_t)remaining;
-  c.dt_ticks = ticks;
-  t->dtbuf[0] = __float_as_uint((float)ticks / kTicksPerSecond);
-  t->dtbuf[1] = __float_as_uint(kDtSentinel);
+}
+
+// rates[k] into the dense source array at cells[k] (inu_set_source_rates).
+__global__ void k_scatter_rates(float* src, const uint32_t* cells, const float* rates, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) src[cells[k]] = rates[k];
 }
 
 // Plain copy of n 16-byte words: the kernel of known traffic that HBM byte
@@ -875,15 +972,21 @@ struct inu_ctx {
   // params
   Phys p{};
   uint32_t flags = INU_FLAG_COMPENSATED | INU_FLAG_ACTIVE_SET;
+  // scenarios: caller cells [cell0[s], cell0[s + 1]), blocks [block0[s], block0[s + 1])
+  uint32_t n_scen = 0;
+  std::vector<uint32_t> cell0, block0;
+  Scen* scen = nullptr;
   // sources
   uint32_t n_src = 0, n_src_blocks = 0, n_src_pairs = 0;
+  std::vector<uint32_t> src_cells;  // internal ids of the source cells, ascending
   // device memory
   std::vector<void*> allocs;
   Args a{};
   Ctl* ctl = nullptr;
   // launch shape
   uint32_t grid_stage = 0;        // default corrector grid (reported by inu_mesh_info)
-  uint32_t grid[2][2][2] = {};    // [stage][compensated][fast order]
+  uint32_t grid[2][2][2][kTiles] = {};  // [stage][compensated][fast order][tile]
+  int tile = 0;                         // stage kernel tile level (inu_autotune)
   size_t lds_stage0 = 0, lds_stage1 = 0;
   // (cell, neighbour) for every neighbour in another block, by cell
   std::vector<std::pair<uint32_t, uint32_t>> cross;
@@ -925,6 +1028,34 @@ int upload(inu_ctx* c, T* dst, const T* src, size_t n) {
   return 0;
 }
 
+// Caller cell arrays (scenarios back to back) to and from the internal layout
+// (each scenario from a block boundary).
+int upload_cells(inu_ctx* c, float* dev, const float* host) {
+  for (uint32_t s = 0; s < c->n_scen; ++s) {
+    const size_t n = c->cell0[s + 1] - c->cell0[s];
+    if (n) HIPCHK(c, hipMemcpyAsync(dev + (size_t)c->block0[s] * kBlockCells, host + c->cell0[s], n * 4,
+                                    hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int download_cells(inu_ctx* c, const float* dev, float* host) {
+  for (uint32_t s = 0; s < c->n_scen; ++s) {
+    const size_t n = c->cell0[s + 1] - c->cell0[s];
+    if (n) HIPCHK(c, hipMemcpyAsync(host + c->cell0[s], dev + (size_t)c->block0[s] * kBlockCells, n * 4,
+                                    hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Internal cell of caller cell u (u < N).
+uint32_t internal_cell(const inu_ctx* c, uint32_t u) {
+  const size_t s = (size_t)(std::upper_bound(c->cell0.begin(), c->cell0.end(), u) - c->cell0.begin()) - 1;
+  return c->block0[s] * kBlockCells + (u - c->cell0[s]);
+}
+
 void free_all(inu_ctx* c) {
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   c->graphs.clear();
@@ -932,6 +1063,8 @@ void free_all(inu_ctx* c) {
   c->allocs.clear();
   c->a = Args{};
   c->ctl = nullptr;
+  c->scen = nullptr;
+  c->n_scen = 0;
   c->has_mesh = false;
 }
 
@@ -959,14 +1092,26 @@ int check_ready(inu_ctx* c) {
 // Kernel launches of one step, in stream order. With `ev` set, events are
 // recorded before the predictor, between the two stages and after the
 // corrector (no host wait).
+template <int ST, int TILE>
+void launch_stage_t(inu_ctx* c, const Args& a, bool comp, bool fm) {
+  const size_t lds = ST == 0 ? c->lds_stage0 : c->lds_stage1;
+  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0][TILE]), b(256);
+  if (comp && fm) hipLaunchKernelGGL((k_stage<ST, true, true, TILE>), g, b, lds, c->stream, a);
+  else if (comp) hipLaunchKernelGGL((k_stage<ST, true, false, TILE>), g, b, lds, c->stream, a);
+  else if (fm) hipLaunchKernelGGL((k_stage<ST, false, true, TILE>), g, b, lds, c->stream, a);
+  else hipLaunchKernelGGL((k_stage<ST, false, false, TILE>), g, b, lds, c->stream, a);
+}
+
+template <int ST>
+void launch_stage(inu_ctx* c, const Args& a, bool comp, bool fm, int tile) {
+  if (tile == 0) launch_stage_t<ST, 0>(c, a, comp, fm);
+  else if (tile == 1) launch_stage_t<ST, 1>(c, a, comp, fm);
+  else launch_stage_t<ST, 2>(c, a, comp, fm);
+}
+
 template <int ST>
 void launch_stage(inu_ctx* c, bool comp, bool fm) {
-  const size_t lds = ST == 0 ? c->lds_stage0 : c->lds_stage1;
-  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0]), b(256);
-  if (comp && fm) hipLaunchKernelGGL((k_stage<ST, true, true>), g, b, lds, c->stream, c->a);
-  else if (comp) hipLaunchKernelGGL((k_stage<ST, true, false>), g, b, lds, c->stream, c->a);
-  else if (fm) hipLaunchKernelGGL((k_stage<ST, false, true>), g, b, lds, c->stream, c->a);
-  else hipLaunchKernelGGL((k_stage<ST, false, false>), g, b, lds, c->stream, c->a);
+  launch_stage<ST>(c, c->a, comp, fm, c->tile);
 }
 
 int launch_step(inu_ctx* c, const hipEvent_t* ev) {
@@ -992,7 +1137,7 @@ int launch_step(inu_ctx* c, const hipEvent_t* ev) {
   }
   if (c->flags & INU_FLAG_ACTIVE_SET)
     hipLaunchKernelGGL(k_dilate, dim3((c->NB + 255) / 256), dim3(256), 0, c->stream, a, 0, 0);
-  hipLaunchKernelGGL(k_step_control, dim3(1), dim3(1), 0, c->stream, a, 1);
+  hipLaunchKernelGGL(k_step_control, dim3(1), dim3(256), 0, c->stream, a, 1);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -1038,16 +1183,47 @@ void inu_destroy(inu_ctx* c) {
 
 const char* inu_last_error(const inu_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* edge_cell_l,
-                 const int32_t* edge_cell_r, const float* edge_nx, const float* edge_ny,
-                 const float* edge_len, const float* edge_dz, const float* cell_area,
-                 const uint32_t* cell_edge_ptr, const uint32_t* cell_edge_idx) {
-  if (!c) return INU_EINVAL;
-  if (n_cells == 0 || n_edges == 0 || !edge_cell_l || !edge_cell_r || !edge_nx || !edge_ny ||
-      !edge_len || !edge_dz || !cell_area || !cell_edge_ptr || !cell_edge_idx)
-    return fail(c, INU_EINVAL, "inu_set_mesh: empty mesh or null array");
-  HIPCHK(c, hipSetDevice(c->device));
-  const uint32_t N = n_cells, E = n_edges, NB = (N + kBlockCells - 1) / kBlockCells;
+}  // extern "C"
+
+namespace {
+
+// One mesh as the caller hands it over (the arguments of inu_set_mesh).
+struct MeshIn {
+  uint32_t N, E;
+  const int32_t *ecl, *ecr;
+  const float *nx, *ny, *len, *dz, *area;
+  const uint32_t *cptr, *cidx;
+};
+
+// Device layout of one mesh on its own (offsets and ids local to the mesh,
+// BlockInfo.meta = real cells only), before it is placed in the union.
+struct Layout {
+  uint32_t N = 0, E = 0, S = 0, NB = 0, n_halo = 0;
+  uint32_t max_local = 1, max_slots = 1, max_ghost = 1;
+  std::vector<BlockInfo> binfo;
+  std::vector<uint4> lrec;
+  std::vector<float> ldz;
+  std::vector<uint32_t> gx;
+  std::vector<uint16_t> lidx;
+  std::vector<int32_t> snb;
+  std::vector<float> slen;
+  std::vector<std::pair<uint32_t, uint32_t>> cross;
+  std::vector<uint32_t> bptr, bidx;
+};
+
+// Validates one mesh and builds its layout.
+int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
+  if (m.N == 0 || m.E == 0 || !m.ecl || !m.ecr || !m.nx || !m.ny || !m.len || !m.dz || !m.area || !m.cptr || !m.cidx)
+    return fail(c, INU_EINVAL, "empty mesh or null array");
+  const uint32_t N = m.N, E = m.E, NB = (N + kBlockCells - 1) / kBlockCells;
+  const int32_t* edge_cell_l = m.ecl;
+  const int32_t* edge_cell_r = m.ecr;
+  const float* edge_nx = m.nx;
+  const float* edge_ny = m.ny;
+  const float* edge_len = m.len;
+  const float* edge_dz = m.dz;
+  const uint32_t* cell_edge_ptr = m.cptr;
+  const uint32_t* cell_edge_idx = m.cidx;
   const uint32_t S = cell_edge_ptr[N];
   if (cell_edge_ptr[0] != 0) return fail(c, INU_EINVAL, "cell_edge_ptr[0] must be 0");
   // Validate the edge order and the signed CSR.
@@ -1085,10 +1261,14 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
                                  "(the face sits on the higher bed)");
   }
   uint32_t max_local = 1, max_slots = 1, max_ghost = 1;
-  std::vector<BlockInfo> binfo(NB);
-  std::vector<uint4> lrec((size_t)E + hp[NB]);
-  std::vector<float> ldz((size_t)E + hp[NB]);
-  std::vector<uint32_t> gxv;
+  std::vector<BlockInfo>& binfo = L.binfo;
+  binfo.assign(NB, BlockInfo{});
+  std::vector<uint4>& lrec = L.lrec;
+  lrec.assign((size_t)E + hp[NB], uint4{});
+  std::vector<float>& ldz = L.ldz;
+  ldz.assign((size_t)E + hp[NB], 0.0f);
+  std::vector<uint32_t>& gxv = L.gx;
+  gxv.clear();
   std::vector<uint32_t> ghosts;
   uint32_t e0 = 0;
   for (uint32_t b = 0; b < NB; ++b) {
@@ -1117,7 +1297,7 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
       ldz[e0 + j] = (edge_dz[2 * e] > 0.0f) ? edge_dz[2 * e] : -edge_dz[2 * e + 1];
     }
     binfo[b] = BlockInfo{e0, n_loc, n_own, (uint32_t)gxv.size(), (uint32_t)ghosts.size(), cell_edge_ptr[cb],
-                         cell_edge_ptr[ce] - cell_edge_ptr[cb], 0u};
+                         cell_edge_ptr[ce] - cell_edge_ptr[cb], ce - cb};
     gxv.insert(gxv.end(), ghosts.begin(), ghosts.end());
     e0 += n_loc;
     max_local = std::max(max_local, n_loc);
@@ -1126,10 +1306,14 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   }
   if (max_local >= 32768u) return fail(c, INU_ERANGE, "a block touches more than 32767 edges");
   // Local CSR: each slot's edge as its position in its block's LDS edge list.
-  std::vector<uint16_t> lidx(S);
-  std::vector<int32_t> snb(S);
-  std::vector<float> slen(S);
-  std::vector<std::pair<uint32_t, uint32_t>> cross;
+  std::vector<uint16_t>& lidx = L.lidx;
+  lidx.assign(S, 0);
+  std::vector<int32_t>& snb = L.snb;
+  snb.assign(S, 0);
+  std::vector<float>& slen = L.slen;
+  slen.assign(S, 0.0f);
+  std::vector<std::pair<uint32_t, uint32_t>>& cross = L.cross;
+  cross.clear();
   for (uint32_t i = 0; i < N; ++i) {
     const uint32_t b = i / kBlockCells;
     if (cell_edge_ptr[i + 1] < cell_edge_ptr[i] || cell_edge_ptr[i + 1] > S)
@@ -1170,7 +1354,10 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
     std::sort(v.begin(), v.end());
     v.erase(std::unique(v.begin(), v.end()), v.end());
   }
-  std::vector<uint32_t> bptr(NB + 1, 0), bidx;
+  std::vector<uint32_t>& bptr = L.bptr;
+  bptr.assign(NB + 1, 0);
+  std::vector<uint32_t>& bidx = L.bidx;
+  bidx.clear();
   for (uint32_t b = 0; b < NB; ++b) {
     std::vector<uint32_t> ring(adj[b]);
     for (uint32_t n : adj[b]) ring.insert(ring.end(), adj[n].begin(), adj[n].end());
@@ -1180,30 +1367,126 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
       if (n != b) bidx.push_back(n);
     bptr[b + 1] = (uint32_t)bidx.size();
   }
+  L.N = N;
+  L.E = E;
+  L.S = S;
+  L.NB = NB;
+  L.n_halo = hp[NB];
+  L.max_local = max_local;
+  L.max_slots = max_slots;
+  L.max_ghost = max_ghost;
+  return 0;
+}
+
+// Places scenario s's copy of mesh scen_mesh[s] at block offset b0[s] of the
+// union and uploads the union. Scenario s owns caller cells
+// [cell0[s], cell0[s + 1]) and internal cells [256 b0[s], 256 b0[s] + N).
+int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& lay, const std::vector<uint32_t>& scen_mesh) {
+  const uint32_t n_scen = (uint32_t)scen_mesh.size();
+  if (n_scen == 0) return fail(c, INU_EINVAL, "no scenarios");
+  if (n_scen > (1u << 22)) return fail(c, INU_ERANGE, "too many scenarios");
+  std::vector<uint32_t> cell0(n_scen + 1, 0), block0(n_scen + 1, 0);
+  uint64_t nb_tot = 0, e_tot = 0, s_tot = 0, l_tot = 0, g_tot = 0, cross_tot = 0, ring_tot = 0, n_tot = 0;
+  uint32_t max_local = 1, max_slots = 1, max_ghost = 1, n_halo = 0;
+  for (uint32_t s = 0; s < n_scen; ++s) {
+    if (scen_mesh[s] >= lay.size()) return fail(c, INU_EINVAL, "scenario_mesh names a mesh out of range");
+    const Layout& L = lay[scen_mesh[s]];
+    cell0[s] = (uint32_t)n_tot;
+    block0[s] = (uint32_t)nb_tot;
+    n_tot += L.N;
+    nb_tot += L.NB;
+    e_tot += L.E;
+    s_tot += L.S;
+    l_tot += L.lrec.size();
+    g_tot += L.gx.size();
+    cross_tot += L.cross.size();
+    ring_tot += L.bidx.size();
+    n_halo += L.n_halo;
+    max_local = std::max(max_local, L.max_local);
+    max_slots = std::max(max_slots, L.max_slots);
+    max_ghost = std::max(max_ghost, L.max_ghost);
+  }
+  cell0[n_scen] = (uint32_t)n_tot;
+  block0[n_scen] = (uint32_t)nb_tot;
+  // Cell ids are int32 and every device array stays below 4 GiB.
+  if (nb_tot * kBlockCells > 0x7fffffffull || l_tot * sizeof(uint4) >= (1ull << 32) || s_tot >= (1ull << 30) ||
+      e_tot > 0xffffffffull)
+    return fail(c, INU_ERANGE, "the meshes are too large for one context");
+  const uint32_t NB = (uint32_t)nb_tot;
+  const size_t NP = (size_t)NB * kBlockCells;
+  std::vector<BlockInfo> binfo(NB);
+  std::vector<uint4> lrec;
+  lrec.reserve(l_tot);
+  std::vector<float> ldz;
+  ldz.reserve(l_tot);
+  std::vector<uint32_t> gx;
+  gx.reserve(g_tot);
+  std::vector<uint32_t> cptr(NP + 1, 0);
+  std::vector<uint16_t> lidx;
+  lidx.reserve(s_tot);
+  std::vector<int32_t> snb;
+  snb.reserve(s_tot);
+  std::vector<float> slen;
+  slen.reserve(s_tot);
+  std::vector<std::pair<uint32_t, uint32_t>> cross;
+  cross.reserve(cross_tot);
+  std::vector<uint32_t> bptr(NB + 1, 0), bidx;
+  bidx.reserve(ring_tot);
+  std::vector<float> area(NP, 0.0f);
+  uint32_t s_off = 0;
+  for (uint32_t s = 0; s < n_scen; ++s) {
+    const Layout& L = lay[scen_mesh[s]];
+    const MeshIn& m = ins[scen_mesh[s]];
+    const uint32_t b0 = block0[s], c0 = b0 * kBlockCells;
+    const uint32_t e_off = (uint32_t)lrec.size(), g_off = (uint32_t)gx.size();
+    for (uint32_t b = 0; b < L.NB; ++b) {
+      BlockInfo bi = L.binfo[b];
+      bi.e0 += e_off;
+      bi.g0 += g_off;
+      bi.s0 += s_off;
+      bi.meta = (s << 9) | bi.meta;
+      binfo[b0 + b] = bi;
+      bptr[b0 + b + 1] = bptr[b0 + b] + (L.bptr[b + 1] - L.bptr[b]);
+    }
+    for (uint32_t n : L.bidx) bidx.push_back(b0 + n);
+    lrec.insert(lrec.end(), L.lrec.begin(), L.lrec.end());
+    ldz.insert(ldz.end(), L.ldz.begin(), L.ldz.end());
+    for (uint32_t g : L.gx) gx.push_back(c0 + g);
+    for (uint32_t i = 0; i < L.NB * kBlockCells; ++i) cptr[c0 + i + 1] = s_off + m.cptr[std::min(i + 1, L.N)];
+    lidx.insert(lidx.end(), L.lidx.begin(), L.lidx.end());
+    for (int32_t n : L.snb) snb.push_back(n < 0 ? n : (int32_t)(c0 + (uint32_t)n));
+    slen.insert(slen.end(), L.slen.begin(), L.slen.end());
+    for (const auto& pr : L.cross) cross.emplace_back(c0 + pr.first, c0 + pr.second);
+    std::copy(m.area, m.area + L.N, area.begin() + c0);
+    s_off += L.S;
+  }
 
   free_all(c);
-  c->N = N;
-  c->E = E;
-  c->S = S;
+  c->N = (uint32_t)n_tot;
+  c->E = (uint32_t)e_tot;
+  c->S = (uint32_t)s_tot;
   c->NB = NB;
   c->max_local = max_local;
-  c->n_halo = hp[NB];
+  c->n_halo = n_halo;
+  c->n_scen = n_scen;
+  c->cell0 = cell0;
+  c->block0 = block0;
   c->n_src = 0;
   c->n_src_blocks = 0;
   c->n_src_pairs = 0;
   Args& a = c->a;
-  const size_t NP = (size_t)NB * kBlockCells;
   int rc = 0;
   rc |= dalloc(c, &a.h, NP); rc |= dalloc(c, &a.hu, NP); rc |= dalloc(c, &a.hv, NP); rc |= dalloc(c, &a.hlo, NP);
   rc |= dalloc(c, &a.h1, NP); rc |= dalloc(c, &a.hu1, NP); rc |= dalloc(c, &a.hv1, NP); rc |= dalloc(c, &a.h1lo, NP);
   rc |= dalloc(c, &a.area, NP); rc |= dalloc(c, &a.nman, NP); rc |= dalloc(c, &a.clamp, NP); rc |= dalloc(c, &a.src, NP);
-  uint32_t* cptr; uint16_t* lid; int32_t* sn; float* sl;
-  rc |= dalloc(c, &cptr, (size_t)N + 1); rc |= dalloc(c, &lid, S); rc |= dalloc(c, &sn, S); rc |= dalloc(c, &sl, S);
+  uint32_t* dcptr; uint16_t* dlid; int32_t* dsn; float* dsl;
+  rc |= dalloc(c, &dcptr, NP + 1); rc |= dalloc(c, &dlid, lidx.size()); rc |= dalloc(c, &dsn, snb.size());
+  rc |= dalloc(c, &dsl, slen.size());
   uint4* dlrec; float* dldz;
   rc |= dalloc(c, &dlrec, lrec.size()); rc |= dalloc(c, &dldz, ldz.size());
   uint32_t *dgx, *dbptr, *dbidx;
   BlockInfo* dbinfo;
-  rc |= dalloc(c, &dbinfo, NB); rc |= dalloc(c, &dgx, gxv.size());
+  rc |= dalloc(c, &dbinfo, NB); rc |= dalloc(c, &dgx, gx.size());
   rc |= dalloc(c, &dbptr, NB + 1); rc |= dalloc(c, &dbidx, bidx.size());
   rc |= dalloc(c, &a.wet, NB); rc |= dalloc(c, &a.active, NB); rc |= dalloc(c, &a.forced, NB);
   rc |= dalloc(c, &a.has_src, NB); rc |= dalloc(c, &a.list, NB); rc |= dalloc(c, &a.idle, NB);
@@ -1211,59 +1494,134 @@ int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* 
   uint2* src_pair;
   rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1); rc |= dalloc(c, &src_pair, 1);
   rc |= dalloc(c, &c->ctl, 1);
+  rc |= dalloc(c, &c->scen, n_scen);
   if (rc) return rc;
-  a.cptr = cptr; a.lidx = lid; a.snb = sn; a.slen = sl;
+  a.cptr = dcptr; a.lidx = dlid; a.snb = dsn; a.slen = dsl;
   a.lrec = dlrec; a.ldz = dldz;
   a.binfo = dbinfo; a.gx = dgx; a.bptr = dbptr; a.bidx = dbidx;
   a.src_cell = src_cell; a.src_blocks = src_blocks; a.src_pair = src_pair; a.n_src_pairs = 0;
   a.ctl = c->ctl;
-  a.N = N; a.NB = NB; a.max_local = max_local; a.max_slots = max_slots; a.max_ghost = max_ghost;
+  a.scen = c->scen;
+  a.n_scen = n_scen;
+  a.N = (uint32_t)NP; a.NB = NB; a.max_local = max_local; a.max_slots = max_slots; a.max_ghost = max_ghost;
   HIPCHK(c, hipMemsetAsync(a.h, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hu, 0, NP * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.hv, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hlo, 0, NP * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.h1, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hu1, 0, NP * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.hv1, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.h1lo, 0, NP * 4, c->stream));
-  HIPCHK(c, hipMemsetAsync(a.area, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.clamp, 0, NP * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.clamp, 0, NP * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.src, 0, NP * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.wet, 0, NB * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.active, 0, NB * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.forced, 0, NB * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.has_src, 0, NB * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(c->ctl, 0, sizeof(Ctl), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->scen, 0, sizeof(Scen) * n_scen, c->stream));
   std::vector<float> nm(NP, 0.03f);
   if ((rc = upload(c, a.nman, nm.data(), NP))) return rc;
-  if ((rc = upload(c, a.area, cell_area, N))) return rc;
-  if ((rc = upload(c, cptr, cell_edge_ptr, (size_t)N + 1))) return rc;
-  if ((rc = upload(c, lid, lidx.data(), S))) return rc;
-  if ((rc = upload(c, sn, snb.data(), S))) return rc;
-  if ((rc = upload(c, sl, slen.data(), S))) return rc;
-  c->cross.swap(cross);
+  if ((rc = upload(c, a.area, area.data(), NP))) return rc;
+  if ((rc = upload(c, dcptr, cptr.data(), NP + 1))) return rc;
+  if ((rc = upload(c, dlid, lidx.data(), lidx.size()))) return rc;
+  if ((rc = upload(c, dsn, snb.data(), snb.size()))) return rc;
+  if ((rc = upload(c, dsl, slen.data(), slen.size()))) return rc;
   if ((rc = upload(c, dlrec, lrec.data(), lrec.size()))) return rc;
   if ((rc = upload(c, dldz, ldz.data(), ldz.size()))) return rc;
   if ((rc = upload(c, dbinfo, binfo.data(), NB))) return rc;
-  if ((rc = upload(c, dgx, gxv.data(), gxv.size()))) return rc;
+  if ((rc = upload(c, dgx, gx.data(), gx.size()))) return rc;
   if ((rc = upload(c, dbptr, bptr.data(), NB + 1))) return rc;
   if ((rc = upload(c, dbidx, bidx.data(), bidx.size()))) return rc;
+  c->cross.swap(cross);
   c->lds_stage0 = lds_words(max_local, max_slots, max_ghost) * sizeof(float);
   c->lds_stage1 = c->lds_stage0;
   if (c->lds_stage1 > 160 * 1024) return fail(c, INU_ERANGE, "a block's edge list does not fit in LDS");
   // Persistent grids: as many workgroups as fit at once, a multiple of the
   // XCD count, per stage kernel variant.
-  const void* variants[2][2][2] = {
-      {{(const void*)k_stage<0, false, false>, (const void*)k_stage<0, false, true>},
-       {(const void*)k_stage<0, true, false>, (const void*)k_stage<0, true, true>}},
-      {{(const void*)k_stage<1, false, false>, (const void*)k_stage<1, false, true>},
-       {(const void*)k_stage<1, true, false>, (const void*)k_stage<1, true, true>}}};
+  const void* variants[2][2][2][kTiles] = {
+      {{{(const void*)k_stage<0, false, false, 0>, (const void*)k_stage<0, false, false, 1>,
+         (const void*)k_stage<0, false, false, 2>},
+        {(const void*)k_stage<0, false, true, 0>, (const void*)k_stage<0, false, true, 1>,
+         (const void*)k_stage<0, false, true, 2>}},
+       {{(const void*)k_stage<0, true, false, 0>, (const void*)k_stage<0, true, false, 1>,
+         (const void*)k_stage<0, true, false, 2>},
+        {(const void*)k_stage<0, true, true, 0>, (const void*)k_stage<0, true, true, 1>,
+         (const void*)k_stage<0, true, true, 2>}}},
+      {{{(const void*)k_stage<1, false, false, 0>, (const void*)k_stage<1, false, false, 1>,
+         (const void*)k_stage<1, false, false, 2>},
+        {(const void*)k_stage<1, false, true, 0>, (const void*)k_stage<1, false, true, 1>,
+         (const void*)k_stage<1, false, true, 2>}},
+       {{(const void*)k_stage<1, true, false, 0>, (const void*)k_stage<1, true, false, 1>,
+         (const void*)k_stage<1, true, false, 2>},
+        {(const void*)k_stage<1, true, true, 0>, (const void*)k_stage<1, true, true, 1>,
+         (const void*)k_stage<1, true, true, 2>}}}};
   for (int st = 0; st < 2; ++st)
     for (int cp = 0; cp < 2; ++cp)
-      for (int fm = 0; fm < 2; ++fm) {
-        int occ = 0;
-        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, variants[st][cp][fm], 256,
-                                                                st ? c->lds_stage1 : c->lds_stage0));
-        uint32_t g = (uint32_t)(std::max(occ, 1) * c->cus);
-        g = std::max<uint32_t>(kXcds, std::min<uint32_t>(g, ((NB + kXcds - 1) / kXcds) * kXcds));
-        c->grid[st][cp][fm] = (g / kXcds) * kXcds;
-      }
-  c->grid_stage = c->grid[1][1][1];
+      for (int fm = 0; fm < 2; ++fm)
+        for (int tl = 0; tl < kTiles; ++tl) {
+          int occ = 0;
+          HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, variants[st][cp][fm][tl], 256,
+                                                                  st ? c->lds_stage1 : c->lds_stage0));
+          uint32_t g = (uint32_t)(std::max(occ, 1) * c->cus);
+          g = std::max<uint32_t>(kXcds, std::min<uint32_t>(g, ((NB + kXcds - 1) / kXcds) * kXcds));
+          c->grid[st][cp][fm][tl] = (g / kXcds) * kXcds;
+        }
+  c->tile = 0;
+  c->grid_stage = c->grid[1][1][1][0];
   c->a.p = c->p;
   c->has_mesh = true;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int inu_set_mesh(inu_ctx* c, uint32_t n_cells, uint32_t n_edges, const int32_t* edge_cell_l,
+                 const int32_t* edge_cell_r, const float* edge_nx, const float* edge_ny,
+                 const float* edge_len, const float* edge_dz, const float* cell_area,
+                 const uint32_t* cell_edge_ptr, const uint32_t* cell_edge_idx) {
+  if (!c) return INU_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<MeshIn> ins{MeshIn{n_cells, n_edges, edge_cell_l, edge_cell_r, edge_nx, edge_ny, edge_len, edge_dz,
+                                 cell_area, cell_edge_ptr, cell_edge_idx}};
+  std::vector<Layout> lay(1);
+  int rc = build_layout(c, ins[0], lay[0]);
+  if (rc) return rc;
+  return assemble(c, ins, lay, std::vector<uint32_t>{0u});
+}
+
+int inu_set_batch(inu_ctx* c, uint32_t n_meshes, const inu_mesh* meshes, uint32_t n_scenarios,
+                  const uint32_t* scenario_mesh) {
+  if (!c) return INU_EINVAL;
+  if (n_meshes == 0 || !meshes || n_scenarios == 0 || !scenario_mesh)
+    return fail(c, INU_EINVAL, "inu_set_batch: no meshes or no scenarios");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<MeshIn> ins(n_meshes);
+  std::vector<Layout> lay(n_meshes);
+  for (uint32_t k = 0; k < n_meshes; ++k) {
+    const inu_mesh& m = meshes[k];
+    ins[k] = MeshIn{m.n_cells, m.n_edges, m.edge_cell_l, m.edge_cell_r, m.nx, m.ny, m.len, m.edge_dz,
+                    m.cell_area, m.cell_edge_ptr, m.cell_edge_idx};
+    int rc = build_layout(c, ins[k], lay[k]);
+    if (rc) {
+      c->err = "mesh " + std::to_string(k) + ": " + c->err;
+      return rc;
+    }
+  }
+  return assemble(c, ins, lay, std::vector<uint32_t>(scenario_mesh, scenario_mesh + n_scenarios));
+}
+
+int inu_scenario_count(inu_ctx* c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  return (int)c->n_scen;
+}
+
+int inu_scenario_info(inu_ctx* c, uint32_t s, uint32_t* out4) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!out4) return fail(c, INU_EINVAL, "null output");
+  if (s >= c->n_scen) return fail(c, INU_EINVAL, "scenario out of range");
+  out4[0] = c->cell0[s];
+  out4[1] = c->cell0[s + 1] - c->cell0[s];
+  out4[2] = c->block0[s];
+  out4[3] = c->block0[s + 1] - c->block0[s];
   return 0;
 }
 
@@ -1303,13 +1661,13 @@ int inu_set_state(inu_ctx* c, const float* h, const float* hu, const float* hv, 
   int rc = check_ready(c);
   if (rc) return rc;
   if (!h || !hu || !hv) return fail(c, INU_EINVAL, "inu_set_state: null array");
-  if ((rc = upload(c, c->a.h, h, c->N))) return rc;
-  if ((rc = upload(c, c->a.hu, hu, c->N))) return rc;
-  if ((rc = upload(c, c->a.hv, hv, c->N))) return rc;
+  if ((rc = upload_cells(c, c->a.h, h))) return rc;
+  if ((rc = upload_cells(c, c->a.hu, hu))) return rc;
+  if ((rc = upload_cells(c, c->a.hv, hv))) return rc;
   if (h_lo) {
-    if ((rc = upload(c, c->a.hlo, h_lo, c->N))) return rc;
+    if ((rc = upload_cells(c, c->a.hlo, h_lo))) return rc;
   } else {
-    HIPCHK(c, hipMemsetAsync(c->a.hlo, 0, (size_t)c->N * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->a.hlo, 0, (size_t)c->NB * kBlockCells * 4, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   return 0;
@@ -1319,7 +1677,9 @@ int inu_set_manning(inu_ctx* c, const float* n) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!n) return fail(c, INU_EINVAL, "inu_set_manning: null array");
-  return upload(c, c->a.nman, n, c->N);
+  for (uint32_t i = 0; i < c->N; ++i)
+    if (!std::isfinite(n[i]) || n[i] < 0.0f) return fail(c, INU_EINVAL, "manning n must be finite and non-negative");
+  return upload_cells(c, c->a.nman, n);
 }
 
 int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* rate) {
@@ -1333,10 +1693,11 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
     if (cell[k] >= c->N) return fail(c, INU_EINVAL, "source cell out of range");
     if (!std::isfinite(rate[k])) return fail(c, INU_EINVAL, "source rate must be finite");
     if (rate[k] == 0.0f) continue;
-    if (dense[cell[k]] != 0.0f) return fail(c, INU_EINVAL, "a source cell is listed twice");
-    dense[cell[k]] = rate[k];
-    cells.push_back(cell[k]);
-    has[cell[k] / kBlockCells] = 1;
+    const uint32_t ic = internal_cell(c, cell[k]);
+    if (dense[ic] != 0.0f) return fail(c, INU_EINVAL, "a source cell is listed twice");
+    dense[ic] = rate[k];
+    cells.push_back(ic);
+    has[ic / kBlockCells] = 1;
   }
   std::sort(cells.begin(), cells.end());
   std::vector<uint2> pairs;
@@ -1375,6 +1736,7 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   if ((rc = upload(c, c->a.has_src, has.data(), has.size()))) return rc;
   c->a.src_cell = dcell;
   c->a.src_blocks = dblk;
+  c->src_cells = cells;
   c->n_src = (uint32_t)cells.size();
   c->n_src_blocks = (uint32_t)blocks.size();
   c->a.n_src = c->n_src;
@@ -1388,7 +1750,8 @@ int inu_ctrl_write(inu_ctx* c, const inu_ctrl* ctrl) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!ctrl) return fail(c, INU_EINVAL, "null ctrl");
-  HIPCHK(c, hipMemcpyAsync(&c->ctl->ctrl, ctrl, sizeof(inu_ctrl), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpy2DAsync(&c->scen[0].ctrl, sizeof(Scen), ctrl, sizeof(inu_ctrl), sizeof(inu_ctrl), c->n_scen,
+                             hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1397,7 +1760,8 @@ int inu_ctrl_read(inu_ctx* c, inu_ctrl* ctrl) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!ctrl) return fail(c, INU_EINVAL, "null ctrl");
-  HIPCHK(c, hipMemcpyAsync(ctrl, &c->ctl->ctrl, sizeof(inu_ctrl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpy2DAsync(ctrl, sizeof(inu_ctrl), &c->scen[0].ctrl, sizeof(Scen), sizeof(inu_ctrl), c->n_scen,
+                             hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1406,7 +1770,8 @@ int inu_dt_words(inu_ctx* c, uint32_t* out4) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!out4) return fail(c, INU_EINVAL, "null output");
-  HIPCHK(c, hipMemcpyAsync(out4, c->ctl->dtbuf, 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpy2DAsync(out4, 16, &c->scen[0].dt_cur, sizeof(Scen), 16, c->n_scen, hipMemcpyDeviceToHost,
+                             c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1434,7 +1799,8 @@ int inu_prologue(inu_ctx* c) {
   const Args& a = c->a;
   const bool act = (c->flags & INU_FLAG_ACTIVE_SET) != 0;
   const uint32_t gb = (c->NB + 255) / 256;
-  hipLaunchKernelGGL(k_block_mark, dim3(gb), dim3(256), 0, c->stream, a, act ? 0 : 1);
+  const uint32_t gm = (std::max(c->NB, c->n_scen) + 255) / 256;
+  hipLaunchKernelGGL(k_block_mark, dim3(gm), dim3(256), 0, c->stream, a, act ? 0 : 1);
   hipLaunchKernelGGL(k_dilate, dim3(gb), dim3(256), 0, c->stream, a, 1, act ? 0 : 1);
   const uint32_t gp = std::min<uint32_t>(c->NB, c->grid_stage);
   if (c->flags & INU_FLAG_EXACT_ORDER)
@@ -1443,7 +1809,7 @@ int inu_prologue(inu_ctx* c) {
     hipLaunchKernelGGL((k_prologue_cfl<true, true>), dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
   else
     hipLaunchKernelGGL((k_prologue_cfl<true, false>), dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
-  hipLaunchKernelGGL(k_step_control, dim3(1), dim3(1), 0, c->stream, a, 0);
+  hipLaunchKernelGGL(k_step_control, dim3(1), dim3(256), 0, c->stream, a, 0);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
@@ -1452,7 +1818,12 @@ int inu_prologue(inu_ctx* c) {
 int inu_retarget(inu_ctx* c) {
   int rc = check_ready(c);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_step_control, dim3(1), dim3(1), 0, c->stream, c->a, 2);
+  // The lists again hold every block whose flag is set (scenarios at their
+  // stop dropped theirs); the flags themselves are kept.
+  const uint32_t gb = (c->NB + 255) / 256;
+  hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(1), 0, c->stream, c->a);
+  hipLaunchKernelGGL(k_dilate, dim3(gb), dim3(256), 0, c->stream, c->a, 1, 1);
+  hipLaunchKernelGGL(k_step_control, dim3(1), dim3(256), 0, c->stream, c->a, 2);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
@@ -1527,7 +1898,7 @@ int inu_download(inu_ctx* c, int which, void* out) {
   if (rc) return rc;
   if (!out) return fail(c, INU_EINVAL, "null output");
   const void* src = nullptr;
-  size_t bytes = (size_t)c->N * 4;
+  size_t bytes = 0;  // 0: a cell field
   switch (which) {
     case INU_FIELD_H: src = c->a.h; break;
     case INU_FIELD_HU: src = c->a.hu; break;
@@ -1543,6 +1914,7 @@ int inu_download(inu_ctx* c, int which, void* out) {
     case INU_FIELD_BLOCK_FORCED: src = c->a.forced; bytes = (size_t)c->NB * 4; break;
     default: return fail(c, INU_EINVAL, "unknown field");
   }
+  if (bytes == 0) return download_cells(c, static_cast<const float*>(src), static_cast<float*>(out));
   HIPCHK(c, hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
@@ -1565,7 +1937,7 @@ int inu_upload(inu_ctx* c, int which, const void* in) {
     case INU_FIELD_CLAMP: dst = c->a.clamp; break;
     default: return fail(c, INU_EINVAL, "field cannot be uploaded");
   }
-  return upload(c, static_cast<float*>(dst), static_cast<const float*>(in), c->N);
+  return upload_cells(c, static_cast<float*>(dst), static_cast<const float*>(in));
 }
 
 int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out7, float g, float dry_tol) {
@@ -1623,6 +1995,90 @@ int inu_stream_copy(inu_ctx* c, uint64_t n_bytes, uint32_t reps, float* ms_out) 
     if (q) (void)hipFree(q);
   if (e != hipSuccess) return fail(c, INU_EHIP, std::string("stream copy: ") + hipGetErrorString(e));
   *ms_out = ms / (float)reps;
+  return 0;
+}
+
+int inu_autotune(inu_ctx* c, uint32_t reps, uint32_t* out4) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!out4 || reps == 0u || reps > 1000u) return fail(c, INU_EINVAL, "inu_autotune: reps in [1, 1000] and an output");
+  uint32_t any_dt = 0;
+  HIPCHK(c, hipMemcpy(&any_dt, &c->ctl->any_dt, 4, hipMemcpyDeviceToHost));
+  if (any_dt == 0u) return fail(c, INU_ESTATE, "inu_autotune: no scenario has a step to take (call after inu_prologue)");
+  const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
+  const bool fm = (c->flags & INU_FLAG_EXACT_ORDER) == 0;
+  // The predictor writes only the predictor state, which the next step
+  // rewrites before reading; the clamp tally stays off while timing.
+  Args a = c->a;
+  a.track_clamp = 0u;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipError_t e = hipEventCreate(&ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+  float best_ms = 0.0f;
+  int best = 0;
+  for (int tl = 0; tl < kTiles && e == hipSuccess; ++tl) {
+    launch_stage<0>(c, a, comp, fm, tl);
+    e = hipEventRecord(ev[0], c->stream);
+    for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) launch_stage<0>(c, a, comp, fm, tl);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(ev[1], c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    out4[1 + tl] = (uint32_t)std::min(4.0e9f, ms * 1000.0f / (float)reps);  // microseconds per launch
+    if (tl == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = tl;
+    }
+  }
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+  c->tile = best;
+  out4[0] = (uint32_t)best;
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  return 0;
+}
+
+int inu_set_tile(inu_ctx* c, uint32_t tile) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (tile >= (uint32_t)kTiles) return fail(c, INU_EINVAL, "tile level out of range");
+  c->tile = (int)tile;
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  return 0;
+}
+
+int inu_set_source_rates(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* rate) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  if (!cell || !rate) return fail(c, INU_EINVAL, "inu_set_source_rates: null array");
+  if (n > c->n_src) return fail(c, INU_EINVAL, "more rates than source cells");
+  std::vector<uint32_t> ic(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    if (cell[k] >= c->N) return fail(c, INU_EINVAL, "source cell out of range");
+    if (!std::isfinite(rate[k])) return fail(c, INU_EINVAL, "source rate must be finite");
+    ic[k] = internal_cell(c, cell[k]);
+    if (!std::binary_search(c->src_cells.begin(), c->src_cells.end(), ic[k]))
+      return fail(c, INU_EINVAL, "inu_set_source_rates: a cell is not a source cell (use inu_set_sources)");
+  }
+  uint32_t* dcell = nullptr;
+  float* drate = nullptr;
+  hipError_t e = hipMallocAsync((void**)&dcell, (size_t)n * 4, c->stream);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&drate, (size_t)n * 4, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dcell, ic.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(drate, rate, (size_t)n * 4, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_scatter_rates, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->a.src, dcell, drate, n);
+    e = hipGetLastError();
+  }
+  if (dcell) (void)hipFreeAsync(dcell, c->stream);
+  if (drate) (void)hipFreeAsync(drate, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("set_source_rates: ") + hipGetErrorString(e));
   return 0;
 }
 

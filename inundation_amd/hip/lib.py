@@ -7,7 +7,9 @@ hard error (there is no CPU fallback on the product path).
 from __future__ import annotations
 
 import ctypes as C
+from collections.abc import Sequence
 from pathlib import Path
+from typing import Any
 
 import numpy as np
 from numpy.typing import NDArray
@@ -47,11 +49,15 @@ EXPORTS = (
     "inu_destroy",
     "inu_last_error",
     "inu_set_mesh",
+    "inu_set_batch",
+    "inu_scenario_count",
+    "inu_scenario_info",
     "inu_mesh_info",
     "inu_set_params",
     "inu_set_state",
     "inu_set_manning",
     "inu_set_sources",
+    "inu_set_source_rates",
     "inu_ctrl_write",
     "inu_ctrl_read",
     "inu_dt_words",
@@ -64,6 +70,8 @@ EXPORTS = (
     "inu_upload",
     "inu_probe",
     "inu_stream_copy",
+    "inu_autotune",
+    "inu_set_tile",
 )
 
 
@@ -80,6 +88,24 @@ class InuCtrl(C.Structure):
         ("last_raw", C.c_float),
         ("cfl_safety", C.c_float),
         ("pad", C.c_uint32),
+    ]
+
+
+class InuMesh(C.Structure):
+    """``inu_mesh``: one mesh of a batch, as the arguments of ``inu_set_mesh``."""
+
+    _fields_ = [
+        ("n_cells", C.c_uint32),
+        ("n_edges", C.c_uint32),
+        ("edge_cell_l", C.c_void_p),
+        ("edge_cell_r", C.c_void_p),
+        ("nx", C.c_void_p),
+        ("ny", C.c_void_p),
+        ("len", C.c_void_p),
+        ("edge_dz", C.c_void_p),
+        ("cell_area", C.c_void_p),
+        ("cell_edge_ptr", C.c_void_p),
+        ("cell_edge_idx", C.c_void_p),
     ]
 
 
@@ -111,11 +137,15 @@ def load() -> C.CDLL:
         "inu_destroy": (None, [_P]),
         "inu_last_error": (C.c_char_p, [_P]),
         "inu_set_mesh": (_I, [_P, _U32, _U32, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+        "inu_set_batch": (_I, [_P, _U32, _P, _U32, _P]),
+        "inu_scenario_count": (_I, [_P]),
+        "inu_scenario_info": (_I, [_P, _U32, _P]),
         "inu_mesh_info": (_I, [_P, _P]),
         "inu_set_params": (_I, [_P, _F, _F, _F, _U32]),
         "inu_set_state": (_I, [_P, _P, _P, _P, _P]),
         "inu_set_manning": (_I, [_P, _P]),
         "inu_set_sources": (_I, [_P, _U32, _P, _P]),
+        "inu_set_source_rates": (_I, [_P, _U32, _P, _P]),
         "inu_ctrl_write": (_I, [_P, C.POINTER(InuCtrl)]),
         "inu_ctrl_read": (_I, [_P, C.POINTER(InuCtrl)]),
         "inu_dt_words": (_I, [_P, _P]),
@@ -128,6 +158,8 @@ def load() -> C.CDLL:
         "inu_upload": (_I, [_P, _I, _P]),
         "inu_probe": (_I, [_P, _U32, _P, _P, _F, _F]),
         "inu_stream_copy": (_I, [_P, C.c_uint64, _U32, _P]),
+        "inu_autotune": (_I, [_P, _U32, _P]),
+        "inu_set_tile": (_I, [_P, _U32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -156,6 +188,7 @@ class Context:
                 raise RuntimeError("no HIP device available for the hip_resident backend")
             raise RuntimeError(f"inu_create failed with code {rc}")
         self._h = h
+        self.n_scenarios = 0
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -186,6 +219,20 @@ class Context:
     def call(self, name: str, *args: object) -> None:
         self.check(getattr(self._lib, name)(self._h, *args))
 
+    @staticmethod
+    def _mesh_arrays(m: dict[str, Any]) -> list[NDArray]:
+        return [
+            np.ascontiguousarray(m["cell_l"], dtype=np.int32),
+            np.ascontiguousarray(m["cell_r"], dtype=np.int32),
+            np.ascontiguousarray(m["nx"], dtype=np.float32),
+            np.ascontiguousarray(m["ny"], dtype=np.float32),
+            np.ascontiguousarray(m["len"], dtype=np.float32),
+            np.ascontiguousarray(m["dz2"], dtype=np.float32),
+            np.ascontiguousarray(m["area"], dtype=np.float32),
+            np.ascontiguousarray(m["cptr"], dtype=np.uint32),
+            np.ascontiguousarray(m["cidx"], dtype=np.uint32),
+        ]
+
     def set_mesh(
         self,
         cell_l: NDArray[np.int32],
@@ -198,20 +245,29 @@ class Context:
         cptr: NDArray[np.uint32],
         cidx: NDArray[np.uint32],
     ) -> None:
-        arrs = [
-            np.ascontiguousarray(cell_l, dtype=np.int32),
-            np.ascontiguousarray(cell_r, dtype=np.int32),
-            np.ascontiguousarray(nx, dtype=np.float32),
-            np.ascontiguousarray(ny, dtype=np.float32),
-            np.ascontiguousarray(length, dtype=np.float32),
-            np.ascontiguousarray(dz2, dtype=np.float32),
-            np.ascontiguousarray(area, dtype=np.float32),
-            np.ascontiguousarray(cptr, dtype=np.uint32),
-            np.ascontiguousarray(cidx, dtype=np.uint32),
-        ]
-        n_cells = arrs[6].size
-        n_edges = arrs[0].size
-        self.call("inu_set_mesh", n_cells, n_edges, *[ptr(a) for a in arrs])
+        arrs = self._mesh_arrays(
+            {"cell_l": cell_l, "cell_r": cell_r, "nx": nx, "ny": ny, "len": length, "dz2": dz2, "area": area,
+             "cptr": cptr, "cidx": cidx}
+        )
+        self.call("inu_set_mesh", arrs[6].size, arrs[0].size, *[ptr(a) for a in arrs])
+        self.n_scenarios = 1
+
+    def set_batch(self, meshes: Sequence[dict[str, Any]], scenario_mesh: Sequence[int]) -> None:
+        """Scenario s on ``meshes[scenario_mesh[s]]`` (each as ``device_mesh_arrays`` returns it)."""
+        keep = [self._mesh_arrays(m) for m in meshes]
+        descs = (InuMesh * len(keep))()
+        for d, arrs in zip(descs, keep):
+            d.n_cells, d.n_edges = arrs[6].size, arrs[0].size
+            (d.edge_cell_l, d.edge_cell_r, d.nx, d.ny, d.len, d.edge_dz, d.cell_area, d.cell_edge_ptr,
+             d.cell_edge_idx) = [ptr(a) for a in arrs]
+        sm = np.ascontiguousarray(scenario_mesh, dtype=np.uint32)
+        self.call("inu_set_batch", len(keep), C.addressof(descs), sm.size, ptr(sm))
+        self.n_scenarios = int(sm.size)
+
+    def scenario_info(self, s: int) -> dict[str, int]:
+        out = np.zeros(4, dtype=np.uint32)
+        self.call("inu_scenario_info", s, ptr(out))
+        return {"cell0": int(out[0]), "cells": int(out[1]), "block0": int(out[2]), "blocks": int(out[3])}
 
     def mesh_info(self) -> dict[str, int]:
         out = np.zeros(6, dtype=np.uint32)
@@ -244,18 +300,38 @@ class Context:
             raise ValueError("cells and rates differ in length")
         self.call("inu_set_sources", c.size, ptr(c) if c.size else None, ptr(r) if r.size else None)
 
-    def ctrl_write(self, ctrl: InuCtrl) -> None:
-        self.call("inu_ctrl_write", C.byref(ctrl))
+    def set_source_rates(self, cells: NDArray[np.uint32], rates: NDArray[np.float32]) -> None:
+        """New rates for cells already in the source set (zero turns a source off)."""
+        c = np.ascontiguousarray(cells, dtype=np.uint32)
+        r = np.ascontiguousarray(rates, dtype=np.float32)
+        if c.size != r.size:
+            raise ValueError("cells and rates differ in length")
+        self.call("inu_set_source_rates", c.size, ptr(c) if c.size else None, ptr(r) if r.size else None)
+
+    def ctrl_write(self, ctrl: InuCtrl | Sequence[InuCtrl]) -> None:
+        """One control block, or one per scenario."""
+        items = [ctrl] if isinstance(ctrl, InuCtrl) else list(ctrl)
+        if len(items) != self.n_scenarios:
+            raise ValueError(f"{len(items)} control blocks for {self.n_scenarios} scenarios")
+        arr = (InuCtrl * len(items))(*items)
+        self.call("inu_ctrl_write", arr)
 
     def ctrl_read(self) -> InuCtrl:
-        c = InuCtrl()
-        self.call("inu_ctrl_read", C.byref(c))
-        return c
+        """The control block of a one-scenario context."""
+        if self.n_scenarios != 1:
+            raise ValueError("ctrl_read() is for one scenario; use ctrls_read()")
+        return self.ctrls_read()[0]
+
+    def ctrls_read(self) -> list[InuCtrl]:
+        arr = (InuCtrl * self.n_scenarios)()
+        self.call("inu_ctrl_read", arr)
+        return list(arr)
 
     def dt_words(self) -> NDArray[np.uint32]:
-        out = np.zeros(4, dtype=np.uint32)
+        """Four words per scenario: this step's dt bits, next-step bound bits, 0, 0 (flat when one scenario)."""
+        out = np.zeros(4 * self.n_scenarios, dtype=np.uint32)
         self.call("inu_dt_words", ptr(out))
-        return out
+        return out if self.n_scenarios == 1 else out.reshape(self.n_scenarios, 4)
 
     def activity(self, reset: bool = False) -> dict[str, int]:
         """Active cells, owned edges and slots summed over the real steps, and the step count."""
@@ -291,6 +367,15 @@ class Context:
         ms = np.zeros(1, dtype=np.float32)
         self.call("inu_stream_copy", int(n_bytes), int(reps), ptr(ms))
         return float(ms[0])
+
+    def autotune(self, reps: int = 5) -> dict[str, Any]:
+        """Pick the stage kernels' tile level by timing the predictor; returns the choice and the timings."""
+        out = np.zeros(4, dtype=np.uint32)
+        self.call("inu_autotune", int(reps), ptr(out))
+        return {"tile": int(out[0]), "us_per_launch": [int(x) for x in out[1:]]}
+
+    def set_tile(self, tile: int) -> None:
+        self.call("inu_set_tile", int(tile))
 
     def probe(self, pairs: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:
         p = np.ascontiguousarray(pairs, dtype=np.float32).reshape(-1, 2)

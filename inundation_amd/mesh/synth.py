@@ -125,49 +125,57 @@ def voronoi_mesh(
 ) -> tuple[NDArray[np.float64], NDArray[np.int32], NDArray[np.int32]]:
     """Lloyd-relaxed Voronoi cells of seeded uniform points in ``[0, length]^2``.
 
-    Points are mirrored across the four sides so every cell is closed and the
-    boundary cells are clipped to the square. Returns ``(verts, faces_flat,
-    face_offsets)`` with the welded Voronoi vertices.
+    Points within a few mean spacings of a side are mirrored across it, so
+    every cell is closed and the boundary cells end exactly on the square (the
+    bisector of a point and its mirror is the side). Returns ``(verts,
+    faces_flat, face_offsets)`` with the welded Voronoi vertices.
     """
     from scipy.spatial import Voronoi
 
     pts = rng.uniform(0.0, length, size=(n_points, 2))
+    band = 4.0 * length / np.sqrt(max(n_points, 1))
 
     def cells(p: NDArray[np.float64]) -> tuple[Voronoi, NDArray[np.int64]]:
-        mirrored = np.concatenate(
-            [
-                p,
-                np.column_stack([-p[:, 0], p[:, 1]]),
-                np.column_stack([2.0 * length - p[:, 0], p[:, 1]]),
-                np.column_stack([p[:, 0], -p[:, 1]]),
-                np.column_stack([p[:, 0], 2.0 * length - p[:, 1]]),
-            ]
-        )
-        vor = Voronoi(mirrored)
+        parts = [p]
+        for axis, side in ((0, 0.0), (0, length), (1, 0.0), (1, length)):
+            near = p[np.abs(p[:, axis] - side) < band]
+            m = near.copy()
+            m[:, axis] = 2.0 * side - m[:, axis]
+            parts.append(m)
+        vor = Voronoi(np.concatenate(parts))
         return vor, np.asarray(vor.point_region[: p.shape[0]], dtype=np.int64)
+
+    def flat_regions(vor: Voronoi, reg: NDArray[np.int64]) -> tuple[NDArray[np.int64], NDArray[np.int64]]:
+        regions = [vor.regions[r] for r in reg.tolist()]
+        deg = np.fromiter((len(r) for r in regions), dtype=np.int64, count=len(regions))
+        offsets = np.zeros(deg.size + 1, dtype=np.int64)
+        np.cumsum(deg, out=offsets[1:])
+        flat = np.fromiter((v for r in regions for v in r), dtype=np.int64, count=int(offsets[-1]))
+        return flat, offsets
 
     for _ in range(lloyd):
         vor, reg = cells(pts)
-        new = np.empty_like(pts)
-        for k, r in enumerate(reg.tolist()):
-            poly = vor.vertices[vor.regions[r]]
-            xs, ys = poly[:, 0], poly[:, 1]
-            cr = xs * np.roll(ys, -1) - np.roll(xs, -1) * ys
-            a6 = 3.0 * cr.sum()
-            new[k, 0] = ((xs + np.roll(xs, -1)) * cr).sum() / a6
-            new[k, 1] = ((ys + np.roll(ys, -1)) * cr).sum() / a6
-        pts = np.clip(new, 1e-9 * length, length * (1.0 - 1e-9))
+        flat, offsets = flat_regions(vor, reg)
+        if (flat < 0).any():
+            raise RuntimeError("an unbounded Voronoi region inside the box")
+        nxt = np.arange(1, flat.size + 1)
+        nxt[offsets[1:] - 1] = offsets[:-1]  # each polygon's last vertex wraps to its first
+        xs, ys = vor.vertices[flat, 0], vor.vertices[flat, 1]
+        xn, yn = xs[nxt], ys[nxt]
+        cr = xs * yn - xn * ys
+        a6 = 3.0 * np.add.reduceat(cr, offsets[:-1])
+        cx = np.add.reduceat((xs + xn) * cr, offsets[:-1]) / a6
+        cy = np.add.reduceat((ys + yn) * cr, offsets[:-1]) / a6
+        pts = np.clip(np.column_stack([cx, cy]), 1e-9 * length, length * (1.0 - 1e-9))
     vor, reg = cells(pts)
-    regions = [vor.regions[r] for r in reg.tolist()]
-    used = np.unique(np.concatenate([np.asarray(r, dtype=np.int64) for r in regions]))
+    flat, offsets = flat_regions(vor, reg)
+    if (flat < 0).any():
+        raise RuntimeError("an unbounded Voronoi region inside the box")
+    used = np.unique(flat)
     remap = np.full(vor.vertices.shape[0], -1, dtype=np.int64)
     remap[used] = np.arange(used.size)
     verts = np.clip(vor.vertices[used], 0.0, length)
-    deg = np.array([len(r) for r in regions], dtype=np.int64)
-    offsets = np.zeros(deg.size + 1, dtype=np.int64)
-    np.cumsum(deg, out=offsets[1:])
-    faces = remap[np.concatenate([np.asarray(r, dtype=np.int64) for r in regions])]
-    return verts, faces.astype(np.int32), offsets.astype(np.int32)
+    return verts, remap[flat].astype(np.int32), offsets.astype(np.int32)
 
 
 def fbm_grid(m: int, hurst: float, rng: np.random.Generator) -> NDArray[np.float64]:

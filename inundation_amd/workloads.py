@@ -129,4 +129,73 @@ def inflow(n_points: int = 1_000_000, seed: int = 1002, length: float = 2000.0) 
     return wl, phases
 
 
-__all__ = ["Workload", "inflow", "lake_at_rest", "rainfall"]
+def _voronoi_terrain(n_points: int, seed: int, length: float) -> MeshGeometry:
+    rng = np.random.default_rng(seed)
+    verts, faces, offsets = voronoi_mesh(n_points, length, rng)
+    cx, cy = _vertex_means(verts, faces, offsets)
+    zb = fbm_terrain(cx, cy, length, rng)
+    return hilbert_reorder(build_geometry(verts, faces, offsets, zb_from_file=zb), verbose=False)[0]
+
+
+@dataclass
+class Scenario:
+    """One flood of the batched workload: its mesh, inflow peak and rain."""
+
+    mesh: int
+    peak: float  # m^3/s at 30 min, triangular over 1 h
+    rain_mm_h: float
+    inflow_cell: int
+
+    def inflow(self, t_mid: float) -> float:
+        """Discharge of the one-minute phase centred on t_mid (s)."""
+        return self.peak * (t_mid / 1800.0 if t_mid <= 1800.0 else max(3600.0 - t_mid, 0.0) / 1800.0)
+
+    def source_pairs(self, geom: MeshGeometry, t_mid: float) -> tuple[NDArray[np.uint32], NDArray[np.float32]]:
+        """(cell, dh/dt) of the phase: uniform rain on every cell plus the inflow cell."""
+        rain = np.float32(self.rain_mm_h / 1000.0 / 3600.0)
+        if rain > 0.0:
+            cells = np.arange(geom.N, dtype=np.uint32)
+            rates = np.full(geom.N, rain, np.float32)
+        else:
+            cells = np.array([self.inflow_cell], np.uint32)
+            rates = np.zeros(1, np.float32)
+        k = int(np.searchsorted(cells, self.inflow_cell))
+        rates[k] = np.float32(rates[k] + np.float32(self.inflow(t_mid)) / geom.area[self.inflow_cell])
+        return cells, rates
+
+
+def batched(
+    n_scenarios: int = 64,
+    sizes: tuple[int, ...] = (50_000, 200_000, 500_000, 1_000_000, 2_000_000),
+    seed: int = 1004,
+    length: float = 2000.0,
+    processes: int = 5,
+) -> tuple[list[MeshGeometry], list[Scenario]]:
+    """The batched-scenario workload: Voronoi meshes (terrain as ``inflow``) and the scenarios on them.
+
+    Scenario k runs on mesh k mod len(sizes), with a seeded inflow peak
+    U[5, 50] m^3/s into the channel near the upstream edge and uniform rain
+    U[0, 80] mm/h. The meshes are built in parallel worker processes (start
+    them before the GPU is used).
+    """
+    import multiprocessing as mp
+
+    args = [(n, seed * 10 + i, length) for i, n in enumerate(sizes)]
+    if processes > 1 and len(sizes) > 1:
+        with mp.get_context("fork").Pool(min(processes, len(sizes))) as pool:
+            geoms = pool.starmap(_voronoi_terrain, args)
+    else:
+        geoms = [_voronoi_terrain(*x) for x in args]
+    x0 = 0.05 * length
+    y0 = 0.5 * length + 0.1 * length * np.sin(2.0 * np.pi * x0 / length)
+    inlet = [int(np.argmin((g.centroid[:, 0] - x0) ** 2 + (g.centroid[:, 1] - y0) ** 2)) for g in geoms]
+    scen = []
+    for k in range(n_scenarios):
+        rng = np.random.default_rng([seed, k])
+        m = k % len(sizes)
+        scen.append(Scenario(mesh=m, peak=float(rng.uniform(5.0, 50.0)), rain_mm_h=float(rng.uniform(0.0, 80.0)),
+                             inflow_cell=inlet[m]))
+    return geoms, scen
+
+
+__all__ = ["Scenario", "Workload", "batched", "inflow", "lake_at_rest", "rainfall"]
