@@ -232,7 +232,7 @@ def main() -> int:
     return 0
 
 
-def _phase_loop(ctx, geoms, scen, n_phases: int, block: int, first: int = 0) -> None:  # noqa: ANN001
+def _phase_loop(ctx, geoms, scen, n_phases: int, block: int, first: int = 0, verbose: bool = False) -> None:  # noqa: ANN001
     """Run every scenario of ctx through the one-minute phases: sources, retarget, steps to the stop."""
     sel = scen[first:first + ctx.n_scenarios]
     base = np.cumsum([0] + [geoms[s.mesh].N for s in sel])
@@ -260,6 +260,8 @@ def _phase_loop(ctx, geoms, scen, n_phases: int, block: int, first: int = 0) -> 
             ctx.steps(block, use_graph=True)
             if all(int(c.now_ticks) >= int(c.stop_ticks) for c in ctx.ctrls_read()):
                 break
+        if verbose:
+            log(f"phase {j + 1}/{n_phases} done")
 
 
 def _start_ctrls(n: int):  # noqa: ANN202
@@ -296,7 +298,7 @@ def bench_batched(args) -> int:  # noqa: ANN001
 
     sizes = tuple(int(x) for x in args.sizes.split(","))
     t0 = time.perf_counter()
-    geoms, scen = batched(n_scenarios=args.scenarios, sizes=sizes)
+    geoms, scen = batched(n_scenarios=args.scenarios, sizes=sizes, progress=log)
     log(f"{len(geoms)} meshes {[g.N for g in geoms]} built in {time.perf_counter() - t0:.1f}s; {len(scen)} scenarios")
     dev = [device_mesh_arrays(g) for g in geoms]
     n_phases = max(1, int(round(args.t_end / 60.0)))
@@ -326,6 +328,7 @@ def bench_batched(args) -> int:  # noqa: ANN001
             _phase_loop(ctx, geoms, scen, n_phases, block, first=k)
             seq_wall += time.perf_counter() - t0
             seq_steps[k] = int(ctx.ctrl_read().steps)
+            log(f"sequential: scenario {k} on mesh {m}: {seq_steps[k]} steps, {seq_wall:.1f}s so far")
         ctx.close()
         log(f"sequential: mesh {m} ({g.N} cells) done, {seq_wall:.1f}s so far")
     seq_cell_steps = sum(seq_steps[k] * geoms[s.mesh].N for k, s in enumerate(scen))
@@ -349,7 +352,7 @@ def bench_batched(args) -> int:  # noqa: ANN001
     log(f"autotune: tile {tune['tile']}, predictor us/launch by tile {tune['us_per_launch']}")
     setup(ctx, scen)
     t0 = time.perf_counter()
-    _phase_loop(ctx, geoms, scen, n_phases, block)
+    _phase_loop(ctx, geoms, scen, n_phases, block, verbose=True)
     bat_wall = time.perf_counter() - t0
     bat_steps = [int(c.steps) for c in ctx.ctrls_read()]
     ctx.close()

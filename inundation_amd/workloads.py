@@ -16,6 +16,7 @@ for the lattice quads).
 
 from __future__ import annotations
 
+from collections.abc import Callable
 from dataclasses import dataclass
 
 import numpy as np
@@ -170,6 +171,7 @@ def batched(
     seed: int = 1004,
     length: float = 2000.0,
     processes: int = 5,
+    progress: Callable[[str], None] | None = None,
 ) -> tuple[list[MeshGeometry], list[Scenario]]:
     """The batched-scenario workload: Voronoi meshes (terrain as ``inflow``) and the scenarios on them.
 
@@ -183,9 +185,18 @@ def batched(
     args = [(n, seed * 10 + i, length) for i, n in enumerate(sizes)]
     if processes > 1 and len(sizes) > 1:
         with mp.get_context("fork").Pool(min(processes, len(sizes))) as pool:
-            geoms = pool.starmap(_voronoi_terrain, args)
+            pending = [pool.apply_async(_voronoi_terrain, x) for x in args]
+            geoms = []
+            for x, p in zip(args, pending):
+                geoms.append(p.get())
+                if progress:
+                    progress(f"mesh of {x[0]} points built")
     else:
-        geoms = [_voronoi_terrain(*x) for x in args]
+        geoms = []
+        for x in args:
+            geoms.append(_voronoi_terrain(*x))
+            if progress:
+                progress(f"mesh of {x[0]} points built")
     x0 = 0.05 * length
     y0 = 0.5 * length + 0.1 * length * np.sin(2.0 * np.pi * x0 / length)
     inlet = [int(np.argmin((g.centroid[:, 0] - x0) ** 2 + (g.centroid[:, 1] - y0) ** 2)) for g in geoms]
