@@ -229,6 +229,46 @@ int inu_autotune(inu_ctx* ctx, uint32_t reps, uint32_t* out4);
 /* Sets the tile level (0, 1 or 2) directly; inu_set_mesh/inu_set_batch reset it to 0. */
 int inu_set_tile(inu_ctx* ctx, uint32_t tile);
 
+/* Bed elevation per cell: recomputes every edge's bed step with the face on
+ * the higher bed (reference mesh/geometry.py:241-254, edge_bed_steps, FP64
+ * then FP32; the left cell on a tie, walls none). For differentiating and
+ * perturbing the bed without a new mesh. */
+int inu_set_bed(inu_ctx* ctx, const float* z);
+
+/* ---- Adjoint (BASELINE differentiation extension; the reference has no
+ * gradient path). Reverse mode of recorded steps, for d loss / d (Manning n,
+ * bed z) of a loss on sampled gauge depths. One scenario; source rates must
+ * be non-negative. The forward is recorded at step 0 after inu_prologue with
+ * inu_adj_steps (CFL-driven steps as inu_steps, plus the dt of each step,
+ * gauge depths h + h_lo after every `every` steps and the state every `seg`
+ * steps); inu_set_source_rates may change rates between recorded steps (it is
+ * replayed). inu_adj_backward replays each `seg`-step segment from its
+ * checkpoint with the recorded dt and rates (device memory: seg x 56 bytes
+ * per cell for the replayed segment) and runs it in reverse, following the
+ * forward's branches. After it the context's state is that of the replay and
+ * must be set again before further steps. */
+int inu_adj_begin(inu_ctx* ctx, uint32_t max_steps, uint32_t seg, uint32_t n_gauges, const uint32_t* gauges,
+                  uint32_t every);
+int inu_adj_steps(inu_ctx* ctx, uint32_t n);
+/* out = gauge depths, (max_steps / every) rows of n_gauges; row r after step (r + 1) every. */
+int inu_adj_series(inu_ctx* ctx, float* out);
+/* seeds = d loss / d series (same shape as inu_adj_series). Out: grad_n and
+ * grad_z per cell; grad_state (NULL, or 3 N words) = d loss / d (h, hu, hv)
+ * of the recording's initial state. */
+int inu_adj_backward(inu_ctx* ctx, const float* seeds, float* grad_n, float* grad_z, float* grad_state);
+/* The recorded steps again from the recording's start, with the recorded dt
+ * and source rates and the current Manning n and bed (for finite
+ * differences); out as inu_adj_series. Leaves the replay's state. */
+int inu_adj_replay(inu_ctx* ctx, float* series_out);
+/* The recorded dt of each step (float bits; max_steps words, 0 past the last). */
+int inu_adj_dt_log(inu_ctx* ctx, uint32_t* out);
+int inu_adj_end(inu_ctx* ctx);
+
+/* Edge-word VJP probe: per row in17 = {hL, huL, hvL, hR, huR, hvR, dzL, dzR,
+ * nx, ny, len, wall, dloss/dword 0..4}; out8 = d loss / d (hL, huL, hvL, hR,
+ * huR, hvR, dzL, dzR). Test support with no reference counterpart. */
+int inu_probe_vjp(inu_ctx* ctx, uint32_t n, const float* in17, float* out8, float g, float dry_tol);
+
 #ifdef __cplusplus
 }
 #endif

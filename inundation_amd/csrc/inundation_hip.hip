@@ -35,6 +35,7 @@
 #include <string>
 #include <vector>
 
+#include "swe_adjoint.h"
 #include "swe_math.h"
 #include "../../include/inundation_hip.h"
 
@@ -118,7 +119,15 @@ struct Args {
   Ctl* ctl;
   Scen* scen;
   uint32_t n_scen;
+  // adjoint tape (recording stage kernels): kTapeWords arrays of N per step slot
+  float* tape;
+  uint32_t tape_slot;
 };
+
+// Tape words per cell and recorded step: stage 0 input (h, hu, hv), its
+// updated state before friction (h', hu', hv') and clamp flag; the same for
+// stage 1.
+constexpr uint32_t kTapeWords = 14;
 
 // ---------------------------------------------------------------------------
 // Reductions
@@ -298,7 +307,7 @@ struct Ahead {
 // issued at once (cells, ghost cells, CSR slots, the thread's first edge
 // records); (B) each edge touching the block is evaluated once into LDS from
 // LDS; (C) each cell sums its edges in CSR order from LDS and updates.
-template <int ST, bool COMP, bool FM, int TILE>
+template <int ST, bool COMP, bool FM, int TILE, bool REC>
 __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, uint32_t gid, float dt,
                                 const Lds& l, Ahead& nx, uint32_t end, uint32_t per) {
   constexpr bool LO = COMP && FM;  // the fast order's face depths read the low words
@@ -404,6 +413,18 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     }
     float hu_new = FM ? fmaf(-dti, sdhu, hui) : hui - dti * sdhu;
     float hv_new = FM ? fmaf(-dti, sdhv, hvi) : hvi - dti * sdhv;
+    if (REC) {
+      // Adjoint tape of this stage: input, updated state before friction, clamp.
+      float* t = a.tape + (size_t)(a.tape_slot * kTapeWords + (ST == 0 ? 0u : 7u)) * a.N + i;
+      const size_t q = a.N;
+      t[0] = hi + loi;
+      t[q] = hui;
+      t[2 * q] = hvi;
+      t[3 * q] = h_new + lo_new;
+      t[4 * q] = hu_new;
+      t[5 * q] = hv_new;
+      t[6 * q] = (h_raw < 0.0f) ? 1.0f : 0.0f;
+    }
     if (a.track_clamp) stg(a.clamp, i, ldg(a.clamp, i) + 0.5f * A * (h_new - h_raw));
     if (h_new < p.dry) {
       hu_new = 0.0f;
@@ -514,7 +535,7 @@ __device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t en
   if (nx.t + per < end) nx.b2 = a.list[nx.t + per];
 }
 
-template <int ST, bool COMP, bool FM, int TILE>
+template <int ST, bool COMP, bool FM, int TILE, bool REC = false>
 __global__ __launch_bounds__(256) void k_stage(const Args a) {
   extern __shared__ float sm[];
   if (a.ctl->any_dt == 0u) return;  // no-op step: every scenario is at its stop
@@ -556,7 +577,7 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
       walk_ahead(a, nx, end, per);
       continue;
     }
-    const uint32_t bb = stage_block<ST, COMP, FM, TILE>(a, b, bi, gid, __uint_as_float(dtw), l, nx, end, per);
+    const uint32_t bb = stage_block<ST, COMP, FM, TILE, REC>(a, b, bi, gid, __uint_as_float(dtw), l, nx, end, per);
     bound = (bb < bound) ? bb : bound;
   }
   if (ST == 1) block_atomic_min(&a.scen[cur_s].dt_next, bound);
@@ -929,6 +950,216 @@ __global__ void k_scatter_rates(float* src, const uint32_t* cells, const float* 
   if (k < n) src[cells[k]] = rates[k];
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint: reverse mode of recorded steps
+// ---------------------------------------------------------------------------
+
+// Arguments of one reverse stage. Adjoint arrays hold three arrays of N
+// words each: d loss / d h, d hu, d hv.
+struct AdjArgs {
+  Args a;
+  const float* tape;     // this step's slot: kTapeWords arrays of N
+  float dt;              // this step's dt
+  const float* lam_out;  // d loss / d stage output
+  const float* lam_add;  // added to the result (the Heun average's direct part) or null
+  float* lam_in;         // d loss / d stage input
+  float* grad_n;         // += d loss / d Manning n, per cell
+  float2* grad_dz;       // += d loss / d (dzL, dzR), per owned edge of the local edge stream
+};
+
+// Reverse of one stage over every block (reference resident.py:401-516 and
+// :187-287 differentiated). Per block: (A) the friction reverse of its cells
+// and ghost cells from the tape gives each one's d loss / d sums; (B) each
+// local edge's word gradients from its two sides, then the edge's VJP into
+// LDS, and owned edges add their bed-step gradients; (C) each cell sums its
+// edges' contributions in CSR order. Deterministic: no atomics.
+template <int ST>
+__global__ __launch_bounds__(256) void k_adj_stage(const AdjArgs x) {
+  extern __shared__ float sm[];
+  const Args& a = x.a;
+  const Lds l = carve(sm, a.max_local, a.max_ghost);
+  const Phys& p = a.p;
+  const uint32_t tid = threadIdx.x;
+  const size_t N = a.N;
+  const float* tp = x.tape + (ST == 0 ? 0u : 7u) * N;
+  for (uint32_t b = blockIdx.x; b < a.NB; b += gridDim.x) {
+    const BlockInfo bi = a.binfo[b];
+    const uint32_t c0 = b << 8, nc = bi_cells(bi);
+    float rh = 0.0f, rhu = 0.0f, rhv = 0.0f;
+    __syncthreads();  // the previous block's LDS reads are done
+    // ---- (A) cells and ghosts -----------------------------------------------
+    for (uint32_t q = tid; q < kBlockCells + bi.n_ghost; q += kBlockCells) {
+      const bool own = q < kBlockCells;
+      if (own && q >= nc) continue;
+      const uint32_t i = own ? c0 + q : a.gx[bi.g0 + (q - kBlockCells)];
+      const float hin = tp[i], huin = tp[N + i], hvin = tp[2 * N + i];
+      const float hp = tp[3 * N + i], hup = tp[4 * N + i], hvp = tp[5 * N + i];
+      const bool clamped = tp[6 * N + i] != 0.0f;
+      float gh, ghu, ghv, gn;
+      inu::adj::friction_vjp(p, hp, hup, hvp, clamped, a.nman[i], x.dt, x.lam_out[i], x.lam_out[N + i],
+                             x.lam_out[2 * N + i], gh, ghu, ghv, gn);
+      if (own) {
+        rh = gh;
+        rhu = ghu;
+        rhv = ghv;
+        x.grad_n[i] += gn;
+      }
+      const float dti = x.dt / a.area[i];
+      l.cell[2 * q] = make_float4(hin, huin, hvin, 0.0f);
+      l.cell[2 * q + 1] = make_float4(-dti * gh, -dti * ghu, -dti * ghv, 0.0f);
+    }
+    __syncthreads();
+    // ---- (B) edges -------------------------------------------------------------
+    for (uint32_t j = tid; j < bi.n_loc; j += kBlockCells) {
+      const EdgeRec r = load_edge(a, bi, j);
+      const uint32_t ql = r.side >> 16, qr = r.side & 0xffffu;
+      const bool wall = qr == kSideWall;
+      const float4 sl = l.cell[2 * ql], bl = l.cell[2 * ql + 1];
+      const float4 sr = wall ? sl : l.cell[2 * qr];
+      const float4 br = wall ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : l.cell[2 * qr + 1];
+      // w0 enters the left sum and leaves the right one; w1, w2 are the
+      // left momentum words, w3, w4 the right ones (subtracted).
+      const float lw[5] = {bl.x - br.x, bl.y, bl.z, -br.y, -br.z};
+      const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
+      const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
+      float gL[3], gR[3], gdz[2];
+      inu::adj::edge_vjp(p, wall, sl.x, sl.y, sl.z, sr.x, sr.y, sr.z, dzL, dzR, r.nx, r.ny, r.len, lw, gL, gR, gdz);
+      l.edge[2 * j] = make_float4(gL[0], gL[1], gL[2], 0.0f);
+      l.edge[2 * j + 1] = make_float4(gR[0], gR[1], gR[2], 0.0f);
+      if (j < bi.n_own && !wall) {
+        float2 acc = x.grad_dz[bi.e0 + j];
+        acc.x += gdz[0];
+        acc.y += gdz[1];
+        x.grad_dz[bi.e0 + j] = acc;
+      }
+    }
+    __syncthreads();
+    // ---- (C) cells ---------------------------------------------------------------
+    if (tid < nc) {
+      const uint32_t i = c0 + tid;
+      float sh = rh, shu = rhu, shv = rhv;
+      const uint32_t k1 = a.cptr[i + 1];
+      for (uint32_t k = a.cptr[i]; k < k1; ++k) {
+        const uint32_t s = a.lidx[k];
+        const float4 g = l.edge[2 * (s >> 1) + (s & 1u)];
+        sh += g.x;
+        shu += g.y;
+        shv += g.z;
+      }
+      if (x.lam_add) {
+        sh += x.lam_add[i];
+        shu += x.lam_add[N + i];
+        shv += x.lam_add[2 * N + i];
+      }
+      x.lam_in[i] = sh;
+      x.lam_in[N + i] = shu;
+      x.lam_in[2 * N + i] = shv;
+    }
+  }
+}
+
+// Reverse of the Heun average (reference resident.py:489-501): from d loss /
+// d step output to the corrector output's and the step input's direct part.
+// Sources are constant additions (non-negative rates, so the source clamp
+// never cuts) and pass the depth gradient through.
+__global__ void k_adj_heun(const float* tape, uint32_t n, float dry, const float* lam, float* lam_c, float* lam_d) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float h0 = tape[i], h2 = tape[10u * (size_t)n + i];
+  const bool wet = 0.5f * (h0 + h2) >= dry;
+  const float bh = 0.5f * lam[i];
+  const float bhu = wet ? 0.5f * lam[n + i] : 0.0f;
+  const float bhv = wet ? 0.5f * lam[2 * (size_t)n + i] : 0.0f;
+  lam_c[i] = bh;
+  lam_c[n + i] = bhu;
+  lam_c[2 * (size_t)n + i] = bhv;
+  lam_d[i] = bh;
+  lam_d[n + i] = bhu;
+  lam_d[2 * (size_t)n + i] = bhv;
+}
+
+// d loss / d sampled gauge depth into d loss / d h.
+__global__ void k_adj_seed(float* lam, const uint32_t* gauges, const float* seed, uint32_t n_g) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_g) lam[gauges[k]] += seed[k];
+}
+
+// Replay of a recorded step: its dt (every block stays listed and active).
+__global__ void k_adj_set_dt(const Args a, uint32_t dt_bits) {
+  a.scen[0].dt_cur = dt_bits;
+  a.ctl->any_dt = 1u;
+}
+
+// Replay set-up: every block listed and active, no lists of skipped blocks.
+__global__ void k_adj_all_blocks(const Args a) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) {
+    a.ctl->cnt_cur = a.NB;
+    a.ctl->idle_cur = 0u;
+  }
+  if (b < a.NB) {
+    a.list[b] = b;
+    a.active[b] = 1u;
+  }
+}
+
+// Recording, before a step: this step's dt, at the index of the step.
+__global__ void k_adj_log(const Args a, uint32_t* log, uint32_t max_steps) {
+  const uint64_t k = a.scen[0].ctrl.steps;
+  if (k < max_steps) log[k] = a.scen[0].dt_cur;
+}
+
+// Recording, after a step: gauge depths every `every` steps, and the state
+// every `seg` steps. A no-op step rewrites the same values.
+__global__ void k_adj_sample(const Args a, float* series, const uint32_t* gauges, uint32_t n_g, uint32_t every,
+                             uint32_t max_samples) {
+  const uint64_t k = a.scen[0].ctrl.steps;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0 || k % every != 0 || k / every > max_samples || j >= n_g) return;
+  const uint32_t g = gauges[j];
+  series[(k / every - 1) * n_g + j] = a.h[g] + a.hlo[g];
+}
+
+// Replay sample: gauge depths into row `row`.
+__global__ void k_adj_sample_row(const Args a, float* series, const uint32_t* gauges, uint32_t n_g, uint32_t row) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_g) return;
+  const uint32_t g = gauges[j];
+  series[(size_t)row * n_g + j] = a.h[g] + a.hlo[g];
+}
+
+__global__ void k_adj_checkpoint(const Args a, float* ck, uint32_t seg, uint32_t max_ck) {
+  const uint64_t k = a.scen[0].ctrl.steps;
+  if (k % seg != 0 || k / seg >= max_ck) return;
+  float* dst = ck + (k / seg) * 4ull * a.N;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < a.N; i += (size_t)gridDim.x * blockDim.x) {
+    dst[i] = a.h[i];
+    dst[a.N + i] = a.hu[i];
+    dst[2 * (size_t)a.N + i] = a.hv[i];
+    dst[3 * (size_t)a.N + i] = a.hlo[i];
+  }
+}
+
+// Edge-word VJP probe: per row in17 = {hL, huL, hvL, hR, huR, hvR, dzL, dzR,
+// nx, ny, len, wall, lw0..lw4}, out8 = {gL, gR, gdzL, gdzR}.
+__global__ void k_probe_vjp(const float* in, float* out, uint32_t n, Phys p) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float* r = in + 17ull * k;
+  float gL[3], gR[3], gdz[2];
+  inu::adj::edge_vjp(p, r[11] != 0.0f, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r + 12, gL,
+                     gR, gdz);
+  float* o = out + 8ull * k;
+  o[0] = gL[0];
+  o[1] = gL[1];
+  o[2] = gL[2];
+  o[3] = gR[0];
+  o[4] = gR[1];
+  o[5] = gR[2];
+  o[6] = gdz[0];
+  o[7] = gdz[1];
+}
+
 // Plain copy of n 16-byte words: the kernel of known traffic that HBM byte
 // counters are scaled against.
 __global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ in, float4* __restrict__ out, size_t n) {
@@ -990,6 +1221,23 @@ struct inu_ctx {
   size_t lds_stage0 = 0, lds_stage1 = 0;
   // (cell, neighbour) for every neighbour in another block, by cell
   std::vector<std::pair<uint32_t, uint32_t>> cross;
+  // Edges (union numbering), for the bed gradient and inu_set_bed: cells
+  // (internal ids, -1 wall), each stream position's edge, each edge's owner
+  // position, and the signed bed steps in stream order.
+  std::vector<int32_t> e_cl, e_cr;
+  std::vector<uint32_t> stream_edge, own_pos;
+  std::vector<float> h_ldz;
+  bool src_nonneg = true;  // every source rate so far is >= 0
+  // adjoint recording
+  struct {
+    bool on = false;
+    uint32_t max_steps = 0, seg = 1, every = 1, n_gauges = 0, max_samples = 0, n_ck = 0;
+    uint32_t* log = nullptr;
+    uint32_t* gauges = nullptr;
+    float* series = nullptr;
+    float* ck = nullptr;
+    std::vector<std::pair<uint32_t, float*>> events;  // (step, dense source rates from that step)
+  } adj;
   // graphs of recorded step blocks, by step count
   std::map<uint32_t, hipGraphExec_t> graphs;
   std::vector<hipEvent_t> events;
@@ -1064,6 +1312,8 @@ void free_all(inu_ctx* c) {
   c->a = Args{};
   c->ctl = nullptr;
   c->scen = nullptr;
+  c->adj.on = false;
+  c->adj.events.clear();
   c->n_scen = 0;
   c->has_mesh = false;
 }
@@ -1209,6 +1459,7 @@ struct Layout {
   std::vector<float> slen;
   std::vector<std::pair<uint32_t, uint32_t>> cross;
   std::vector<uint32_t> bptr, bidx;
+  std::vector<uint32_t> stream_edge;  // local edge stream position -> edge
 };
 
 // Validates one mesh and builds its layout.
@@ -1269,6 +1520,7 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
   ldz.assign((size_t)E + hp[NB], 0.0f);
   std::vector<uint32_t>& gxv = L.gx;
   gxv.clear();
+  L.stream_edge.assign((size_t)E + hp[NB], 0u);
   std::vector<uint32_t> ghosts;
   uint32_t e0 = 0;
   for (uint32_t b = 0; b < NB; ++b) {
@@ -1294,6 +1546,7 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
       memcpy(&by, &edge_ny[e], 4);
       memcpy(&bl, &edge_len[e], 4);
       lrec[e0 + j] = make_uint4((ql << 16) | qr, bx, by, bl);
+      L.stream_edge[e0 + j] = e;
       ldz[e0 + j] = (edge_dz[2 * e] > 0.0f) ? edge_dz[2 * e] : -edge_dz[2 * e + 1];
     }
     binfo[b] = BlockInfo{e0, n_loc, n_own, (uint32_t)gxv.size(), (uint32_t)ghosts.size(), cell_edge_ptr[cb],
@@ -1433,7 +1686,13 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   std::vector<uint32_t> bptr(NB + 1, 0), bidx;
   bidx.reserve(ring_tot);
   std::vector<float> area(NP, 0.0f);
-  uint32_t s_off = 0;
+  std::vector<int32_t> e_cl, e_cr;
+  e_cl.reserve(e_tot);
+  e_cr.reserve(e_tot);
+  std::vector<uint32_t> stream_edge;
+  stream_edge.reserve(l_tot);
+  std::vector<uint32_t> own_pos(e_tot, 0u);
+  uint32_t s_off = 0, ed_off = 0;
   for (uint32_t s = 0; s < n_scen; ++s) {
     const Layout& L = lay[scen_mesh[s]];
     const MeshIn& m = ins[scen_mesh[s]];
@@ -1449,6 +1708,15 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
       bptr[b0 + b + 1] = bptr[b0 + b] + (L.bptr[b + 1] - L.bptr[b]);
     }
     for (uint32_t n : L.bidx) bidx.push_back(b0 + n);
+    for (uint32_t b = 0; b < L.NB; ++b)
+      for (uint32_t j = 0; j < L.binfo[b].n_own; ++j)
+        own_pos[ed_off + L.stream_edge[L.binfo[b].e0 + j]] = e_off + L.binfo[b].e0 + j;
+    for (uint32_t e : L.stream_edge) stream_edge.push_back(ed_off + e);
+    for (uint32_t e = 0; e < L.E; ++e) {
+      e_cl.push_back((int32_t)(c0 + (uint32_t)m.ecl[e]));
+      e_cr.push_back(m.ecr[e] < 0 ? -1 : (int32_t)(c0 + (uint32_t)m.ecr[e]));
+    }
+    ed_off += L.E;
     lrec.insert(lrec.end(), L.lrec.begin(), L.lrec.end());
     ldz.insert(ldz.end(), L.ldz.begin(), L.ldz.end());
     for (uint32_t g : L.gx) gx.push_back(c0 + g);
@@ -1528,6 +1796,12 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   if ((rc = upload(c, dbptr, bptr.data(), NB + 1))) return rc;
   if ((rc = upload(c, dbidx, bidx.data(), bidx.size()))) return rc;
   c->cross.swap(cross);
+  c->e_cl.swap(e_cl);
+  c->e_cr.swap(e_cr);
+  c->stream_edge.swap(stream_edge);
+  c->own_pos.swap(own_pos);
+  c->h_ldz.swap(ldz);
+  c->src_nonneg = true;
   c->lds_stage0 = lds_words(max_local, max_slots, max_ghost) * sizeof(float);
   c->lds_stage1 = c->lds_stage0;
   if (c->lds_stage1 > 160 * 1024) return fail(c, INU_ERANGE, "a block's edge list does not fit in LDS");
@@ -1686,6 +1960,7 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   int rc = check_ready(c);
   if (rc) return rc;
   if (n && (!cell || !rate)) return fail(c, INU_EINVAL, "inu_set_sources: null array");
+  if (c->adj.on) return fail(c, INU_ESTATE, "inu_set_sources during an adjoint recording (use inu_set_source_rates)");
   std::vector<float> dense((size_t)c->NB * kBlockCells, 0.0f);
   std::vector<uint32_t> has(c->NB, 0), cells;
   cells.reserve(n);
@@ -1737,6 +2012,9 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   c->a.src_cell = dcell;
   c->a.src_blocks = dblk;
   c->src_cells = cells;
+  c->src_nonneg = true;
+  for (uint32_t k = 0; k < n; ++k)
+    if (rate[k] < 0.0f) c->src_nonneg = false;
   c->n_src = (uint32_t)cells.size();
   c->n_src_blocks = (uint32_t)blocks.size();
   c->a.n_src = c->n_src;
@@ -2051,6 +2329,8 @@ int inu_set_tile(inu_ctx* c, uint32_t tile) {
   return 0;
 }
 
+static bool rc_nonneg_required(const inu_ctx* c) { return !c->src_nonneg; }
+
 int inu_set_source_rates(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* rate) {
   int rc = check_ready(c);
   if (rc) return rc;
@@ -2079,6 +2359,397 @@ int inu_set_source_rates(inu_ctx* c, uint32_t n, const uint32_t* cell, const flo
   if (drate) (void)hipFreeAsync(drate, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(c, INU_EHIP, std::string("set_source_rates: ") + hipGetErrorString(e));
+  for (uint32_t k = 0; k < n; ++k)
+    if (rate[k] < 0.0f) c->src_nonneg = false;
+  if (c->adj.on) {
+    // The recording replays these rates from the current step on.
+    if (rc_nonneg_required(c)) return fail(c, INU_EINVAL, "the adjoint needs non-negative source rates");
+    Scen sc;
+    HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
+    const uint32_t step = (uint32_t)sc.ctrl.steps;
+    const size_t NP = (size_t)c->NB * kBlockCells;
+    float* ev = nullptr;
+    if (!c->adj.events.empty() && c->adj.events.back().first == step) {
+      ev = c->adj.events.back().second;
+    } else {
+      if ((rc = dalloc(c, &ev, NP))) return rc;
+      c->adj.events.emplace_back(step, ev);
+    }
+    HIPCHK(c, hipMemcpy(ev, c->a.src, NP * 4, hipMemcpyDeviceToDevice));
+  }
+  return 0;
+}
+
+int inu_set_bed(inu_ctx* c, const float* z) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!z) return fail(c, INU_EINVAL, "inu_set_bed: null array");
+  std::vector<float> zi((size_t)c->NB * kBlockCells, 0.0f);
+  for (uint32_t s = 0; s < c->n_scen; ++s)
+    std::copy(z + c->cell0[s], z + c->cell0[s + 1], zi.begin() + (size_t)c->block0[s] * kBlockCells);
+  for (uint32_t i = 0; i < c->N; ++i)
+    if (!std::isfinite(z[i])) return fail(c, INU_EINVAL, "bed elevation must be finite");
+  // Face on the higher bed, steps in FP64 then FP32 (reference mesh/geometry.py:241-254).
+  for (size_t q = 0; q < c->stream_edge.size(); ++q) {
+    const uint32_t e = c->stream_edge[q];
+    const int32_t r = c->e_cr[e];
+    if (r < 0) {
+      c->h_ldz[q] = 0.0f;
+      continue;
+    }
+    const double zl = zi[c->e_cl[e]], zr = zi[r];
+    const double zf = std::max(zl, zr);
+    const float dzl = (float)(zf - zl), dzr = (float)(zf - zr);
+    c->h_ldz[q] = (dzl > 0.0f) ? dzl : -dzr;
+  }
+  return upload(c, const_cast<float*>(c->a.ldz), c->h_ldz.data(), c->h_ldz.size());
+}
+
+int inu_adj_begin(inu_ctx* c, uint32_t max_steps, uint32_t seg, uint32_t n_gauges, const uint32_t* gauges,
+                  uint32_t every) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (c->n_scen != 1) return fail(c, INU_EINVAL, "the adjoint records one scenario");
+  if (max_steps == 0 || seg == 0 || every == 0 || n_gauges == 0 || !gauges)
+    return fail(c, INU_EINVAL, "inu_adj_begin: steps, segment, sampling interval and gauges must be given");
+  if (!c->src_nonneg) return fail(c, INU_EINVAL, "the adjoint needs non-negative source rates");
+  for (uint32_t k = 0; k < n_gauges; ++k)
+    if (gauges[k] >= c->N) return fail(c, INU_EINVAL, "gauge cell out of range");
+  Scen sc;
+  HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
+  if (sc.ctrl.steps != 0ull) return fail(c, INU_ESTATE, "start the recording at step 0 (after inu_prologue)");
+  auto& d = c->adj;
+  d.on = false;
+  d.events.clear();
+  d.max_steps = max_steps;
+  d.seg = seg;
+  d.every = every;
+  d.n_gauges = n_gauges;
+  d.max_samples = max_steps / every;
+  d.n_ck = (max_steps + seg - 1) / seg + 1;
+  const size_t NP = (size_t)c->NB * kBlockCells;
+  std::vector<uint32_t> gi(n_gauges);
+  for (uint32_t k = 0; k < n_gauges; ++k) gi[k] = internal_cell(c, gauges[k]);
+  rc |= dalloc(c, &d.log, max_steps);
+  rc |= dalloc(c, &d.gauges, n_gauges);
+  rc |= dalloc(c, &d.series, (size_t)std::max(d.max_samples, 1u) * n_gauges);
+  rc |= dalloc(c, &d.ck, (size_t)d.n_ck * 4 * NP);
+  float* ev = nullptr;
+  rc |= dalloc(c, &ev, NP);
+  if (rc) return rc;
+  if ((rc = upload(c, d.gauges, gi.data(), gi.size()))) return rc;
+  HIPCHK(c, hipMemsetAsync(d.series, 0, (size_t)std::max(d.max_samples, 1u) * n_gauges * 4, c->stream));
+  const Args& a = c->a;
+  HIPCHK(c, hipMemcpyAsync(d.ck, a.h, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.ck + NP, a.hu, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.ck + 2 * NP, a.hv, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.ck + 3 * NP, a.hlo, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ev, a.src, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  d.events.emplace_back(0u, ev);
+  d.on = true;
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  return 0;
+}
+
+int inu_adj_steps(inu_ctx* c, uint32_t n) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  auto& d = c->adj;
+  if (!d.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
+  if (n == 0) return 0;
+  const Args& a = c->a;
+  const uint32_t key = n | 0x80000000u;
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (uint32_t k = 0; k < n; ++k) {
+      hipLaunchKernelGGL(k_adj_log, dim3(1), dim3(1), 0, c->stream, a, d.log, d.max_steps);
+      if ((rc = launch_step(c, nullptr))) {
+        hipGraph_t dummy = nullptr;
+        (void)hipStreamEndCapture(c->stream, &dummy);
+        if (dummy) (void)hipGraphDestroy(dummy);
+        return rc;
+      }
+      hipLaunchKernelGGL(k_adj_sample, dim3((d.n_gauges + 255) / 256), dim3(256), 0, c->stream, a, d.series,
+                         d.gauges, d.n_gauges, d.every, d.max_samples);
+      hipLaunchKernelGGL(k_adj_checkpoint, dim3(std::min<uint32_t>(c->NB, 1024u)), dim3(256), 0, c->stream, a,
+                         d.ck, d.seg, d.n_ck);
+    }
+    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(c, INU_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    it = c->graphs.emplace(key, ex).first;
+  }
+  HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inu_adj_series(inu_ctx* c, float* out) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  auto& d = c->adj;
+  if (!d.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
+  if (!out) return fail(c, INU_EINVAL, "null output");
+  HIPCHK(c, hipMemcpy(out, d.series, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int inu_adj_backward(inu_ctx* c, const float* seeds, float* grad_n, float* grad_z, float* grad_state) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  auto& d = c->adj;
+  if (!d.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
+  if (!seeds || !grad_n || !grad_z) return fail(c, INU_EINVAL, "null array");
+  Scen sc;
+  HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
+  const uint64_t K64 = sc.ctrl.steps;
+  if (K64 == 0 || K64 > d.max_steps) return fail(c, INU_ESTATE, "recorded steps must be in [1, max_steps]");
+  const uint32_t K = (uint32_t)K64;
+  const size_t NP = (size_t)c->NB * kBlockCells;
+  const size_t L = c->h_ldz.size();
+  std::vector<uint32_t> log(K);
+  HIPCHK(c, hipMemcpy(log.data(), d.log, (size_t)K * 4, hipMemcpyDeviceToHost));
+  for (uint32_t k = 0; k < K; ++k)
+    if (log[k] == 0u) return fail(c, INU_ESTATE, "a recorded step has dt = 0 (the recording ran past its stop)");
+  // Work memory of the reverse pass.
+  std::vector<void*> mem;
+  auto grab = [&](size_t bytes) -> void* {
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
+    mem.push_back(q);
+    return q;
+  };
+  auto release = [&]() {
+    (void)hipStreamSynchronize(c->stream);
+    for (void* q : mem) (void)hipFree(q);
+  };
+  float* tape = static_cast<float*>(grab((size_t)d.seg * kTapeWords * NP * 4));
+  float* lam = static_cast<float*>(grab(3 * NP * 4));
+  float* lc = static_cast<float*>(grab(3 * NP * 4));
+  float* ld = static_cast<float*>(grab(3 * NP * 4));
+  float* l1 = static_cast<float*>(grab(3 * NP * 4));
+  float* gn = static_cast<float*>(grab(NP * 4));
+  float2* gdz = static_cast<float2*>(grab(std::max<size_t>(L, 1) * 8));
+  float* dseed = static_cast<float*>(grab((size_t)std::max(d.max_samples, 1u) * d.n_gauges * 4));
+  if (!tape || !lam || !lc || !ld || !l1 || !gn || !gdz || !dseed) {
+    release();
+    return fail(c, INU_EHIP, "adjoint: out of device memory (the tape needs segment x 56 B per cell)");
+  }
+  hipError_t e = hipSuccess;
+  auto ok = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  ok(hipMemsetAsync(lam, 0, 3 * NP * 4, c->stream));
+  ok(hipMemsetAsync(gn, 0, NP * 4, c->stream));
+  ok(hipMemsetAsync(gdz, 0, std::max<size_t>(L, 1) * 8, c->stream));
+  ok(hipMemsetAsync(tape, 0, (size_t)d.seg * kTapeWords * NP * 4, c->stream));
+  ok(hipMemcpyAsync(dseed, seeds, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyHostToDevice, c->stream));
+  const Args& a = c->a;
+  hipLaunchKernelGGL(k_adj_all_blocks, dim3((c->NB + 255) / 256), dim3(256), 0, c->stream, a);
+  const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
+  const bool fm = (c->flags & INU_FLAG_EXACT_ORDER) == 0;
+  int occ = 0;
+  ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_adj_stage<0>, 256, c->lds_stage1));
+  const uint32_t grid_adj = std::max<uint32_t>(1u, std::min<uint32_t>(c->NB, (uint32_t)std::max(occ, 1) * c->cus));
+  const uint32_t gc = (uint32_t)((NP + 255) / 256);
+  const uint32_t n_seg = (K + d.seg - 1) / d.seg;
+  for (uint32_t cs = n_seg; cs-- > 0 && e == hipSuccess;) {
+    const uint32_t k0 = cs * d.seg, k1 = std::min(K, k0 + d.seg);
+    const float* ck = d.ck + (size_t)cs * 4 * NP;
+    ok(hipMemcpyAsync(a.h, ck, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+    ok(hipMemcpyAsync(a.hu, ck + NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+    ok(hipMemcpyAsync(a.hv, ck + 2 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+    ok(hipMemcpyAsync(a.hlo, ck + 3 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+    // Replay the segment with the recorded dt and source rates, writing the tape.
+    for (uint32_t k = k0; k < k1; ++k) {
+      const float* rates = nullptr;
+      for (const auto& evk : d.events)
+        if ((k == k0 && evk.first <= k) || (k > k0 && evk.first == k)) rates = evk.second;
+      if (rates) ok(hipMemcpyAsync(a.src, rates, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+      hipLaunchKernelGGL(k_adj_set_dt, dim3(1), dim3(1), 0, c->stream, a, log[k]);
+      Args ar = a;
+      ar.tape = tape;
+      ar.tape_slot = k - k0;
+      const dim3 b(256);
+      const dim3 g0(c->grid[0][comp ? 1 : 0][fm ? 1 : 0][0]), g1(c->grid[1][comp ? 1 : 0][fm ? 1 : 0][0]);
+      if (comp && fm) {
+        hipLaunchKernelGGL((k_stage<0, true, true, 0, true>), g0, b, c->lds_stage0, c->stream, ar);
+        hipLaunchKernelGGL((k_stage<1, true, true, 0, true>), g1, b, c->lds_stage1, c->stream, ar);
+      } else if (comp) {
+        hipLaunchKernelGGL((k_stage<0, true, false, 0, true>), g0, b, c->lds_stage0, c->stream, ar);
+        hipLaunchKernelGGL((k_stage<1, true, false, 0, true>), g1, b, c->lds_stage1, c->stream, ar);
+      } else if (fm) {
+        hipLaunchKernelGGL((k_stage<0, false, true, 0, true>), g0, b, c->lds_stage0, c->stream, ar);
+        hipLaunchKernelGGL((k_stage<1, false, true, 0, true>), g1, b, c->lds_stage1, c->stream, ar);
+      } else {
+        hipLaunchKernelGGL((k_stage<0, false, false, 0, true>), g0, b, c->lds_stage0, c->stream, ar);
+        hipLaunchKernelGGL((k_stage<1, false, false, 0, true>), g1, b, c->lds_stage1, c->stream, ar);
+      }
+    }
+    // Reverse the segment.
+    for (uint32_t k = k1; k-- > k0;) {
+      const float* slot = tape + (size_t)(k - k0) * kTapeWords * NP;
+      if ((k + 1) % d.every == 0 && (k + 1) / d.every <= d.max_samples)
+        hipLaunchKernelGGL(k_adj_seed, dim3((d.n_gauges + 255) / 256), dim3(256), 0, c->stream, lam, d.gauges,
+                           dseed + (size_t)((k + 1) / d.every - 1) * d.n_gauges, d.n_gauges);
+      hipLaunchKernelGGL(k_adj_heun, dim3(gc), dim3(256), 0, c->stream, slot, (uint32_t)NP, c->p.dry, lam, lc, ld);
+      float dtk;
+      memcpy(&dtk, &log[k], 4);
+      AdjArgs x{a, slot, dtk, lc, nullptr, l1, gn, gdz};
+      hipLaunchKernelGGL(k_adj_stage<1>, dim3(grid_adj), dim3(256), c->lds_stage1, c->stream, x);
+      x.lam_out = l1;
+      x.lam_add = ld;
+      x.lam_in = lam;
+      hipLaunchKernelGGL(k_adj_stage<0>, dim3(grid_adj), dim3(256), c->lds_stage1, c->stream, x);
+    }
+    ok(hipGetLastError());
+  }
+  ok(hipStreamSynchronize(c->stream));
+  if (e != hipSuccess) {
+    release();
+    return fail(c, INU_EHIP, std::string("adjoint: ") + hipGetErrorString(e));
+  }
+  // Manning gradient, and the state gradient, in the caller's cell order.
+  rc = download_cells(c, gn, grad_n);
+  if (!rc && grad_state)
+    for (int q = 0; q < 3 && !rc; ++q) rc = download_cells(c, lam + q * NP, grad_state + (size_t)q * c->N);
+  // Bed gradient: d loss / d (dzL, dzR) of each edge, through the face-on-the-higher-bed rule.
+  std::vector<float2> hg(L);
+  if (!rc && L) {
+    e = hipMemcpy(hg.data(), gdz, L * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(c, INU_EHIP, std::string("adjoint: ") + hipGetErrorString(e));
+  }
+  release();
+  if (rc) return rc;
+  std::vector<double> gz((size_t)c->NB * kBlockCells, 0.0);
+  for (size_t ed = 0; ed < c->own_pos.size(); ++ed) {
+    const int32_t r = c->e_cr[ed];
+    if (r < 0) continue;
+    const uint32_t q = c->own_pos[ed];
+    const int32_t l = c->e_cl[ed];
+    if (c->h_ldz[q] > 0.0f) {  // zR > zL: dzL = zR - zL
+      gz[r] += hg[q].x;
+      gz[l] -= hg[q].x;
+    } else {  // zL >= zR: dzR = zL - zR
+      gz[l] += hg[q].y;
+      gz[r] -= hg[q].y;
+    }
+  }
+  for (uint32_t s = 0; s < c->n_scen; ++s)
+    for (uint32_t i = c->cell0[s]; i < c->cell0[s + 1]; ++i)
+      grad_z[i] = (float)gz[(size_t)c->block0[s] * kBlockCells + (i - c->cell0[s])];
+  return 0;
+}
+
+int inu_adj_replay(inu_ctx* c, float* series_out) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  auto& d = c->adj;
+  if (!d.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
+  if (!series_out) return fail(c, INU_EINVAL, "null output");
+  Scen sc;
+  HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
+  const uint32_t K = (uint32_t)std::min<uint64_t>(sc.ctrl.steps, d.max_steps);
+  if (K == 0) return fail(c, INU_ESTATE, "nothing recorded");
+  std::vector<uint32_t> log(K);
+  HIPCHK(c, hipMemcpy(log.data(), d.log, (size_t)K * 4, hipMemcpyDeviceToHost));
+  const size_t NP = (size_t)c->NB * kBlockCells;
+  float* series = nullptr;
+  HIPCHK(c, hipMalloc(&series, (size_t)std::max(d.max_samples, 1u) * d.n_gauges * 4));
+  hipError_t e = hipMemsetAsync(series, 0, (size_t)std::max(d.max_samples, 1u) * d.n_gauges * 4, c->stream);
+  auto ok = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  const Args& a = c->a;
+  const float* ck = d.ck;
+  ok(hipMemcpyAsync(a.h, ck, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  ok(hipMemcpyAsync(a.hu, ck + NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  ok(hipMemcpyAsync(a.hv, ck + 2 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  ok(hipMemcpyAsync(a.hlo, ck + 3 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_adj_all_blocks, dim3((c->NB + 255) / 256), dim3(256), 0, c->stream, a);
+  const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
+  const bool fm = (c->flags & INU_FLAG_EXACT_ORDER) == 0;
+  for (uint32_t k = 0; k < K && e == hipSuccess; ++k) {
+    for (const auto& evk : d.events)
+      if (evk.first == k) ok(hipMemcpyAsync(a.src, evk.second, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_adj_set_dt, dim3(1), dim3(1), 0, c->stream, a, log[k]);
+    launch_stage<0>(c, a, comp, fm, c->tile);
+    launch_stage<1>(c, a, comp, fm, c->tile);
+    if ((k + 1) % d.every == 0 && (k + 1) / d.every <= d.max_samples)
+      hipLaunchKernelGGL(k_adj_sample_row, dim3((d.n_gauges + 255) / 256), dim3(256), 0, c->stream, a, series,
+                         d.gauges, d.n_gauges, (k + 1) / d.every - 1);
+  }
+  ok(hipGetLastError());
+  ok(hipStreamSynchronize(c->stream));
+  if (e == hipSuccess)
+    e = hipMemcpy(series_out, series, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(series);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("adjoint replay: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int inu_adj_dt_log(inu_ctx* c, uint32_t* out) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!c->adj.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
+  if (!out) return fail(c, INU_EINVAL, "null output");
+  HIPCHK(c, hipMemcpy(out, c->adj.log, (size_t)c->adj.max_steps * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int inu_adj_end(inu_ctx* c) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  auto& d = c->adj;
+  for (void* q : {(void*)d.log, (void*)d.gauges, (void*)d.series, (void*)d.ck}) {
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
+    if (q && it != c->allocs.end()) {
+      (void)hipFree(q);
+      c->allocs.erase(it);
+    }
+  }
+  for (auto& ev : d.events) {
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)ev.second);
+    if (it != c->allocs.end()) {
+      (void)hipFree(ev.second);
+      c->allocs.erase(it);
+    }
+  }
+  d.events.clear();
+  d.log = nullptr;
+  d.gauges = nullptr;
+  d.series = nullptr;
+  d.ck = nullptr;
+  d.on = false;
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  return 0;
+}
+
+int inu_probe_vjp(inu_ctx* c, uint32_t n, const float* in17, float* out8, float g, float dry_tol) {
+  if (!c) return INU_EINVAL;
+  if (!in17 || !out8) return fail(c, INU_EINVAL, "null array");
+  if (n == 0) return 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  float *din = nullptr, *dout = nullptr;
+  HIPCHK(c, hipMalloc(&din, (size_t)n * 17 * 4));
+  hipError_t e = hipMalloc(&dout, (size_t)n * 8 * 4);
+  if (e == hipSuccess) e = hipMemcpy(din, in17, (size_t)n * 17 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_probe_vjp, dim3((n + 255) / 256), dim3(256), 0, c->stream, din, dout, n,
+                       make_phys(g, dry_tol, 0.8f));
+    e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpy(out8, dout, (size_t)n * 8 * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(din);
+  if (dout) (void)hipFree(dout);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("probe_vjp: ") + hipGetErrorString(e));
   return 0;
 }
 

@@ -35,9 +35,9 @@ struct Phys {
   float sqrt_g;    // sqrt(g)
 };
 
-__device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }
-__device__ __forceinline__ float gmax(float x, float y) { return (x < y) ? y : x; }
-__device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+__host__ __device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }
+__host__ __device__ __forceinline__ float gmax(float x, float y) { return (x < y) ? y : x; }
+__host__ __device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
 
 __device__ __forceinline__ float safe_vel(const Phys& p, float h, float hq) {
   return (h > p.dry) ? (2.0f * h * hq) / (h * h + gmax(h * h, p.e_film2)) : 0.0f;

@@ -72,6 +72,15 @@ EXPORTS = (
     "inu_stream_copy",
     "inu_autotune",
     "inu_set_tile",
+    "inu_set_bed",
+    "inu_adj_begin",
+    "inu_adj_steps",
+    "inu_adj_series",
+    "inu_adj_backward",
+    "inu_adj_replay",
+    "inu_adj_dt_log",
+    "inu_adj_end",
+    "inu_probe_vjp",
 )
 
 
@@ -160,6 +169,15 @@ def load() -> C.CDLL:
         "inu_stream_copy": (_I, [_P, C.c_uint64, _U32, _P]),
         "inu_autotune": (_I, [_P, _U32, _P]),
         "inu_set_tile": (_I, [_P, _U32]),
+        "inu_set_bed": (_I, [_P, _P]),
+        "inu_adj_begin": (_I, [_P, _U32, _U32, _U32, _P, _U32]),
+        "inu_adj_steps": (_I, [_P, _U32]),
+        "inu_adj_series": (_I, [_P, _P]),
+        "inu_adj_backward": (_I, [_P, _P, _P, _P, _P]),
+        "inu_adj_replay": (_I, [_P, _P]),
+        "inu_adj_dt_log": (_I, [_P, _P]),
+        "inu_adj_end": (_I, [_P]),
+        "inu_probe_vjp": (_I, [_P, _U32, _P, _P, _F, _F]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -376,6 +394,54 @@ class Context:
 
     def set_tile(self, tile: int) -> None:
         self.call("inu_set_tile", int(tile))
+
+    def set_bed(self, z: NDArray[np.float32]) -> None:
+        """Bed elevation per cell; every edge's bed step is recomputed."""
+        a = np.ascontiguousarray(z, dtype=np.float32)
+        self.call("inu_set_bed", ptr(a))
+
+    def adj_begin(self, max_steps: int, seg: int, gauges: NDArray[np.uint32], every: int) -> None:
+        g = np.ascontiguousarray(gauges, dtype=np.uint32)
+        self._adj = (int(max_steps) // int(every), g.size)
+        self.call("inu_adj_begin", int(max_steps), int(seg), g.size, ptr(g), int(every))
+
+    def adj_steps(self, n: int) -> None:
+        self.call("inu_adj_steps", int(n))
+
+    def adj_series(self) -> NDArray[np.float32]:
+        out = np.zeros(self._adj, dtype=np.float32)
+        self.call("inu_adj_series", ptr(out))
+        return out
+
+    def adj_backward(self, seeds: NDArray[np.float32], n_cells: int) -> tuple[NDArray, NDArray, NDArray]:
+        """(d loss / d n, d loss / d z, d loss / d initial (h, hu, hv)) for seeds = d loss / d series."""
+        sd = np.ascontiguousarray(seeds, dtype=np.float32).reshape(self._adj)
+        gn = np.zeros(n_cells, np.float32)
+        gz = np.zeros(n_cells, np.float32)
+        gs = np.zeros((3, n_cells), np.float32)
+        self.call("inu_adj_backward", ptr(sd), ptr(gn), ptr(gz), ptr(gs))
+        return gn, gz, gs
+
+    def adj_replay(self) -> NDArray[np.float32]:
+        """Gauge series of the recorded steps replayed with the current n and bed."""
+        out = np.zeros(self._adj, dtype=np.float32)
+        self.call("inu_adj_replay", ptr(out))
+        return out
+
+    def adj_dts(self, max_steps: int) -> NDArray[np.float32]:
+        """Recorded dt of each step (0 past the last recorded step)."""
+        out = np.zeros(int(max_steps), dtype=np.uint32)
+        self.call("inu_adj_dt_log", ptr(out))
+        return out.view(np.float32)
+
+    def adj_end(self) -> None:
+        self.call("inu_adj_end")
+
+    def probe_vjp(self, rows: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:
+        r = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, 17)
+        out = np.empty((r.shape[0], 8), dtype=np.float32)
+        self.call("inu_probe_vjp", r.shape[0], ptr(r), ptr(out), g, dry_tol)
+        return out
 
     def probe(self, pairs: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:
         p = np.ascontiguousarray(pairs, dtype=np.float32).reshape(-1, 2)
