@@ -82,11 +82,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument(
         "--workload",
-        choices=("rainfall", "batched"),
+        choices=("rainfall", "batched", "adjoint"),
         default="rainfall",
         help="rainfall: the plain 16 M-cell step (default); batched: scenarios on five Voronoi meshes, batched "
-        "against sequential",
+        "against sequential; adjoint: Manning/bed gradients of a gauge misfit on a 250 k-cell Voronoi flood",
     )
+    ap.add_argument("--cells", type=int, default=250_000, help="adjoint: Voronoi points")
+    ap.add_argument("--fd-cells", type=int, default=20, help="adjoint: cells checked by central differences")
     ap.add_argument("--scenarios", type=int, default=64, help="batched: number of scenarios")
     ap.add_argument("--t-end", type=float, default=3600.0, help="batched: simulated seconds per scenario")
     ap.add_argument("--sizes", type=str, default="50000,200000,500000,1000000,2000000",
@@ -103,13 +105,15 @@ def main() -> int:
     if args.gpus != 1:
         print(
             "bench.py measures one MI355X: the solver has no multi-GPU path "
-            "(sharding by Hilbert block ranges is described in DESIGN.md); run with --gpus 1",
+            "(it would shard by contiguous Hilbert block ranges, one halo exchange per stage); run with --gpus 1",
             file=sys.stderr,
         )
         return 2
 
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     from inundation_amd.hip import lib as L
     from inundation_amd.hip.solver import device_mesh_arrays
@@ -393,6 +397,215 @@ def bench_batched(args) -> int:  # noqa: ANN001
     }
     print(json.dumps(out))
     return 0
+
+
+def bench_adjoint(args) -> int:  # noqa: ANN001
+    """Gauge-misfit gradients w.r.t. Manning n and bed z: the adjoint against the plain forward run.
+
+    A truth run with the smooth 'true' n field picks 32 gauges among the cells
+    it wets and records their depths every 10 steps; the calibration run
+    starts from n = 0.04. Timed: the plain forward (hipGraph steps, no
+    recording), and the gradient (recorded forward + backward: segment replays
+    from checkpoints every 100 steps and the reverse kernels). Central
+    differences (eps = 1e-3) of replays of the recording (same dt sequence and
+    source events) check the gradient on ``--fd-cells`` cells drawn among those
+    whose n-gradient exceeds 1e-3 of the largest.
+    """
+    from inundation_amd.hip import lib as L
+    from inundation_amd.hip.adjoint import GaugeRun
+    from inundation_amd.hip.solver import device_mesh_arrays
+    from inundation_amd.workloads import calibration
+
+    t0 = time.perf_counter()
+    cal = calibration(n_points=args.cells, steps=args.steps)
+    geom = cal.geom
+    log(f"calibration mesh: {geom.N} cells built in {time.perf_counter() - t0:.1f}s")
+    d = device_mesh_arrays(geom)
+    ctx = L.Context(0)
+    ctx.set_mesh(d["cell_l"], d["cell_r"], d["nx"], d["ny"], d["len"], d["dz2"], d["area"], d["cptr"], d["cidx"])
+    zero = np.zeros(geom.N, np.float32)
+    run = GaugeRun(ctx=ctx, state0=(zero, zero, zero), gauges=np.array([cal.inflow_cell], np.uint32),
+                   every=cal.every, n_steps=cal.steps, seg=cal.seg, dry_tol=1e-4, cfl=0.8, dt_max=1.0, dt_init=0.1,
+                   source_cells=cal.source_cells(), rates_at=cal.rates_at, graph_steps=50)
+    z = geom.zb.astype(np.float32)
+    # Truth run: gauges among the cells it wets, then their observed depths.
+    run.forward_plain(cal.n_true, z)
+    h_truth = ctx.download(L.FIELD_H, geom.N)
+    run.gauges = cal.pick_gauges(h_truth)
+    obs = run.forward(cal.n_true, z).astype(np.float64)
+    t_truth = ctx.ctrl_read().now_ticks / float(1 << 20)
+    log(f"truth run: {cal.steps} steps, {t_truth:.1f} s simulated, {int((h_truth > 1e-4).sum())} wet cells, "
+        f"gauge depths {obs[-1].min():.3f}..{obs[-1].max():.3f} m")
+
+    n0 = np.full(geom.N, cal.n_start, np.float32)
+    reps = 3
+    t_plain = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run.forward_plain(n0, z)
+        t_plain.append(time.perf_counter() - t0)
+    h_end = ctx.download(L.FIELD_H, geom.N)
+    active_frac = float(ctx.download(L.FIELD_BLOCK_ACTIVE, -(-geom.N // 256), np.uint32).astype(bool).mean())
+    t_fwd, t_bwd = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        series = run.forward(n0, z)
+        t1 = time.perf_counter()
+        seeds = (2.0 * (series.astype(np.float64) - obs)).astype(np.float32)
+        gn, gz = run.backward(seeds)
+        t2 = time.perf_counter()
+        t_fwd.append(t1 - t0)
+        t_bwd.append(t2 - t1)
+    loss = float(((series.astype(np.float64) - obs) ** 2).sum())
+    plain, fwd, bwd = min(t_plain), min(t_fwd), min(t_bwd)
+    ratio = (fwd + bwd) / plain
+    log(f"plain forward {plain * 1e3:.1f} ms, recorded forward {fwd * 1e3:.1f} ms, backward {bwd * 1e3:.1f} ms: "
+        f"gradient = {ratio:.2f}x forward; loss {loss:.4e}; wet cells at the end {int((h_end > 1e-4).sum())}")
+
+    # Central differences of replays (the recorded dt sequence and source events).
+    def loss_of(nn, zz) -> float:  # noqa: ANN001
+        s_ = run.replay(nn, zz).astype(np.float64)
+        return float(((s_ - obs) ** 2).sum())
+
+    replay_same = np.array_equal(run.replay(n0, z).view(np.uint32), series.view(np.uint32))
+    rng = np.random.default_rng([cal.seed, 2])
+    fd = {}
+    eps, eps2 = 1e-3, 3e-4
+    for name, vec, grad in (("n", n0, gn), ("z", z, gz)):
+        cand = np.flatnonzero(np.abs(grad) > 0.05 * np.abs(grad).max())
+        cells = np.sort(rng.choice(cand, min(args.fd_cells, cand.size), replace=False))
+        rows = []
+        for i in cells.tolist():
+            f = []
+            for e in (eps, eps2):
+                up, dn = vec.copy(), vec.copy()
+                up[i] += e
+                dn[i] -= e
+                if name == "n":
+                    f.append((loss_of(up, z) - loss_of(dn, z)) / (2 * e))
+                else:
+                    f.append((loss_of(n0, up) - loss_of(n0, dn)) / (2 * e))
+            rows.append((i, float(grad[i]), f[0], f[1]))
+        g_ = np.array([r[1] for r in rows])
+        f_ = np.array([r[2] for r in rows])
+        f2 = np.array([r[3] for r in rows])
+        rel = np.abs(g_ - f_) / np.maximum(np.abs(f_), 1e-30)
+        fd[name] = {
+            "cells": int(cells.size),
+            "drawn_from": f"{cand.size} cells with |gradient| > 5% of the largest",
+            "rel_err_norm": float(np.linalg.norm(g_ - f_) / np.linalg.norm(f_)),
+            "rel_err_median": float(np.median(rel)),
+            "rel_err_max": float(rel.max()),
+            "fd_eps_spread_norm": float(np.linalg.norm(f2 - f_) / np.linalg.norm(f_)),
+        }
+        log(f"FD {name}: {fd[name]}")
+        for r in rows[:5]:
+            log(f"   cell {r[0]}: adjoint {r[1]:.6e} fd(1e-3) {r[2]:.6e} fd(3e-4) {r[3]:.6e}")
+    stats = ctx.adj_stats()
+    cpu = None if args.no_cpu else cpu_baseline_adjoint(ctx, cal, run, n0, z, args.cpu_seconds)
+    ctx.close()
+    cell_steps = geom.N * cal.steps
+    out = {
+        "metric": METRIC,
+        "value": cell_steps / (fwd + bwd),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": cal.steps,
+        "warmup": 1,
+        "ms_per_step": (fwd + bwd) / cal.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"adjoint calibration: {geom.N} Voronoi cells over 2 km fBm with a carved channel, "
+            f"20 m^3/s-peak inflow hydrograph, true n smooth in [0.02,0.08], start n=0.04, {cal.n_gauges} gauges "
+            f"every {cal.every} steps, {cal.steps} steps, checkpoints every {cal.seg}",
+            "cells": geom.N,
+            "simulated_s": t_truth,
+            "value_is": "cell-steps of the calibration run differentiated per second (recorded forward + backward)",
+        },
+        "adjoint": {
+            "plain_forward_s": plain,
+            "recorded_forward_s": fwd,
+            "backward_s": bwd,
+            "cost_over_forward": ratio,
+            "loss": loss,
+            "replay_bitwise": bool(replay_same),
+            "active_block_fraction_at_end": active_frac,
+            "fd_eps": [eps, eps2],
+            "fd": fd,
+            "thin_layer_depth_m": 100.0 * run.dry_tol,
+            "reverse_stats": stats,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def cpu_baseline_adjoint(ctx, cal, run, n0, z, budget_s: float) -> dict:  # noqa: ANN001
+    """The FP64 oracle's reverse-mode gradient (torch autodiff on the host cores) on a bounded slice.
+
+    The slice is the calibration run's first K steps from the dry start (the
+    recorded dt and source rate), with the loss taken at the 32 cells nearest
+    the inlet every 5 steps (sum of squared depths) so that it sees water this
+    early. The same slice runs through the GPU adjoint, and the two gradients
+    are compared. About 1.7 GB of host memory per step at 250 k cells.
+    """
+    import torch
+
+    from inundation_amd.hip.adjoint import GaugeRun
+    from inundation_amd.hip.solver import device_mesh_arrays
+    from oracle import adjoint64 as A
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    torch.set_num_threads(cores)
+    geom = cal.geom
+    k_steps = 20 if budget_s >= 10.0 else 10
+    every = 5
+    d2 = (geom.centroid[:, 0] - geom.centroid[cal.inflow_cell, 0]) ** 2 + (
+        geom.centroid[:, 1] - geom.centroid[cal.inflow_cell, 1]
+    ) ** 2
+    gauges = np.sort(np.argsort(d2)[:32]).astype(np.uint32)
+    zero = np.zeros(geom.N, np.float32)
+    sl = GaugeRun(ctx=ctx, state0=(zero, zero, zero), gauges=gauges, every=every, n_steps=k_steps, seg=10,
+                  dry_tol=run.dry_tol, cfl=run.cfl, dt_max=run.dt_max, dt_init=run.dt_init,
+                  source_cells=cal.source_cells(), rates_at=cal.rates_at, graph_steps=k_steps)
+    series = sl.forward(n0, z)
+    gn, gz = sl.backward((2.0 * series.astype(np.float64)).astype(np.float32))
+    dts = sl.dts().astype(np.float64)
+    t_end = float(dts.sum())
+    if t_end >= 60.0:
+        raise RuntimeError("the CPU slice crosses a hydrograph phase")
+    src = np.zeros(geom.N)
+    src[cal.inflow_cell] = float(cal.rates_at(0.0)[0])
+    m = device_mesh_arrays(geom)
+    t0 = time.perf_counter()
+    _, gz64, gn64 = A.misfit_and_grads(m, z.astype(np.float64), n0.astype(np.float64),
+                                       tuple(np.zeros(geom.N) for _ in range(3)), dts, src, gauges,
+                                       np.zeros((k_steps // every, gauges.size)), every, thin=100.0 * run.dry_tol)
+    dt = time.perf_counter() - t0
+    err_n = float(np.linalg.norm(gn - gn64) / np.linalg.norm(gn64))
+    err_z = float(np.linalg.norm(gz - gz64) / np.linalg.norm(gz64))
+    log(f"cpu FP64 adjoint: {k_steps} steps in {dt:.2f}s on {cores} threads; GPU vs FP64 gradient "
+        f"(rel norm) n {err_n:.2e}, z {err_z:.2e}")
+    return {
+        "value": geom.N * k_steps / dt,
+        "unit": "cell-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"FP64 reverse-mode gradient (oracle, torch autodiff) of the first {k_steps} steps of the same "
+        f"{geom.N}-cell run from the dry start (recorded dt, loss at the 32 cells nearest the inlet every {every} "
+        f"steps); the GPU adjoint of the same slice differs from it by {err_n:.2e} (n) and {err_z:.2e} (z) in "
+        f"relative norm",
+        "gpu_vs_fp64": {"n": err_n, "z": err_z},
+    }
 
 
 def cpu_baseline(ctx, wl, m, info, budget_s: float) -> dict:  # noqa: ANN001

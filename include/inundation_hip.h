@@ -116,7 +116,7 @@ int inu_set_mesh(inu_ctx* ctx, uint32_t n_cells, uint32_t n_edges, const int32_t
                  const float* edge_len, const float* edge_dz, const float* cell_area,
                  const uint32_t* cell_edge_ptr, const uint32_t* cell_edge_idx);
 
-/* Batched scenarios (BASELINE batching extension; no reference counterpart:
+/* Batched scenarios (an extension with no reference counterpart:
  * the reference runs one scenario per solver). n_scenarios scenarios, scenario
  * s on mesh scenario_mesh[s] of the n_meshes meshes, each with its own state,
  * sources, control block and CFL dt, all advanced by the same launches.
@@ -217,7 +217,7 @@ int inu_probe(inu_ctx* ctx, uint32_t n, const float* pairs, float* out7, float g
  * buffers. */
 int inu_stream_copy(inu_ctx* ctx, uint64_t n_bytes, uint32_t reps, float* ms_out);
 
-/* Autotuner of the stage kernels' tiles (BASELINE batching extension; no
+/* Autotuner of the stage kernels' tiles (batching extension; no
  * reference counterpart). Times `reps` launches of the predictor at each tile
  * level on the current state (after inu_prologue; the state is left as it
  * was) and keeps the fastest for both stages: out4 = {chosen level,
@@ -235,18 +235,22 @@ int inu_set_tile(inu_ctx* ctx, uint32_t tile);
  * perturbing the bed without a new mesh. */
 int inu_set_bed(inu_ctx* ctx, const float* z);
 
-/* ---- Adjoint (BASELINE differentiation extension; the reference has no
- * gradient path). Reverse mode of recorded steps, for d loss / d (Manning n,
- * bed z) of a loss on sampled gauge depths. One scenario; source rates must
- * be non-negative. The forward is recorded at step 0 after inu_prologue with
- * inu_adj_steps (CFL-driven steps as inu_steps, plus the dt of each step,
- * gauge depths h + h_lo after every `every` steps and the state every `seg`
- * steps); inu_set_source_rates may change rates between recorded steps (it is
+/* ---- Adjoint (an extension: the reference has no gradient path). Reverse
+ * mode of recorded steps, for d loss / d (Manning n, bed z) of a loss on
+ * sampled gauge depths. One scenario; source rates must be non-negative;
+ * gauge cells distinct. The forward is recorded at step 0 after inu_prologue
+ * with inu_adj_steps: CFL-driven steps as inu_steps, plus each step's dt and
+ * active-set lists (listed blocks, idle source blocks), gauge depths h + h_lo
+ * after every `every` steps and the state every `seg` steps;
+ * inu_set_source_rates may change rates between recorded steps (they are
  * replayed). inu_adj_backward replays each `seg`-step segment from its
- * checkpoint with the recorded dt and rates (device memory: seg x 56 bytes
- * per cell for the replayed segment) and runs it in reverse, following the
- * forward's branches. After it the context's state is that of the replay and
- * must be set again before further steps. */
+ * checkpoint with the recorded dt, rates and lists (device memory: seg x 56
+ * bytes per cell for the segment's tape) and runs it in reverse over the
+ * listed blocks, following the forward's branches; blocks the forward skipped
+ * keep their state, so their gradient passes through. Replays and reverse
+ * segments run as hipGraphs cached with the recording. After a replay or
+ * backward the context's state is that of the replay and must be set again
+ * before further steps. */
 int inu_adj_begin(inu_ctx* ctx, uint32_t max_steps, uint32_t seg, uint32_t n_gauges, const uint32_t* gauges,
                   uint32_t every);
 int inu_adj_steps(inu_ctx* ctx, uint32_t n);
@@ -262,6 +266,17 @@ int inu_adj_backward(inu_ctx* ctx, const float* seeds, float* grad_n, float* gra
 int inu_adj_replay(inu_ctx* ctx, float* series_out);
 /* The recorded dt of each step (float bits; max_steps words, 0 past the last). */
 int inu_adj_dt_log(inu_ctx* ctx, uint32_t* out);
+/* Counters of the last inu_adj_backward: listed block-steps reversed, idle
+ * source block-steps passed through, and edges whose gradient reached a cell
+ * outside the listed blocks (film on the active set's boundary; those few
+ * additions are atomic, so their order, and only theirs, may vary). */
+int inu_adj_stats(inu_ctx* ctx, unsigned long long* out3);
+/* Face depth below which a side's velocity and celerity are held in the
+ * reverse pass (its depth still enters the fluxes): u = q / h and sqrt(g h)
+ * have slopes of order 1 / h in a thin layer, which the reverse pass would
+ * compound at every wetting front. 0 holds none; negative restores the
+ * default, 100 x dry_tol. Also used by inu_probe_vjp. */
+int inu_adj_set_thin(inu_ctx* ctx, float depth);
 int inu_adj_end(inu_ctx* ctx);
 
 /* Edge-word VJP probe: per row in17 = {hL, huL, hvL, hR, huR, hvR, dzL, dzR,

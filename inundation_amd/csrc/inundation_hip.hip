@@ -119,9 +119,20 @@ struct Args {
   Ctl* ctl;
   Scen* scen;
   uint32_t n_scen;
-  // adjoint tape (recording stage kernels): kTapeWords arrays of N per step slot
+  // adjoint tape (recording stage kernels): kTapeWords arrays of N per step
+  // slot; the slot is the replay cursor's step minus its segment start
   float* tape;
-  uint32_t tape_slot;
+  const struct AdjCur* adj_cur;
+};
+
+// Device-side cursor of an adjoint replay or reverse pass: the step being
+// replayed or reversed, the first step of its segment (tape slot = k - k0),
+// and counters of the reverse pass: listed block-steps, idle source
+// block-steps, and edges whose gradient reached a cell outside the listed
+// blocks (film on the active set's boundary).
+struct AdjCur {
+  uint32_t k, k0;
+  unsigned long long stats[3];
 };
 
 // Tape words per cell and recorded step: stage 0 input (h, hu, hv), its
@@ -246,6 +257,19 @@ __device__ __forceinline__ float4 load_ghost(const Args& a, uint32_t g) {
   return make_float4(ldg(a.h, g), ldg(a.hu, g), ldg(a.hv, g), LO ? ldg(a.hlo, g) : 0.0f);
 }
 
+// Adjoint tape: a ghost cell whose block is not listed this step keeps its
+// step state through both stages; its stage input is recorded for the reverse
+// pass (listed cells are recorded by their own block).
+template <int ST, bool COMP>
+__device__ __forceinline__ void rec_ghost(const Args& a, uint32_t g, float4 s) {
+  if (ldg(a.active, g >> 8) != 0u) return;
+  const uint32_t slot = a.adj_cur->k - a.adj_cur->k0;
+  float* t = a.tape + (size_t)(slot * kTapeWords + (ST == 0 ? 0u : 7u)) * a.N + g;
+  t[0] = COMP ? ldg(a.h, g) + ldg(a.hlo, g) : s.x;
+  t[a.N] = s.y;
+  t[2 * (size_t)a.N] = s.z;
+}
+
 // Stage input of one cell or ghost into LDS record q, with the fast order's
 // per-depth vel_factor, square root and hydrostatic pressure.
 template <bool FM>
@@ -356,9 +380,16 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   if (nx.t + per < end) nx.b2 = a.list[nx.t + per];
   __syncthreads();  // the previous block's LDS reads are done
   if (tid < nc) put_cell<FM>(l, p, tid, make_float4(hi, hui, hvi, loi));
-  if (tid < bi.n_ghost) put_cell<FM>(l, p, kBlockCells + tid, gs);
-  for (uint32_t g = tid + kBlockCells; g < bi.n_ghost; g += kBlockCells)
-    put_cell<FM>(l, p, kBlockCells + g, load_ghost<ST, LO>(a, ldg(a.gx, bi.g0 + g)));
+  if (tid < bi.n_ghost) {
+    put_cell<FM>(l, p, kBlockCells + tid, gs);
+    if (REC) rec_ghost<ST, COMP>(a, gid, gs);
+  }
+  for (uint32_t g = tid + kBlockCells; g < bi.n_ghost; g += kBlockCells) {
+    const uint32_t gg = ldg(a.gx, bi.g0 + g);
+    const float4 s = load_ghost<ST, LO>(a, gg);
+    put_cell<FM>(l, p, kBlockCells + g, s);
+    if (REC) rec_ghost<ST, COMP>(a, gg, s);
+  }
 #pragma unroll
   for (uint32_t m = 0; m < kSlotsAhead; ++m)
     if (tid + m * kBlockCells < bi.n_slots) l.s_slot[tid + m * kBlockCells] = (uint16_t)sl[m];
@@ -415,7 +446,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     float hv_new = FM ? fmaf(-dti, sdhv, hvi) : hvi - dti * sdhv;
     if (REC) {
       // Adjoint tape of this stage: input, updated state before friction, clamp.
-      float* t = a.tape + (size_t)(a.tape_slot * kTapeWords + (ST == 0 ? 0u : 7u)) * a.N + i;
+      const uint32_t slot = a.adj_cur->k - a.adj_cur->k0;
+      float* t = a.tape + (size_t)(slot * kTapeWords + (ST == 0 ? 0u : 7u)) * a.N + i;
       const size_t q = a.N;
       t[0] = hi + loi;
       t[q] = hui;
@@ -955,24 +987,44 @@ __global__ void k_scatter_rates(float* src, const uint32_t* cells, const float* 
 // ---------------------------------------------------------------------------
 
 // Arguments of one reverse stage. Adjoint arrays hold three arrays of N
-// words each: d loss / d h, d hu, d hv.
+// words each: d loss / d h, d hu, d hv. The step comes from the cursor: its dt
+// from the log, its tape slot k - k0, its listed blocks from `list` (built by
+// k_adj_lists).
 struct AdjArgs {
   Args a;
-  const float* tape;     // this step's slot: kTapeWords arrays of N
-  float dt;              // this step's dt
-  const float* lam_out;  // d loss / d stage output
-  const float* lam_add;  // added to the result (the Heun average's direct part) or null
-  float* lam_in;         // d loss / d stage input
+  const float* tape;     // kTapeWords arrays of N per slot
+  const uint32_t* log;   // dt bits per step
+  AdjCur* cur;
+  float* lam;            // d loss / d step output; d loss / d step input after stage 0
+  float* l1;             // d loss / d stage-0 output (written by stage 1, read by stage 0)
   float* grad_n;         // += d loss / d Manning n, per cell
-  float2* grad_dz;       // += d loss / d (dzL, dzR), per owned edge of the local edge stream
+  float2* grad_dz;       // += d loss / d (dzL, dzR), per local edge stream position
 };
 
-// Reverse of one stage over every block (reference resident.py:401-516 and
-// :187-287 differentiated). Per block: (A) the friction reverse of its cells
-// and ghost cells from the tape gives each one's d loss / d sums; (B) each
-// local edge's word gradients from its two sides, then the edge's VJP into
-// LDS, and owned edges add their bed-step gradients; (C) each cell sums its
-// edges' contributions in CSR order. Deterministic: no atomics.
+// d loss / d corrector output of cell i from d loss / d step output (the Heun
+// average, reference resident.py:489-501): half of each, the momenta's only
+// where the averaged depth is wet (a dry average zeroes them). Sources are
+// constant additions (non-negative rates, so the source clamp never cuts)
+// and pass the depth gradient through. The same half is the step input's
+// direct part.
+__device__ __forceinline__ float3 heun_half(const float* tp, const float* lam, size_t N, uint32_t i, float dry) {
+  const float h0 = tp[i], h2 = tp[10 * N + i];
+  const bool wet = 0.5f * (h0 + h2) >= dry;
+  return make_float3(0.5f * lam[i], wet ? 0.5f * lam[N + i] : 0.0f, wet ? 0.5f * lam[2 * N + i] : 0.0f);
+}
+
+// Reverse of one stage over the listed blocks (reference resident.py:401-516
+// and :187-287 differentiated). Per block: (A) the friction reverse of its
+// cells and of its listed ghost cells from the tape gives each one's update
+// weight (-dt / area times d loss / d sums); a ghost whose block is not listed
+// was not updated by this stage and weighs zero; (B) each local edge's word
+// gradients from its two sides' weights, then the edge's VJP into LDS; the
+// edge's bed-step gradient is kept by its owner or, when the owner is not
+// listed, by this block; a nonzero gradient of a side outside the listed
+// blocks (film there) is added to that cell's d loss / d step input at once,
+// since such a cell keeps its state through the step; (C) each cell sums its
+// edges' contributions in CSR order. Deterministic except for those
+// boundary additions, which are counted.
 template <int ST>
 __global__ __launch_bounds__(256) void k_adj_stage(const AdjArgs x) {
   extern __shared__ float sm[];
@@ -981,8 +1033,15 @@ __global__ __launch_bounds__(256) void k_adj_stage(const AdjArgs x) {
   const Phys& p = a.p;
   const uint32_t tid = threadIdx.x;
   const size_t N = a.N;
-  const float* tp = x.tape + (ST == 0 ? 0u : 7u) * N;
-  for (uint32_t b = blockIdx.x; b < a.NB; b += gridDim.x) {
+  const uint32_t k = x.cur->k;
+  const float* slot = x.tape + (size_t)(k - x.cur->k0) * kTapeWords * N;
+  const float* tp = slot + (ST == 0 ? 0u : 7u) * N;
+  const float dt = __uint_as_float(x.log[k]);
+  const float* lam_out = (ST == 0) ? x.l1 : nullptr;
+  uint32_t n_cross = 0;
+  const uint32_t cnt = a.ctl->cnt_cur;
+  for (uint32_t t = blockIdx.x; t < cnt; t += gridDim.x) {
+    const uint32_t b = a.list[t];
     const BlockInfo bi = a.binfo[b];
     const uint32_t c0 = b << 8, nc = bi_cells(bi);
     float rh = 0.0f, rhu = 0.0f, rhv = 0.0f;
@@ -992,20 +1051,25 @@ __global__ __launch_bounds__(256) void k_adj_stage(const AdjArgs x) {
       const bool own = q < kBlockCells;
       if (own && q >= nc) continue;
       const uint32_t i = own ? c0 + q : a.gx[bi.g0 + (q - kBlockCells)];
+      const bool listed = own || a.active[i >> 8] != 0u;
       const float hin = tp[i], huin = tp[N + i], hvin = tp[2 * N + i];
-      const float hp = tp[3 * N + i], hup = tp[4 * N + i], hvp = tp[5 * N + i];
-      const bool clamped = tp[6 * N + i] != 0.0f;
-      float gh, ghu, ghv, gn;
-      inu::adj::friction_vjp(p, hp, hup, hvp, clamped, a.nman[i], x.dt, x.lam_out[i], x.lam_out[N + i],
-                             x.lam_out[2 * N + i], gh, ghu, ghv, gn);
-      if (own) {
-        rh = gh;
-        rhu = ghu;
-        rhv = ghv;
-        x.grad_n[i] += gn;
+      float gh = 0.0f, ghu = 0.0f, ghv = 0.0f;
+      if (listed) {
+        const float hp = tp[3 * N + i], hup = tp[4 * N + i], hvp = tp[5 * N + i];
+        const bool clamped = tp[6 * N + i] != 0.0f;
+        const float3 lo = (ST == 1) ? heun_half(slot, x.lam, N, i, p.dry)
+                                    : make_float3(lam_out[i], lam_out[N + i], lam_out[2 * N + i]);
+        float gn;
+        inu::adj::friction_vjp(p, hp, hup, hvp, clamped, a.nman[i], dt, lo.x, lo.y, lo.z, gh, ghu, ghv, gn);
+        if (own) {
+          rh = gh;
+          rhu = ghu;
+          rhv = ghv;
+          x.grad_n[i] += gn;
+        }
       }
-      const float dti = x.dt / a.area[i];
-      l.cell[2 * q] = make_float4(hin, huin, hvin, 0.0f);
+      const float dti = dt / a.area[i];
+      l.cell[2 * q] = make_float4(hin, huin, hvin, listed ? 1.0f : 0.0f);
       l.cell[2 * q + 1] = make_float4(-dti * gh, -dti * ghu, -dti * ghv, 0.0f);
     }
     __syncthreads();
@@ -1026,11 +1090,28 @@ __global__ __launch_bounds__(256) void k_adj_stage(const AdjArgs x) {
       inu::adj::edge_vjp(p, wall, sl.x, sl.y, sl.z, sr.x, sr.y, sr.z, dzL, dzR, r.nx, r.ny, r.len, lw, gL, gR, gdz);
       l.edge[2 * j] = make_float4(gL[0], gL[1], gL[2], 0.0f);
       l.edge[2 * j + 1] = make_float4(gR[0], gR[1], gR[2], 0.0f);
-      if (j < bi.n_own && !wall) {
+      // Edges with only the right cell here are owned by the left cell's block.
+      const bool keep_dz = j < bi.n_own || sl.w == 0.0f;
+      if (keep_dz && !wall) {
         float2 acc = x.grad_dz[bi.e0 + j];
         acc.x += gdz[0];
         acc.y += gdz[1];
         x.grad_dz[bi.e0 + j] = acc;
+      }
+      // A side outside the listed blocks is a ghost cell (own cells are listed).
+      if (sl.w == 0.0f && (gL[0] != 0.0f || gL[1] != 0.0f || gL[2] != 0.0f)) {
+        const uint32_t i = a.gx[bi.g0 + (ql - kBlockCells)];
+        atomicAdd(&x.lam[i], gL[0]);
+        atomicAdd(&x.lam[N + i], gL[1]);
+        atomicAdd(&x.lam[2 * N + i], gL[2]);
+        ++n_cross;
+      }
+      if (!wall && sr.w == 0.0f && (gR[0] != 0.0f || gR[1] != 0.0f || gR[2] != 0.0f)) {
+        const uint32_t i = a.gx[bi.g0 + (qr - kBlockCells)];
+        atomicAdd(&x.lam[i], gR[0]);
+        atomicAdd(&x.lam[N + i], gR[1]);
+        atomicAdd(&x.lam[2 * N + i], gR[2]);
+        ++n_cross;
       }
     }
     __syncthreads();
@@ -1039,105 +1120,137 @@ __global__ __launch_bounds__(256) void k_adj_stage(const AdjArgs x) {
       const uint32_t i = c0 + tid;
       float sh = rh, shu = rhu, shv = rhv;
       const uint32_t k1 = a.cptr[i + 1];
-      for (uint32_t k = a.cptr[i]; k < k1; ++k) {
-        const uint32_t s = a.lidx[k];
-        const float4 g = l.edge[2 * (s >> 1) + (s & 1u)];
+      for (uint32_t kk = a.cptr[i]; kk < k1; ++kk) {
+        const uint32_t sidx = a.lidx[kk];
+        const float4 g = l.edge[2 * (sidx >> 1) + (sidx & 1u)];
         sh += g.x;
         shu += g.y;
         shv += g.z;
       }
-      if (x.lam_add) {
-        sh += x.lam_add[i];
-        shu += x.lam_add[N + i];
-        shv += x.lam_add[2 * N + i];
+      if (ST == 0) {
+        // The step input's direct part, from the Heun average.
+        const float3 d = heun_half(slot, x.lam, N, i, p.dry);
+        sh += d.x;
+        shu += d.y;
+        shv += d.z;
+        x.lam[i] = sh;
+        x.lam[N + i] = shu;
+        x.lam[2 * N + i] = shv;
+      } else {
+        x.l1[i] = sh;
+        x.l1[N + i] = shu;
+        x.l1[2 * N + i] = shv;
       }
-      x.lam_in[i] = sh;
-      x.lam_in[N + i] = shu;
-      x.lam_in[2 * N + i] = shv;
+    }
+  }
+  if (n_cross) atomicAdd(&x.cur->stats[2], (unsigned long long)n_cross);
+}
+
+// What a recording keeps per step (k_adj_record).
+struct AdjRec {
+  uint32_t* log;       // dt bits per step
+  uint8_t* codes;      // per step and block: 1 listed, 2 idle source block, 0 neither
+  float* series;       // gauge depths, one row per sample
+  const uint32_t* gauges;
+  float* ck;           // state checkpoints (h, hu, hv, h_lo), one every seg steps
+  uint32_t max_steps, n_g, every, max_samples, seg, n_ck;
+};
+
+// Recording, at the start and after every step (k = steps taken): the next
+// step's dt and its lists (the rows start zeroed), gauge depths every
+// `every` steps, the state every `seg` steps. A no-op step rewrites row k
+// with its own (empty) lists, which no replay reads.
+__global__ __launch_bounds__(256) void k_adj_record(const Args a, const AdjRec r) {
+  const uint64_t k = a.scen[0].ctrl.steps;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  if (k < r.max_steps) {
+    if (tid == 0) r.log[k] = a.scen[0].dt_cur;
+    uint8_t* row = r.codes + k * a.NB;
+    const uint32_t na = a.ctl->cnt_cur, ni = a.ctl->idle_cur;
+    for (uint32_t t = tid; t < na; t += stride) row[a.list[t]] = 1u;
+    for (uint32_t t = tid; t < ni; t += stride) row[a.idle[t]] = 2u;
+  }
+  if (k > 0 && k % r.every == 0 && k / r.every <= r.max_samples)
+    for (uint32_t j = tid; j < r.n_g; j += stride) {
+      const uint32_t g = r.gauges[j];
+      r.series[(k / r.every - 1) * r.n_g + j] = a.h[g] + a.hlo[g];
+    }
+  if (k % r.seg == 0 && k / r.seg < r.n_ck) {
+    float* dst = r.ck + (k / r.seg) * 4ull * a.N;
+    for (size_t i = tid; i < a.N; i += stride) {
+      dst[i] = a.h[i];
+      dst[a.N + i] = a.hu[i];
+      dst[2 * (size_t)a.N + i] = a.hv[i];
+      dst[3 * (size_t)a.N + i] = a.hlo[i];
     }
   }
 }
 
-// Reverse of the Heun average (reference resident.py:489-501): from d loss /
-// d step output to the corrector output's and the step input's direct part.
-// Sources are constant additions (non-negative rates, so the source clamp
-// never cuts) and pass the depth gradient through.
-__global__ void k_adj_heun(const float* tape, uint32_t n, float dry, const float* lam, float* lam_c, float* lam_d) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float h0 = tape[i], h2 = tape[10u * (size_t)n + i];
-  const bool wet = 0.5f * (h0 + h2) >= dry;
-  const float bh = 0.5f * lam[i];
-  const float bhu = wet ? 0.5f * lam[n + i] : 0.0f;
-  const float bhv = wet ? 0.5f * lam[2 * (size_t)n + i] : 0.0f;
-  lam_c[i] = bh;
-  lam_c[n + i] = bhu;
-  lam_c[2 * (size_t)n + i] = bhv;
-  lam_d[i] = bh;
-  lam_d[n + i] = bhu;
-  lam_d[2 * (size_t)n + i] = bhv;
-}
+// Replay and reverse of step k = cursor: its dt, its active flags and lists
+// from the recorded codes (the counts start at zero), and, in the reverse
+// pass (seed != null), d loss / d the gauge depths sampled after the step
+// (gauges are distinct).
+struct AdjStep {
+  const uint8_t* codes;
+  const uint32_t* log;
+  const AdjCur* cur;
+  const float* seed;  // per sample row, or null
+  const uint32_t* gauges;
+  float* lam;
+  uint32_t n_g, every, max_samples;
+};
 
-// d loss / d sampled gauge depth into d loss / d h.
-__global__ void k_adj_seed(float* lam, const uint32_t* gauges, const float* seed, uint32_t n_g) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n_g) lam[gauges[k]] += seed[k];
-}
-
-// Replay of a recorded step: its dt (every block stays listed and active).
-__global__ void k_adj_set_dt(const Args a, uint32_t dt_bits) {
-  a.scen[0].dt_cur = dt_bits;
-  a.ctl->any_dt = 1u;
-}
-
-// Replay set-up: every block listed and active, no lists of skipped blocks.
-__global__ void k_adj_all_blocks(const Args a) {
+__global__ __launch_bounds__(256) void k_adj_lists(const Args a, const AdjStep s) {
+  const uint32_t k = s.cur->k;
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b == 0) {
-    a.ctl->cnt_cur = a.NB;
-    a.ctl->idle_cur = 0u;
-  }
+  uint32_t code = 0u;
   if (b < a.NB) {
-    a.list[b] = b;
-    a.active[b] = 1u;
+    code = s.codes[(size_t)k * a.NB + b];
+    a.active[b] = code == 1u ? 1u : 0u;
+  }
+  wave_append(code == 1u, b, &a.ctl->cnt_cur, a.list);
+  wave_append(code == 2u, b, &a.ctl->idle_cur, a.idle);
+  if (b == 0) {
+    a.scen[0].dt_cur = s.log[k];
+    a.ctl->any_dt = 1u;
+  }
+  if (s.seed && (k + 1) % s.every == 0 && (k + 1) / s.every <= s.max_samples) {
+    const float* row = s.seed + (size_t)((k + 1) / s.every - 1) * s.n_g;
+    for (uint32_t j = b; j < s.n_g; j += gridDim.x * blockDim.x) s.lam[s.gauges[j]] += row[j];
   }
 }
 
-// Recording, before a step: this step's dt, at the index of the step.
-__global__ void k_adj_log(const Args a, uint32_t* log, uint32_t max_steps) {
-  const uint64_t k = a.scen[0].ctrl.steps;
-  if (k < max_steps) log[k] = a.scen[0].dt_cur;
-}
-
-// Recording, after a step: gauge depths every `every` steps, and the state
-// every `seg` steps. A no-op step rewrites the same values.
-__global__ void k_adj_sample(const Args a, float* series, const uint32_t* gauges, uint32_t n_g, uint32_t every,
-                             uint32_t max_samples) {
-  const uint64_t k = a.scen[0].ctrl.steps;
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k == 0 || k % every != 0 || k / every > max_samples || j >= n_g) return;
-  const uint32_t g = gauges[j];
-  series[(k / every - 1) * n_g + j] = a.h[g] + a.hlo[g];
-}
-
-// Replay sample: gauge depths into row `row`.
-__global__ void k_adj_sample_row(const Args a, float* series, const uint32_t* gauges, uint32_t n_g, uint32_t row) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_g) return;
-  const uint32_t g = gauges[j];
-  series[(size_t)row * n_g + j] = a.h[g] + a.hlo[g];
-}
-
-__global__ void k_adj_checkpoint(const Args a, float* ck, uint32_t seg, uint32_t max_ck) {
-  const uint64_t k = a.scen[0].ctrl.steps;
-  if (k % seg != 0 || k / seg >= max_ck) return;
-  float* dst = ck + (k / seg) * 4ull * a.N;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < a.N; i += (size_t)gridDim.x * blockDim.x) {
-    dst[i] = a.h[i];
-    dst[a.N + i] = a.hu[i];
-    dst[2 * (size_t)a.N + i] = a.hv[i];
-    dst[3 * (size_t)a.N + i] = a.hlo[i];
+// End of a replayed (dir = +1) or reversed (dir = -1) step: the replay's
+// gauge sample (series != null), the reverse pass's list counters, the
+// counts cleared for the next k_adj_lists, the cursor moved.
+__global__ __launch_bounds__(256) void k_adj_end(const Args a, AdjCur* cur, int dir, float* series,
+                                                 const uint32_t* gauges, uint32_t n_g, uint32_t every,
+                                                 uint32_t max_samples) {
+  const uint32_t k = cur->k;
+  if (series && (k + 1) % every == 0 && (k + 1) / every <= max_samples)
+    for (uint32_t j = threadIdx.x; j < n_g; j += blockDim.x) {
+      const uint32_t g = gauges[j];
+      series[(size_t)((k + 1) / every - 1) * n_g + j] = a.h[g] + a.hlo[g];
+    }
+  __syncthreads();  // every thread has read the cursor
+  if (threadIdx.x == 0) {
+    if (dir < 0) {
+      cur->stats[0] += a.ctl->cnt_cur;
+      cur->stats[1] += a.ctl->idle_cur;
+    }
+    a.ctl->cnt_cur = 0u;
+    a.ctl->idle_cur = 0u;
+    cur->k = k + (uint32_t)dir;
   }
+}
+
+// Cursor at step k of the segment starting at k0, list counts cleared.
+__global__ void k_adj_cursor(const Args a, AdjCur* cur, uint32_t k, uint32_t k0) {
+  cur->k = k;
+  cur->k0 = k0;
+  a.ctl->cnt_cur = 0u;
+  a.ctl->idle_cur = 0u;
 }
 
 // Edge-word VJP probe: per row in17 = {hL, huL, hvL, hR, huR, hvR, dzL, dzR,
@@ -1233,11 +1346,23 @@ struct inu_ctx {
     bool on = false;
     uint32_t max_steps = 0, seg = 1, every = 1, n_gauges = 0, max_samples = 0, n_ck = 0;
     uint32_t* log = nullptr;
+    uint8_t* codes = nullptr;  // per step and block: 1 listed, 2 idle source block
     uint32_t* gauges = nullptr;
     float* series = nullptr;
     float* ck = nullptr;
+    AdjCur* cur = nullptr;
     std::vector<std::pair<uint32_t, float*>> events;  // (step, dense source rates from that step)
+    // Work of the replay and reverse passes, kept with the recording (its
+    // graphs point at them): tape of one segment, d loss / d state, d loss /
+    // d stage-0 output, parameter gradients, gauge seeds, replayed series.
+    float *tape = nullptr, *lam = nullptr, *l1 = nullptr, *gn = nullptr, *dseed = nullptr, *rseries = nullptr;
+    float2* gdz = nullptr;
+    uint32_t grid_adj = 1;
+    unsigned long long stats[3] = {0ull, 0ull, 0ull};
   } adj;
+  // Reverse pass: face depth below which a side's velocity and celerity are
+  // held; negative: kAdjThinDry x dry_tol.
+  float adj_thin = -1.0f;
   // graphs of recorded step blocks, by step count
   std::map<uint32_t, hipGraphExec_t> graphs;
   std::vector<hipEvent_t> events;
@@ -1330,6 +1455,7 @@ Phys make_phys(float g, float dry, float cfl) {
   p.sqrt_g_e = std::sqrt(g / p.e_film);
   p.inv_dry = 1.0f / dry;
   p.sqrt_g = std::sqrt(g);
+  p.adj_thin = 0.0f;
   return p;
 }
 
@@ -1364,6 +1490,26 @@ void launch_stage(inu_ctx* c, bool comp, bool fm) {
   launch_stage<ST>(c, c->a, comp, fm, c->tile);
 }
 
+// Sources of the blocks listed in `idle` (k_source_idle).
+void launch_source_idle(inu_ctx* c, const Args& a, bool comp, bool fm) {
+  const uint32_t gb = std::max(1u, std::min<uint32_t>(c->n_src_blocks, 8u * (uint32_t)c->cus));
+  if (comp && fm) hipLaunchKernelGGL((k_source_idle<true, true>), dim3(gb), dim3(256), 0, c->stream, a);
+  else if (comp) hipLaunchKernelGGL((k_source_idle<true, false>), dim3(gb), dim3(256), 0, c->stream, a);
+  else if (fm) hipLaunchKernelGGL((k_source_idle<false, true>), dim3(gb), dim3(256), 0, c->stream, a);
+  else hipLaunchKernelGGL((k_source_idle<false, false>), dim3(gb), dim3(256), 0, c->stream, a);
+}
+
+// A stage that also writes the adjoint tape (a.tape at the cursor's slot).
+template <int ST>
+void launch_stage_rec(inu_ctx* c, const Args& a, bool comp, bool fm) {
+  const size_t lds = ST == 0 ? c->lds_stage0 : c->lds_stage1;
+  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0][0]), b(256);
+  if (comp && fm) hipLaunchKernelGGL((k_stage<ST, true, true, 0, true>), g, b, lds, c->stream, a);
+  else if (comp) hipLaunchKernelGGL((k_stage<ST, true, false, 0, true>), g, b, lds, c->stream, a);
+  else if (fm) hipLaunchKernelGGL((k_stage<ST, false, true, 0, true>), g, b, lds, c->stream, a);
+  else hipLaunchKernelGGL((k_stage<ST, false, false, 0, true>), g, b, lds, c->stream, a);
+}
+
 int launch_step(inu_ctx* c, const hipEvent_t* ev) {
   const Args& a = c->a;
   const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
@@ -1374,11 +1520,7 @@ int launch_step(inu_ctx* c, const hipEvent_t* ev) {
   launch_stage<1>(c, comp, fm);
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   if (c->n_src) {
-    const uint32_t gb = std::max(1u, std::min<uint32_t>(c->n_src_blocks, 8u * (uint32_t)c->cus));
-    if (comp && fm) hipLaunchKernelGGL((k_source_idle<true, true>), dim3(gb), dim3(256), 0, c->stream, a);
-    else if (comp) hipLaunchKernelGGL((k_source_idle<true, false>), dim3(gb), dim3(256), 0, c->stream, a);
-    else if (fm) hipLaunchKernelGGL((k_source_idle<false, true>), dim3(gb), dim3(256), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_source_idle<false, false>), dim3(gb), dim3(256), 0, c->stream, a);
+    launch_source_idle(c, a, comp, fm);
     if (c->n_src_pairs) {
       const uint32_t gs = (c->n_src_pairs + 255) / 256;
       if (fm) hipLaunchKernelGGL((k_source_fix<true>), dim3(gs), dim3(256), 0, c->stream, a);
@@ -2405,6 +2547,177 @@ int inu_set_bed(inu_ctx* c, const float* z) {
   return upload(c, const_cast<float*>(c->a.ldz), c->h_ldz.data(), c->h_ldz.size());
 }
 
+}  // extern "C"
+
+namespace {
+
+// Frees a recording's buffers (log, lists, gauges, series, checkpoints,
+// source events, reverse-pass work) and every cached graph.
+static void adj_release(inu_ctx* c) {
+  auto& d = c->adj;
+  auto drop = [&](void* q) {
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
+    if (q && it != c->allocs.end()) {
+      (void)hipFree(q);
+      c->allocs.erase(it);
+    }
+  };
+  for (void* q : {(void*)d.log, (void*)d.codes, (void*)d.gauges, (void*)d.series, (void*)d.ck, (void*)d.cur,
+                  (void*)d.tape, (void*)d.lam, (void*)d.l1, (void*)d.gn, (void*)d.dseed, (void*)d.rseries,
+                  (void*)d.gdz})
+    drop(q);
+  for (auto& ev : d.events) drop((void*)ev.second);
+  d.events.clear();
+  d.log = nullptr;
+  d.codes = nullptr;
+  d.gauges = nullptr;
+  d.series = nullptr;
+  d.ck = nullptr;
+  d.cur = nullptr;
+  d.tape = d.lam = d.l1 = d.gn = d.dseed = d.rseries = nullptr;
+  d.gdz = nullptr;
+  d.on = false;
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+}
+
+// Graph keys of the adjoint's step blocks (length in the low 28 bits).
+constexpr uint32_t kKeyRecord = 0x80000000u, kKeyReplayRec = 0x40000000u, kKeyReplay = 0x20000000u,
+                   kKeyReverse = 0x10000000u, kKeyLenMask = 0x0fffffffu;
+
+// Runs enqueue(m) from a graph cached under key | m, captured on first use.
+template <class F>
+static int run_graph(inu_ctx* c, uint32_t key, uint32_t m, F enqueue) {
+  auto it = c->graphs.find(key | m);
+  if (it == c->graphs.end()) {
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(m);
+    const hipError_t e_end = hipStreamEndCapture(c->stream, &g);
+    if (rc || e_end != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : fail(c, INU_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e_end));
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(c, INU_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    it = c->graphs.emplace(key | m, ex).first;
+  }
+  HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+  return 0;
+}
+
+// Default thin-layer depth of the reverse pass, in dry_tol: 1 cm at the usual
+// 1e-4 m (on the 250 k-cell inflow flood the gradient's distance to central
+// differences is smallest near there).
+constexpr float kAdjThinDry = 100.0f;
+
+static float adj_thin_depth(const inu_ctx* c) {
+  return c->adj_thin < 0.0f ? kAdjThinDry * c->p.dry : c->adj_thin;
+}
+
+static AdjStep adj_step_args(inu_ctx* c, bool reverse) {
+  const auto& d = c->adj;
+  return AdjStep{d.codes, d.log, d.cur, reverse ? d.dseed : nullptr, d.gauges, d.lam, d.n_gauges, d.every,
+                 d.max_samples};
+}
+
+// m recorded steps replayed from the cursor on: each step's lists and dt from
+// the recording, the two stages (writing the tape when rec), the idle
+// blocks' sources; series != null samples the gauges.
+static int adj_enqueue_replay(inu_ctx* c, uint32_t m, bool rec, float* series) {
+  const auto& d = c->adj;
+  Args ar = c->a;
+  ar.tape = d.tape;
+  ar.adj_cur = d.cur;
+  const AdjStep st = adj_step_args(c, false);
+  const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
+  const bool fm = (c->flags & INU_FLAG_EXACT_ORDER) == 0;
+  const uint32_t gl = (c->NB + 255) / 256;
+  for (uint32_t j = 0; j < m; ++j) {
+    hipLaunchKernelGGL(k_adj_lists, dim3(gl), dim3(256), 0, c->stream, ar, st);
+    if (rec) {
+      launch_stage_rec<0>(c, ar, comp, fm);
+      launch_stage_rec<1>(c, ar, comp, fm);
+    } else {
+      launch_stage<0>(c, ar, comp, fm, c->tile);
+      launch_stage<1>(c, ar, comp, fm, c->tile);
+    }
+    if (c->n_src) launch_source_idle(c, ar, comp, fm);
+    hipLaunchKernelGGL(k_adj_end, dim3(1), dim3(256), 0, c->stream, ar, d.cur, 1, series, d.gauges, d.n_gauges,
+                       d.every, d.max_samples);
+  }
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// m recorded steps reversed from the cursor down.
+static int adj_enqueue_reverse(inu_ctx* c, uint32_t m) {
+  const auto& d = c->adj;
+  AdjArgs x{c->a, d.tape, d.log, d.cur, d.lam, d.l1, d.gn, d.gdz};
+  x.a.p.adj_thin = adj_thin_depth(c);
+  const AdjStep st = adj_step_args(c, true);
+  const uint32_t gl = (c->NB + 255) / 256;
+  for (uint32_t j = 0; j < m; ++j) {
+    hipLaunchKernelGGL(k_adj_lists, dim3(gl), dim3(256), 0, c->stream, c->a, st);
+    hipLaunchKernelGGL(k_adj_stage<1>, dim3(d.grid_adj), dim3(256), c->lds_stage1, c->stream, x);
+    hipLaunchKernelGGL(k_adj_stage<0>, dim3(d.grid_adj), dim3(256), c->lds_stage1, c->stream, x);
+    hipLaunchKernelGGL(k_adj_end, dim3(1), dim3(256), 0, c->stream, c->a, d.cur, -1, nullptr, d.gauges, d.n_gauges,
+                       d.every, d.max_samples);
+  }
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// The source rates in effect at step k: the last event at or before it.
+static const float* adj_rates_at(const inu_ctx* c, uint32_t k) {
+  const float* r = nullptr;
+  for (const auto& ev : c->adj.events)
+    if (ev.first <= k) r = ev.second;
+  return r;
+}
+
+// Replays steps [ka, kb) of the segment starting at k0 from the state in
+// place (the state of step ka, with the rates in effect at ka), changing the
+// rates at each recorded event.
+static int adj_replay_range(inu_ctx* c, uint32_t ka, uint32_t kb, uint32_t k0, bool rec, float* series) {
+  auto& d = c->adj;
+  const size_t NP = (size_t)c->NB * kBlockCells;
+  hipLaunchKernelGGL(k_adj_cursor, dim3(1), dim3(1), 0, c->stream, c->a, d.cur, ka, k0);
+  uint32_t k = ka;
+  while (k < kb) {
+    uint32_t e = kb;
+    for (const auto& ev : d.events)
+      if (ev.first > k && ev.first < e) e = ev.first;
+    int rc = run_graph(c, rec ? kKeyReplayRec : kKeyReplay, e - k,
+                       [&](uint32_t m) { return adj_enqueue_replay(c, m, rec, series); });
+    if (rc) return rc;
+    k = e;
+    if (k < kb) HIPCHK(c, hipMemcpyAsync(c->a.src, adj_rates_at(c, k), NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  }
+  return 0;
+}
+
+// Restores checkpoint cs (the state of step cs * seg) and the rates in effect then.
+static int adj_restore(inu_ctx* c, uint32_t cs) {
+  const auto& d = c->adj;
+  const size_t NP = (size_t)c->NB * kBlockCells;
+  const float* ck = d.ck + (size_t)cs * 4 * NP;
+  const Args& a = c->a;
+  HIPCHK(c, hipMemcpyAsync(a.h, ck, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(a.hu, ck + NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(a.hv, ck + 2 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(a.hlo, ck + 3 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  const float* rates = adj_rates_at(c, cs * d.seg);
+  if (rates) HIPCHK(c, hipMemcpyAsync(a.src, rates, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int inu_adj_begin(inu_ctx* c, uint32_t max_steps, uint32_t seg, uint32_t n_gauges, const uint32_t* gauges,
                   uint32_t every) {
   int rc = check_ready(c);
@@ -2412,44 +2725,52 @@ int inu_adj_begin(inu_ctx* c, uint32_t max_steps, uint32_t seg, uint32_t n_gauge
   if (c->n_scen != 1) return fail(c, INU_EINVAL, "the adjoint records one scenario");
   if (max_steps == 0 || seg == 0 || every == 0 || n_gauges == 0 || !gauges)
     return fail(c, INU_EINVAL, "inu_adj_begin: steps, segment, sampling interval and gauges must be given");
+  if (max_steps > kKeyLenMask) return fail(c, INU_EINVAL, "inu_adj_begin: too many steps");
   if (!c->src_nonneg) return fail(c, INU_EINVAL, "the adjoint needs non-negative source rates");
-  for (uint32_t k = 0; k < n_gauges; ++k)
-    if (gauges[k] >= c->N) return fail(c, INU_EINVAL, "gauge cell out of range");
+  std::vector<uint32_t> sorted(gauges, gauges + n_gauges);
+  std::sort(sorted.begin(), sorted.end());
+  if (sorted.back() >= c->N) return fail(c, INU_EINVAL, "gauge cell out of range");
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+    return fail(c, INU_EINVAL, "gauge cells must be distinct");
   Scen sc;
   HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
   if (sc.ctrl.steps != 0ull) return fail(c, INU_ESTATE, "start the recording at step 0 (after inu_prologue)");
+  adj_release(c);
   auto& d = c->adj;
-  d.on = false;
-  d.events.clear();
   d.max_steps = max_steps;
-  d.seg = seg;
+  d.seg = std::min(seg, max_steps);
   d.every = every;
   d.n_gauges = n_gauges;
   d.max_samples = max_steps / every;
-  d.n_ck = (max_steps + seg - 1) / seg + 1;
+  d.n_ck = (max_steps + d.seg - 1) / d.seg + 1;
   const size_t NP = (size_t)c->NB * kBlockCells;
   std::vector<uint32_t> gi(n_gauges);
   for (uint32_t k = 0; k < n_gauges; ++k) gi[k] = internal_cell(c, gauges[k]);
   rc |= dalloc(c, &d.log, max_steps);
+  rc |= dalloc(c, &d.codes, (size_t)max_steps * c->NB);
   rc |= dalloc(c, &d.gauges, n_gauges);
   rc |= dalloc(c, &d.series, (size_t)std::max(d.max_samples, 1u) * n_gauges);
   rc |= dalloc(c, &d.ck, (size_t)d.n_ck * 4 * NP);
+  rc |= dalloc(c, &d.cur, 1);
   float* ev = nullptr;
   rc |= dalloc(c, &ev, NP);
-  if (rc) return rc;
-  if ((rc = upload(c, d.gauges, gi.data(), gi.size()))) return rc;
-  HIPCHK(c, hipMemsetAsync(d.series, 0, (size_t)std::max(d.max_samples, 1u) * n_gauges * 4, c->stream));
-  const Args& a = c->a;
-  HIPCHK(c, hipMemcpyAsync(d.ck, a.h, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(d.ck + NP, a.hu, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(d.ck + 2 * NP, a.hv, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(d.ck + 3 * NP, a.hlo, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(ev, a.src, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (rc) {
+    if (ev) c->adj.events.emplace_back(0u, ev);
+    adj_release(c);
+    return rc;
+  }
   d.events.emplace_back(0u, ev);
+  if ((rc = upload(c, d.gauges, gi.data(), gi.size()))) return rc;
+  HIPCHK(c, hipMemsetAsync(d.codes, 0, (size_t)max_steps * c->NB, c->stream));
+  HIPCHK(c, hipMemsetAsync(d.series, 0, (size_t)std::max(d.max_samples, 1u) * n_gauges * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(d.cur, 0, sizeof(AdjCur), c->stream));
+  HIPCHK(c, hipMemcpyAsync(ev, c->a.src, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  const AdjRec r{d.log, d.codes, d.series, d.gauges, d.ck, d.max_steps, d.n_gauges, d.every, d.max_samples, d.seg,
+                 d.n_ck};
+  hipLaunchKernelGGL(k_adj_record, dim3(std::min<uint32_t>(c->NB, 1024u)), dim3(256), 0, c->stream, c->a, r);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   d.on = true;
-  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
-  c->graphs.clear();
   return 0;
 }
 
@@ -2459,33 +2780,19 @@ int inu_adj_steps(inu_ctx* c, uint32_t n) {
   auto& d = c->adj;
   if (!d.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
   if (n == 0) return 0;
-  const Args& a = c->a;
-  const uint32_t key = n | 0x80000000u;
-  auto it = c->graphs.find(key);
-  if (it == c->graphs.end()) {
-    hipGraph_t g = nullptr;
-    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (uint32_t k = 0; k < n; ++k) {
-      hipLaunchKernelGGL(k_adj_log, dim3(1), dim3(1), 0, c->stream, a, d.log, d.max_steps);
-      if ((rc = launch_step(c, nullptr))) {
-        hipGraph_t dummy = nullptr;
-        (void)hipStreamEndCapture(c->stream, &dummy);
-        if (dummy) (void)hipGraphDestroy(dummy);
-        return rc;
-      }
-      hipLaunchKernelGGL(k_adj_sample, dim3((d.n_gauges + 255) / 256), dim3(256), 0, c->stream, a, d.series,
-                         d.gauges, d.n_gauges, d.every, d.max_samples);
-      hipLaunchKernelGGL(k_adj_checkpoint, dim3(std::min<uint32_t>(c->NB, 1024u)), dim3(256), 0, c->stream, a,
-                         d.ck, d.seg, d.n_ck);
+  if (n > kKeyLenMask) return fail(c, INU_EINVAL, "inu_adj_steps: too many steps at once");
+  const AdjRec r{d.log, d.codes, d.series, d.gauges, d.ck, d.max_steps, d.n_gauges, d.every, d.max_samples, d.seg,
+                 d.n_ck};
+  rc = run_graph(c, kKeyRecord, n, [&](uint32_t m) {
+    for (uint32_t k = 0; k < m; ++k) {
+      int r2 = launch_step(c, nullptr);
+      if (r2) return r2;
+      hipLaunchKernelGGL(k_adj_record, dim3(std::min<uint32_t>(c->NB, 1024u)), dim3(256), 0, c->stream, c->a, r);
     }
-    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
-    hipGraphExec_t ex = nullptr;
-    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(c, INU_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    it = c->graphs.emplace(key, ex).first;
-  }
-  HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+    HIPCHK(c, hipGetLastError());
+    return 0;
+  });
+  if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -2500,138 +2807,86 @@ int inu_adj_series(inu_ctx* c, float* out) {
   return 0;
 }
 
+// Steps recorded so far, each with a nonzero dt.
+static int adj_recorded_steps(inu_ctx* c, uint32_t* K) {
+  auto& d = c->adj;
+  Scen sc;
+  HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
+  if (sc.ctrl.steps == 0 || sc.ctrl.steps > d.max_steps)
+    return fail(c, INU_ESTATE, "recorded steps must be in [1, max_steps]");
+  *K = (uint32_t)sc.ctrl.steps;
+  std::vector<uint32_t> log(*K);
+  HIPCHK(c, hipMemcpy(log.data(), d.log, (size_t)*K * 4, hipMemcpyDeviceToHost));
+  for (uint32_t k = 0; k < *K; ++k)
+    if (log[k] == 0u) return fail(c, INU_ESTATE, "a recorded step has dt = 0 (the recording ran past its stop)");
+  return 0;
+}
+
+// Reverse-pass work buffers, allocated once per recording.
+static int adj_work(inu_ctx* c) {
+  auto& d = c->adj;
+  if (d.tape) return 0;
+  const size_t NP = (size_t)c->NB * kBlockCells;
+  const size_t L = std::max<size_t>(c->h_ldz.size(), 1);
+  int rc = 0;
+  rc |= dalloc(c, &d.tape, (size_t)d.seg * kTapeWords * NP);
+  rc |= dalloc(c, &d.lam, 3 * NP);
+  rc |= dalloc(c, &d.l1, 3 * NP);
+  rc |= dalloc(c, &d.gn, NP);
+  rc |= dalloc(c, &d.gdz, L);
+  rc |= dalloc(c, &d.dseed, (size_t)std::max(d.max_samples, 1u) * d.n_gauges);
+  rc |= dalloc(c, &d.rseries, (size_t)std::max(d.max_samples, 1u) * d.n_gauges);
+  if (rc) return fail(c, INU_EHIP, "adjoint: out of device memory (the tape needs segment x 56 B per cell)");
+  int occ = 0;
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_adj_stage<0>, 256, c->lds_stage1));
+  d.grid_adj = std::max<uint32_t>(1u, std::min<uint32_t>(c->NB, (uint32_t)std::max(occ, 1) * c->cus));
+  return 0;
+}
+
 int inu_adj_backward(inu_ctx* c, const float* seeds, float* grad_n, float* grad_z, float* grad_state) {
   int rc = check_ready(c);
   if (rc) return rc;
   auto& d = c->adj;
   if (!d.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
   if (!seeds || !grad_n || !grad_z) return fail(c, INU_EINVAL, "null array");
-  Scen sc;
-  HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
-  const uint64_t K64 = sc.ctrl.steps;
-  if (K64 == 0 || K64 > d.max_steps) return fail(c, INU_ESTATE, "recorded steps must be in [1, max_steps]");
-  const uint32_t K = (uint32_t)K64;
+  uint32_t K = 0;
+  if ((rc = adj_recorded_steps(c, &K)) || (rc = adj_work(c))) return rc;
   const size_t NP = (size_t)c->NB * kBlockCells;
   const size_t L = c->h_ldz.size();
-  std::vector<uint32_t> log(K);
-  HIPCHK(c, hipMemcpy(log.data(), d.log, (size_t)K * 4, hipMemcpyDeviceToHost));
-  for (uint32_t k = 0; k < K; ++k)
-    if (log[k] == 0u) return fail(c, INU_ESTATE, "a recorded step has dt = 0 (the recording ran past its stop)");
-  // Work memory of the reverse pass.
-  std::vector<void*> mem;
-  auto grab = [&](size_t bytes) -> void* {
-    void* q = nullptr;
-    if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
-    mem.push_back(q);
-    return q;
-  };
-  auto release = [&]() {
-    (void)hipStreamSynchronize(c->stream);
-    for (void* q : mem) (void)hipFree(q);
-  };
-  float* tape = static_cast<float*>(grab((size_t)d.seg * kTapeWords * NP * 4));
-  float* lam = static_cast<float*>(grab(3 * NP * 4));
-  float* lc = static_cast<float*>(grab(3 * NP * 4));
-  float* ld = static_cast<float*>(grab(3 * NP * 4));
-  float* l1 = static_cast<float*>(grab(3 * NP * 4));
-  float* gn = static_cast<float*>(grab(NP * 4));
-  float2* gdz = static_cast<float2*>(grab(std::max<size_t>(L, 1) * 8));
-  float* dseed = static_cast<float*>(grab((size_t)std::max(d.max_samples, 1u) * d.n_gauges * 4));
-  if (!tape || !lam || !lc || !ld || !l1 || !gn || !gdz || !dseed) {
-    release();
-    return fail(c, INU_EHIP, "adjoint: out of device memory (the tape needs segment x 56 B per cell)");
-  }
-  hipError_t e = hipSuccess;
-  auto ok = [&](hipError_t r) {
-    if (e == hipSuccess) e = r;
-  };
-  ok(hipMemsetAsync(lam, 0, 3 * NP * 4, c->stream));
-  ok(hipMemsetAsync(gn, 0, NP * 4, c->stream));
-  ok(hipMemsetAsync(gdz, 0, std::max<size_t>(L, 1) * 8, c->stream));
-  ok(hipMemsetAsync(tape, 0, (size_t)d.seg * kTapeWords * NP * 4, c->stream));
-  ok(hipMemcpyAsync(dseed, seeds, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyHostToDevice, c->stream));
-  const Args& a = c->a;
-  hipLaunchKernelGGL(k_adj_all_blocks, dim3((c->NB + 255) / 256), dim3(256), 0, c->stream, a);
-  const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
-  const bool fm = (c->flags & INU_FLAG_EXACT_ORDER) == 0;
-  int occ = 0;
-  ok(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_adj_stage<0>, 256, c->lds_stage1));
-  const uint32_t grid_adj = std::max<uint32_t>(1u, std::min<uint32_t>(c->NB, (uint32_t)std::max(occ, 1) * c->cus));
-  const uint32_t gc = (uint32_t)((NP + 255) / 256);
+  HIPCHK(c, hipMemsetAsync(d.lam, 0, 3 * NP * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(d.gn, 0, NP * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(d.gdz, 0, std::max<size_t>(L, 1) * 8, c->stream));
+  HIPCHK(c, hipMemsetAsync(d.cur, 0, sizeof(AdjCur), c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.dseed, seeds, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyHostToDevice,
+                           c->stream));
   const uint32_t n_seg = (K + d.seg - 1) / d.seg;
-  for (uint32_t cs = n_seg; cs-- > 0 && e == hipSuccess;) {
+  for (uint32_t cs = n_seg; cs-- > 0;) {
     const uint32_t k0 = cs * d.seg, k1 = std::min(K, k0 + d.seg);
-    const float* ck = d.ck + (size_t)cs * 4 * NP;
-    ok(hipMemcpyAsync(a.h, ck, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-    ok(hipMemcpyAsync(a.hu, ck + NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-    ok(hipMemcpyAsync(a.hv, ck + 2 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-    ok(hipMemcpyAsync(a.hlo, ck + 3 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-    // Replay the segment with the recorded dt and source rates, writing the tape.
-    for (uint32_t k = k0; k < k1; ++k) {
-      const float* rates = nullptr;
-      for (const auto& evk : d.events)
-        if ((k == k0 && evk.first <= k) || (k > k0 && evk.first == k)) rates = evk.second;
-      if (rates) ok(hipMemcpyAsync(a.src, rates, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-      hipLaunchKernelGGL(k_adj_set_dt, dim3(1), dim3(1), 0, c->stream, a, log[k]);
-      Args ar = a;
-      ar.tape = tape;
-      ar.tape_slot = k - k0;
-      const dim3 b(256);
-      const dim3 g0(c->grid[0][comp ? 1 : 0][fm ? 1 : 0][0]), g1(c->grid[1][comp ? 1 : 0][fm ? 1 : 0][0]);
-      if (comp && fm) {
-        hipLaunchKernelGGL((k_stage<0, true, true, 0, true>), g0, b, c->lds_stage0, c->stream, ar);
-        hipLaunchKernelGGL((k_stage<1, true, true, 0, true>), g1, b, c->lds_stage1, c->stream, ar);
-      } else if (comp) {
-        hipLaunchKernelGGL((k_stage<0, true, false, 0, true>), g0, b, c->lds_stage0, c->stream, ar);
-        hipLaunchKernelGGL((k_stage<1, true, false, 0, true>), g1, b, c->lds_stage1, c->stream, ar);
-      } else if (fm) {
-        hipLaunchKernelGGL((k_stage<0, false, true, 0, true>), g0, b, c->lds_stage0, c->stream, ar);
-        hipLaunchKernelGGL((k_stage<1, false, true, 0, true>), g1, b, c->lds_stage1, c->stream, ar);
-      } else {
-        hipLaunchKernelGGL((k_stage<0, false, false, 0, true>), g0, b, c->lds_stage0, c->stream, ar);
-        hipLaunchKernelGGL((k_stage<1, false, false, 0, true>), g1, b, c->lds_stage1, c->stream, ar);
-      }
-    }
-    // Reverse the segment.
-    for (uint32_t k = k1; k-- > k0;) {
-      const float* slot = tape + (size_t)(k - k0) * kTapeWords * NP;
-      if ((k + 1) % d.every == 0 && (k + 1) / d.every <= d.max_samples)
-        hipLaunchKernelGGL(k_adj_seed, dim3((d.n_gauges + 255) / 256), dim3(256), 0, c->stream, lam, d.gauges,
-                           dseed + (size_t)((k + 1) / d.every - 1) * d.n_gauges, d.n_gauges);
-      hipLaunchKernelGGL(k_adj_heun, dim3(gc), dim3(256), 0, c->stream, slot, (uint32_t)NP, c->p.dry, lam, lc, ld);
-      float dtk;
-      memcpy(&dtk, &log[k], 4);
-      AdjArgs x{a, slot, dtk, lc, nullptr, l1, gn, gdz};
-      hipLaunchKernelGGL(k_adj_stage<1>, dim3(grid_adj), dim3(256), c->lds_stage1, c->stream, x);
-      x.lam_out = l1;
-      x.lam_add = ld;
-      x.lam_in = lam;
-      hipLaunchKernelGGL(k_adj_stage<0>, dim3(grid_adj), dim3(256), c->lds_stage1, c->stream, x);
-    }
-    ok(hipGetLastError());
+    // Replay the segment from its checkpoint, writing the tape, then reverse it.
+    if ((rc = adj_restore(c, cs)) || (rc = adj_replay_range(c, k0, k1, k0, true, nullptr))) return rc;
+    hipLaunchKernelGGL(k_adj_cursor, dim3(1), dim3(1), 0, c->stream, c->a, d.cur, k1 - 1, k0);
+    rc = run_graph(c, kKeyReverse, k1 - k0, [&](uint32_t m) { return adj_enqueue_reverse(c, m); });
+    if (rc) return rc;
   }
-  ok(hipStreamSynchronize(c->stream));
-  if (e != hipSuccess) {
-    release();
-    return fail(c, INU_EHIP, std::string("adjoint: ") + hipGetErrorString(e));
-  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  AdjCur cur;
+  HIPCHK(c, hipMemcpy(&cur, d.cur, sizeof(AdjCur), hipMemcpyDeviceToHost));
+  for (int q = 0; q < 3; ++q) d.stats[q] = cur.stats[q];
   // Manning gradient, and the state gradient, in the caller's cell order.
-  rc = download_cells(c, gn, grad_n);
+  rc = download_cells(c, d.gn, grad_n);
   if (!rc && grad_state)
-    for (int q = 0; q < 3 && !rc; ++q) rc = download_cells(c, lam + q * NP, grad_state + (size_t)q * c->N);
-  // Bed gradient: d loss / d (dzL, dzR) of each edge, through the face-on-the-higher-bed rule.
-  std::vector<float2> hg(L);
-  if (!rc && L) {
-    e = hipMemcpy(hg.data(), gdz, L * 8, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = fail(c, INU_EHIP, std::string("adjoint: ") + hipGetErrorString(e));
-  }
-  release();
+    for (int q = 0; q < 3 && !rc; ++q) rc = download_cells(c, d.lam + q * NP, grad_state + (size_t)q * c->N);
   if (rc) return rc;
-  std::vector<double> gz((size_t)c->NB * kBlockCells, 0.0);
-  for (size_t ed = 0; ed < c->own_pos.size(); ++ed) {
+  // Bed gradient: d loss / d (dzL, dzR) of each edge, from the stream
+  // position that kept it, through the face-on-the-higher-bed rule.
+  std::vector<float2> hg(L);
+  if (L) HIPCHK(c, hipMemcpy(hg.data(), d.gdz, L * 8, hipMemcpyDeviceToHost));
+  std::vector<double> gz(NP, 0.0);
+  for (size_t q = 0; q < L; ++q) {
+    if (hg[q].x == 0.0f && hg[q].y == 0.0f) continue;
+    const uint32_t ed = c->stream_edge[q];
     const int32_t r = c->e_cr[ed];
     if (r < 0) continue;
-    const uint32_t q = c->own_pos[ed];
     const int32_t l = c->e_cl[ed];
     if (c->h_ldz[q] > 0.0f) {  // zR > zL: dzL = zR - zL
       gz[r] += hg[q].x;
@@ -2653,44 +2908,12 @@ int inu_adj_replay(inu_ctx* c, float* series_out) {
   auto& d = c->adj;
   if (!d.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
   if (!series_out) return fail(c, INU_EINVAL, "null output");
-  Scen sc;
-  HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
-  const uint32_t K = (uint32_t)std::min<uint64_t>(sc.ctrl.steps, d.max_steps);
-  if (K == 0) return fail(c, INU_ESTATE, "nothing recorded");
-  std::vector<uint32_t> log(K);
-  HIPCHK(c, hipMemcpy(log.data(), d.log, (size_t)K * 4, hipMemcpyDeviceToHost));
-  const size_t NP = (size_t)c->NB * kBlockCells;
-  float* series = nullptr;
-  HIPCHK(c, hipMalloc(&series, (size_t)std::max(d.max_samples, 1u) * d.n_gauges * 4));
-  hipError_t e = hipMemsetAsync(series, 0, (size_t)std::max(d.max_samples, 1u) * d.n_gauges * 4, c->stream);
-  auto ok = [&](hipError_t r) {
-    if (e == hipSuccess) e = r;
-  };
-  const Args& a = c->a;
-  const float* ck = d.ck;
-  ok(hipMemcpyAsync(a.h, ck, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  ok(hipMemcpyAsync(a.hu, ck + NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  ok(hipMemcpyAsync(a.hv, ck + 2 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  ok(hipMemcpyAsync(a.hlo, ck + 3 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  hipLaunchKernelGGL(k_adj_all_blocks, dim3((c->NB + 255) / 256), dim3(256), 0, c->stream, a);
-  const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
-  const bool fm = (c->flags & INU_FLAG_EXACT_ORDER) == 0;
-  for (uint32_t k = 0; k < K && e == hipSuccess; ++k) {
-    for (const auto& evk : d.events)
-      if (evk.first == k) ok(hipMemcpyAsync(a.src, evk.second, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-    hipLaunchKernelGGL(k_adj_set_dt, dim3(1), dim3(1), 0, c->stream, a, log[k]);
-    launch_stage<0>(c, a, comp, fm, c->tile);
-    launch_stage<1>(c, a, comp, fm, c->tile);
-    if ((k + 1) % d.every == 0 && (k + 1) / d.every <= d.max_samples)
-      hipLaunchKernelGGL(k_adj_sample_row, dim3((d.n_gauges + 255) / 256), dim3(256), 0, c->stream, a, series,
-                         d.gauges, d.n_gauges, (k + 1) / d.every - 1);
-  }
-  ok(hipGetLastError());
-  ok(hipStreamSynchronize(c->stream));
-  if (e == hipSuccess)
-    e = hipMemcpy(series_out, series, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyDeviceToHost);
-  (void)hipFree(series);
-  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("adjoint replay: ") + hipGetErrorString(e));
+  uint32_t K = 0;
+  if ((rc = adj_recorded_steps(c, &K)) || (rc = adj_work(c))) return rc;
+  HIPCHK(c, hipMemsetAsync(d.rseries, 0, (size_t)std::max(d.max_samples, 1u) * d.n_gauges * 4, c->stream));
+  if ((rc = adj_restore(c, 0)) || (rc = adj_replay_range(c, 0, K, 0, false, d.rseries))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(series_out, d.rseries, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -2703,32 +2926,32 @@ int inu_adj_dt_log(inu_ctx* c, uint32_t* out) {
   return 0;
 }
 
+int inu_adj_stats(inu_ctx* c, unsigned long long* out3) {
+  if (!c) return INU_EINVAL;
+  if (!out3) return fail(c, INU_EINVAL, "null output");
+  for (int q = 0; q < 3; ++q) out3[q] = c->adj.stats[q];
+  return 0;
+}
+
+int inu_adj_set_thin(inu_ctx* c, float depth) {
+  if (!c) return INU_EINVAL;
+  if (std::isnan(depth) || !std::isfinite(depth)) return fail(c, INU_EINVAL, "thin-layer depth must be finite");
+  c->adj_thin = depth;
+  for (auto it = c->graphs.begin(); it != c->graphs.end();) {
+    if ((it->first & ~kKeyLenMask) == kKeyReverse) {
+      (void)hipGraphExecDestroy(it->second);
+      it = c->graphs.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  return 0;
+}
+
 int inu_adj_end(inu_ctx* c) {
   int rc = check_ready(c);
   if (rc) return rc;
-  auto& d = c->adj;
-  for (void* q : {(void*)d.log, (void*)d.gauges, (void*)d.series, (void*)d.ck}) {
-    auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
-    if (q && it != c->allocs.end()) {
-      (void)hipFree(q);
-      c->allocs.erase(it);
-    }
-  }
-  for (auto& ev : d.events) {
-    auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)ev.second);
-    if (it != c->allocs.end()) {
-      (void)hipFree(ev.second);
-      c->allocs.erase(it);
-    }
-  }
-  d.events.clear();
-  d.log = nullptr;
-  d.gauges = nullptr;
-  d.series = nullptr;
-  d.ck = nullptr;
-  d.on = false;
-  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
-  c->graphs.clear();
+  adj_release(c);
   return 0;
 }
 
@@ -2742,8 +2965,9 @@ int inu_probe_vjp(inu_ctx* c, uint32_t n, const float* in17, float* out8, float 
   hipError_t e = hipMalloc(&dout, (size_t)n * 8 * 4);
   if (e == hipSuccess) e = hipMemcpy(din, in17, (size_t)n * 17 * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_probe_vjp, dim3((n + 255) / 256), dim3(256), 0, c->stream, din, dout, n,
-                       make_phys(g, dry_tol, 0.8f));
+    Phys pp = make_phys(g, dry_tol, 0.8f);
+    pp.adj_thin = c->adj_thin < 0.0f ? kAdjThinDry * dry_tol : c->adj_thin;
+    hipLaunchKernelGGL(k_probe_vjp, dim3((n + 255) / 256), dim3(256), 0, c->stream, din, dout, n, pp);
     e = hipStreamSynchronize(c->stream);
   }
   if (e == hipSuccess) e = hipMemcpy(out8, dout, (size_t)n * 8 * 4, hipMemcpyDeviceToHost);

@@ -33,6 +33,7 @@ struct Phys {
   float sqrt_g_e;  // sqrt(g / e_film)
   float inv_dry;   // 1 / dry_tol
   float sqrt_g;    // sqrt(g)
+  float adj_thin;  // adjoint only: face depth below which a side's velocity and celerity are held
 };
 
 __host__ __device__ __forceinline__ float gmin(float x, float y) { return (y < x) ? y : x; }

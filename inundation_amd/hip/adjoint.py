@@ -13,6 +13,7 @@ resident Heun step as ``hip_resident`` (reference
 
 from __future__ import annotations
 
+from collections.abc import Callable
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -42,20 +43,26 @@ class GaugeRun:
     source_cells: NDArray[np.uint32] | None = None
     # (first step, rates of source_cells) per phase, the first at step 0
     phases: list[tuple[int, NDArray[np.float32]]] = field(default_factory=list)
+    # or rates as a function of simulated time (s), looked up every graph_steps steps
+    rates_at: Callable[[float], NDArray[np.float32]] | None = None
     graph_steps: int = 50
 
     @property
     def n_cells(self) -> int:
         return int(self.state0[0].size)
 
-    def forward(self, n: NDArray[np.float32], z: NDArray[np.float32]) -> NDArray[np.float32]:
+    def _start(self, n: NDArray[np.float32], z: NDArray[np.float32]) -> NDArray[np.uint32]:
         c = self.ctx
+        c.adj_end()  # sources may not be replaced while a recording is open
         c.set_params(self.g, self.dry_tol, self.cfl, self.flags)
         c.set_manning(np.asarray(n, np.float32))
         c.set_bed(np.asarray(z, np.float32))
         c.set_state(*self.state0, None)
         cells = np.empty(0, np.uint32) if self.source_cells is None else self.source_cells
-        rates0 = self.phases[0][1] if self.phases else np.empty(0, np.float32)
+        if self.rates_at is not None:
+            rates0 = self.rates_at(0.0)
+        else:
+            rates0 = self.phases[0][1] if self.phases else np.empty(0, np.float32)
         c.set_sources(cells, rates0)
         ctrl = L.InuCtrl()
         ctrl.now_ticks = 0
@@ -67,7 +74,24 @@ class GaugeRun:
         ctrl.cfl_safety = 1.0
         c.ctrl_write(ctrl)
         c.prologue()
-        c.adj_begin(self.n_steps, self.seg, self.gauges, self.every)
+        return cells
+
+    def _drive(self, cells: NDArray[np.uint32], step: Callable[[int], None]) -> None:
+        """Run n_steps in chunks of at most graph_steps, changing the source rates between chunks."""
+        c = self.ctx
+        if self.rates_at is not None:
+            last = self.rates_at(0.0)
+            done = 0
+            while done < self.n_steps:
+                if done:
+                    r = self.rates_at(c.ctrl_read().now_ticks / float(1 << 20))
+                    if not np.array_equal(r, last):
+                        c.set_source_rates(cells, r)
+                        last = r
+                m = min(self.graph_steps, self.n_steps - done)
+                step(m)
+                done += m
+            return
         starts = [p[0] for p in self.phases[1:]] + [self.n_steps]
         done = 0
         for k, end in enumerate(starts):
@@ -75,9 +99,23 @@ class GaugeRun:
                 c.set_source_rates(cells, self.phases[k][1])
             while done < end:
                 m = min(self.graph_steps, end - done)
-                c.adj_steps(m)
+                step(m)
                 done += m
-        return c.adj_series()
+
+    def forward(self, n: NDArray[np.float32], z: NDArray[np.float32]) -> NDArray[np.float32]:
+        """The recorded run: gauge depths (samples x gauges); the adjoint's tape is kept on the device."""
+        cells = self._start(n, z)
+        self.ctx.adj_begin(self.n_steps, self.seg, self.gauges, self.every)
+        self._drive(cells, self.ctx.adj_steps)
+        return self.ctx.adj_series()
+
+    def forward_plain(self, n: NDArray[np.float32], z: NDArray[np.float32]) -> None:
+        """The same run without recording (the plain resident step): the cost the adjoint is compared with.
+
+        Ends the context's recording.
+        """
+        cells = self._start(n, z)
+        self._drive(cells, self.ctx.steps)
 
     def backward(self, seeds: NDArray[np.float32]) -> tuple[NDArray[np.float32], NDArray[np.float32]]:
         gn, gz, _ = self.ctx.adj_backward(seeds, self.n_cells)

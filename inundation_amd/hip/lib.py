@@ -79,6 +79,8 @@ EXPORTS = (
     "inu_adj_backward",
     "inu_adj_replay",
     "inu_adj_dt_log",
+    "inu_adj_stats",
+    "inu_adj_set_thin",
     "inu_adj_end",
     "inu_probe_vjp",
 )
@@ -176,6 +178,8 @@ def load() -> C.CDLL:
         "inu_adj_backward": (_I, [_P, _P, _P, _P, _P]),
         "inu_adj_replay": (_I, [_P, _P]),
         "inu_adj_dt_log": (_I, [_P, _P]),
+        "inu_adj_stats": (_I, [_P, _P]),
+        "inu_adj_set_thin": (_I, [_P, _F]),
         "inu_adj_end": (_I, [_P]),
         "inu_probe_vjp": (_I, [_P, _U32, _P, _P, _F, _F]),
     }
@@ -433,6 +437,16 @@ class Context:
         out = np.zeros(int(max_steps), dtype=np.uint32)
         self.call("inu_adj_dt_log", ptr(out))
         return out.view(np.float32)
+
+    def adj_stats(self) -> dict[str, int]:
+        """Counters of the last backward pass (see inu_adj_stats)."""
+        out = np.zeros(3, dtype=np.uint64)
+        self.call("inu_adj_stats", ptr(out))
+        return {"listed_block_steps": int(out[0]), "idle_block_steps": int(out[1]), "boundary_film_edges": int(out[2])}
+
+    def adj_set_thin(self, depth: float) -> None:
+        """Face depth below which the reverse pass holds a side's velocity and celerity."""
+        self.call("inu_adj_set_thin", float(depth))
 
     def adj_end(self) -> None:
         self.call("inu_adj_end")

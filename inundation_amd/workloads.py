@@ -8,7 +8,11 @@
   tests);
 - ``inflow``: a Lloyd-relaxed Voronoi mesh over 2 km of fBm terrain with a
   carved channel and one point source with a triangular hydrograph (peak
-  20 m^3/s at 30 min over 1 h), n = 0.035.
+  20 m^3/s at 30 min over 1 h), n = 0.035;
+- ``batched``: 64 scenarios over five such meshes (50 k to 2 M cells), each
+  with its own inflow peak and rain;
+- ``calibration``: ``inflow`` on 250 k cells with a smooth 'true' Manning
+  field in [0.02, 0.08], 32 gauges, 2000 steps, for the adjoint.
 
 Terrain is evaluated at the mean of each cell's vertices (the polygon centroid
 for the lattice quads).
@@ -23,7 +27,7 @@ import numpy as np
 from numpy.typing import NDArray
 
 from .mesh.geometry import MeshGeometry, build_geometry, hilbert_reorder
-from .mesh.synth import fbm_terrain, gaussian_bumps, lattice_tri_quad, quad_grid, voronoi_mesh
+from .mesh.synth import fbm_terrain, gaussian_bumps, lattice_tri_quad, quad_grid, sample_bilinear, voronoi_mesh
 from .workflow import PointSource, SimulationPhase, build_phase_source_rate
 
 
@@ -209,4 +213,78 @@ def batched(
     return geoms, scen
 
 
-__all__ = ["Scenario", "Workload", "batched", "inflow", "lake_at_rest", "rainfall"]
+def smooth_field(
+    x: NDArray[np.float64],
+    y: NDArray[np.float64],
+    length: float,
+    rng: np.random.Generator,
+    lo: float,
+    hi: float,
+    sigma: float = 150.0,
+    grid: int = 256,
+) -> NDArray[np.float32]:
+    """Gaussian-filtered white noise (``sigma`` metres) mapped linearly onto [lo, hi] over the cells."""
+    noise = rng.standard_normal((grid, grid))
+    k = np.fft.fftfreq(grid, d=length / grid)
+    gain = np.exp(-2.0 * np.pi**2 * sigma**2 * (k[:, None] ** 2 + k[None, :] ** 2))
+    f = np.fft.ifft2(np.fft.fft2(noise) * gain).real
+    v = sample_bilinear(f, length, x, y)
+    v = (v - v.min()) / max(float(v.max() - v.min()), 1e-300)
+    return (lo + (hi - lo) * v).astype(np.float32)
+
+
+@dataclass
+class Calibration:
+    """Manning calibration: a 'true' n field, gauges, and the inflow of ``inflow`` on a Voronoi mesh."""
+
+    geom: MeshGeometry
+    n_true: NDArray[np.float32]
+    inflow_cell: int
+    steps: int = 2000
+    seg: int = 100
+    every: int = 10
+    n_gauges: int = 32
+    n_start: float = 0.04
+    seed: int = 1005
+    peak: float = 20.0
+
+    def inflow(self, t: float) -> float:
+        """Discharge (m^3/s) of the one-minute hydrograph phase holding time t (s)."""
+        t_mid = (np.floor(t / 60.0) + 0.5) * 60.0
+        return self.peak * (t_mid / 1800.0 if t_mid <= 1800.0 else max(3600.0 - t_mid, 0.0) / 1800.0)
+
+    def source_cells(self) -> NDArray[np.uint32]:
+        return np.array([self.inflow_cell], np.uint32)
+
+    def rates_at(self, t: float) -> NDArray[np.float32]:
+        """dh/dt of the source cell in the phase holding t."""
+        return np.array([np.float32(self.inflow(t)) / self.geom.area[self.inflow_cell]], np.float32)
+
+    def pick_gauges(self, h_truth: NDArray[np.float32], min_depth: float = 0.05) -> NDArray[np.uint32]:
+        """``n_gauges`` seeded cells among those deeper than ``min_depth`` at the end of the truth run."""
+        wet = np.flatnonzero(h_truth > min_depth)
+        if wet.size < self.n_gauges:
+            raise ValueError(f"only {wet.size} cells are wet in the truth run")
+        rng = np.random.default_rng([self.seed, 1])
+        return np.sort(rng.choice(wet, self.n_gauges, replace=False)).astype(np.uint32)
+
+
+def calibration(n_points: int = 250_000, seed: int = 1005, length: float = 2000.0, steps: int = 2000) -> Calibration:
+    """The adjoint calibration workload: ``inflow``'s terrain and hydrograph on ``n_points`` Voronoi cells.
+
+    The true Manning field is seeded smooth noise (150 m Gaussian filter)
+    mapped onto [0.02, 0.08]; the calibration starts from n = 0.04.
+    """
+    geom = _voronoi_terrain(n_points, seed, length)
+    rng = np.random.default_rng([seed, 0])
+    cx = geom.centroid[:, 0].astype(np.float64)
+    cy = geom.centroid[:, 1].astype(np.float64)
+    n_true = smooth_field(cx, cy, length, rng, 0.02, 0.08)
+    x0 = 0.05 * length
+    y0 = 0.5 * length + 0.1 * length * np.sin(2.0 * np.pi * x0 / length)
+    inlet = int(np.argmin((cx - x0) ** 2 + (cy - y0) ** 2))
+    return Calibration(geom=geom, n_true=n_true, inflow_cell=inlet, steps=steps, seed=seed)
+
+
+__all__ = ["Calibration", "Scenario", "Workload", "batched", "calibration", "inflow", "lake_at_rest", "rainfall",
+           "smooth_field"]

@@ -14,7 +14,11 @@ adjoint at the scheme's kinks:
   the higher bed, the left one on a tie; walls have no step;
 - the square root of a non-positive depth has derivative 0 (a dry side);
 - wet/dry tests, clamps and the HLLC branch choice take the branch the
-  forward values take.
+  forward values take;
+- held (no derivative through them): the weights of the wet/dry speed
+  blend, the star state of a degenerate HLLC star region (|S - SM| at the
+  1e-14 guard), and, below ``thin`` face depth, a side's velocity and
+  celerity (its depth still enters).
 
 Scheme (as ``oracle/ref64.py``): Audusse face depths ``max(0, h - dz)``,
 desingularised velocities (reference ``flux.py:63-66``), HLLC with blended
@@ -96,27 +100,41 @@ def _vel(h: torch.Tensor, q: torch.Tensor, dry: float) -> torch.Tensor:
     return torch.where(h > dry, 2.0 * h * q / (hh + torch.clamp(hh, min=e * e)), torch.zeros_like(h))
 
 
-def _speeds(hL, unL, hR, unR, g: float, dry: float):  # noqa: ANN001, ANN202
+def _hold(x: torch.Tensor, where: torch.Tensor) -> torch.Tensor:
+    """x with no derivative where ``where`` holds."""
+    return torch.where(where, x.detach(), x)
+
+
+def _speeds(hL, unL, hR, unR, g: float, dry: float, thinL=None, thinR=None):  # noqa: ANN001, ANN202
     sqL, sqR = _sqrt0(hL), _sqrt0(hR)
+    hLc, hRc = hL, hR
+    if thinL is not None:
+        sqL, sqR = _hold(sqL, thinL), _hold(sqR, thinR)
+        hLc, hRc = _hold(hL, thinL), _hold(hR, thinR)
     cL, cR = np.sqrt(g) * sqL, np.sqrt(g) * sqR
     den = sqL + sqR
     ok = den > 1e-12
     u_roe = torch.where(ok, (sqL * unL + sqR * unR) / torch.where(ok, den, torch.ones_like(den)), torch.zeros_like(den))
-    c_roe = _sqrt0(0.5 * g * (hL + hR))
+    c_roe = _sqrt0(0.5 * g * (hLc + hRc))
     sl = torch.minimum(unL - cL, u_roe - c_roe)
     sr = torch.maximum(unR + cR, u_roe + c_roe)
-    wl = torch.clamp(hL / dry - 1.0, 0.0, 1.0)
-    wr = torch.clamp(hR / dry - 1.0, 0.0, 1.0)
+    # The blend weights are the wet/dry decision: held.
+    wl = torch.clamp(hL / dry - 1.0, 0.0, 1.0).detach()
+    wr = torch.clamp(hR / dry - 1.0, 0.0, 1.0).detach()
     dl = (1.0 - wl) * wr
     dr = (1.0 - wr) * wl
     roe = 1.0 - dl - dr
     return (roe * sl + dl * (unR - 2.0 * cR) + dr * (unL - cL), roe * sr + dl * (unR + cR) + dr * (unL + 2.0 * cL))
 
 
-def _hllc(hL, hqnL, hqtL, hR, hqnR, hqtR, g: float, dry: float):  # noqa: ANN001, ANN202
+def _hllc(hL, hqnL, hqtL, hR, hqnR, hqtR, g: float, dry: float, thin: float = 0.0):  # noqa: ANN001, ANN202
     unL, utL = _vel(hL, hqnL, dry), _vel(hL, hqtL, dry)
     unR, utR = _vel(hR, hqnR, dry), _vel(hR, hqtR, dry)
-    SL, SR = _speeds(hL, unL, hR, unR, g, dry)
+    thinL = thinR = None
+    if thin > 0.0:
+        thinL, thinR = hL < thin, hR < thin
+        unL, utL, unR, utR = _hold(unL, thinL), _hold(utL, thinL), _hold(unR, thinR), _hold(utR, thinR)
+    SL, SR = _speeds(hL, unL, hR, unR, g, dry, thinL, thinR)
     num = SL * hR * (unR - SR) - SR * hL * (unL - SL)
     den = hR * (unR - SR) - hL * (unL - SL)
     ok = den.abs() > 1e-12
@@ -124,8 +142,9 @@ def _hllc(hL, hqnL, hqtL, hR, hqnR, hqtR, g: float, dry: float):  # noqa: ANN001
     FL = (hL * unL, hL * unL * unL + 0.5 * g * hL * hL, hL * unL * utL)
     FR = (hR * unR, hR * unR * unR + 0.5 * g * hR * hR, hR * unR * utR)
     dSL, dSR = SL - SM, SR - SM
-    kL = hL * (SL - unL) / torch.where(dSL.abs() > 1e-14, dSL, torch.full_like(dSL, 1e-14))
-    kR = hR * (SR - unR) / torch.where(dSR.abs() > 1e-14, dSR, torch.full_like(dSR, 1e-14))
+    okL, okR = dSL.abs() > 1e-14, dSR.abs() > 1e-14
+    kL = _hold(hL * (SL - unL) / torch.where(okL, dSL, torch.full_like(dSL, 1e-14)), ~okL)
+    kR = _hold(hR * (SR - unR) / torch.where(okR, dSR, torch.full_like(dSR, 1e-14)), ~okR)
     FsL = (FL[0] + SL * (kL - hL), FL[1] + SL * (kL * SM - hL * unL), FL[2] + SL * (kL * utL - hL * utL))
     FsR = (FR[0] + SR * (kR - hR), FR[1] + SR * (kR * SM - hR * unR), FR[2] + SR * (kR * utR - hR * utR))
     out = [
@@ -135,7 +154,7 @@ def _hllc(hL, hqnL, hqtL, hR, hqnR, hqtR, g: float, dry: float):  # noqa: ANN001
     return out[0], out[1], out[2]
 
 
-def edge_words(m: MeshT, h, hu, hv, dzl, dzr, g: float, dry: float):  # noqa: ANN001, ANN201
+def edge_words(m: MeshT, h, hu, hv, dzl, dzr, g: float, dry: float, thin: float = 0.0):  # noqa: ANN001, ANN201
     """(5, E) words: mass, left momentum x/y, right momentum x/y (times length)."""
     hL, huL, hvL = h[m.cl], hu[m.cl], hv[m.cl]
     qn = huL * m.nx + hvL * m.ny
@@ -149,7 +168,7 @@ def edge_words(m: MeshT, h, hu, hv, dzl, dzr, g: float, dry: float):  # noqa: AN
     uR, vR = _vel(hR, huR, dry), _vel(hR, hvR, dry)
     huLs, hvLs, huRs, hvRs = hLs * uL, hLs * vL, hRs * uR, hRs * vR
     Fh, Fhn, Fht = _hllc(hLs, huLs * m.nx + hvLs * m.ny, -huLs * m.ny + hvLs * m.nx,
-                         hRs, huRs * m.nx + hvRs * m.ny, -huRs * m.ny + hvRs * m.nx, g, dry)
+                         hRs, huRs * m.nx + hvRs * m.ny, -huRs * m.ny + hvRs * m.nx, g, dry, thin)
     Fh = torch.where(m.wall, torch.zeros_like(Fh), Fh)
     Fhu = Fhn * m.nx - Fht * m.ny
     Fhv = Fhn * m.ny + Fht * m.nx
@@ -161,8 +180,8 @@ def edge_words(m: MeshT, h, hu, hv, dzl, dzr, g: float, dry: float):  # noqa: AN
     return torch.where(live[None, :], w, torch.zeros_like(w))
 
 
-def stage(m: MeshT, h, hu, hv, n, dzl, dzr, dt: float, g: float, dry: float):  # noqa: ANN001, ANN201
-    w = edge_words(m, h, hu, hv, dzl, dzr, g, dry)
+def stage(m: MeshT, h, hu, hv, n, dzl, dzr, dt: float, g: float, dry: float, thin: float = 0.0):  # noqa: ANN001, ANN201
+    w = edge_words(m, h, hu, hv, dzl, dzr, g, dry, thin)
     e = m.slot_edge
     r = m.slot_right
     sdh = torch.zeros_like(h).index_add(0, m.slot_cell, torch.where(r, -w[0, e], w[0, e]))
@@ -187,9 +206,10 @@ def stage(m: MeshT, h, hu, hv, n, dzl, dzr, dt: float, g: float, dry: float):  #
     return h1, hu1 / den, hv1 / den
 
 
-def heun_step(m: MeshT, h, hu, hv, n, dzl, dzr, dt: float, g: float, dry: float, src=None):  # noqa: ANN001, ANN201
-    h1, hu1, hv1 = stage(m, h, hu, hv, n, dzl, dzr, dt, g, dry)
-    h2, hu2, hv2 = stage(m, h1, hu1, hv1, n, dzl, dzr, dt, g, dry)
+def heun_step(m: MeshT, h, hu, hv, n, dzl, dzr, dt: float, g: float, dry: float, src=None,  # noqa: ANN001, ANN201
+              thin: float = 0.0):
+    h1, hu1, hv1 = stage(m, h, hu, hv, n, dzl, dzr, dt, g, dry, thin)
+    h2, hu2, hv2 = stage(m, h1, hu1, hv1, n, dzl, dzr, dt, g, dry, thin)
     ho = 0.5 * (h + h2)
     wet = ho >= dry
     zero = torch.zeros_like(ho)
@@ -212,6 +232,7 @@ def misfit_and_grads(
     sample_every: int,
     g: float = 9.81,
     dry: float = 1e-4,
+    thin: float = 0.0,
 ) -> tuple[float, np.ndarray, np.ndarray]:
     """Loss and its FP64 gradients with respect to z and n.
 
@@ -228,7 +249,7 @@ def misfit_and_grads(
     loss = torch.zeros((), dtype=F64)
     k = 0
     for step, dt in enumerate(np.asarray(dts, np.float64).tolist(), start=1):
-        h, hu, hv = heun_step(mesh, h, hu, hv, nt, dzl, dzr, dt, g, dry, srct)
+        h, hu, hv = heun_step(mesh, h, hu, hv, nt, dzl, dzr, dt, g, dry, srct, thin)
         if step % sample_every == 0:
             loss = loss + ((h[gi] - ot[k]) ** 2).sum()
             k += 1

@@ -86,10 +86,10 @@ def test_gradients_match_central_differences(case) -> None:  # noqa: ANN001
 def test_a_planted_friction_mistake_fails(case, monkeypatch) -> None:  # noqa: ANN001
     real_stage = A.stage
 
-    def wrong(m, h, hu, hv, n, dzl, dzr, dt, g, dry):  # noqa: ANN001, ANN202
+    def wrong(m, h, hu, hv, n, dzl, dzr, dt, g, dry, thin=0.0):  # noqa: ANN001, ANN202
         # Manning gradient of the wrong power: n enters as n^2 * n / n0 with n0 detached.
         n_bad = n * n.detach() / n.detach().clamp(min=1e-30) ** 0.5 * n.detach() ** 0.5
-        return real_stage(m, h, hu, hv, n + 0.5 * (n_bad - n_bad.detach()), dzl, dzr, dt, g, dry)
+        return real_stage(m, h, hu, hv, n + 0.5 * (n_bad - n_bad.detach()), dzl, dzr, dt, g, dry, thin)
 
     def grads(*a):  # noqa: ANN002, ANN202
         monkeypatch.setattr(A, "stage", wrong)
