@@ -1465,6 +1465,21 @@ int check_ready(inu_ctx* c) {
   return 0;
 }
 
+// A copy on the solver stream, waited for: ordered after every kernel and
+// copy already queued there.
+int copy_sync(inu_ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (bytes == 0) return 0;
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, kind, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// The scenarios' records (control block and dt words), copied whole.
+int scen_read(inu_ctx* c, std::vector<Scen>& out) {
+  out.resize(c->n_scen);
+  return copy_sync(c, out.data(), c->scen, sizeof(Scen) * c->n_scen, hipMemcpyDeviceToHost);
+}
+
 // Kernel launches of one step, in stream order. With `ev` set, events are
 // recorded before the predictor, between the two stages and after the
 // corrector (no host wait).
@@ -2170,19 +2185,19 @@ int inu_ctrl_write(inu_ctx* c, const inu_ctrl* ctrl) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!ctrl) return fail(c, INU_EINVAL, "null ctrl");
-  HIPCHK(c, hipMemcpy2DAsync(&c->scen[0].ctrl, sizeof(Scen), ctrl, sizeof(inu_ctrl), sizeof(inu_ctrl), c->n_scen,
-                             hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return 0;
+  std::vector<Scen> sc;
+  if ((rc = scen_read(c, sc))) return rc;
+  for (uint32_t s = 0; s < c->n_scen; ++s) sc[s].ctrl = ctrl[s];
+  return copy_sync(c, c->scen, sc.data(), sizeof(Scen) * c->n_scen, hipMemcpyHostToDevice);
 }
 
 int inu_ctrl_read(inu_ctx* c, inu_ctrl* ctrl) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!ctrl) return fail(c, INU_EINVAL, "null ctrl");
-  HIPCHK(c, hipMemcpy2DAsync(ctrl, sizeof(inu_ctrl), &c->scen[0].ctrl, sizeof(Scen), sizeof(inu_ctrl), c->n_scen,
-                             hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<Scen> sc;
+  if ((rc = scen_read(c, sc))) return rc;
+  for (uint32_t s = 0; s < c->n_scen; ++s) ctrl[s] = sc[s].ctrl;
   return 0;
 }
 
@@ -2190,9 +2205,9 @@ int inu_dt_words(inu_ctx* c, uint32_t* out4) {
   int rc = check_ready(c);
   if (rc) return rc;
   if (!out4) return fail(c, INU_EINVAL, "null output");
-  HIPCHK(c, hipMemcpy2DAsync(out4, 16, &c->scen[0].dt_cur, sizeof(Scen), 16, c->n_scen, hipMemcpyDeviceToHost,
-                             c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<Scen> sc;
+  if ((rc = scen_read(c, sc))) return rc;
+  for (uint32_t s = 0; s < c->n_scen; ++s) memcpy(out4 + 4 * s, &sc[s].dt_cur, 16);
   return 0;
 }
 
@@ -2372,12 +2387,13 @@ int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out7, float g, 
     (void)hipFree(din);
     return fail(c, INU_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
-  e = hipMemcpy(din, pairs, (size_t)n * 8, hipMemcpyHostToDevice);
+  e = hipMemcpyAsync(din, pairs, (size_t)n * 8, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_probe, dim3((n + 255) / 256), dim3(256), 0, c->stream, din, dout, n, make_phys(g, dry_tol, 0.8f));
     e = hipStreamSynchronize(c->stream);
   }
-  if (e == hipSuccess) e = hipMemcpy(out7, dout, (size_t)n * 4 * kProbeOut, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpyAsync(out7, dout, (size_t)n * 4 * kProbeOut, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(din);
   (void)hipFree(dout);
   if (e != hipSuccess) return fail(c, INU_EHIP, std::string("probe: ") + hipGetErrorString(e));
@@ -2423,7 +2439,7 @@ int inu_autotune(inu_ctx* c, uint32_t reps, uint32_t* out4) {
   if (rc) return rc;
   if (!out4 || reps == 0u || reps > 1000u) return fail(c, INU_EINVAL, "inu_autotune: reps in [1, 1000] and an output");
   uint32_t any_dt = 0;
-  HIPCHK(c, hipMemcpy(&any_dt, &c->ctl->any_dt, 4, hipMemcpyDeviceToHost));
+  if ((rc = copy_sync(c, &any_dt, &c->ctl->any_dt, 4, hipMemcpyDeviceToHost))) return rc;
   if (any_dt == 0u) return fail(c, INU_ESTATE, "inu_autotune: no scenario has a step to take (call after inu_prologue)");
   const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
   const bool fm = (c->flags & INU_FLAG_EXACT_ORDER) == 0;
@@ -2506,9 +2522,9 @@ int inu_set_source_rates(inu_ctx* c, uint32_t n, const uint32_t* cell, const flo
   if (c->adj.on) {
     // The recording replays these rates from the current step on.
     if (rc_nonneg_required(c)) return fail(c, INU_EINVAL, "the adjoint needs non-negative source rates");
-    Scen sc;
-    HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
-    const uint32_t step = (uint32_t)sc.ctrl.steps;
+    std::vector<Scen> sc;
+    if ((rc = scen_read(c, sc))) return rc;
+    const uint32_t step = (uint32_t)sc[0].ctrl.steps;
     const size_t NP = (size_t)c->NB * kBlockCells;
     float* ev = nullptr;
     if (!c->adj.events.empty() && c->adj.events.back().first == step) {
@@ -2517,7 +2533,7 @@ int inu_set_source_rates(inu_ctx* c, uint32_t n, const uint32_t* cell, const flo
       if ((rc = dalloc(c, &ev, NP))) return rc;
       c->adj.events.emplace_back(step, ev);
     }
-    HIPCHK(c, hipMemcpy(ev, c->a.src, NP * 4, hipMemcpyDeviceToDevice));
+    if ((rc = copy_sync(c, ev, c->a.src, NP * 4, hipMemcpyDeviceToDevice))) return rc;
   }
   return 0;
 }
@@ -2732,9 +2748,9 @@ int inu_adj_begin(inu_ctx* c, uint32_t max_steps, uint32_t seg, uint32_t n_gauge
   if (sorted.back() >= c->N) return fail(c, INU_EINVAL, "gauge cell out of range");
   if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
     return fail(c, INU_EINVAL, "gauge cells must be distinct");
-  Scen sc;
-  HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
-  if (sc.ctrl.steps != 0ull) return fail(c, INU_ESTATE, "start the recording at step 0 (after inu_prologue)");
+  std::vector<Scen> sc;
+  if ((rc = scen_read(c, sc))) return rc;
+  if (sc[0].ctrl.steps != 0ull) return fail(c, INU_ESTATE, "start the recording at step 0 (after inu_prologue)");
   adj_release(c);
   auto& d = c->adj;
   d.max_steps = max_steps;
@@ -2803,20 +2819,21 @@ int inu_adj_series(inu_ctx* c, float* out) {
   auto& d = c->adj;
   if (!d.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
   if (!out) return fail(c, INU_EINVAL, "null output");
-  HIPCHK(c, hipMemcpy(out, d.series, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyDeviceToHost));
+  return copy_sync(c, out, d.series, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyDeviceToHost);
   return 0;
 }
 
 // Steps recorded so far, each with a nonzero dt.
 static int adj_recorded_steps(inu_ctx* c, uint32_t* K) {
   auto& d = c->adj;
-  Scen sc;
-  HIPCHK(c, hipMemcpy(&sc, c->scen, sizeof(Scen), hipMemcpyDeviceToHost));
-  if (sc.ctrl.steps == 0 || sc.ctrl.steps > d.max_steps)
+  std::vector<Scen> sc;
+  int rc = scen_read(c, sc);
+  if (rc) return rc;
+  if (sc[0].ctrl.steps == 0 || sc[0].ctrl.steps > d.max_steps)
     return fail(c, INU_ESTATE, "recorded steps must be in [1, max_steps]");
-  *K = (uint32_t)sc.ctrl.steps;
+  *K = (uint32_t)sc[0].ctrl.steps;
   std::vector<uint32_t> log(*K);
-  HIPCHK(c, hipMemcpy(log.data(), d.log, (size_t)*K * 4, hipMemcpyDeviceToHost));
+  if ((rc = copy_sync(c, log.data(), d.log, (size_t)*K * 4, hipMemcpyDeviceToHost))) return rc;
   for (uint32_t k = 0; k < *K; ++k)
     if (log[k] == 0u) return fail(c, INU_ESTATE, "a recorded step has dt = 0 (the recording ran past its stop)");
   return 0;
@@ -2870,7 +2887,7 @@ int inu_adj_backward(inu_ctx* c, const float* seeds, float* grad_n, float* grad_
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   AdjCur cur;
-  HIPCHK(c, hipMemcpy(&cur, d.cur, sizeof(AdjCur), hipMemcpyDeviceToHost));
+  if ((rc = copy_sync(c, &cur, d.cur, sizeof(AdjCur), hipMemcpyDeviceToHost))) return rc;
   for (int q = 0; q < 3; ++q) d.stats[q] = cur.stats[q];
   // Manning gradient, and the state gradient, in the caller's cell order.
   rc = download_cells(c, d.gn, grad_n);
@@ -2880,7 +2897,7 @@ int inu_adj_backward(inu_ctx* c, const float* seeds, float* grad_n, float* grad_
   // Bed gradient: d loss / d (dzL, dzR) of each edge, from the stream
   // position that kept it, through the face-on-the-higher-bed rule.
   std::vector<float2> hg(L);
-  if (L) HIPCHK(c, hipMemcpy(hg.data(), d.gdz, L * 8, hipMemcpyDeviceToHost));
+  if (L && (rc = copy_sync(c, hg.data(), d.gdz, L * 8, hipMemcpyDeviceToHost))) return rc;
   std::vector<double> gz(NP, 0.0);
   for (size_t q = 0; q < L; ++q) {
     if (hg[q].x == 0.0f && hg[q].y == 0.0f) continue;
@@ -2913,8 +2930,7 @@ int inu_adj_replay(inu_ctx* c, float* series_out) {
   HIPCHK(c, hipMemsetAsync(d.rseries, 0, (size_t)std::max(d.max_samples, 1u) * d.n_gauges * 4, c->stream));
   if ((rc = adj_restore(c, 0)) || (rc = adj_replay_range(c, 0, K, 0, false, d.rseries))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(series_out, d.rseries, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyDeviceToHost));
-  return 0;
+  return copy_sync(c, series_out, d.rseries, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyDeviceToHost);
 }
 
 int inu_adj_dt_log(inu_ctx* c, uint32_t* out) {
@@ -2922,8 +2938,7 @@ int inu_adj_dt_log(inu_ctx* c, uint32_t* out) {
   if (rc) return rc;
   if (!c->adj.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
   if (!out) return fail(c, INU_EINVAL, "null output");
-  HIPCHK(c, hipMemcpy(out, c->adj.log, (size_t)c->adj.max_steps * 4, hipMemcpyDeviceToHost));
-  return 0;
+  return copy_sync(c, out, c->adj.log, (size_t)c->adj.max_steps * 4, hipMemcpyDeviceToHost);
 }
 
 int inu_adj_stats(inu_ctx* c, unsigned long long* out3) {
@@ -2963,14 +2978,15 @@ int inu_probe_vjp(inu_ctx* c, uint32_t n, const float* in17, float* out8, float 
   float *din = nullptr, *dout = nullptr;
   HIPCHK(c, hipMalloc(&din, (size_t)n * 17 * 4));
   hipError_t e = hipMalloc(&dout, (size_t)n * 8 * 4);
-  if (e == hipSuccess) e = hipMemcpy(din, in17, (size_t)n * 17 * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpyAsync(din, in17, (size_t)n * 17 * 4, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
     Phys pp = make_phys(g, dry_tol, 0.8f);
     pp.adj_thin = c->adj_thin < 0.0f ? kAdjThinDry * dry_tol : c->adj_thin;
     hipLaunchKernelGGL(k_probe_vjp, dim3((n + 255) / 256), dim3(256), 0, c->stream, din, dout, n, pp);
     e = hipStreamSynchronize(c->stream);
   }
-  if (e == hipSuccess) e = hipMemcpy(out8, dout, (size_t)n * 8 * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpyAsync(out8, dout, (size_t)n * 8 * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(din);
   if (dout) (void)hipFree(dout);
   if (e != hipSuccess) return fail(c, INU_EHIP, std::string("probe_vjp: ") + hipGetErrorString(e));
