@@ -413,6 +413,11 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // ---- (C) cells -----------------------------------------------------------
   bool wet = false;
   uint32_t bound = kNoBound;
+  // The corrector's dt bound is cfl A / lam over the cell's wave speeds and,
+  // for a wetted source cell, over its source speeds: the smaller bound is
+  // the one over the larger lam (the same rounded numerator), so one
+  // division takes both.
+  float lam_b = 0.0f;
   if (tid < nc) {
     float sdh = 0.0f, sdhu = 0.0f, sdhv = 0.0f, lam = 0.0f;
     for (uint32_t k = k0; k < k1; ++k) {
@@ -498,7 +503,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
         hvf = 0.0f;
       }
       wet = hf >= p.dry;
-      if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
+      if (lam > 1e-30f) lam_b = lam;
       // Sources of this block (reference resident.py:695-708), after the
       // corrector; they change depth only.
       if (rate != 0.0f) {
@@ -547,12 +552,10 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
           if (FM) lam = fmaf(fmaf(kf, c, u), sd.y, lam);
           else lam += (u + kf * c) * sd.y;
         }
-        if (lam > 1e-30f && A > 1e-10f) {
-          const uint32_t bb = __float_as_uint(p.cfl * A / lam);
-          bound = (bb < bound) ? bb : bound;
-        }
+        if (lam > 1e-30f) lam_b = inu::gmax(lam_b, lam);
       }
     }
+    if (tid < nc && lam_b > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam_b);
   }
   return bound;
 }

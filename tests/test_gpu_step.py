@@ -58,10 +58,15 @@ def test_probe_is_bitwise_the_cpu(ctx) -> None:  # noqa: ANN001
         10.0 ** rng.uniform(-6, 3, n // 4),
         rng.normal(0, 100, n // 4),
         1.0 + rng.uniform(-1e-6, 1e-6, n // 4),
+        # square roots across the whole range: subnormals, the library's
+        # rescaled band below 2^-96, its edge, signed zeros
+        (10.0 ** rng.uniform(-45.5, 38, n // 4)).astype(np.float32),
+        np.float32(2.0 ** -96) * (1.0 + rng.uniform(-1e-3, 1e-3, 1024)).astype(np.float32),
+        np.array([0.0, -0.0, 2.0 ** -149, 2.0 ** -126, 2.0 ** -96, 3.4e38], np.float32),
     ]).astype(np.float32)
     b = np.concatenate([
         10.0 ** rng.uniform(-9, -1, n // 2),
-        rng.normal(0, 1, n // 2),
+        rng.normal(0, 1, a.size - n // 2),
     ]).astype(np.float32)
     b[b == 0] = 1.0
     pairs = np.column_stack([a, b]).astype(np.float32)
