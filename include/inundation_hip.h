@@ -64,6 +64,14 @@ extern "C" {
 #define INU_FIELD_BLOCK_ACTIVE 10 /* uint32 per block */
 #define INU_FIELD_BLOCK_FORCED 11 /* uint32 per block */
 
+/* Open boundary conditions (reference tuning.py:64-68, the OPEN_* constants of
+ * shaders/flux.py:194-202). */
+#define INU_OPEN_WALL 0u
+#define INU_OPEN_FREE 1u   /* zero-gradient outflow */
+#define INU_OPEN_NORMAL 2u /* outflow at the Manning velocity; value sqrt(S) / n */
+#define INU_OPEN_STAGE 3u  /* a level beyond the edge; value the depth over the cell's bed */
+#define INU_OPEN_INFLOW 4u /* value the discharge per unit width */
+
 /* The 48-byte resident control block: the integer clock and step planning
  * (reference RESIDENT_CTRL_DTYPE, gpu_resident_batch.py:65-77; the CTRL block
  * of STEP_CONTROL_GLSL, resident.py:823-833). */
@@ -109,8 +117,11 @@ const char* inu_last_error(const inu_ctx* ctx);
  * gpu_resident_batch.py:160-282). Edges must be sorted by the block of their
  * left cell; edge_dz holds 2E interleaved (dzL, dzR) words (solver.py:233);
  * cell_edge_idx holds (edge << 1) | is_right in that sorted edge numbering
- * (blocks.py:33-44). edge_cell_r is a cell or -1 (wall); open edges are not
- * implemented. Resets the state, parameters and sources. */
+ * (blocks.py:33-44). edge_cell_r is a cell, -1 (a wall) or -2 - k (open edge
+ * k, k in 0..n_open-1, each once, on an edge without a bed step: reference
+ * solver.py:239-243). Open edges need a context of one scenario; they start
+ * closed (INU_OPEN_WALL), in one stretch. Resets the state, parameters and
+ * sources. */
 int inu_set_mesh(inu_ctx* ctx, uint32_t n_cells, uint32_t n_edges, const int32_t* edge_cell_l,
                  const int32_t* edge_cell_r, const float* edge_nx, const float* edge_ny,
                  const float* edge_len, const float* edge_dz, const float* cell_area,
@@ -209,6 +220,15 @@ int inu_upload(inu_ctx* ctx, int which, const void* in);
  * solver.py:449-467). */
 int inu_probe(inu_ctx* ctx, uint32_t n, const float* pairs, float* out7, float g, float dry_tol);
 
+/* Test probe of the open boundary formulas (the reference tests them the same
+ * way: tests/unit/test_gpu_open_ghost.py:46-68 runs open_ghost and
+ * boundary_mass_flux, flux.py:224-281, on the device). Case k is (h, hu, hv,
+ * nx, ny, fh) of the cell inside an open edge of condition kinds[k] and value
+ * values[k]; out4 gets the ghost (h, hu, hv) and the edge's mass flux for an
+ * HLLC mass flux fh. */
+int inu_probe_open(inu_ctx* ctx, uint32_t n, const float* cases6, const uint32_t* kinds, const float* values,
+                   float* out4, float g, float dry_tol);
+
 /* A plain device copy of n_bytes (a positive multiple of 16), launched
  * `reps` times after one untimed launch; *ms_out = mean duration of one copy.
  * Measurement support with no counterpart in the reference: a kernel of known
@@ -228,6 +248,26 @@ int inu_autotune(inu_ctx* ctx, uint32_t reps, uint32_t* out4);
 
 /* Sets the tile level (0, 1 or 2) directly; inu_set_mesh/inu_set_batch reset it to 0. */
 int inu_set_tile(inu_ctx* ctx, uint32_t tile);
+
+/* Open boundary stretches: open edge k belongs to stretch s when
+ * stretch_ptr[s] <= k < stretch_ptr[s + 1] (n_stretches + 1 words, from 0 to
+ * the open edge count; reference SWESolver's open_edges, solver.py:140-165,
+ * one edge array per stretch, numbered in stretch order, and
+ * gpu_resident_batch.py:250-263). Clears the volume tally. */
+int inu_set_open_stretches(inu_ctx* ctx, uint32_t n_stretches, const uint32_t* stretch_ptr);
+
+/* Condition (INU_OPEN_*) and value of every open edge, before a run or a
+ * phase; replaces set_open_conditions (reference solver.py:427-438,
+ * gpu_resident_batch.py:447-458). A block whose open edge can bring water in
+ * (inflow or stage with value > 0) counts as wet while these hold; call
+ * inu_prologue (or inu_retarget) before stepping so the lists see it. */
+int inu_set_open_conditions(inu_ctx* ctx, uint32_t n_open, const uint32_t* kinds, const float* values);
+
+/* Volume [m^3] out of and into the mesh through each stretch since the last
+ * inu_set_state or inu_set_open_stretches: out2 = (out, in) per stretch, for
+ * n_stretches stretches (the count last set; reference download_open_volumes,
+ * gpu_resident_batch.py:460-467; the tally of resident.py:613-658). */
+int inu_open_volumes(inu_ctx* ctx, uint32_t n_stretches, double* out2);
 
 /* Bed elevation per cell: recomputes every edge's bed step with the face on
  * the higher bed (reference mesh/geometry.py:241-254, edge_bed_steps, FP64

@@ -119,10 +119,27 @@ struct Args {
   Ctl* ctl;
   Scen* scen;
   uint32_t n_scen;
+  // open boundary edges (device record, read only where they are handled)
+  const struct OpenDev* open;
+  uint32_t n_open;
   // adjoint tape (recording stage kernels): kTapeWords arrays of N per step
   // slot; the slot is the replay cursor's step minus its segment start
   float* tape;
   const struct AdjCur* adj_cur;
+};
+
+// Open boundary edges: condition and value per open edge; each open edge's
+// six words per stage (from k_open_words) and its mass word (flux x length)
+// per stage, for the boundary tally; per block, its open edges
+// [bptr[b], bptr[b + 1]) as (local edge position, open edge index) (the edge
+// stream holds them as walls).
+struct OpenDev {
+  const uint32_t* kind;
+  const float* value;
+  float4* words;
+  float* flux;
+  const uint32_t* bptr;
+  const uint2* jk;
 };
 
 // Device-side cursor of an adjoint replay or reverse pass: the step being
@@ -279,8 +296,94 @@ __device__ __forceinline__ void put_cell(const Lds& l, const Phys& p, uint32_t q
                                           p.half_g * s.x * s.x, 0.0f);
 }
 
+// The words of an open edge with right side (hR, huR, hvR) beyond it; `wall`
+// passes no mass; fh receives the HLLC mass flux. The face depths are the
+// cell depths (no bed step).
+template <bool FM>
+__device__ __forceinline__ void open_words(const Phys& p, float4 sl, float4 xl, float hR, float huR, float hvR,
+                                           bool wall, const EdgeRec& r, float w[6], float& fh) {
+  if (FM) {
+    const inu::fast::Side L{sl.x, sl.y, sl.z, 0.0f, xl.x, xl.y, xl.z};
+    const inu::fast::Side R{hR, huR, hvR, 0.0f, inu::fast::vel_factor(p, hR), sqrtf(inu::gmax(hR, 0.0f)),
+                            p.half_g * hR * hR};
+    inu::fast::edge_words(p, wall, L, R, 0.0f, 0.0f, r.nx, r.ny, r.len, w, &fh);
+  } else {
+    inu::edge_words(p, wall, sl.x, sl.y, sl.z, hR, huR, hvR, 0.0f, 0.0f, r.nx, r.ny, r.len, w, &fh);
+  }
+}
+
+// Words of an open edge (reference resident.py:130-142 and flux.py:271-296)
+// from its inside cell's stage input (sl, and xl = its vel_factor, square root
+// and pressure in the fast order): the ghost state of its condition; a dry
+// edge passes nothing unless an inflow keeps it live; an outflow-only edge
+// whose flux would draw water in is redone against the wall ghost; otherwise
+// the mass word is the condition's boundary flux.
+template <bool FM>
+__device__ void open_edge_words(const Phys& p, uint32_t kind, float value, float4 sl, float4 xl, const EdgeRec& r,
+                                float w[6]) {
+  float hG, huG, hvG;
+  inu::ghost_state(p, kind, value, sl.x, sl.y, sl.z, r.nx, r.ny, hG, huG, hvG);
+  for (int q = 0; q < 6; ++q) w[q] = 0.0f;
+  if (!(sl.x < p.dry && hG < p.dry) || inu::open_forced(kind, value)) {
+    float fh;
+    open_words<FM>(p, sl, xl, hG, huG, hvG, false, r, w, fh);
+    if (inu::open_outflow_only(kind) && fh < 0.0f) {
+      inu::ghost_state(p, inu::kOpenWall, 0.0f, sl.x, sl.y, sl.z, r.nx, r.ny, hG, huG, hvG);
+      open_words<FM>(p, sl, xl, hG, huG, hvG, true, r, w, fh);
+    } else {
+      w[0] = inu::boundary_mass(kind, value, fh) * r.len;
+    }
+  }
+}
+
+// Open edge words of stage ST, one thread per open edge, before the stage
+// kernel (which copies them into its LDS edge list), from the inside cell's
+// stage input as the stage kernel loads it; the mass word also goes to the
+// boundary tally. A cell whose block is not visited gets words nobody reads.
+template <int ST, bool FM>
+__global__ void k_open_words(const Args a, const uint32_t* open_cell, const float4* open_geom) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n_open || a.ctl->any_dt == 0u) return;
+  const Phys& p = a.p;
+  const uint32_t i = open_cell[k];
+  const float h = (ST == 0) ? a.h[i] : a.h1[i];
+  const float4 sl = make_float4(h, (ST == 0) ? a.hu[i] : a.hu1[i], (ST == 0) ? a.hv[i] : a.hv1[i], 0.0f);
+  const float4 xl = FM ? make_float4(inu::fast::vel_factor(p, h), sqrtf(inu::gmax(h, 0.0f)), p.half_g * h * h, 0.0f)
+                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float4 g = open_geom[k];
+  EdgeRec r;
+  r.side = 0u;
+  r.nx = g.x;
+  r.ny = g.y;
+  r.len = g.z;
+  r.d = 0.0f;
+  float w[6];
+  open_edge_words<FM>(p, a.open->kind[k], a.open->value[k], sl, xl, r, w);
+  float4* o = a.open->words + 2 * ((size_t)ST * a.n_open + k);
+  o[0] = make_float4(w[0], w[1], w[2], w[3]);
+  o[1] = make_float4(w[4], w[5], 0.0f, 0.0f);
+  a.open->flux[(size_t)ST * a.n_open + k] = w[0];
+}
+
+// Open edges of block b, after its edge list is complete: their words from
+// k_open_words replace the wall words. Only a mesh with open edges pays for
+// the pass.
+template <int ST>
+__device__ __forceinline__ void open_pass(const Args& a, const Lds& l, uint32_t b) {
+  const uint32_t t1 = a.open->bptr[b + 1];
+  for (uint32_t t = a.open->bptr[b] + threadIdx.x; t < t1; t += kBlockCells) {
+    const uint2 jk = a.open->jk[t];
+    const float4* o = a.open->words + 2 * ((size_t)ST * a.n_open + jk.y);
+    const float4 w0 = o[0], w1 = o[1];
+    l.edge[2 * jk.x] = w0;
+    l.edge[2 * jk.x + 1].x = w1.x;
+    l.edge[2 * jk.x + 1].y = w1.y;
+  }
+}
+
 // Words of local edge j into LDS (zeros when both sides are dry); both sides
-// are read from LDS.
+// are read from LDS. An open edge is stored as a wall; open_pass replaces its
+// words.
 template <int ST, bool FM, bool LO>
 __device__ __forceinline__ void edge_eval(const Args& a, const Lds& l, const EdgeRec& r, uint32_t j) {
   const uint32_t ql = r.side >> 16;
@@ -409,6 +512,10 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     nx.dtw = a.scen[bi_scen(nx.bi)].dt_cur;
   }
   __syncthreads();
+  if (a.n_open != 0u) {
+    open_pass<ST>(a, l, b);
+    __syncthreads();
+  }
 
   // ---- (C) cells -----------------------------------------------------------
   bool wet = false;
@@ -880,6 +987,8 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
       const uint32_t qr = r.side & 0xffffu;
       const float4 sl = l.cell[2 * ql];
       float4 sr;
+      const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
+      const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
       if (qr == kSideWall) {
         sr = sl;
         if (FM) inu::fast::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
@@ -887,8 +996,6 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
       } else {
         sr = l.cell[2 * qr];
       }
-      const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
-      const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
       float contrib = 0.0f;
       if (sl.x >= p.dry || sr.x >= p.dry) {
         contrib = (FM ? inu::fast::wave_bound(p, sl.x, sl.y, sl.z, dzL, sr.x, sr.y, sr.z, dzR, r.nx, r.ny, sl.w, sr.w)
@@ -898,6 +1005,25 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
       l.edge[2 * j].x = contrib;
     }
     __syncthreads();
+    if (a.n_open != 0u) {
+      // Open edges: the wave bound against the ghost of their condition
+      // (reference resident.py:556-574).
+      for (uint32_t t = a.open->bptr[b] + tid; t < a.open->bptr[b + 1]; t += kBlockCells) {
+        const uint2 jk = a.open->jk[t];
+        const EdgeRec r = load_edge(a, bi, jk.x);
+        const float4 sl = l.cell[2 * (r.side >> 16)];
+        float4 sr = sl;
+        inu::ghost_state(p, a.open->kind[jk.y], a.open->value[jk.y], sl.x, sl.y, sl.z, r.nx, r.ny, sr.x, sr.y, sr.z);
+        float contrib = 0.0f;
+        if (sl.x >= p.dry || sr.x >= p.dry) {
+          contrib = (FM ? inu::fast::wave_bound(p, sl.x, sl.y, sl.z, 0.0f, sr.x, sr.y, sr.z, 0.0f, r.nx, r.ny, sl.w, sl.w)
+                        : inu::wave_bound(p, sl.x, sl.y, sl.z, 0.0f, sr.x, sr.y, sr.z, 0.0f, r.nx, r.ny)) *
+                    r.len;
+        }
+        l.edge[2 * jk.x].x = contrib;
+      }
+      __syncthreads();
+    }
     uint32_t bound = kNoBound;
     if (tid < nc) {
       const uint32_t i = c0 + tid;
@@ -976,6 +1102,33 @@ __global__ void k_step_control(const Args a, int mode) {
       }
     }
     t->any_dt = any_moving ? 1u : 0u;
+  }
+}
+
+// Volume through each open stretch after a step (reference resident.py:
+// 613-658, run after each flux stage): each stage moves dt/2 times its mass
+// word; out and in are kept apart, each as an exact pair of float32 words
+// (hi, lo). One thread per stretch sums its edges in a fixed order, stage 0
+// then stage 1; an edge of a block not visited this step moved nothing.
+__global__ void k_open_tally(const Args a, const uint32_t* stretch_ptr, const uint32_t* open_cell, float* tally,
+                             uint32_t n_stretch) {
+  const uint32_t st = blockIdx.x * blockDim.x + threadIdx.x;
+  if (st >= n_stretch) return;
+  const uint32_t dtw = a.scen[0].dt_cur;
+  if (dtw == 0u) return;
+  const float half_dt = 0.5f * __uint_as_float(dtw);
+  for (uint32_t stage = 0; stage < 2u; ++stage) {
+    for (uint32_t k = stretch_ptr[st]; k < stretch_ptr[st + 1]; ++k) {
+      if (a.active[open_cell[k] >> 8] == 0u) continue;
+      const float v = half_dt * a.open->flux[(size_t)stage * a.n_open + k];
+      const uint32_t o = 4u * st + ((v < 0.0f) ? 2u : 0u);
+      float sum, err, hi, lo;
+      inu::two_sum(tally[o], fabsf(v), sum, err);
+      err += tally[o + 1];
+      inu::two_sum(sum, err, hi, lo);
+      tally[o] = hi;
+      tally[o + 1] = lo;
+    }
   }
 }
 
@@ -1302,6 +1455,19 @@ __global__ void k_probe(const float* in, float* out, uint32_t n, Phys p) {
   o[6] = (x > 0.0f) ? inu::fast::film_drag(p, x, y) : 0.0f;
 }
 
+// Open boundary probe: cases (h, hu, hv, nx, ny, fh) with a condition and
+// value each -> the ghost state (hG, huG, hvG) and the boundary mass flux of
+// an HLLC mass flux fh.
+__global__ void k_probe_open(const float* in, const uint32_t* kind, const float* value, float* out, uint32_t n,
+                             Phys p) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float* c = in + 6 * (size_t)k;
+  float* o = out + 4 * (size_t)k;
+  inu::ghost_state(p, kind[k], value[k], c[0], c[1], c[2], c[3], c[4], o[0], o[1], o[2]);
+  o[3] = inu::boundary_mass(kind[k], value[k], c[5]);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1366,6 +1532,14 @@ struct inu_ctx {
   // Reverse pass: face depth below which a side's velocity and celerity are
   // held; negative: kAdjThinDry x dry_tol.
   float adj_thin = -1.0f;
+  // Open boundaries: open edge k's inside cell (internal id), the stretches
+  // (open edges [ptr[s], ptr[s + 1]) form stretch s) and their volume tally.
+  uint32_t n_open = 0, n_stretch = 0;
+  std::vector<uint32_t> open_cell_h;
+  uint32_t *open_cell = nullptr, *stretch_ptr = nullptr;
+  float* tally = nullptr;
+  float4* open_geom = nullptr;
+  OpenDev open_h{};  // host copy of the device record's pointers
   // graphs of recorded step blocks, by step count
   std::map<uint32_t, hipGraphExec_t> graphs;
   std::vector<hipEvent_t> events;
@@ -1528,15 +1702,28 @@ void launch_stage_rec(inu_ctx* c, const Args& a, bool comp, bool fm) {
   else hipLaunchKernelGGL((k_stage<ST, false, false, 0, true>), g, b, lds, c->stream, a);
 }
 
+// Open edge words of stage ST (k_open_words), before that stage's kernel.
+template <int ST>
+void launch_open_words(inu_ctx* c, bool fm) {
+  const dim3 g((c->n_open + 255) / 256), b(256);
+  if (fm) hipLaunchKernelGGL((k_open_words<ST, true>), g, b, 0, c->stream, c->a, c->open_cell, c->open_geom);
+  else hipLaunchKernelGGL((k_open_words<ST, false>), g, b, 0, c->stream, c->a, c->open_cell, c->open_geom);
+}
+
 int launch_step(inu_ctx* c, const hipEvent_t* ev) {
   const Args& a = c->a;
   const bool comp = (c->flags & INU_FLAG_COMPENSATED) != 0;
   const bool fm = (c->flags & INU_FLAG_EXACT_ORDER) == 0;
+  if (c->n_open) launch_open_words<0>(c, fm);
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   launch_stage<0>(c, comp, fm);
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  if (c->n_open) launch_open_words<1>(c, fm);
   launch_stage<1>(c, comp, fm);
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  if (c->n_open)
+    hipLaunchKernelGGL(k_open_tally, dim3((c->n_stretch + 63) / 64), dim3(64), 0, c->stream, a, c->stretch_ptr,
+                       c->open_cell, c->tally, c->n_stretch);
   if (c->n_src) {
     launch_source_idle(c, a, comp, fm);
     if (c->n_src_pairs) {
@@ -1620,6 +1807,10 @@ struct Layout {
   std::vector<std::pair<uint32_t, uint32_t>> cross;
   std::vector<uint32_t> bptr, bidx;
   std::vector<uint32_t> stream_edge;  // local edge stream position -> edge
+  std::vector<uint32_t> open_cell;    // open edge k -> its (inside) cell
+  std::vector<float4> open_geom;      // open edge k -> (nx, ny, length, 0)
+  std::vector<uint32_t> open_bptr;    // per block, its open edges in open_jk
+  std::vector<uint2> open_jk;         // (local stream position, open edge index)
 };
 
 // Validates one mesh and builds its layout.
@@ -1642,7 +1833,8 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
   for (uint32_t e = 0; e < E; ++e) {
     const int32_t l = edge_cell_l[e], r = edge_cell_r[e];
     if (l < 0 || (uint32_t)l >= N) return fail(c, INU_EINVAL, "edge_cell_l out of range");
-    if (r < -1 || r >= (int64_t)N) return fail(c, INU_EINVAL, "edge_cell_r out of range (open edges are not supported)");
+    if (r >= (int64_t)N || (r < -1 && (int64_t)(-2 - (int64_t)r) >= (int64_t)E))
+      return fail(c, INU_EINVAL, "edge_cell_r out of range (a cell, -1 for a wall, -2 - k for open edge k)");
     if (e > 0 && (uint32_t)l / kBlockCells < (uint32_t)edge_cell_l[e - 1] / kBlockCells)
       return fail(c, INU_EINVAL, "edges must be sorted by the block of their left cell");
     eo[(uint32_t)l / kBlockCells + 1]++;
@@ -1671,6 +1863,28 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
       return fail(c, INU_EINVAL, "edge_dz: both steps must be finite and non-negative with one of them zero "
                                  "(the face sits on the higher bed)");
   }
+  // Open edges: each index 0..n_open-1 once, on an edge without a bed step.
+  {
+    std::vector<int64_t> open_edge;
+    for (uint32_t e = 0; e < E; ++e) {
+      const int32_t r = edge_cell_r[e];
+      if (r >= -1) continue;
+      const uint32_t k = (uint32_t)(-2 - (int64_t)r);
+      if (k >= open_edge.size()) open_edge.resize(k + 1, -1);
+      if (open_edge[k] >= 0) return fail(c, INU_EINVAL, "two edges name the same open edge index");
+      if (edge_dz[2 * e] != 0.0f || edge_dz[2 * e + 1] != 0.0f)
+        return fail(c, INU_EINVAL, "an open boundary edge has a bed step");
+      open_edge[k] = e;
+    }
+    L.open_cell.assign(open_edge.size(), 0u);
+    L.open_geom.assign(open_edge.size(), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (size_t k = 0; k < open_edge.size(); ++k) {
+      if (open_edge[k] < 0) return fail(c, INU_EINVAL, "open edge indices must be 0..n_open-1 with none missing");
+      const int64_t e = open_edge[k];
+      L.open_cell[k] = (uint32_t)edge_cell_l[e];
+      L.open_geom[k] = make_float4(edge_nx[e], edge_ny[e], edge_len[e], 0.0f);
+    }
+  }
   uint32_t max_local = 1, max_slots = 1, max_ghost = 1;
   std::vector<BlockInfo>& binfo = L.binfo;
   binfo.assign(NB, BlockInfo{});
@@ -1682,6 +1896,8 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
   gxv.clear();
   L.stream_edge.assign((size_t)E + hp[NB], 0u);
   std::vector<uint32_t> ghosts;
+  L.open_bptr.assign(NB + 1, 0u);
+  L.open_jk.clear();
   uint32_t e0 = 0;
   for (uint32_t b = 0; b < NB; ++b) {
     const uint32_t cb = b * kBlockCells, ce = std::min(cb + kBlockCells, N);
@@ -1701,6 +1917,7 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
       const uint32_t e = (j < n_own) ? eo[b] + j : hx[hp[b] + (j - n_own)];
       const uint32_t ql = local(edge_cell_l[e]);
       const uint32_t qr = (edge_cell_r[e] < 0) ? kSideWall : local(edge_cell_r[e]);
+      if (edge_cell_r[e] < -1) L.open_jk.push_back(make_uint2(j, (uint32_t)(-2 - (int64_t)edge_cell_r[e])));
       uint32_t bx, by, bl;
       memcpy(&bx, &edge_nx[e], 4);
       memcpy(&by, &edge_ny[e], 4);
@@ -1711,6 +1928,7 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
     }
     binfo[b] = BlockInfo{e0, n_loc, n_own, (uint32_t)gxv.size(), (uint32_t)ghosts.size(), cell_edge_ptr[cb],
                          cell_edge_ptr[ce] - cell_edge_ptr[cb], ce - cb};
+    L.open_bptr[b + 1] = (uint32_t)L.open_jk.size();
     gxv.insert(gxv.end(), ghosts.begin(), ghosts.end());
     e0 += n_loc;
     max_local = std::max(max_local, n_loc);
@@ -1821,6 +2039,9 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   }
   cell0[n_scen] = (uint32_t)n_tot;
   block0[n_scen] = (uint32_t)nb_tot;
+  uint32_t n_open = 0;
+  for (uint32_t s = 0; s < n_scen; ++s) n_open += (uint32_t)lay[scen_mesh[s]].open_cell.size();
+  if (n_open && n_scen != 1) return fail(c, INU_EINVAL, "open boundaries take a context of one scenario");
   // Cell ids are int32 and every device array stays below 4 GiB.
   if (nb_tot * kBlockCells > 0x7fffffffull || l_tot * sizeof(uint4) >= (1ull << 32) || s_tot >= (1ull << 30) ||
       e_tot > 0xffffffffull)
@@ -1874,7 +2095,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
     for (uint32_t e : L.stream_edge) stream_edge.push_back(ed_off + e);
     for (uint32_t e = 0; e < L.E; ++e) {
       e_cl.push_back((int32_t)(c0 + (uint32_t)m.ecl[e]));
-      e_cr.push_back(m.ecr[e] < 0 ? -1 : (int32_t)(c0 + (uint32_t)m.ecr[e]));
+      e_cr.push_back(m.ecr[e] < 0 ? m.ecr[e] : (int32_t)(c0 + (uint32_t)m.ecr[e]));
     }
     ed_off += L.E;
     lrec.insert(lrec.end(), L.lrec.begin(), L.lrec.end());
@@ -1923,6 +2144,31 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1); rc |= dalloc(c, &src_pair, 1);
   rc |= dalloc(c, &c->ctl, 1);
   rc |= dalloc(c, &c->scen, n_scen);
+  c->n_open = n_open;
+  c->n_stretch = n_open ? 1u : 0u;
+  c->open_cell_h.clear();
+  c->open_cell = c->stretch_ptr = nullptr;
+  c->tally = nullptr;
+  c->open_geom = nullptr;
+  c->open_h = OpenDev{};
+  a.open = nullptr;
+  a.n_open = n_open;
+  if (n_open) {
+    OpenDev& o = c->open_h;
+    uint32_t *okind, *obptr;
+    float *oval, *oflux;
+    float4* owords;
+    uint2* ojk;
+    OpenDev* odev;
+    rc |= dalloc(c, &okind, n_open); rc |= dalloc(c, &oval, n_open); rc |= dalloc(c, &oflux, 2 * (size_t)n_open);
+    rc |= dalloc(c, &owords, 4 * (size_t)n_open); rc |= dalloc(c, &obptr, NB + 1); rc |= dalloc(c, &ojk, n_open);
+    rc |= dalloc(c, &odev, 1);
+    rc |= dalloc(c, &c->open_cell, n_open); rc |= dalloc(c, &c->stretch_ptr, n_open + 1);
+    rc |= dalloc(c, &c->tally, 4 * (size_t)n_open); rc |= dalloc(c, &c->open_geom, n_open);
+    o = OpenDev{okind, oval, owords, oflux, obptr, ojk};
+    a.open = odev;
+    for (uint32_t cl : lay[scen_mesh[0]].open_cell) c->open_cell_h.push_back(cl);  // one scenario: no offset
+  }
   if (rc) return rc;
   a.cptr = dcptr; a.lidx = dlid; a.snb = dsn; a.slen = dsl;
   a.lrec = dlrec; a.ldz = dldz;
@@ -1942,6 +2188,23 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   HIPCHK(c, hipMemsetAsync(a.forced, 0, NB * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.has_src, 0, NB * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(c->ctl, 0, sizeof(Ctl), c->stream));
   HIPCHK(c, hipMemsetAsync(c->scen, 0, sizeof(Scen) * n_scen, c->stream));
+  if (n_open) {
+    // Every open edge starts closed (a wall) and the mesh is one stretch.
+    const OpenDev& o = c->open_h;
+    if ((rc = upload(c, const_cast<OpenDev*>(a.open), &o, 1))) return rc;
+    HIPCHK(c, hipMemsetAsync(const_cast<uint32_t*>(o.kind), 0, n_open * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(const_cast<float*>(o.value), 0, n_open * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(o.flux, 0, 2 * (size_t)n_open * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->tally, 0, 4 * (size_t)n_open * 4, c->stream));
+    const uint32_t ptr2[2] = {0u, n_open};
+    if ((rc = upload(c, c->stretch_ptr, ptr2, 2))) return rc;
+    if ((rc = upload(c, c->open_cell, c->open_cell_h.data(), n_open))) return rc;
+    if ((rc = upload(c, c->open_geom, lay[scen_mesh[0]].open_geom.data(), n_open))) return rc;
+    if ((rc = upload(c, const_cast<uint32_t*>(o.bptr), lay[scen_mesh[0]].open_bptr.data(), NB + 1))) return rc;
+    if ((rc = upload(c, const_cast<uint2*>(o.jk), lay[scen_mesh[0]].open_jk.data(), n_open))) return rc;
+    HIPCHK(c, hipMemsetAsync(o.words, 0, 4 * (size_t)n_open * sizeof(float4), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   std::vector<float> nm(NP, 0.03f);
   if ((rc = upload(c, a.nman, nm.data(), NP))) return rc;
   if ((rc = upload(c, a.area, area.data(), NP))) return rc;
@@ -2102,6 +2365,12 @@ int inu_set_state(inu_ctx* c, const float* h, const float* hu, const float* hv, 
     if ((rc = upload_cells(c, c->a.hlo, h_lo))) return rc;
   } else {
     HIPCHK(c, hipMemsetAsync(c->a.hlo, 0, (size_t)c->NB * kBlockCells * 4, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  // A new initial state starts the open-boundary volume tally again
+  // (reference gpu_resident_batch.py:440-445).
+  if (c->n_open) {
+    HIPCHK(c, hipMemsetAsync(c->tally, 0, 4 * (size_t)c->n_stretch * 4, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   return 0;
@@ -2403,6 +2672,34 @@ int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out7, float g, 
   return 0;
 }
 
+int inu_probe_open(inu_ctx* c, uint32_t n, const float* cases6, const uint32_t* kinds, const float* values,
+                   float* out4, float g, float dry_tol) {
+  if (!c) return INU_EINVAL;
+  if (!cases6 || !kinds || !values || !out4) return fail(c, INU_EINVAL, "null array");
+  if (n == 0) return 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t in_b = (size_t)n * 24, k_b = (size_t)n * 4, out_b = (size_t)n * 16;
+  char* buf = nullptr;
+  HIPCHK(c, hipMalloc(&buf, in_b + 2 * k_b + out_b));
+  float* din = reinterpret_cast<float*>(buf);
+  uint32_t* dk = reinterpret_cast<uint32_t*>(buf + in_b);
+  float* dv = reinterpret_cast<float*>(buf + in_b + k_b);
+  float* dout = reinterpret_cast<float*>(buf + in_b + 2 * k_b);
+  hipError_t e = hipMemcpyAsync(din, cases6, in_b, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dk, kinds, k_b, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dv, values, k_b, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_probe_open, dim3((n + 255) / 256), dim3(256), 0, c->stream, din, dk, dv, dout, n,
+                       make_phys(g, dry_tol, 0.8f));
+    e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out4, dout, out_b, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("probe_open: ") + hipGetErrorString(e));
+  return 0;
+}
+
 int inu_stream_copy(inu_ctx* c, uint64_t n_bytes, uint32_t reps, float* ms_out) {
   if (!c) return INU_EINVAL;
   if (!ms_out || reps == 0u || n_bytes == 0u || n_bytes % 16u != 0u)
@@ -2541,6 +2838,62 @@ int inu_set_source_rates(inu_ctx* c, uint32_t n, const uint32_t* cell, const flo
   return 0;
 }
 
+int inu_set_open_stretches(inu_ctx* c, uint32_t n_stretches, const uint32_t* stretch_ptr) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (c->n_open == 0) return fail(c, INU_EINVAL, "the mesh has no open edges");
+  if (n_stretches == 0 || !stretch_ptr) return fail(c, INU_EINVAL, "inu_set_open_stretches: no stretches");
+  if (n_stretches > c->n_open) return fail(c, INU_EINVAL, "more stretches than open edges");
+  if (stretch_ptr[0] != 0 || stretch_ptr[n_stretches] != c->n_open)
+    return fail(c, INU_EINVAL, "stretch_ptr must run from 0 to the open edge count");
+  for (uint32_t s = 0; s < n_stretches; ++s)
+    if (stretch_ptr[s + 1] <= stretch_ptr[s]) return fail(c, INU_EINVAL, "every stretch needs an open edge");
+  if ((rc = upload(c, c->stretch_ptr, stretch_ptr, n_stretches + 1))) return rc;
+  c->n_stretch = n_stretches;
+  HIPCHK(c, hipMemsetAsync(c->tally, 0, 4 * (size_t)n_stretches * 4, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  return 0;
+}
+
+int inu_set_open_conditions(inu_ctx* c, uint32_t n, const uint32_t* kinds, const float* values) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (n != c->n_open) return fail(c, INU_EINVAL, "one condition per open edge");
+  if (n == 0) return 0;
+  if (!kinds || !values) return fail(c, INU_EINVAL, "inu_set_open_conditions: null array");
+  for (uint32_t k = 0; k < n; ++k) {
+    if (kinds[k] > INU_OPEN_INFLOW) return fail(c, INU_EINVAL, "unknown open boundary condition");
+    if (!std::isfinite(values[k])) return fail(c, INU_EINVAL, "open boundary values must be finite");
+  }
+  if ((rc = upload(c, const_cast<uint32_t*>(c->open_h.kind), kinds, n))) return rc;
+  if ((rc = upload(c, const_cast<float*>(c->open_h.value), values, n))) return rc;
+  // A block with an edge that can bring water in from outside counts as wet
+  // while these conditions hold (reference gpu_resident_batch.py:447-458).
+  std::vector<uint32_t> forced(c->NB, 0u);
+  for (uint32_t k = 0; k < n; ++k)
+    if ((kinds[k] == INU_OPEN_INFLOW || kinds[k] == INU_OPEN_STAGE) && values[k] > 0.0f)
+      forced[c->open_cell_h[k] / kBlockCells] = 1u;
+  return upload(c, c->a.forced, forced.data(), forced.size());
+}
+
+int inu_open_volumes(inu_ctx* c, uint32_t n_stretches, double* out2) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (n_stretches != c->n_stretch)
+    return fail(c, INU_EINVAL, "inu_open_volumes: n_stretches is not the stretch count");
+  if (n_stretches == 0) return 0;
+  if (!out2) return fail(c, INU_EINVAL, "null output");
+  std::vector<float> t(4 * (size_t)c->n_stretch);
+  if ((rc = copy_sync(c, t.data(), c->tally, t.size() * 4, hipMemcpyDeviceToHost))) return rc;
+  for (uint32_t s = 0; s < c->n_stretch; ++s) {
+    out2[2 * s] = (double)t[4 * s] + (double)t[4 * s + 1];
+    out2[2 * s + 1] = (double)t[4 * s + 2] + (double)t[4 * s + 3];
+  }
+  return 0;
+}
+
 int inu_set_bed(inu_ctx* c, const float* z) {
   int rc = check_ready(c);
   if (rc) return rc;
@@ -2554,7 +2907,7 @@ int inu_set_bed(inu_ctx* c, const float* z) {
   for (size_t q = 0; q < c->stream_edge.size(); ++q) {
     const uint32_t e = c->stream_edge[q];
     const int32_t r = c->e_cr[e];
-    if (r < 0) {
+    if (r < 0) {  // a wall or an open edge: no bed step
       c->h_ldz[q] = 0.0f;
       continue;
     }
@@ -2742,6 +3095,7 @@ int inu_adj_begin(inu_ctx* c, uint32_t max_steps, uint32_t seg, uint32_t n_gauge
   int rc = check_ready(c);
   if (rc) return rc;
   if (c->n_scen != 1) return fail(c, INU_EINVAL, "the adjoint records one scenario");
+  if (c->n_open) return fail(c, INU_EINVAL, "the adjoint does not differentiate open boundaries");
   if (max_steps == 0 || seg == 0 || every == 0 || n_gauges == 0 || !gauges)
     return fail(c, INU_EINVAL, "inu_adj_begin: steps, segment, sampling interval and gauges must be given");
   if (max_steps > kKeyLenMask) return fail(c, INU_EINVAL, "inu_adj_begin: too many steps");

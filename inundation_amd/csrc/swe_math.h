@@ -148,13 +148,15 @@ __device__ __forceinline__ void wall_ghost(float huL, float hvL, float nx, float
 // The per-edge words of one live edge: mass flux, left momentum (x, y), right
 // momentum (x, y), all times the edge length, and the CFL term Smax * len
 // (reference resident.py:165-184 with flux.py:98-169, 271-272).
+// fh, if given, receives the HLLC mass flux before a wall zeroes it.
 __device__ __forceinline__ void edge_words(const Phys& p, bool wall, float hL, float huL, float hvL,
                                            float hR, float huR, float hvR, float dzL, float dzR,
-                                           float nx, float ny, float len, float w[6]) {
+                                           float nx, float ny, float len, float w[6], float* fh = nullptr) {
   float hLs, hqnL, hqtL, hRs, hqnR, hqtR;
   face_states(p, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx, ny, hLs, hqnL, hqtL, hRs, hqnR, hqtR);
   float Fh, Fhn, Fht, Smax;
   hllc(p, hLs, hqnL, hqtL, hRs, hqnR, hqtR, Fh, Fhn, Fht, Smax);
+  if (fh) *fh = Fh;
   if (wall) Fh = 0.0f;  // a wall passes no mass
   const float Fhu = Fhn * nx - Fht * ny;
   const float Fhv = Fhn * ny + Fht * nx;
@@ -214,6 +216,85 @@ __device__ __forceinline__ float cbrt_ieee(float x) {
 
 // h^(4/3) as h * cbrt(h): the Manning friction depth factor.
 __device__ __forceinline__ float pow43(float h) { return h * cbrt_ieee(h); }
+
+// ---------------------------------------------------------------------------
+// Open boundaries (reference shaders/flux.py:194-282): the state beyond an open
+// edge, from the cell inside it. Both arithmetic orders use these, in the
+// reference's literal order; the two fractional powers are formed from the
+// IEEE-only cube root (h^(2/3) = cbrt(h)^2, q^(1/3) = cbrt(q)).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kOpenWall = 0u, kOpenFree = 1u, kOpenNormal = 2u, kOpenStage = 3u, kOpenInflow = 4u;
+constexpr int kOpenNewtonSteps = 4;
+
+// h^(2/3), 0 below the smallest normal float.
+__device__ __forceinline__ float pow23(float h) {
+  if (!(h >= 1.17549435e-38f)) return 0.0f;
+  const float c = cbrt_ieee(h);
+  return c * c;
+}
+
+// Depth beyond an inflow edge of discharge q (> 0) per unit width: the root of
+// -q/hG + 2 sqrt(g hG) = riemann, the outgoing characteristic, by Newton in
+// s = sqrt(hG) from below, each step at least halving s.
+__device__ __forceinline__ float inflow_depth(const Phys& p, float q, float riemann) {
+  const float root_g = p.sqrt_g;
+  float s = gmax(cbrt_ieee(q / (2.0f * root_g)), riemann / (2.0f * root_g));
+  for (int k = 0; k < kOpenNewtonSteps; ++k) {
+    const float f = 2.0f * root_g * s - q / (s * s) - riemann;
+    const float df = 2.0f * root_g + 2.0f * q / (s * s * s);
+    s = gmax(s - f / df, 0.5f * s);
+  }
+  return s * s;
+}
+
+// Ghost state beyond an open edge of condition `kind`. `value` is sqrt(S)/n
+// for normal depth, the depth over the cell's bed for a stage, the discharge
+// per unit width for an inflow; wall and free outflow take none.
+__device__ __forceinline__ void ghost_state(const Phys& p, uint32_t kind, float value, float hL, float huL, float hvL,
+                                            float nx, float ny, float& hG, float& huG, float& hvG) {
+  const float uL = safe_vel(p, hL, huL);
+  const float vL = safe_vel(p, hL, hvL);
+  const float unL = uL * nx + vL * ny;
+  float ut = -uL * ny + vL * nx;
+  const float cL = sqrtf(p.g * gmax(hL, 0.0f));
+  float un = -unL;
+  hG = hL;
+  if (kind == kOpenFree) {
+    un = fabsf(unL);
+  } else if (kind == kOpenNormal) {
+    un = value * pow23(gmax(hL, 0.0f));
+  } else if (kind == kOpenStage) {
+    // A level cannot reach a boundary against supercritical outflow.
+    if (hL < p.dry || unL < cL) {
+      hG = value;
+      un = unL + 2.0f * (cL - sqrtf(p.g * value));
+    } else {
+      un = unL;
+    }
+  } else if (kind == kOpenInflow && value > 0.0f) {
+    hG = inflow_depth(p, value, unL + 2.0f * cL);
+    un = -value / hG;
+    ut = 0.0f;
+  }
+  huG = hG * (un * nx - ut * ny);
+  hvG = hG * (un * ny + ut * nx);
+}
+
+// An inflow keeps its edge live while dry, or its volume would be lost.
+__device__ __forceinline__ bool open_forced(uint32_t kind, float value) { return kind == kOpenInflow && value > 0.0f; }
+
+// Free outflow and normal depth never draw water in.
+__device__ __forceinline__ bool open_outflow_only(uint32_t kind) { return kind == kOpenFree || kind == kOpenNormal; }
+
+// Mass flux through an open edge whose HLLC mass flux is fh: outflow
+// conditions pass none inward, a stage passes fh, an inflow exactly its
+// discharge, a closed edge nothing.
+__device__ __forceinline__ float boundary_mass(uint32_t kind, float value, float fh) {
+  if (kind == kOpenFree || kind == kOpenNormal) return gmax(fh, 0.0f);
+  if (kind == kOpenStage) return fh;
+  if (kind == kOpenInflow) return -value;
+  return 0.0f;
+}
 
 // ---------------------------------------------------------------------------
 // Fast arithmetic: the same scheme with shared reciprocals and explicit FMAs.
@@ -362,7 +443,8 @@ struct Side {
 // depth as face depth, so its vel_factor and square root are the cell's; at
 // most one side steps, and only its face depth needs new ones.
 __device__ __forceinline__ void edge_words(const Phys& p, bool wall, const Side& L, const Side& R, float dzL,
-                                           float dzR, float nx, float ny, float len, float w[6]) {
+                                           float dzR, float nx, float ny, float len, float w[6],
+                                           float* fh = nullptr) {
   Face f;
   f.hLs = (dzL == 0.0f) ? L.h : gmax(0.0f, R.h + (((L.h - R.h) - dzL) + (L.lo - R.lo)));
   f.hRs = (dzR == 0.0f) ? R.h : gmax(0.0f, L.h + (((R.h - L.h) - dzR) + (R.lo - L.lo)));
@@ -385,6 +467,7 @@ __device__ __forceinline__ void edge_words(const Phys& p, bool wall, const Side&
   const float sqR = stepR ? sqs : R.sq;
   float Fh, Fhn, Fht, Smax;
   hllc_vel(p, f.hLs, sqL, f.qnL * rL, f.qtL * rL, f.hRs, sqR, f.qnR * rR, f.qtR * rR, Fh, Fhn, Fht, Smax);
+  if (fh) *fh = Fh;
   if (wall) Fh = 0.0f;
   // Normal momentum flux plus the hydrostatic correction of each side,
   // grouped as (Fhn - P(face depth)) + P(cell depth): still water gives

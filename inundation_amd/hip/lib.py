@@ -29,6 +29,7 @@ FLAG_ACTIVE_SET = 4
 FLAG_TRACK_CLAMP = 8
 FLAG_EXACT_ORDER = 16
 
+OPEN_WALL, OPEN_FREE, OPEN_NORMAL, OPEN_STAGE, OPEN_INFLOW = 0, 1, 2, 3, 4
 FIELD_H = 0
 FIELD_HU = 1
 FIELD_HV = 2
@@ -81,6 +82,10 @@ EXPORTS = (
     "inu_adj_dt_log",
     "inu_adj_stats",
     "inu_adj_set_thin",
+    "inu_set_open_stretches",
+    "inu_set_open_conditions",
+    "inu_open_volumes",
+    "inu_probe_open",
     "inu_adj_end",
     "inu_probe_vjp",
 )
@@ -180,6 +185,10 @@ def load() -> C.CDLL:
         "inu_adj_dt_log": (_I, [_P, _P]),
         "inu_adj_stats": (_I, [_P, _P]),
         "inu_adj_set_thin": (_I, [_P, _F]),
+        "inu_set_open_stretches": (_I, [_P, _U32, _P]),
+        "inu_set_open_conditions": (_I, [_P, _U32, _P, _P]),
+        "inu_open_volumes": (_I, [_P, _U32, _P]),
+        "inu_probe_open": (_I, [_P, _U32, _P, _P, _P, _P, _F, _F]),
         "inu_adj_end": (_I, [_P]),
         "inu_probe_vjp": (_I, [_P, _U32, _P, _P, _F, _F]),
     }
@@ -451,10 +460,42 @@ class Context:
     def adj_end(self) -> None:
         self.call("inu_adj_end")
 
+    def set_open_stretches(self, stretch_ptr: NDArray[np.uint32]) -> None:
+        """Open edges [ptr[s], ptr[s + 1]) form stretch s."""
+        p = np.ascontiguousarray(stretch_ptr, dtype=np.uint32)
+        self.call("inu_set_open_stretches", p.size - 1, ptr(p))
+
+    def set_open_conditions(self, kinds: NDArray[np.uint32], values: NDArray[np.float32]) -> None:
+        k = np.ascontiguousarray(kinds, dtype=np.uint32)
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        if k.size != v.size:
+            raise ValueError("one value per open edge")
+        self.call("inu_set_open_conditions", k.size, ptr(k) if k.size else None, ptr(v) if v.size else None)
+
+    def open_volumes(self, n_stretches: int) -> NDArray[np.float64]:
+        """(stretches, 2) volume [m^3] out of and into the mesh through each stretch."""
+        out = np.zeros((int(n_stretches), 2), dtype=np.float64)
+        self.call("inu_open_volumes", out.shape[0], ptr(out) if out.size else None)
+        return out
+
     def probe_vjp(self, rows: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:
         r = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, 17)
         out = np.empty((r.shape[0], 8), dtype=np.float32)
         self.call("inu_probe_vjp", r.shape[0], ptr(r), ptr(out), g, dry_tol)
+        return out
+
+    def probe_open(
+        self, cases: NDArray[np.float32], kinds: NDArray[np.uint32], values: NDArray[np.float32], g: float,
+        dry_tol: float,
+    ) -> NDArray[np.float32]:
+        """Cases (h, hu, hv, nx, ny, fh) -> ghost (h, hu, hv) and boundary mass flux."""
+        c = np.ascontiguousarray(cases, dtype=np.float32).reshape(-1, 6)
+        k = np.ascontiguousarray(kinds, dtype=np.uint32).ravel()
+        v = np.ascontiguousarray(values, dtype=np.float32).ravel()
+        if not (k.size == v.size == c.shape[0]):
+            raise ValueError("one kind and one value per case")
+        out = np.empty((c.shape[0], 4), dtype=np.float32)
+        self.call("inu_probe_open", c.shape[0], ptr(c), ptr(k), ptr(v), ptr(out), g, dry_tol)
         return out
 
     def probe(self, pairs: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:

@@ -66,6 +66,35 @@ def device_mesh_arrays(geom: MeshGeometry) -> dict[str, Any]:
     }
 
 
+def mark_open_edges(
+    geom: MeshGeometry, m: dict[str, Any], open_edges: Sequence[NDArray[np.int64]]
+) -> NDArray[np.uint32]:
+    """Open edge k as ``cell_r == -2 - k`` in ``m`` (the device edge order), stretches in order.
+
+    Reference ``vulkan/solver.py:165, 238-243`` and ``gpu_resident_batch.py:176-181``:
+    ``open_edges`` holds each stretch's edges as indices into ``geom``; every
+    other boundary edge stays a wall. Returns the stretch pointer (open edges
+    ``[ptr[s], ptr[s + 1])`` form stretch s).
+    """
+    stretches = [np.asarray(e, dtype=np.int64).ravel() for e in open_edges]
+    edges = np.concatenate([np.empty(0, dtype=np.int64), *stretches])
+    if edges.size:
+        if edges.min() < 0 or edges.max() >= geom.E:
+            raise ValueError("open edge index out of range")
+        if np.unique(edges).size != edges.size:
+            raise ValueError("an edge is in more than one open stretch")
+        if np.any(np.asarray(geom.edge_cellR)[edges] != -1):
+            raise ValueError("open edges must be boundary edges")
+        if any(st.size == 0 for st in stretches):
+            raise ValueError("every open stretch needs an edge")
+        position = np.empty(geom.E, dtype=np.int64)
+        position[m["order"]] = np.arange(geom.E)
+        cell_r = m["cell_r"].copy()
+        cell_r[position[edges]] = (-2 - np.arange(edges.size)).astype(np.int32)
+        m["cell_r"] = cell_r
+    return np.r_[0, np.cumsum([st.size for st in stretches], dtype=np.int64)].astype(np.uint32)
+
+
 class SWESolverHipResident:
     """Record once, replay many steps; dt and time stay on the GPU."""
 
@@ -97,8 +126,6 @@ class SWESolverHipResident:
             raise ValueError(
                 f"hip_resident needs workgroup_size={ACTIVE_BLOCK_CELLS}, got {workgroup_size}"
             )
-        if any(np.asarray(e).size for e in open_edges):
-            raise ValueError("hip_resident does not implement open boundaries yet")
         self.geom = geom
         self.N = geom.N
         self.E = geom.E
@@ -114,9 +141,17 @@ class SWESolverHipResident:
         self._hv0 = np.asarray(hv0, dtype=np.float32).ravel().copy()
         self._ctx = L.Context(device)
         m = device_mesh_arrays(geom)
+        stretch_ptr = mark_open_edges(geom, m, open_edges)
+        self._open_stretches = [np.asarray(e, dtype=np.int64).ravel() for e in open_edges]
+        self.n_open = int(stretch_ptr[-1])
+        self._open_cells = np.asarray(geom.edge_cellL)[
+            np.concatenate([np.empty(0, dtype=np.int64), *self._open_stretches])
+        ]
         self._ctx.set_mesh(
             m["cell_l"], m["cell_r"], m["nx"], m["ny"], m["len"], m["dz2"], m["area"], m["cptr"], m["cidx"]
         )
+        if self.n_open:
+            self._ctx.set_open_stretches(stretch_ptr)
         self._ctx.set_manning(np.asarray(n0, dtype=np.float32))
         self._apply_params()
         self._ctx.set_state(self._h0, self._hu0, self._hv0, None)
@@ -153,8 +188,23 @@ class SWESolverHipResident:
         self.reset(self._h0, self._hu0, self._hv0)
 
     def set_open_conditions(self, kinds: NDArray[np.uint32], values: NDArray[np.float32]) -> None:
-        if np.asarray(kinds).size or np.asarray(values).size:
-            raise ValueError("expected 0 open edges")
+        """Every open edge's condition (``L.OPEN_*``) and value, in stretch order, before a run.
+
+        Reference ``vulkan/solver.py:427-437`` and ``gpu_resident_batch.py:447-458``:
+        a block with an edge that can bring water in stays active while these hold.
+        """
+        kinds = np.asarray(kinds, dtype=np.uint32).ravel()
+        values = np.asarray(values, dtype=np.float32).ravel()
+        if kinds.size != self.n_open or values.size != self.n_open:
+            raise ValueError(
+                f"expected {self.n_open} open edges, got {kinds.size} kinds and {values.size} values"
+            )
+        if self.n_open:
+            self._ctx.set_open_conditions(kinds, values)
+
+    def download_open_volumes(self) -> NDArray[np.float64]:
+        """``(stretches, 2)`` volume [m^3] out of and into the mesh since the start."""
+        return self._ctx.open_volumes(len(self._open_stretches))
 
     def _check_error_free_arithmetic(self) -> None:
         """Refuse to run if two-sum lost its error term (reference solver.py:449-467)."""
@@ -257,7 +307,8 @@ class SWESolverHipResident:
         self._apply_params()
         if not resume:
             self._reset_to_initial()
-        if self.compensated:
+            self.open_volume_snapshots = []
+        if self.compensated or self.n_open:
             self._check_error_free_arithmetic()
         self._set_sources(source_rate)
 
@@ -270,6 +321,8 @@ class SWESolverHipResident:
             if capture_momentum_snapshots:
                 hu_snaps.append(self.download_hu())
                 hv_snaps.append(self.download_hv())
+            if self.n_open:
+                self.open_volume_snapshots.append(self.download_open_volumes())
 
         capture()
         ctrl = L.InuCtrl()

@@ -46,6 +46,15 @@ DT_SENTINEL = 1e30
 # Backends whose run() returns hu/hv snapshots (reference tuning.py:57-61 plus
 # this backend).
 MOMENTUM_SNAPSHOT_BACKENDS = frozenset({"hip_resident"})
+# Backends that take open boundaries (reference tuning.py:61).
+OPEN_BOUNDARY_BACKENDS = frozenset({"hip_resident"})
+
+# Open boundary conditions, the word per open edge (reference tuning.py:64-68).
+OPEN_WALL = 0
+OPEN_FREE = 1
+OPEN_NORMAL = 2
+OPEN_STAGE = 3
+OPEN_INFLOW = 4
 
 
 def uint_as_float_bits(n: int) -> float:

@@ -8,10 +8,12 @@ loading (``:450-596``) and ``SWEWorkflow.prepare()/run()`` (``:658-969``).
 
 ``solver_impl`` accepts the reference's names; ``"hip_resident"`` (the
 default here) is the MI355X backend. The Vulkan backends need Kompute, which
-this package does not ship, and ``prepare()`` says so. Open boundaries and
-mesh-file loading are not implemented by this package: ``prepare()`` takes
-the geometry (``MeshGeometry`` + permutation) or a geometry-cache artifact
-written by ``inundation_amd.mesh.cache``.
+this package does not ship, and ``prepare()`` says so. Open boundaries follow
+``:74-121, 258-289, 361-435``: named stretches picked by a polyline, a
+condition per stretch and phase, and the volume through each stretch at every
+snapshot. Mesh-file loading is not implemented by this package:
+``prepare()`` takes the geometry (``MeshGeometry`` + permutation) or a
+geometry-cache artifact written by ``inundation_amd.mesh.cache``.
 """
 
 from __future__ import annotations
@@ -25,7 +27,18 @@ from typing import TYPE_CHECKING, Any, Literal, cast
 
 import numpy as np
 
-from .tuning import DRY_TOL_DEFAULT, GRAVITY_G, MOMENTUM_SNAPSHOT_BACKENDS
+from .mesh.boundaries import conveyance_split, select_boundary_edges
+from .tuning import (
+    DRY_TOL_DEFAULT,
+    GRAVITY_G,
+    MOMENTUM_SNAPSHOT_BACKENDS,
+    OPEN_BOUNDARY_BACKENDS,
+    OPEN_FREE,
+    OPEN_INFLOW,
+    OPEN_NORMAL,
+    OPEN_STAGE,
+    OPEN_WALL,
+)
 
 if TYPE_CHECKING:
     from collections.abc import Mapping, Sequence
@@ -48,12 +61,62 @@ class PointSource:
 
 
 @dataclass(frozen=True)
+class OpenBoundary:
+    """A stretch of mesh boundary that phases can open (reference ``workflow.py:74-86``).
+
+    ``line`` is a polyline in the mesh file's coordinates; the boundary edges
+    whose midpoints lie within ``tolerance_m`` of it form the stretch, which
+    phases and results call ``name``.
+    """
+
+    name: str
+    line: Sequence[tuple[float, float]]
+    tolerance_m: float
+
+
+@dataclass(frozen=True)
+class Wall:
+    """The stretch stays closed for this phase."""
+
+
+@dataclass(frozen=True)
+class FreeOutflow:
+    """Zero-gradient outflow: exact for supercritical flow, partly reflecting for subcritical."""
+
+
+@dataclass(frozen=True)
+class NormalDepth:
+    """Outflow at the Manning velocity of the local depth for an energy slope."""
+
+    energy_slope: float
+
+
+@dataclass(frozen=True)
+class Stage:
+    """A water level beyond the boundary [m, the bed's datum]."""
+
+    elevation_m: float
+
+
+@dataclass(frozen=True)
+class Inflow:
+    """Discharge into the mesh [m^3/s], spread over the stretch by Manning conveyance."""
+
+    discharge_m3s: float
+    energy_slope: float
+
+
+BoundaryCondition = Wall | FreeOutflow | NormalDepth | Stage | Inflow
+
+
+@dataclass(frozen=True)
 class SimulationPhase:
     """One phase in a multi-phase hydrograph schedule."""
 
     duration_s: float
     sources: Sequence[PointSource]
-    boundaries: Mapping[str, Any] = field(default_factory=dict)
+    # The condition of every stretch of WorkflowConfig.open_boundaries, by name.
+    boundaries: Mapping[str, BoundaryCondition] = field(default_factory=dict)
 
 
 @dataclass(frozen=True)
@@ -80,7 +143,7 @@ class WorkflowConfig:
     geometry_cache_dir: str | os.PathLike[str] | None = None
     geometry_cache_source: str | os.PathLike[str] | None = None
     capture_momentum_snapshots: bool = False
-    open_boundaries: Sequence[Any] = ()
+    open_boundaries: Sequence[OpenBoundary] = ()
     steps_per_graph: int | None = None
     deterministic: bool | None = None
     compensated: bool | None = None
@@ -111,7 +174,16 @@ class WorkflowConfig:
                 f"{self.solver_impl!r}; use one of {sorted(MOMENTUM_SNAPSHOT_BACKENDS)}"
             )
         if self.open_boundaries:
-            raise ValueError("open_boundaries are not implemented by the hip_resident backend")
+            if self.solver_impl not in OPEN_BOUNDARY_BACKENDS:
+                raise ValueError(
+                    f"open_boundaries is not supported by solver_impl {self.solver_impl!r}; "
+                    f"use one of {sorted(OPEN_BOUNDARY_BACKENDS)}"
+                )
+            if self.deterministic is False:
+                raise ValueError("open_boundaries need deterministic accumulation")
+            names = [b.name for b in self.open_boundaries]
+            if len(set(names)) != len(names):
+                raise ValueError(f"open boundary names repeat: {sorted(names)}")
 
 
 @dataclass
@@ -130,7 +202,115 @@ class WorkflowResult:
     hu_snapshots: list[NDArray[np.float32]] | None = None
     hv_snapshots: list[NDArray[np.float32]] | None = None
     clamp_mass_m3: float | None = None
+    # Per open stretch, (snapshots, 2) cumulative volume [m^3] out of and into
+    # the mesh through its edges, aligned with snap_times.
     boundary_volumes_m3: dict[str, NDArray[np.float64]] = field(default_factory=dict)
+
+
+def inflow_phases(
+    upstream: WorkflowResult,
+    outlet: str,
+    into: str,
+    energy_slope: float,
+    other_boundaries: Mapping[str, BoundaryCondition] | None = None,
+    sources: Sequence[PointSource] = (),
+) -> list[SimulationPhase]:
+    """Phases that pass the volume leaving an upstream basin into the next one.
+
+    Reference ``workflow.py:258-289``: one phase per output window of the
+    upstream run, at the window's mean outflow through ``outlet``, entering
+    through ``into``, so each window's volume passes whole.
+    ``other_boundaries`` gives the downstream basin's other stretches.
+    """
+    t = np.asarray(upstream.snap_times, dtype=np.float64)
+    out = upstream.boundary_volumes_m3[outlet][:, 0]
+    # A phase's first snapshot repeats the previous phase's last one.
+    ends = np.flatnonzero(np.diff(t) > 0.0) + 1
+    starts = np.r_[0, ends[:-1]]
+    phases = []
+    for a, b in zip(starts.tolist(), ends.tolist(), strict=True):
+        span = float(t[b] - t[a])
+        cond = {**(other_boundaries or {}), into: Inflow(float((out[b] - out[a]) / span), energy_slope)}
+        phases.append(SimulationPhase(span, sources, cond))
+    return phases
+
+
+def open_stretch_edges(geom: MeshGeometry, boundaries: Sequence[OpenBoundary]) -> list[NDArray[np.int64]]:
+    """Each stretch's boundary edges (reference ``workflow.py:361-377``); empty or shared picks raise."""
+    stretches = [select_boundary_edges(geom, b.line, b.tolerance_m) for b in boundaries]
+    for b, edges in zip(boundaries, stretches, strict=True):
+        if edges.size == 0:
+            raise ValueError(
+                f"open boundary {b.name!r} selects no boundary edges within {b.tolerance_m} m of its line"
+            )
+    picked = np.concatenate(stretches) if stretches else np.empty(0, dtype=np.int64)
+    if np.unique(picked).size != picked.size:
+        raise ValueError("open boundaries select the same edge twice")
+    return stretches
+
+
+def open_conditions(
+    geom: MeshGeometry,
+    manning: NDArray[np.float32],
+    boundary: OpenBoundary,
+    condition: BoundaryCondition,
+    edges: NDArray[np.int64],
+) -> tuple[int, NDArray[np.float32]]:
+    """Condition code and per-edge value of one stretch (reference ``workflow.py:380-410``).
+
+    The value is what the ghost state reads: sqrt(S)/n for normal depth, the
+    depth of the stage over the edge cell's bed (at least 0), the inflow's
+    discharge per unit width from ``conveyance_split``.
+    """
+    cells = np.asarray(geom.edge_cellL)[edges]
+    fields = [float(getattr(condition, f)) for f in getattr(condition, "__dataclass_fields__", {})]
+    if not np.isfinite(fields).all():
+        raise ValueError(f"open boundary {boundary.name!r} has a non-finite value: {condition}")
+    if isinstance(condition, Wall):
+        return OPEN_WALL, np.zeros(edges.size, dtype=np.float32)
+    if isinstance(condition, FreeOutflow):
+        return OPEN_FREE, np.zeros(edges.size, dtype=np.float32)
+    if isinstance(condition, NormalDepth):
+        if condition.energy_slope <= 0.0:
+            raise ValueError(f"open boundary {boundary.name!r} needs a positive energy slope")
+        return OPEN_NORMAL, (np.sqrt(condition.energy_slope) / manning[cells]).astype(np.float32)
+    if isinstance(condition, Stage):
+        depth = condition.elevation_m - np.asarray(geom.zb)[cells].astype(np.float64)
+        return OPEN_STAGE, np.maximum(depth, 0.0).astype(np.float32)
+    if not isinstance(condition, Inflow):
+        raise TypeError(f"open boundary {boundary.name!r}: unknown condition {condition!r}")
+    if condition.energy_slope <= 0.0:
+        raise ValueError(f"open boundary {boundary.name!r} needs a positive energy slope")
+    if condition.discharge_m3s < 0.0:
+        raise ValueError(f"open boundary {boundary.name!r} has a negative inflow")
+    return OPEN_INFLOW, conveyance_split(
+        condition.discharge_m3s,
+        np.asarray(geom.edge_len)[edges],
+        np.asarray(geom.zb)[cells],
+        manning[cells],
+        condition.energy_slope,
+    )
+
+
+def phase_open_conditions(
+    geom: MeshGeometry,
+    manning: NDArray[np.float32],
+    boundaries: Sequence[OpenBoundary],
+    stretches: Sequence[NDArray[np.int64]],
+    phase: SimulationPhase,
+) -> tuple[NDArray[np.uint32], NDArray[np.float32]]:
+    """Per-edge condition codes and values of one phase, in stretch order (reference ``:413-435``)."""
+    names = {b.name for b in boundaries}
+    if set(phase.boundaries) != names:
+        raise ValueError(
+            f"phase boundaries {sorted(phase.boundaries)} must name every open boundary: {sorted(names)}"
+        )
+    kinds, values = [], []
+    for b, edges in zip(boundaries, stretches, strict=True):
+        kind, value = open_conditions(geom, manning, b, phase.boundaries[b.name], edges)
+        kinds.append(np.full(edges.size, kind, dtype=np.uint32))
+        values.append(value)
+    return np.concatenate(kinds), np.concatenate(values)
 
 
 def _point_in_polygon(point: NDArray[np.float64], polygon: NDArray[np.float64]) -> bool:
@@ -317,6 +497,8 @@ class SWEWorkflow:
         self.geom: MeshGeometry | None = None
         self.perm: NDArray[np.int32] | None = None
         self.solver: Any = None
+        self._open_stretches: list[NDArray[np.int64]] = []
+        self._manning: NDArray[np.float32] | None = None
 
     def prepare(self, geometry: MeshGeometry | None = None, perm: NDArray[np.int32] | None = None) -> None:
         """Geometry (given, or from ``geometry_cache_source``), initial state, solver."""
@@ -363,6 +545,8 @@ class SWEWorkflow:
                 n0 = n_loaded
         from .hip.solver import SWESolverHipResident
 
+        self._open_stretches = open_stretch_edges(geom, cfg.open_boundaries)
+        self._manning = np.asarray(n0, dtype=np.float32)
         solver = SWESolverHipResident(
             geom,
             None,
@@ -374,6 +558,7 @@ class SWEWorkflow:
             dry_tol=cfg.dry_tol,
             cfl=cfg.cfl,
             workgroup_size=cfg.workgroup_size,
+            open_edges=self._open_stretches,
             device=cfg.device,
         )
         if cfg.track_clamp_mass:
@@ -406,10 +591,14 @@ class SWEWorkflow:
         for k, phase in enumerate(phases):
             if phase.duration_s <= 0.0:
                 raise ValueError(f"Phase {k} has non-positive duration.")
-            if phase.boundaries:
-                raise ValueError("open boundaries are not implemented by the hip_resident backend")
             t_end = t + phase.duration_s
             src = build_phase_source_rate(self.geom, phase, valid)
+            if self._open_stretches:
+                self.solver.set_open_conditions(
+                    *phase_open_conditions(
+                        self.geom, self._manning, cfg.open_boundaries, self._open_stretches, phase
+                    )
+                )
             res = self.solver.run(
                 t_end=t_end,
                 output_interval_s=cfg.output_interval_s,
@@ -441,6 +630,10 @@ class SWEWorkflow:
         clamp = None
         if cfg.track_clamp_mass:
             clamp = float(np.asarray(self.solver.download_clamp_mass(), dtype=np.float64).sum())
+        volumes: dict[str, NDArray[np.float64]] = {}
+        if self._open_stretches:
+            stacked = np.stack(self.solver.open_volume_snapshots)
+            volumes = {b.name: stacked[:, k] for k, b in enumerate(cfg.open_boundaries)}
         if cfg.progress:
             sys.stdout.write(f"[workflow] {self.solver.steps_total} steps in {wall:.2f}s\n")
         return WorkflowResult(
@@ -456,11 +649,23 @@ class SWEWorkflow:
             hu_snapshots=hu_s,
             hv_snapshots=hv_s,
             clamp_mass_m3=clamp,
+            boundary_volumes_m3=volumes,
         )
 
 
 __all__ = [
+    "BoundaryCondition",
+    "FreeOutflow",
+    "Inflow",
+    "NormalDepth",
+    "OpenBoundary",
     "PointSource",
+    "Stage",
+    "Wall",
+    "inflow_phases",
+    "open_conditions",
+    "open_stretch_edges",
+    "phase_open_conditions",
     "SWEWorkflow",
     "SimulationPhase",
     "WorkflowConfig",

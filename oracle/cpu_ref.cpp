@@ -18,6 +18,13 @@
 //   cfl_edge_gather / cfl_gather / dt_reset   resident.py:528-610, cfl.py:130-166
 //   record order             inundation/vulkan/gpu_resident_batch.py:489-513, 567-599
 //   HLLC, reconstruction, film drag, two-sum   inundation/vulkan/shaders/flux.py:43-185, 301-320
+//   open boundaries: open_forced / inflow_depth / ghost_state /
+//                            boundary_mass_flux flux.py:206-281 and the
+//                            outflow-only redo OPEN_FLUX_MASS_GLSL :284-295;
+//                            the open ghost and forced-edge skip in the flux
+//                            kernel resident.py:126-145; the prologue ghost
+//                            :556-566; the stretch tally :613-658; forced
+//                            blocks gpu_resident_batch.py:447-458
 //
 // Two arithmetic orders, as in the backend (flag 16 selects the first): the
 // reference's literal order (one IEEE division or sqrt per occurrence, no
@@ -29,7 +36,8 @@
 // of the step has been applied (the reference reads it while other source
 // threads may still be writing it, so its result there depends on thread
 // timing). Friction's h^(4/3) is h * cbrt(h) with the IEEE-only cube root of
-// the backend, restated here.
+// the backend, restated here; so are the open-boundary powers: h^(2/3) is
+// cbrt(h)^2, and an inflow's depth is a fixed four-step Newton solve.
 //
 // Build: see oracle/build.py (g++ -O2 -fopenmp -ffp-contract=off
 // -fno-fast-math -fno-math-errno). FP32 arithmetic, no contraction.
@@ -215,6 +223,76 @@ inline float cbrt_ieee(float x) {
 }
 inline float pow43(float h) { return h * cbrt_ieee(h); }
 
+// ---- open boundaries (flux.py:188-272) ------------------------------------
+
+constexpr uint32_t kOpenWall = 0u, kOpenFree = 1u, kOpenNormal = 2u, kOpenStage = 3u, kOpenInflow = 4u;
+
+// h^(2/3) as cbrt(h)^2; 0 below the smallest normal float.
+inline float pow23(float h) {
+  if (!(h >= 1.17549435e-38f)) return 0.0f;
+  float c = cbrt_ieee(h);
+  return c * c;
+}
+
+// Depth beyond an inflow edge of discharge q > 0 per unit width: the root of
+// -q/hG + 2 sqrt(g hG) = riemann (the outgoing characteristic), Newton in
+// s = sqrt(hG) from the larger of the critical-flow and still-water starts,
+// four steps, each keeping at least half of s (flux.py:209-223), the cube root
+// being the IEEE-only one above.
+inline float inflow_depth(const Pc& pc, float q, float riemann) {
+  const float root_g = std::sqrt(pc.g);
+  float s = gmax(cbrt_ieee(q / (2.0f * root_g)), riemann / (2.0f * root_g));
+  for (int k = 0; k < 4; ++k) {
+    float f = 2.0f * root_g * s - q / (s * s) - riemann;
+    float df = 2.0f * root_g + 2.0f * q / (s * s * s);
+    s = gmax(s - f / df, 0.5f * s);
+  }
+  return s * s;
+}
+
+// flux.py:224-256: the state beyond an open edge; value is sqrt(S)/n for
+// normal depth, the depth over the cell's bed for a stage, the discharge per
+// unit width for an inflow.
+inline void ghost_state(const Pc& pc, uint32_t kind, float value, float hL, float huL, float hvL, float nx,
+                        float ny, float& hG, float& huG, float& hvG) {
+  float uL = safe_vel(pc, hL, huL);
+  float vL = safe_vel(pc, hL, hvL);
+  float unL = uL * nx + vL * ny;
+  float ut = -uL * ny + vL * nx;
+  float cL = std::sqrt(pc.g * gmax(hL, 0.0f));
+  float un = -unL;
+  hG = hL;
+  if (kind == kOpenFree) {
+    un = std::fabs(unL);
+  } else if (kind == kOpenNormal) {
+    un = value * pow23(gmax(hL, 0.0f));
+  } else if (kind == kOpenStage) {
+    if (hL < pc.dry || unL < cL) {
+      hG = value;
+      un = unL + 2.0f * (cL - std::sqrt(pc.g * value));
+    } else {
+      un = unL;
+    }
+  } else if (kind == kOpenInflow && value > 0.0f) {
+    hG = inflow_depth(pc, value, unL + 2.0f * cL);
+    un = -value / hG;
+    ut = 0.0f;
+  }
+  huG = hG * (un * nx - ut * ny);
+  hvG = hG * (un * ny + ut * nx);
+}
+
+inline bool open_forced(uint32_t kind, float value) { return kind == kOpenInflow && value > 0.0f; }
+inline bool outflow_only(uint32_t kind) { return kind == kOpenFree || kind == kOpenNormal; }
+
+// flux.py:271-281: mass flux through an open edge of HLLC mass flux fh.
+inline float boundary_mass_flux(uint32_t kind, float value, float fh) {
+  if (kind == kOpenFree || kind == kOpenNormal) return gmax(fh, 0.0f);
+  if (kind == kOpenStage) return fh;
+  if (kind == kOpenInflow) return -value;
+  return 0.0f;
+}
+
 // ---- the backend's fast arithmetic, restated -------------------------------
 // Same scheme, algebraically unchanged: one division per side for the
 // desingularised velocities, c = sqrt(g) sqrt(h), h * (1 / dry_tol), one star
@@ -375,6 +453,10 @@ struct cref {
   std::vector<uint32_t> blk;  // 3 words per block: wet, active, forced
   std::vector<uint32_t> src_cell;
   std::vector<float> src_rate;
+  // open edges: edge e with ecR[e] = -2 - k is open edge k
+  std::vector<uint32_t> okind, sptr;
+  std::vector<float> oval, tally;  // tally: per stretch out (hi, lo), in (hi, lo)
+  std::vector<uint32_t> oedge;     // the edge of open edge k
   uint32_t dtbuf[4] = {0, 0, 0, 0};
   Ctrl ctrl{};
   Pc pc{9.81f, 1e-4f, 0.8f};
@@ -399,6 +481,53 @@ inline void load_state(const cref& r, int stage, int c, float& h_, float& hu_, f
   }
 }
 
+// The six words of one edge (resident.py:146-184): mass, left momentum x, y,
+// right momentum x, y, CFL term; fh receives the HLLC mass flux, and a wall
+// passes no mass.
+void edge_words(const cref& r, const fastm::K& k, bool wall, float hL, float huL, float hvL, float loL, float hR,
+                float huR, float hvR, float loR, float dzL, float dzR, float nx_, float ny_, float len, float* o,
+                float& fh) {
+  const Pc& pc = r.pc;
+  if (r.fast) {
+    float hLs, qnL, qtL, hRs, qnR, qtR;
+    fastm::faces(k, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx_, ny_, loL, loR, hLs, qnL, qtL, hRs, qnR, qtR);
+    float rL = fastm::vel_factor(k, hLs), rR = fastm::vel_factor(k, hRs);
+    float Fh, Fhn, Fht, Smax;
+    fastm::hllc(k, hLs, qnL * rL, qtL * rL, hRs, qnR * rR, qtR * rR, Fh, Fhn, Fht, Smax);
+    fh = Fh;
+    if (wall) Fh = 0.0f;
+    // Normal momentum flux plus the hydrostatic correction, grouped as
+    // (Fhn - P(face depth)) + P(cell depth): still water gives exactly the
+    // cell's own pressure on every edge.
+    float nL = (Fhn - k.half_g * hLs * hLs) + k.half_g * hL * hL;
+    float nR = (Fhn - k.half_g * hRs * hRs) + k.half_g * hR * hR;
+    float tx = Fht * nx_, ty = Fht * ny_;
+    o[0] = Fh * len;
+    o[1] = std::fma(nL, nx_, -ty) * len;
+    o[2] = std::fma(nL, ny_, tx) * len;
+    o[3] = std::fma(nR, nx_, -ty) * len;
+    o[4] = std::fma(nR, ny_, tx) * len;
+    o[5] = Smax * len;
+    return;
+  }
+  float hLs, hqnL, hqtL, hRs, hqnR, hqtR;
+  face_states(pc, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx_, ny_, hLs, hqnL, hqtL, hRs, hqnR, hqtR);
+  float Fh, Fhn, Fht, Smax;
+  hllc(pc, hLs, hqnL, hqtL, hRs, hqnR, hqtR, Fh, Fhn, Fht, Smax);
+  fh = Fh;
+  if (wall) Fh = 0.0f;  // boundary_mass_flux of a wall
+  float Fhu = Fhn * nx_ - Fht * ny_;
+  float Fhv = Fhn * ny_ + Fht * nx_;
+  float SL_g = 0.5f * pc.g * (hL * hL - hLs * hLs);
+  float SR_g = 0.5f * pc.g * (hR * hR - hRs * hRs);
+  o[0] = Fh * len;
+  o[1] = Fhu * len + SL_g * nx_ * len;
+  o[2] = Fhv * len + SL_g * ny_ * len;
+  o[3] = Fhu * len + SR_g * nx_ * len;
+  o[4] = Fhv * len + SR_g * ny_ * len;
+  o[5] = Smax * len;
+}
+
 void flux(cref& r, int stage) {
   if (r.dtbuf[0] == 0u) return;
   const Pc pc = r.pc;
@@ -417,6 +546,8 @@ void flux(cref& r, int stage) {
       load_state(r, stage, cL, hL, huL, hvL);
       const float loL = load_lo(r, stage, cL);
       const float loR = (cR >= 0) ? load_lo(r, stage, cR) : loL;
+      const bool open = cR < -1;
+      const uint32_t ok = open ? (uint32_t)(-2 - cR) : 0u;
       if (cR == -1) {
         hR = hL;
         if (r.fast) {
@@ -425,54 +556,59 @@ void flux(cref& r, int stage) {
           huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
           hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
         }
+      } else if (open) {
+        ghost_state(pc, r.okind[ok], r.oval[ok], hL, huL, hvL, nx_, ny_, hR, huR, hvR);
       } else {
         load_state(r, stage, cR, hR, huR, hvR);
       }
-      if (hL < pc.dry && hR < pc.dry) continue;
-      float dzL = r.edz[2 * eid], dzR = r.edz[2 * eid + 1];
-      if (r.fast) {
-        const fastm::K& k = kk;
-        float hLs, qnL, qtL, hRs, qnR, qtR;
-        fastm::faces(k, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx_, ny_, loL, loR, hLs, qnL, qtL, hRs, qnR, qtR);
-        float rL = fastm::vel_factor(k, hLs), rR = fastm::vel_factor(k, hRs);
-        float Fh, Fhn, Fht, Smax;
-        fastm::hllc(k, hLs, qnL * rL, qtL * rL, hRs, qnR * rR, qtR * rR, Fh, Fhn, Fht, Smax);
-        if (cR < 0) Fh = 0.0f;
-        // Normal momentum flux plus the hydrostatic correction, grouped as
-        // (Fhn - P(face depth)) + P(cell depth): still water gives exactly the
-        // cell's own pressure on every edge.
-        float nL = (Fhn - k.half_g * hLs * hLs) + k.half_g * hL * hL;
-        float nR = (Fhn - k.half_g * hRs * hRs) + k.half_g * hR * hR;
-        float tx = Fht * nx_, ty = Fht * ny_;
-        __atomic_store_n(&r.touched[cL], (uint8_t)1, __ATOMIC_RELAXED);
-        if (cR >= 0) __atomic_store_n(&r.touched[cR], (uint8_t)1, __ATOMIC_RELAXED);
-        float* o = &r.ef[(size_t)kWords * eid];
-        o[0] = Fh * len;
-        o[1] = std::fma(nL, nx_, -ty) * len;
-        o[2] = std::fma(nL, ny_, tx) * len;
-        o[3] = std::fma(nR, nx_, -ty) * len;
-        o[4] = std::fma(nR, ny_, tx) * len;
-        if (stage == 1) o[kCfl] = Smax * len;
+      float* o = &r.ef[(size_t)kWords * eid];
+      if (hL < pc.dry && hR < pc.dry && !(open && open_forced(r.okind[ok], r.oval[ok]))) {
+        // An open edge clears its words; the gather skips other dry edges.
+        if (open)
+          for (uint32_t w = 0; w < kWords; ++w) o[w] = 0.0f;
         continue;
       }
-      float hLs, hqnL, hqtL, hRs, hqnR, hqtR;
-      face_states(pc, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx_, ny_, hLs, hqnL, hqtL, hRs, hqnR, hqtR);
-      float Fh, Fhn, Fht, Smax;
-      hllc(pc, hLs, hqnL, hqtL, hRs, hqnR, hqtR, Fh, Fhn, Fht, Smax);
-      if (cR < 0) Fh = 0.0f;  // boundary_mass_flux of a wall
-      float Fhu = Fhn * nx_ - Fht * ny_;
-      float Fhv = Fhn * ny_ + Fht * nx_;
-      float SL_g = 0.5f * pc.g * (hL * hL - hLs * hLs);
-      float SR_g = 0.5f * pc.g * (hR * hR - hRs * hRs);
+      float dzL = r.edz[2 * eid], dzR = r.edz[2 * eid + 1];
+      float w[kWords], fh;
+      edge_words(r, kk, cR == -1, hL, huL, hvL, loL, hR, huR, hvR, loR, dzL, dzR, nx_, ny_, len, w, fh);
+      if (open) {
+        const uint32_t kind = r.okind[ok];
+        if (outflow_only(kind) && fh < 0.0f) {
+          // Drawing water in through an outflow-only edge: the whole flux is
+          // redone against the wall ghost and passes no mass.
+          ghost_state(pc, kOpenWall, 0.0f, hL, huL, hvL, nx_, ny_, hR, huR, hvR);
+          edge_words(r, kk, true, hL, huL, hvL, loL, hR, huR, hvR, loR, dzL, dzR, nx_, ny_, len, w, fh);
+        } else {
+          w[0] = boundary_mass_flux(kind, r.oval[ok], fh) * len;
+        }
+      }
       __atomic_store_n(&r.touched[cL], (uint8_t)1, __ATOMIC_RELAXED);
       if (cR >= 0) __atomic_store_n(&r.touched[cR], (uint8_t)1, __ATOMIC_RELAXED);
-      float* o = &r.ef[(size_t)kWords * eid];
-      o[0] = Fh * len;
-      o[1] = Fhu * len + SL_g * nx_ * len;
-      o[2] = Fhv * len + SL_g * ny_ * len;
-      o[3] = Fhu * len + SR_g * nx_ * len;
-      o[4] = Fhv * len + SR_g * ny_ * len;
-      if (stage == 1) o[kCfl] = Smax * len;
+      for (uint32_t q = 0; q < 5; ++q) o[q] = w[q];
+      if (stage == 1) o[kCfl] = w[5];
+    }
+  }
+}
+
+// Volume through each stretch after one flux stage (resident.py:613-658):
+// dt/2 of each open edge's mass word, out and in apart, each kept as an exact
+// pair of words; an edge of an inactive block moved nothing.
+void open_tally(cref& r) {
+  if (r.dtbuf[0] == 0u || r.oedge.empty()) return;
+  const float half_dt = 0.5f * bitsf(r.dtbuf[0]);
+  const uint32_t n_st = (uint32_t)r.sptr.size() - 1;
+  for (uint32_t st = 0; st < n_st; ++st) {
+    for (uint32_t k = r.sptr[st]; k < r.sptr[st + 1]; ++k) {
+      const uint32_t e = r.oedge[k];
+      if (!block_active(r, (uint32_t)r.ecL[e] >> 8)) continue;
+      float v = half_dt * r.ef[(size_t)kWords * e];
+      uint32_t o = 4u * st + ((v < 0.0f) ? 2u : 0u);
+      float sum, err, hi, lo;
+      two_sum(r.tally[o], std::fabs(v), sum, err);
+      err += r.tally[o + 1];
+      two_sum(sum, err, hi, lo);
+      r.tally[o] = hi;
+      r.tally[o + 1] = lo;
     }
   }
 }
@@ -683,6 +819,9 @@ void cfl_prologue(cref& r) {
         huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
         hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
       }
+    } else if (cR < -1) {
+      const uint32_t k = (uint32_t)(-2 - cR);
+      ghost_state(pc, r.okind[k], r.oval[k], hL, huL, hvL, nx_, ny_, hR, huR, hvR);
     } else {
       hR = r.h[cR]; huR = r.hu[cR]; hvR = r.hv[cR];
     }
@@ -770,6 +909,18 @@ cref* cref_create(uint32_t N, uint32_t E, const int32_t* ecL, const int32_t* ecR
   r->ef.assign((size_t)kWords * E, 0.0f);
   r->touched.assign(N, 0);
   r->blk.assign(3 * (size_t)r->NB, 0u);
+  // Open edges (ecR = -2 - k) start closed, in one stretch.
+  for (uint32_t e = 0; e < E; ++e) {
+    if (ecR[e] >= -1) continue;
+    const uint32_t k = (uint32_t)(-2 - ecR[e]);
+    if (k >= r->oedge.size()) r->oedge.resize(k + 1, 0u);
+    r->oedge[k] = e;
+  }
+  const uint32_t n_open = (uint32_t)r->oedge.size();
+  r->okind.assign(n_open, kOpenWall);
+  r->oval.assign(n_open, 0.0f);
+  r->sptr = {0u, n_open};
+  r->tally.assign(n_open ? 4 : 0, 0.0f);
   return r;
 }
 
@@ -806,7 +957,31 @@ void cref_set_state(cref* r, const float* h, const float* hu, const float* hv, c
   if (hlo) r->hlo.assign(hlo, hlo + r->N);
   else r->hlo.assign(r->N, 0.0f);
   r->h1lo.assign(r->N, 0.0f);
+  std::fill(r->tally.begin(), r->tally.end(), 0.0f);
 }
+
+// Stretch s holds open edges [ptr[s], ptr[s + 1]); clears the tally.
+void cref_set_open_stretches(cref* r, uint32_t n, const uint32_t* ptr) {
+  r->sptr.assign(ptr, ptr + n + 1);
+  r->tally.assign(4 * (size_t)n, 0.0f);
+}
+
+// Condition and value per open edge; a block whose open edge can bring water
+// in (inflow or stage above zero) is forced live (gpu_resident_batch.py:447-458).
+void cref_set_open_conditions(cref* r, const uint32_t* kinds, const float* values) {
+  const size_t n = r->oedge.size();
+  r->okind.assign(kinds, kinds + n);
+  r->oval.assign(values, values + n);
+  for (uint32_t b = 0; b < r->NB; ++b) r->blk[3 * b + 2] = 0u;
+  for (size_t k = 0; k < n; ++k)
+    if ((kinds[k] == kOpenInflow || kinds[k] == kOpenStage) && values[k] > 0.0f)
+      r->blk[3 * ((uint32_t)r->ecL[r->oedge[k]] >> 8) + 2] = 1u;
+}
+
+uint32_t cref_open_count(const cref* r) { return (uint32_t)r->oedge.size(); }
+
+// Out and in volume per stretch as (hi, lo) float pairs: 4 words per stretch.
+void cref_open_tally(const cref* r, float* out4) { std::copy(r->tally.begin(), r->tally.end(), out4); }
 
 void cref_set_manning(cref* r, const float* n) { r->nman.assign(n, n + r->N); }
 
@@ -835,6 +1010,7 @@ void cref_steps(cref* r, uint32_t n) {
   for (uint32_t k = 0; k < n; ++k) {
     for (int stage = 0; stage < 2; ++stage) {
       flux(*r, stage);
+      open_tally(*r);
       update(*r, stage);
     }
     if (!r->src_cell.empty()) source(*r);
@@ -852,6 +1028,19 @@ void cref_download(const cref* r, int which, void* out) {
   } else if (which >= 9 && which <= 11) {
     uint32_t* o = static_cast<uint32_t*>(out);
     for (uint32_t b = 0; b < r->NB; ++b) o[b] = r->blk[3 * b + (which - 9)];
+  }
+}
+
+// Same probe as the backend's inu_probe_open: case k = (h, hu, hv, nx, ny, fh)
+// -> ghost (h, hu, hv) and boundary mass flux.
+void cref_probe_open(uint32_t n, const float* cases6, const uint32_t* kinds, const float* values, float* out4,
+                     float g, float dry) {
+  Pc pc{g, dry, 0.8f};
+  for (uint32_t k = 0; k < n; ++k) {
+    const float* c = cases6 + 6 * (size_t)k;
+    float* o = out4 + 4 * (size_t)k;
+    ghost_state(pc, kinds[k], values[k], c[0], c[1], c[2], c[3], c[4], o[0], o[1], o[2]);
+    o[3] = boundary_mass_flux(kinds[k], values[k], c[5]);
   }
 }
 

@@ -55,6 +55,11 @@ def load() -> C.CDLL:
         "cref_steps": (None, [_P, _U]),
         "cref_download": (None, [_P, C.c_int, _P]),
         "cref_probe": (None, [_U, _P, _P, _F, _F]),
+        "cref_probe_open": (None, [_U, _P, _P, _P, _P, _F, _F]),
+        "cref_set_open_stretches": (None, [_P, _U, _P]),
+        "cref_set_open_conditions": (None, [_P, _P, _P]),
+        "cref_open_count": (_U, [_P]),
+        "cref_open_tally": (None, [_P, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -81,6 +86,17 @@ def probe(pairs: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.fl
     p = np.ascontiguousarray(pairs, dtype=np.float32).reshape(-1, 2)
     out = np.empty((p.shape[0], 7), dtype=np.float32)
     load().cref_probe(p.shape[0], _p(p), _p(out), g, dry_tol)
+    return out
+
+
+def probe_open(cases, kinds, values, g: float, dry_tol: float) -> NDArray[np.float32]:  # noqa: ANN001
+    """Cases (h, hu, hv, nx, ny, fh) -> ghost (h, hu, hv) and boundary mass flux."""
+    c = np.ascontiguousarray(cases, dtype=np.float32).reshape(-1, 6)
+    k = np.ascontiguousarray(kinds, dtype=np.uint32).ravel()
+    v = np.ascontiguousarray(values, dtype=np.float32).ravel()
+    assert k.size == v.size == c.shape[0]
+    out = np.empty((c.shape[0], 4), dtype=np.float32)
+    load().cref_probe_open(c.shape[0], _p(c), _p(k), _p(v), _p(out), g, dry_tol)
     return out
 
 
@@ -168,10 +184,38 @@ class CpuRef:
         del use_graph
         self._lib.cref_steps(self._h, n)
 
+    @property
+    def n_open(self) -> int:
+        return int(self._lib.cref_open_count(self._h))
+
+    def set_open_stretches(self, stretch_ptr) -> None:  # noqa: ANN001
+        p = np.ascontiguousarray(stretch_ptr, dtype=np.uint32)
+        self._n_stretch = p.size - 1
+        self._lib.cref_set_open_stretches(self._h, p.size - 1, _p(p))
+
+    def set_open_conditions(self, kinds, values) -> None:  # noqa: ANN001
+        k = np.ascontiguousarray(kinds, dtype=np.uint32)
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        assert k.size == v.size == self.n_open
+        self._lib.cref_set_open_conditions(self._h, _p(k), _p(v))
+
+    def open_tally(self) -> NDArray[np.float32]:
+        """Per stretch: out (hi, lo), in (hi, lo) as float32 words."""
+        n = getattr(self, "_n_stretch", 1 if self.n_open else 0)
+        out = np.zeros(4 * n, dtype=np.float32)
+        if n:
+            self._lib.cref_open_tally(self._h, _p(out))
+        return out.reshape(n, 4)
+
+    def open_volumes(self) -> NDArray[np.float64]:
+        """Per stretch (out, in) [m^3], each as hi + lo in FP64."""
+        t = self.open_tally().astype(np.float64)
+        return np.stack([t[:, 0] + t[:, 1], t[:, 2] + t[:, 3]], axis=1)
+
     def download(self, which: int, n: int, dtype: type = np.float32) -> NDArray:
         out = np.empty(n, dtype=dtype)
         self._lib.cref_download(self._h, which, _p(out))
         return out
 
 
-__all__ = ["CpuRef", "load", "max_threads", "probe", "set_threads"]
+__all__ = ["CpuRef", "load", "max_threads", "probe", "probe_open", "set_threads"]

@@ -28,6 +28,27 @@ def valley_grid(nx: int = 256, ny: int = 128, dx: float = 2.0, datum: float = 0.
     return hilbert_reorder(geom, verbose=False)[0]
 
 
+def channel_grid(
+    cells: int = 300, across: int = 20, length: float = 3000.0, width: float = 200.0, slope: float = 1e-3
+) -> MeshGeometry:
+    """A straight channel of square cells, bed ``10 - slope * x``, Hilbert-ordered.
+
+    The reference's open-boundary test channel (``tests/unit/test_gpu_open_boundaries.py:41-56``:
+    3 km by 200 m in 300 by 20 cells, slope 1e-3).
+    """
+    dx = length / cells
+    assert abs(dx - width / across) < 1e-9, "square cells"
+    verts, faces = quad_grid(cells, across, dx)
+    cx = (np.arange(cells * across) % cells + 0.5) * dx
+    geom = build_geometry(verts.astype(np.float32), faces, zb_from_file=(10.0 - slope * cx).astype(np.float32))
+    return hilbert_reorder(geom, verbose=False)[0]
+
+
+def channel_end(x: float, width: float = 200.0, dx: float = 10.0) -> tuple[list[tuple[float, float]], float]:
+    """The line across the channel at ``x`` and its pick tolerance (a quarter cell)."""
+    return [(x, 0.0), (x, width)], 0.25 * dx
+
+
 def _extent(g: MeshGeometry) -> tuple[float, float]:
     v = g.cell_vertices
     return float(v[:, 0].max()), float(v[:, 1].max())
