@@ -24,6 +24,9 @@ CFL_DEFAULT = 0.8
 # decays at a rate (reference tuning.py:37-39).
 FILM_DEPTHS = 4.0
 
+# A mesh polygon needs at least this many vertices (reference tuning.py:23).
+MIN_POLYGON_VERTICES = 3
+
 # Cells below this area are refused by build_geometry; the CFL reductions skip
 # cells at or below it (reference tuning.py:88-92).
 MIN_CELL_AREA = 1e-10

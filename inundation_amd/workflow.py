@@ -11,9 +11,10 @@ default here) is the MI355X backend. The Vulkan backends need Kompute, which
 this package does not ship, and ``prepare()`` says so. Open boundaries follow
 ``:74-121, 258-289, 361-435``: named stretches picked by a polyline, a
 condition per stretch and phase, and the volume through each stretch at every
-snapshot. Mesh-file loading is not implemented by this package:
-``prepare()`` takes the geometry (``MeshGeometry`` + permutation) or a
-geometry-cache artifact written by ``inundation_amd.mesh.cache``.
+snapshot. Geometry comes from ``mesh_source`` (OBJ or (Geo)Parquet, through
+the key-addressed ``geometry_cache_dir`` when set) or from a prebuilt
+``geometry_cache_source`` artifact (``:658-760``); ``prepare()`` also takes
+a built ``MeshGeometry`` directly.
 """
 
 from __future__ import annotations
@@ -154,6 +155,11 @@ class WorkflowConfig:
     device: int = 0
 
     def __post_init__(self) -> None:
+        if self.mesh_source is None and self.geometry_cache_source is None:
+            raise ValueError(
+                "mesh_source is required unless geometry_cache_source points at a "
+                "prebuilt geometry-cache artifact"
+            )
         if self.manning_n <= 0:
             raise ValueError(f"manning_n must be positive, got {self.manning_n}")
         if self.output_interval_s <= 0:
@@ -489,6 +495,57 @@ def load_initial_state(
     return h, hu, hv, n_mann
 
 
+def load_geometry(cfg: WorkflowConfig) -> tuple[MeshGeometry, NDArray[np.int32]]:
+    """The solver's geometry and ``perm[new] = old`` (reference ``workflow.py:658-752``).
+
+    A ``geometry_cache_source`` artifact wins and the mesh file is never
+    opened; its cell order must match ``use_hilbert_reorder``. Otherwise the
+    mesh file is loaded, built and reordered, through the directory cache
+    when ``geometry_cache_dir`` is set (a hit skips the build, a miss stores
+    the result).
+    """
+    from .mesh.cache import (
+        geometry_cache_key,
+        geometry_from_mesh_file,
+        load_geometry_cache,
+        load_geometry_cache_artifact,
+        save_geometry_cache,
+    )
+
+    def log(msg: str) -> None:
+        if cfg.progress:
+            sys.stdout.write(f"[prepare] {msg}\n")
+            sys.stdout.flush()
+
+    if cfg.geometry_cache_source is not None:
+        geom, perm, meta = load_geometry_cache_artifact(cfg.geometry_cache_source)
+        if meta.use_hilbert_reorder != cfg.use_hilbert_reorder:
+            raise ValueError(
+                f"geometry cache artifact {cfg.geometry_cache_source} was built with "
+                f"use_hilbert_reorder={meta.use_hilbert_reorder}, but WorkflowConfig.use_hilbert_reorder "
+                f"is {cfg.use_hilbert_reorder} — rebuild the artifact or match the config"
+            )
+        log(f"geometry cache artifact loaded ({cfg.geometry_cache_source}; built from {meta.mesh_source})")
+        return geom, perm
+    if cfg.mesh_source is None:
+        raise RuntimeError("mesh_source is None without a geometry cache source")
+    key = ""
+    if cfg.geometry_cache_dir is not None:
+        key = geometry_cache_key(cfg.mesh_source, use_hilbert_reorder=cfg.use_hilbert_reorder)
+        hit = load_geometry_cache(cfg.geometry_cache_dir, key)
+        if hit is not None:
+            log(f"geometry cache hit ({cfg.geometry_cache_dir}, key={key[:12]}...)")
+            return hit
+        log(f"geometry cache miss ({cfg.geometry_cache_dir}, key={key[:12]}...) — building")
+    t0 = time.perf_counter()
+    geom, perm = geometry_from_mesh_file(cfg.mesh_source, use_hilbert_reorder=cfg.use_hilbert_reorder)
+    log(f"geometry built in {time.perf_counter() - t0:.1f}s ({cfg.mesh_source})")
+    if cfg.geometry_cache_dir is not None:
+        path = save_geometry_cache(cfg.geometry_cache_dir, key, geom, perm)
+        log(f"geometry cache saved ({path})")
+    return geom, perm
+
+
 class SWEWorkflow:
     """Prepare once, run phase schedules (reference ``workflow.py:628-969``)."""
 
@@ -501,7 +558,7 @@ class SWEWorkflow:
         self._manning: NDArray[np.float32] | None = None
 
     def prepare(self, geometry: MeshGeometry | None = None, perm: NDArray[np.int32] | None = None) -> None:
-        """Geometry (given, or from ``geometry_cache_source``), initial state, solver."""
+        """Geometry (given, or loaded as ``load_geometry`` does), initial state, solver."""
         cfg = self.config
         if cfg.solver_impl != "hip_resident":
             raise RuntimeError(
@@ -509,20 +566,7 @@ class SWEWorkflow:
                 "this package provides 'hip_resident'"
             )
         if geometry is None:
-            if cfg.geometry_cache_source is None:
-                raise ValueError(
-                    "pass the geometry to prepare() or set geometry_cache_source "
-                    "(mesh-file loading is not part of this package)"
-                )
-            from .mesh.cache import load_geometry_cache_artifact
-
-            geometry, perm, meta = load_geometry_cache_artifact(cfg.geometry_cache_source)
-            if meta.use_hilbert_reorder != cfg.use_hilbert_reorder:
-                raise ValueError(
-                    f"geometry cache artifact {cfg.geometry_cache_source} was built with "
-                    f"use_hilbert_reorder={meta.use_hilbert_reorder}, but "
-                    f"WorkflowConfig.use_hilbert_reorder is {cfg.use_hilbert_reorder}"
-                )
+            geometry, perm = load_geometry(cfg)
         geom = geometry
         if perm is None:
             perm = np.arange(geom.N, dtype=np.int32)
@@ -663,6 +707,7 @@ __all__ = [
     "Stage",
     "Wall",
     "inflow_phases",
+    "load_geometry",
     "open_conditions",
     "open_stretch_edges",
     "phase_open_conditions",

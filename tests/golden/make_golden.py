@@ -15,7 +15,12 @@ inputs and outputs as ``.npz`` data:
 - ``cpu_step_valley.npz``: 8-step FP64 trajectories of the four step-parity
   cases (reference ``test_gpu_step_parity.py:81-115``) on a 32 x 16 valley,
   and the FP64 CFL bound;
-- ``hllc_vectors.npz``: HLLC fluxes and speeds of seeded random face states.
+- ``hllc_vectors.npz``: HLLC fluxes and speeds of seeded random face states;
+- ``artifact_v6_patch.npz``: a geometry-cache artifact written by the
+  reference's ``build_geometry_cache`` from a small OBJ mesh, and
+  ``mesh_obj_cache.npz``: that OBJ's text, the reference's loader output
+  and its cache keys (``python tests/golden/make_golden.py cache`` writes
+  only these two).
 
 Only data is written; nothing of the reference's source is copied.
 """
@@ -179,11 +184,67 @@ def hllc_fixtures(cpu_step) -> None:  # noqa: ANN001
     print("hllc", m)
 
 
+# A 3 x 2 patch of quads and triangles with vertex elevations, one face
+# clockwise, one vertex listed but unused, "v/vt" face tokens.
+OBJ_TEXT = """\
+# mixed patch
+v 0 0 1.0
+v 1 0 1.5
+v 2 0 2.0
+v 3 0 2.5
+v 0 1 1.2
+v 1 1 1.7
+v 2 1 2.2
+v 3 1 2.7
+v 0 2 1.4
+v 1 2 1.9
+v 2 2 2.4
+v 3 2 2.9
+v 9 9 9.0
+f 1 2 6 5
+f 2/1 3/2 7/3
+f 2 7 6
+f 3 4 8 7
+f 5 6 10 9
+f 6 10 11
+f 6 7 11
+f 7 8 12 11
+"""
+
+
+def cache_fixtures() -> None:
+    import tempfile
+
+    sys.path.insert(0, str(REF))
+    import inundation.mesh.cache as cache
+    import inundation.mesh.loader as loader
+
+    with tempfile.TemporaryDirectory() as tmp:
+        obj = Path(tmp) / "patch.obj"
+        obj.write_text(OBJ_TEXT, encoding="ascii")
+        art = cache.build_geometry_cache(obj, Path(tmp) / "patch.geometry.npz", use_hilbert_reorder=True)
+        (OUT / "artifact_v6_patch.npz").write_bytes(Path(art).read_bytes())
+        v, f, o, zb, n = loader.load_mesh_file(str(obj))
+        assert n is None
+        np.savez_compressed(
+            OUT / "mesh_obj_cache.npz",
+            obj_text=np.frombuffer(OBJ_TEXT.encode("ascii"), dtype=np.uint8),
+            verts=v, faces=f, offsets=o, zb=zb,
+            key_hilbert=np.array(cache.geometry_cache_key(obj, use_hilbert_reorder=True)),
+            key_plain=np.array(cache.geometry_cache_key(obj, use_hilbert_reorder=False)),
+        )
+    print("cache", cache.GEOMETRY_CACHE_VERSION)
+
+
 def main() -> None:
+    if sys.argv[1:] == ["cache"]:
+        cache_fixtures()
+        return
     geometry, blocks, cpu_step = _ref_modules()
     geometry_fixtures(geometry, blocks)
     cpu_step_fixtures(geometry, cpu_step)
     hllc_fixtures(cpu_step)
+    cache_fixtures()
 
 
 if __name__ == "__main__":

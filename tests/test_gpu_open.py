@@ -171,16 +171,29 @@ def test_stretches_split_the_tally(ctx, channel) -> None:  # noqa: ANN001
 # ---- the workflow -----------------------------------------------------------
 
 
-def _workflow(g, initial_depth: float | None = None, **cfg):  # noqa: ANN001, ANN202
+@pytest.fixture(scope="module")
+def channel_file(tmp_path_factory):  # noqa: ANN001, ANN201
+    """The channel as a mesh file, as the reference's tests write it."""
+    from inundation_amd.mesh.loader import write_mesh_parquet
+    from inundation_amd.mesh.synth import quad_grid
+
+    verts, faces = quad_grid(300, 20, 10.0)
+    cx = (np.arange(300 * 20) % 300 + 0.5) * 10.0
+    path = tmp_path_factory.mktemp("open") / "channel.parquet"
+    write_mesh_parquet(path, verts, faces, (10.0 - SLOPE * cx).astype(np.float32))
+    return str(path)
+
+
+def _workflow(path: str, initial_depth: float | None = None, **cfg):  # noqa: ANN001, ANN202
     from inundation_amd.workflow import OpenBoundary, SWEWorkflow, WorkflowConfig
 
     state = None
     if initial_depth is not None:
-        h = np.full(g.N, initial_depth, dtype=np.float32)
+        h = np.full(300 * 20, initial_depth, dtype=np.float32)
         state = {"h": h, "hu": np.zeros_like(h), "hv": np.zeros_like(h)}
     wf = SWEWorkflow(
         WorkflowConfig(
-            mesh_source=None,
+            mesh_source=path,
             manning_n=MANNING,
             output_interval_s=5.0,
             dt_max=1.0,
@@ -189,6 +202,7 @@ def _workflow(g, initial_depth: float | None = None, **cfg):  # noqa: ANN001, AN
             track_clamp_mass=True,
             dry_tol=DRY,
             initial_state_source=state,
+            initial_state_order="original",
             open_boundaries=[
                 OpenBoundary("up", *channel_end(0.0)),
                 OpenBoundary("down", *channel_end(3000.0)),
@@ -196,15 +210,14 @@ def _workflow(g, initial_depth: float | None = None, **cfg):  # noqa: ANN001, AN
             **cfg,
         )
     )
-    wf.prepare(g)
+    wf.prepare()
     return wf
 
 
-def test_workflow_volumes_line_up_with_the_snapshots(channel) -> None:  # noqa: ANN001
+def test_workflow_volumes_line_up_with_the_snapshots(channel_file) -> None:  # noqa: ANN001
     from inundation_amd.workflow import Inflow, SimulationPhase, Wall
 
-    g = channel[0]
-    wf = _workflow(g)
+    wf = _workflow(channel_file)
     res = wf.run([
         SimulationPhase(12.0, [], {"up": Inflow(50.0, SLOPE), "down": Wall()}),
         SimulationPhase(8.0, [], {"up": Inflow(20.0, SLOPE), "down": Wall()}),
@@ -220,12 +233,11 @@ def test_workflow_volumes_line_up_with_the_snapshots(channel) -> None:  # noqa: 
     wf.solver.close()
 
 
-def test_workflow_balance_closes_while_water_enters_and_leaves(channel) -> None:  # noqa: ANN001
+def test_workflow_balance_closes_while_water_enters_and_leaves(channel_file) -> None:  # noqa: ANN001
     from inundation_amd.workflow import Inflow, NormalDepth, SimulationPhase
 
-    g = channel[0]
-    wf = _workflow(g, initial_depth=0.5)
-    start = 0.5 * float(g.area.astype(np.float64).sum())
+    wf = _workflow(channel_file, initial_depth=0.5)
+    start = 0.5 * float(wf.geom.area.astype(np.float64).sum())
     res = wf.run([SimulationPhase(120.0, [], {"up": Inflow(50.0, SLOPE), "down": NormalDepth(SLOPE)})])
     went_in = res.boundary_volumes_m3["up"][-1, 1]
     went_out = res.boundary_volumes_m3["down"][-1, 0]
@@ -234,11 +246,10 @@ def test_workflow_balance_closes_while_water_enters_and_leaves(channel) -> None:
     wf.solver.close()
 
 
-def test_closing_a_stretch_releases_its_blocks(channel) -> None:  # noqa: ANN001
+def test_closing_a_stretch_releases_its_blocks(channel_file) -> None:  # noqa: ANN001
     from inundation_amd.workflow import Inflow, SimulationPhase, Wall
 
-    g = channel[0]
-    wf = _workflow(g)
+    wf = _workflow(channel_file)
     wf.run([SimulationPhase(5.0, [], {"up": Inflow(50.0, SLOPE), "down": Wall()})])
     assert wf.solver.download_block_flags()[:, 2].sum() > 0
     wf.run([SimulationPhase(5.0, [], {"up": Wall(), "down": Wall()})])
@@ -246,16 +257,16 @@ def test_closing_a_stretch_releases_its_blocks(channel) -> None:  # noqa: ANN001
     wf.solver.close()
 
 
-def test_repeat_runs_are_bit_identical(channel) -> None:  # noqa: ANN001
+def test_repeat_runs_are_bit_identical(channel_file) -> None:  # noqa: ANN001
     from inundation_amd.workflow import Inflow, SimulationPhase, Stage
 
-    g = channel[0]
+    n_cells = 300 * 20
     phases = [SimulationPhase(30.0, [], {"up": Inflow(50.0, SLOPE), "down": Stage(7.3)})]
     bits = []
     for _ in range(2):
-        wf = _workflow(g)
+        wf = _workflow(channel_file)
         r = wf.run(phases)
         bits.append(r.h_final.tobytes() + b"".join(v.tobytes() for v in r.boundary_volumes_m3.values()))
         wf.solver.close()
     assert bits[0] == bits[1]
-    assert math.isfinite(float(np.frombuffer(bits[0][: 4 * g.N], np.float32).sum()))
+    assert math.isfinite(float(np.frombuffer(bits[0][: 4 * n_cells], np.float32).sum()))
