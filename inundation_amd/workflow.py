@@ -419,7 +419,11 @@ def load_initial_state(
 ) -> tuple[NDArray[np.float32], NDArray[np.float32], NDArray[np.float32], NDArray[np.float32] | None] | None:
     """Optional per-cell initial state (h or wse, hu/hv or u/v, optional n_mann).
 
-    Reference ``workflow.py:450-596`` for dict, ``.npz`` and ``.npy`` sources.
+    Reference ``workflow.py:450-596``: a dict, ``.npz``, ``.npy`` ((N, 3) or
+    (N, 4) columns), ``.csv`` or ``.parquet`` (through pandas). Metadata
+    entries (``saved_order``, ``mesh_source``, ``t_end_s``, ``dt_max``,
+    ``cfl_interval``) are ignored; an ``id_col`` column maps rows to cells;
+    ``order="original"`` takes rows in the mesh file's cell order.
     """
     if source is None:
         return None
@@ -440,8 +444,15 @@ def load_initial_state(
             state = {"h": arr[:, 0], "hu": arr[:, 1], "hv": arr[:, 2]}
             if arr.shape[1] == 4:
                 state["n_mann"] = arr[:, 3]
+        elif suffix in (".csv", ".parquet"):
+            try:
+                import pandas as pd
+            except ImportError as exc:
+                raise RuntimeError("pandas is required to load .csv/.parquet initial state files") from exc
+            df = pd.read_csv(path) if suffix == ".csv" else pd.read_parquet(path)
+            state = {c: df[c].to_numpy() for c in df.columns}
         else:
-            raise ValueError(f"Unsupported initial state format: {suffix}. Use .npz or .npy")
+            raise ValueError(f"Unsupported initial state format: {suffix}. Use .npz, .npy, .csv, or .parquet")
     valid = {"h", "wse", "hu", "hv", "u", "v", "n_mann", id_col}
     state = {k: v for k, v in state.items() if k not in INITIAL_STATE_METADATA_KEYS}
     bad = sorted(k for k in state if k not in valid)
@@ -493,6 +504,38 @@ def load_initial_state(
     if n_mann is not None:
         n_mann = np.maximum(n_mann.astype(np.float32), np.float32(1e-4))
     return h, hu, hv, n_mann
+
+
+def save_warm_start(
+    path: str | os.PathLike[str],
+    result: WorkflowResult,
+    perm: NDArray[np.int32],
+    *,
+    order: str = "original",
+    **metadata: object,
+) -> Path:
+    """A run's final state as an ``.npz`` warm start that ``initial_state_source`` loads.
+
+    ``order="original"`` writes rows in the mesh file's cell order (load it
+    with ``initial_state_order="original"``), ``"solver"`` in the solver's.
+    ``saved_order`` and any ``metadata`` (for example ``mesh_source``,
+    ``t_end_s``) ride along and are ignored by the loader.
+    """
+    if result.hu_final is None or result.hv_final is None:
+        raise ValueError("the result holds no final momentum")
+    if order not in ("solver", "original"):
+        raise ValueError("order must be 'solver' or 'original'")
+    fields = {"h": result.h_final, "hu": result.hu_final, "hv": result.hv_final}
+    if order == "original":
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(perm.size, dtype=perm.dtype)
+        fields = {k: v[inv] for k, v in fields.items()}
+    bad = sorted(set(metadata) - INITIAL_STATE_METADATA_KEYS)
+    if bad:
+        raise ValueError(f"unknown warm-start metadata {bad}; allowed: {sorted(INITIAL_STATE_METADATA_KEYS)}")
+    out = Path(os.fspath(path))
+    np.savez(out, saved_order=np.asarray(order), **{k: np.asarray(v) for k, v in metadata.items()}, **fields)
+    return out
 
 
 def load_geometry(cfg: WorkflowConfig) -> tuple[MeshGeometry, NDArray[np.int32]]:
@@ -711,6 +754,7 @@ __all__ = [
     "open_conditions",
     "open_stretch_edges",
     "phase_open_conditions",
+    "save_warm_start",
     "SWEWorkflow",
     "SimulationPhase",
     "WorkflowConfig",
