@@ -80,6 +80,7 @@ def main() -> int:
     ap.add_argument("--scale", type=int, default=1, help="shrink the lattice by this factor (testing)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-unordered", action="store_true", help="skip the unordered-accumulation comparison run")
     ap.add_argument(
         "--workload",
         choices=("rainfall", "batched", "adjoint"),
@@ -180,6 +181,33 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(ctx, wl, m, info, args.cpu_seconds)
 
+    # The same run with unordered accumulation (INU_FLAG_DETERMINISTIC off,
+    # the reference's atomic variant): its speed, and how far it drifts from
+    # the fixed-order state over the run.
+    unordered = None
+    if not args.no_unordered:
+        start(ctx, wl, t_stop)
+        ctx.steps_timed(args.steps)
+        h_fixed = ctx.download(L.FIELD_H, n)
+        ctx.set_params(9.81, wl.dry_tol, wl.cfl, flags & ~L.FLAG_DETERMINISTIC)
+        start(ctx, wl, t_stop)
+        if args.warmup:
+            ctx.steps_timed(args.warmup)
+        start(ctx, wl, t_stop)
+        t0 = time.perf_counter()
+        u_pred, u_corr, _ = ctx.steps_timed(args.steps)
+        u_wall = time.perf_counter() - t0
+        h_u = ctx.download(L.FIELD_H, n)
+        ctx.set_params(9.81, wl.dry_tol, wl.cfl, flags)
+        unordered = {
+            "ms_per_step": u_wall / args.steps * 1e3,
+            "predictor_ms": u_pred / args.steps,
+            "corrector_ms": u_corr / args.steps,
+            "value": args.steps * n / u_wall,
+            "max_abs_depth_diff_m": float(np.abs(h_u.astype(np.float64) - h_fixed).max()),
+        }
+        log(f"unordered accumulation: {unordered}")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -230,6 +258,7 @@ def main() -> int:
             "active_cell_steps_per_s_over_stage_kernels": act["cells"] / ((ms_pred + ms_corr) * 1e-3),
         },
         "cpu_baseline": cpu,
+        "unordered_accumulation": unordered,
     }
     print(json.dumps(out))
     ctx.close()

@@ -42,7 +42,11 @@ extern "C" {
 
 /* inu_set_params flags: the solver attributes the workflow sets after
  * construction (reference gpu_resident_batch.py:143-158, workflow.py:816-826). */
-#define INU_FLAG_DETERMINISTIC 1u /* fixed-order CSR gather (the only mode implemented) */
+/* Fixed-order CSR gather (bit-reproducible). Off: each edge's words are added
+ * into its cells' sums as threads arrive (the reference's atomic variant,
+ * resident.py:110-180), results vary in the last bits from run to run; not
+ * with open boundaries or an adjoint recording. */
+#define INU_FLAG_DETERMINISTIC 1u
 #define INU_FLAG_COMPENSATED 2u   /* depth as a compensated float32 pair */
 #define INU_FLAG_ACTIVE_SET 4u    /* skip blocks beyond the two-ring of water */
 #define INU_FLAG_TRACK_CLAMP 8u   /* accumulate the positivity-clamp mass */

@@ -381,17 +381,17 @@ __device__ __forceinline__ void open_pass(const Args& a, const Lds& l, uint32_t 
   }
 }
 
-// Words of local edge j into LDS (zeros when both sides are dry); both sides
-// are read from LDS. An open edge is stored as a wall; open_pass replaces its
+// The six words of a local edge (zeros when both sides are dry), both sides
+// read from LDS. An open edge is stored as a wall; open_pass replaces its
 // words.
-template <int ST, bool FM, bool LO>
-__device__ __forceinline__ void edge_eval(const Args& a, const Lds& l, const EdgeRec& r, uint32_t j) {
+template <bool FM, bool LO>
+__device__ __forceinline__ void edge_words_lds(const Args& a, const Lds& l, const EdgeRec& r, float w[6]) {
   const uint32_t ql = r.side >> 16;
   const uint32_t qr = r.side & 0xffffu;
   const bool wall = qr == kSideWall;
   const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
   const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
-  float w[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  for (int q = 0; q < 6; ++q) w[q] = 0.0f;
   const float4 sl = l.cell[2 * ql];
   float4 sr;
   if (wall) {
@@ -412,8 +412,43 @@ __device__ __forceinline__ void edge_eval(const Args& a, const Lds& l, const Edg
       inu::edge_words(a.p, wall, sl.x, sl.y, sl.z, sr.x, sr.y, sr.z, dzL, dzR, r.nx, r.ny, r.len, w);
     }
   }
+}
+
+// Words of local edge j into LDS, for the fixed-order gather.
+template <int ST, bool FM, bool LO>
+__device__ __forceinline__ void edge_eval(const Args& a, const Lds& l, const EdgeRec& r, uint32_t j) {
+  float w[6];
+  edge_words_lds<FM, LO>(a, l, r, w);
   l.edge[2 * j] = make_float4(w[0], w[1], w[2], w[3]);
   l.edge[2 * j + 1] = make_float4(w[4], w[5], __uint_as_float(r.side), r.len);
+}
+
+// Unordered accumulation (the reference's atomic flux variant,
+// resident.py:110-180): the edge's words are added at once into the LDS sums
+// of its sides in this block (mass, momentum x, y, wave speeds), in whatever
+// order the threads arrive, so results vary in the last bits from run to run.
+// Ghost sides are summed by their own block.
+template <int ST, bool FM, bool LO>
+__device__ __forceinline__ void edge_accumulate(const Args& a, const Lds& l, const EdgeRec& r) {
+  float w[6];
+  edge_words_lds<FM, LO>(a, l, r, w);
+  if (w[0] == 0.0f && w[1] == 0.0f && w[2] == 0.0f && w[3] == 0.0f && w[4] == 0.0f && w[5] == 0.0f) return;
+  // Four sums per cell, each a run of 256 words (one LDS bank per cell).
+  float* acc = reinterpret_cast<float*>(l.edge);
+  const uint32_t ql = r.side >> 16;
+  const uint32_t qr = r.side & 0xffffu;
+  if (ql < kBlockCells) {
+    atomicAdd(acc + ql, w[0]);
+    atomicAdd(acc + kBlockCells + ql, w[1]);
+    atomicAdd(acc + 2 * kBlockCells + ql, w[2]);
+    if (ST == 1) atomicAdd(acc + 3 * kBlockCells + ql, w[5]);
+  }
+  if (qr < kBlockCells) {
+    atomicAdd(acc + qr, -w[0]);
+    atomicAdd(acc + kBlockCells + qr, -w[3]);
+    atomicAdd(acc + 2 * kBlockCells + qr, -w[4]);
+    if (ST == 1) atomicAdd(acc + 3 * kBlockCells + qr, w[5]);
+  }
 }
 
 // The next blocks of a workgroup's walk over the list: the block after the
@@ -434,7 +469,7 @@ struct Ahead {
 // issued at once (cells, ghost cells, CSR slots, the thread's first edge
 // records); (B) each edge touching the block is evaluated once into LDS from
 // LDS; (C) each cell sums its edges in CSR order from LDS and updates.
-template <int ST, bool COMP, bool FM, int TILE, bool REC>
+template <int ST, bool COMP, bool FM, int TILE, bool REC, bool ATOM>
 __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, uint32_t gid, float dt,
                                 const Lds& l, Ahead& nx, uint32_t end, uint32_t per) {
   constexpr bool LO = COMP && FM;  // the fast order's face depths read the low words
@@ -454,7 +489,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   uint32_t sl[kSlotsAhead];
 #pragma unroll
   for (uint32_t m = 0; m < kSlotsAhead; ++m)
-    if (tid + m * kBlockCells < bi.n_slots) sl[m] = ldg(a.lidx, bi.s0 + tid + m * kBlockCells);
+    if (!ATOM && tid + m * kBlockCells < bi.n_slots) sl[m] = ldg(a.lidx, bi.s0 + tid + m * kBlockCells);
   float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (tid < bi.n_ghost) gs = load_ghost<ST, LO>(a, gid);
   const uint32_t i = c0 + tid;
@@ -466,8 +501,10 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     hui = ldg((ST == 0) ? a.hu : a.hu1, i);
     hvi = ldg((ST == 0) ? a.hv : a.hv1, i);
     if (COMP) loi = ldg((ST == 0) ? a.hlo : a.h1lo, i);
-    k0 = ldg(a.cptr, i) - bi.s0;
-    k1 = ldg(a.cptr, i + 1) - bi.s0;
+    if (!ATOM) {
+      k0 = ldg(a.cptr, i) - bi.s0;
+      k1 = ldg(a.cptr, i + 1) - bi.s0;
+    }
     A = ldg(a.area, i);
     nm = ldg(a.nman, i);
     if (ST == 1) {
@@ -493,19 +530,30 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     put_cell<FM>(l, p, kBlockCells + g, s);
     if (REC) rec_ghost<ST, COMP>(a, gg, s);
   }
+  if (ATOM) {
+    float* acc = reinterpret_cast<float*>(l.edge);
+    for (uint32_t k = 0; k < 4; ++k) acc[k * kBlockCells + tid] = 0.0f;
+  } else {
 #pragma unroll
-  for (uint32_t m = 0; m < kSlotsAhead; ++m)
-    if (tid + m * kBlockCells < bi.n_slots) l.s_slot[tid + m * kBlockCells] = (uint16_t)sl[m];
-  for (uint32_t k = tid + kSlotsAhead * kBlockCells; k < bi.n_slots; k += kBlockCells)
-    l.s_slot[k] = ldg(a.lidx, bi.s0 + k);
+    for (uint32_t m = 0; m < kSlotsAhead; ++m)
+      if (tid + m * kBlockCells < bi.n_slots) l.s_slot[tid + m * kBlockCells] = (uint16_t)sl[m];
+    for (uint32_t k = tid + kSlotsAhead * kBlockCells; k < bi.n_slots; k += kBlockCells)
+      l.s_slot[k] = ldg(a.lidx, bi.s0 + k);
+  }
   __syncthreads();
 
   // ---- (B) edges -----------------------------------------------------------
 #pragma unroll
-  for (uint32_t m = 0; m < kEdgesAhead; ++m)
-    if (tid + m * kBlockCells < n_loc) edge_eval<ST, FM, LO>(a, l, er[m], tid + m * kBlockCells);
-  for (uint32_t j = tid + kEdgesAhead * kBlockCells; j < n_loc; j += kBlockCells)
-    edge_eval<ST, FM, LO>(a, l, load_edge(a, bi, j), j);
+  for (uint32_t m = 0; m < kEdgesAhead; ++m) {
+    if (tid + m * kBlockCells < n_loc) {
+      if (ATOM) edge_accumulate<ST, FM, LO>(a, l, er[m]);
+      else edge_eval<ST, FM, LO>(a, l, er[m], tid + m * kBlockCells);
+    }
+  }
+  for (uint32_t j = tid + kEdgesAhead * kBlockCells; j < n_loc; j += kBlockCells) {
+    if (ATOM) edge_accumulate<ST, FM, LO>(a, l, load_edge(a, bi, j));
+    else edge_eval<ST, FM, LO>(a, l, load_edge(a, bi, j), j);
+  }
   // Walk ahead: this thread's ghost cell id of the next block, its dt.
   if (nx.t < end) {
     if (tid < nx.bi.n_ghost) nx.gid = ldg(a.gx, nx.bi.g0 + tid);
@@ -527,7 +575,14 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   float lam_b = 0.0f;
   if (tid < nc) {
     float sdh = 0.0f, sdhu = 0.0f, sdhv = 0.0f, lam = 0.0f;
-    for (uint32_t k = k0; k < k1; ++k) {
+    if (ATOM) {
+      const float* acc = reinterpret_cast<const float*>(l.edge);
+      sdh = acc[tid];
+      sdhu = acc[kBlockCells + tid];
+      sdhv = acc[2 * kBlockCells + tid];
+      lam = acc[3 * kBlockCells + tid];
+    }
+    for (uint32_t k = k0; !ATOM && k < k1; ++k) {
       const uint32_t s = l.s_slot[k];
       const uint32_t j = s >> 1;
       const float4 wa = l.edge[2 * j];
@@ -677,7 +732,7 @@ __device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t en
   if (nx.t + per < end) nx.b2 = a.list[nx.t + per];
 }
 
-template <int ST, bool COMP, bool FM, int TILE, bool REC = false>
+template <int ST, bool COMP, bool FM, int TILE, bool REC = false, bool ATOM = false>
 __global__ __launch_bounds__(256) void k_stage(const Args a) {
   extern __shared__ float sm[];
   if (a.ctl->any_dt == 0u) return;  // no-op step: every scenario is at its stop
@@ -719,7 +774,8 @@ __global__ __launch_bounds__(256) void k_stage(const Args a) {
       walk_ahead(a, nx, end, per);
       continue;
     }
-    const uint32_t bb = stage_block<ST, COMP, FM, TILE, REC>(a, b, bi, gid, __uint_as_float(dtw), l, nx, end, per);
+    const uint32_t bb =
+        stage_block<ST, COMP, FM, TILE, REC, ATOM>(a, b, bi, gid, __uint_as_float(dtw), l, nx, end, per);
     bound = (bb < bound) ? bb : bound;
   }
   if (ST == 1) block_atomic_min(&a.scen[cur_s].dt_next, bound);
@@ -1670,9 +1726,21 @@ void launch_stage_t(inu_ctx* c, const Args& a, bool comp, bool fm) {
   else hipLaunchKernelGGL((k_stage<ST, false, false, TILE>), g, b, lds, c->stream, a);
 }
 
+// Unordered accumulation (INU_FLAG_DETERMINISTIC off), tile level 0.
+template <int ST>
+void launch_stage_atom(inu_ctx* c, const Args& a, bool comp, bool fm) {
+  const size_t lds = ST == 0 ? c->lds_stage0 : c->lds_stage1;
+  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0][0]), b(256);
+  if (comp && fm) hipLaunchKernelGGL((k_stage<ST, true, true, 0, false, true>), g, b, lds, c->stream, a);
+  else if (comp) hipLaunchKernelGGL((k_stage<ST, true, false, 0, false, true>), g, b, lds, c->stream, a);
+  else if (fm) hipLaunchKernelGGL((k_stage<ST, false, true, 0, false, true>), g, b, lds, c->stream, a);
+  else hipLaunchKernelGGL((k_stage<ST, false, false, 0, false, true>), g, b, lds, c->stream, a);
+}
+
 template <int ST>
 void launch_stage(inu_ctx* c, const Args& a, bool comp, bool fm, int tile) {
-  if (tile == 0) launch_stage_t<ST, 0>(c, a, comp, fm);
+  if (!(c->flags & INU_FLAG_DETERMINISTIC)) launch_stage_atom<ST>(c, a, comp, fm);
+  else if (tile == 0) launch_stage_t<ST, 0>(c, a, comp, fm);
   else if (tile == 1) launch_stage_t<ST, 1>(c, a, comp, fm);
   else launch_stage_t<ST, 2>(c, a, comp, fm);
 }
@@ -2037,6 +2105,9 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
     max_slots = std::max(max_slots, L.max_slots);
     max_ghost = std::max(max_ghost, L.max_ghost);
   }
+  // The edge words' LDS region also holds the unordered sums of a block's
+  // cells (one float4 each).
+  max_local = std::max(max_local, kBlockCells / 2);
   cell0[n_scen] = (uint32_t)n_tot;
   block0[n_scen] = (uint32_t)nb_tot;
   uint32_t n_open = 0;
@@ -2336,8 +2407,8 @@ int inu_set_params(inu_ctx* c, float g, float dry_tol, float cfl, uint32_t flags
   if (rc) return rc;
   if (!(g > 0.0f) || !(dry_tol > 0.0f) || !(cfl > 0.0f) || !std::isfinite(g) || !std::isfinite(dry_tol) || !std::isfinite(cfl))
     return fail(c, INU_EINVAL, "g, dry_tol and cfl must be finite and positive");
-  if (!(flags & INU_FLAG_DETERMINISTIC))
-    return fail(c, INU_EINVAL, "only deterministic (fixed-order gather) accumulation is implemented");
+  if (!(flags & INU_FLAG_DETERMINISTIC) && c->n_open)
+    return fail(c, INU_EINVAL, "open boundaries need deterministic accumulation");
   if ((flags ^ c->flags) & ~INU_FLAG_TRACK_CLAMP) {
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
     c->graphs.clear();
@@ -3096,6 +3167,8 @@ int inu_adj_begin(inu_ctx* c, uint32_t max_steps, uint32_t seg, uint32_t n_gauge
   if (rc) return rc;
   if (c->n_scen != 1) return fail(c, INU_EINVAL, "the adjoint records one scenario");
   if (c->n_open) return fail(c, INU_EINVAL, "the adjoint does not differentiate open boundaries");
+  if (!(c->flags & INU_FLAG_DETERMINISTIC))
+    return fail(c, INU_EINVAL, "the adjoint replays the fixed-order gather: set INU_FLAG_DETERMINISTIC");
   if (max_steps == 0 || seg == 0 || every == 0 || n_gauges == 0 || !gauges)
     return fail(c, INU_EINVAL, "inu_adj_begin: steps, segment, sampling interval and gauges must be given");
   if (max_steps > kKeyLenMask) return fail(c, INU_EINVAL, "inu_adj_begin: too many steps");

@@ -161,9 +161,10 @@ class SWESolverHipResident:
     # -- parameters ---------------------------------------------------------
 
     def _flags(self) -> int:
-        if not self.deterministic:
-            raise ValueError("hip_resident implements deterministic (fixed-order) accumulation only")
-        f = L.FLAG_DETERMINISTIC
+        """Kernel flags; ``deterministic = False`` sums edge words in arrival order (not bit-reproducible)."""
+        if self.n_open and not self.deterministic:
+            raise ValueError("open boundaries need deterministic accumulation")
+        f = L.FLAG_DETERMINISTIC if self.deterministic else 0
         if self.compensated:
             f |= L.FLAG_COMPENSATED
         if self.active_set:

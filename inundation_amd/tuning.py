@@ -20,6 +20,13 @@ DRY_TOL_DEFAULT = 1e-4
 # dt_i = CFL * A_i / sum_j(max|S_j| * len_j) over the cell's edges, with the wave
 # speeds the flux assumed (reference tuning.py:28-33).
 CFL_DEFAULT = 0.8
+# CFL number of the host-driven backends, which reuse one dt for cfl_interval
+# steps (reference tuning.py:34-36); their host loops stop below
+# CFL_EPSILON_MIN and ignore bounds above CFL_SANITY_MAX (an all-dry mesh)
+# (tuning.py:43-45).
+CFL_HOST_DRIVEN = 0.45
+CFL_EPSILON_MIN = 1e-10
+CFL_SANITY_MAX = 1e10
 # Below FILM_DEPTHS * dry_tol velocities are desingularised and film momentum
 # decays at a rate (reference tuning.py:37-39).
 FILM_DEPTHS = 4.0
@@ -47,10 +54,12 @@ ACTIVE_BLOCK_CELLS = 256
 DT_SENTINEL = 1e30
 
 # Backends whose run() returns hu/hv snapshots (reference tuning.py:57-61 plus
-# this backend).
-MOMENTUM_SNAPSHOT_BACKENDS = frozenset({"hip_resident"})
+# this backend). The reference's backend names run on the HIP kernels here:
+# gpu_resident_batch as hip_resident, baseline and fixed_dt_batch_barrier as
+# the host-driven loops.
+MOMENTUM_SNAPSHOT_BACKENDS = frozenset({"hip_resident", "gpu_resident_batch", "baseline"})
 # Backends that take open boundaries (reference tuning.py:61).
-OPEN_BOUNDARY_BACKENDS = frozenset({"hip_resident"})
+OPEN_BOUNDARY_BACKENDS = frozenset({"hip_resident", "gpu_resident_batch", "baseline"})
 
 # Open boundary conditions, the word per open edge (reference tuning.py:64-68).
 OPEN_WALL = 0
@@ -120,6 +129,9 @@ def output_window_stops(start_ticks: int, end_ticks: int, interval_ticks: int) -
 __all__ = [
     "ACTIVE_BLOCK_CELLS",
     "CFL_DEFAULT",
+    "CFL_EPSILON_MIN",
+    "CFL_HOST_DRIVEN",
+    "CFL_SANITY_MAX",
     "DRY_TOL_DEFAULT",
     "DT_SENTINEL",
     "FILM_DEPTHS",

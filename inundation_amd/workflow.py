@@ -6,9 +6,11 @@ Mirrors the reference's ``inundation/workflow.py``: the frozen
 area-proportional point-source split (``:317-358, 599-625``), initial-state
 loading (``:450-596``) and ``SWEWorkflow.prepare()/run()`` (``:658-969``).
 
-``solver_impl`` accepts the reference's names; ``"hip_resident"`` (the
-default here) is the MI355X backend. The Vulkan backends need Kompute, which
-this package does not ship, and ``prepare()`` says so. Open boundaries follow
+``solver_impl`` accepts the reference's names and runs each on the HIP
+kernels: ``"hip_resident"`` (the default) and ``"gpu_resident_batch"`` are
+the resident solver, ``"baseline"`` and ``"fixed_dt_batch_barrier"`` the
+reference's host-driven loops (``inundation_amd.hip.host_driven``), kept
+for A/B checks. Open boundaries follow
 ``:74-121, 258-289, 361-435``: named stretches picked by a polyline, a
 condition per stretch and phase, and the volume through each stretch at every
 snapshot. Geometry comes from ``mesh_source`` (OBJ or (Geo)Parquet, through
@@ -603,11 +605,6 @@ class SWEWorkflow:
     def prepare(self, geometry: MeshGeometry | None = None, perm: NDArray[np.int32] | None = None) -> None:
         """Geometry (given, or loaded as ``load_geometry`` does), initial state, solver."""
         cfg = self.config
-        if cfg.solver_impl != "hip_resident":
-            raise RuntimeError(
-                f"solver_impl {cfg.solver_impl!r} is a Vulkan/Kompute backend of the reference; "
-                "this package provides 'hip_resident'"
-            )
         if geometry is None:
             geometry, perm = load_geometry(cfg)
         geom = geometry
@@ -630,32 +627,36 @@ class SWEWorkflow:
             h0, hu0, hv0, n_loaded = loaded
             if n_loaded is not None:
                 n0 = n_loaded
+        from .hip.host_driven import SWESolverHipHostDriven
         from .hip.solver import SWESolverHipResident
 
         self._open_stretches = open_stretch_edges(geom, cfg.open_boundaries)
         self._manning = np.asarray(n0, dtype=np.float32)
-        solver = SWESolverHipResident(
-            geom,
-            None,
-            h0,
-            hu0,
-            hv0,
-            n0,
-            g=cfg.gravity,
-            dry_tol=cfg.dry_tol,
-            cfl=cfg.cfl,
-            workgroup_size=cfg.workgroup_size,
-            open_edges=self._open_stretches,
-            device=cfg.device,
-        )
+        args = (geom, None, h0, hu0, hv0, n0)
+        kwargs: dict[str, Any] = {
+            "g": cfg.gravity,
+            "dry_tol": cfg.dry_tol,
+            "cfl": cfg.cfl,
+            "workgroup_size": cfg.workgroup_size,
+            "open_edges": self._open_stretches,
+            "device": cfg.device,
+        }
+        resident = cfg.solver_impl in ("hip_resident", "gpu_resident_batch")
+        if resident:
+            solver: Any = SWESolverHipResident(*args, **kwargs)
+        else:
+            solver = SWESolverHipHostDriven(*args, mode=cfg.solver_impl, **kwargs)
         if cfg.track_clamp_mass:
             solver.track_clamp = True
-        if cfg.steps_per_graph is not None:
-            solver.steps_per_graph = cfg.steps_per_graph
-        if cfg.deterministic is not None:
-            solver.deterministic = cfg.deterministic
-        if cfg.compensated is not None:
-            solver.compensated = cfg.compensated
+        # The resident backend takes the run-recording attributes
+        # (reference workflow.py:816-826).
+        if resident:
+            if cfg.steps_per_graph is not None:
+                solver.steps_per_graph = cfg.steps_per_graph
+            if cfg.deterministic is not None:
+                solver.deterministic = cfg.deterministic
+            if cfg.compensated is not None:
+                solver.compensated = cfg.compensated
         self.geom = geom
         self.perm = perm
         self.solver = solver
@@ -696,7 +697,7 @@ class SWEWorkflow:
                 resume=k > 0,
                 t_start=t,
                 cfl_interval=cfg.cfl_interval,
-                output_every_steps=None,
+                output_every_steps=cfg.output_every_steps if cfg.solver_impl == "fixed_dt_batch_barrier" else None,
                 capture_momentum_snapshots=cfg.capture_momentum_snapshots,
             )
             if len(res) == 4:
