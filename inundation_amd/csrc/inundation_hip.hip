@@ -573,6 +573,9 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // the one over the larger lam (the same rounded numerator), so one
   // division takes both.
   float lam_b = 0.0f;
+  // The corrector's fast order takes the source dt bound of all-wet blocks
+  // from the cells' perimeters.
+  constexpr bool PERIM = ST == 1 && FM && !ATOM;
   if (tid < nc) {
     float sdh = 0.0f, sdhu = 0.0f, sdhv = 0.0f, lam = 0.0f;
     if (ATOM) {
@@ -690,9 +693,23 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     }
   }
   if (ST == 1) {
-    const int any_wet = __syncthreads_or(wet ? 1 : 0);
-    if (tid == 0) a.wet[b] = any_wet ? 1u : 0u;
-    if (a.n_src != 0u) {
+    const int n_wet = __syncthreads_count(wet ? 1 : 0);
+    if (tid == 0) a.wet[b] = n_wet != 0 ? 1u : 0u;
+    if (PERIM && a.n_src != 0u && (uint32_t)n_wet == nc) {
+      // Every cell of the block is wet after the sources, so no neighbour in
+      // it is dry and each edge term of the bound below is (u + c) len: the
+      // sum is (u + c) times the perimeter. A dry neighbour in another block
+      // is k_source_fix's, as below. The perimeter is the cell's edge lengths
+      // summed in CSR order.
+      if (tid < nc && rate != 0.0f) {
+        float perim = 0.0f;
+        for (uint32_t k = k0; k < k1; ++k) perim += l.edge[2 * (l.s_slot[k] >> 1) + 1].w;
+        const float u = sqrtf(fmaf(hui, hui, hvi * hvi)) / hi;
+        const float c = p.sqrt_g * sqrtf(hi);
+        const float lam = (u + c) * perim;
+        if (lam > 1e-30f) lam_b = inu::gmax(lam_b, lam);
+      }
+    } else if (a.n_src != 0u) {
       // Next-step dt bound of the block's wetted source cells
       // (resident.py:709-728) from the post-source depths, with every
       // neighbour in another block taken as wet; k_source_fix recomputes the
@@ -733,7 +750,7 @@ __device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t en
 }
 
 template <int ST, bool COMP, bool FM, int TILE, bool REC = false, bool ATOM = false>
-__global__ __launch_bounds__(256) void k_stage(const Args a) {
+__global__ __launch_bounds__(256, TILE == 0 ? 5 : 1) void k_stage(const Args a) {
   extern __shared__ float sm[];
   if (a.ctl->any_dt == 0u) return;  // no-op step: every scenario is at its stop
   const Lds l = carve(sm, a.max_local, a.max_ghost);
@@ -828,12 +845,16 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
     __syncthreads();  // the previous block's s_h reads are done
     s_h[threadIdx.x] = hn;
     if (__syncthreads_or(wet ? 1 : 0) && threadIdx.x == 0) a.wet[b] = 1u;
+    // Every cell of the block wet after the sources: the fast order takes
+    // (u + c) times the perimeter, as the corrector does.
+    const bool all_wet =
+        __syncthreads_count((threadIdx.x >= bi_cells(bi) || hn >= p.dry) ? 1 : 0) == (int)blockDim.x;
     if (wet) {
       const float hui = a.hu[i];
       const float hvi = a.hv[i];
       const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
       const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
-      float lam = 0.0f;
+      float lam = 0.0f, perim = 0.0f;
       const uint32_t s1 = a.cptr[i + 1];
       for (uint32_t s = a.cptr[i]; s < s1; ++s) {
         const uint32_t q = (uint32_t)a.snb[s] - c0;  // walls and other blocks fall outside
@@ -841,7 +862,9 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
         const float k = dry_side ? 2.0f : 1.0f;
         if (FM) lam = fmaf(fmaf(k, c, u), a.slen[s], lam);
         else lam += (u + k * c) * a.slen[s];
+        perim += a.slen[s];
       }
+      if (FM && all_wet) lam = (u + c) * perim;
       const float A = a.area[i];
       if (lam > 1e-30f && A > 1e-10f) {
         const uint32_t bb = __float_as_uint(p.cfl * A / lam);

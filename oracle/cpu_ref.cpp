@@ -749,7 +749,13 @@ void source(cref& r) {
     r.h[i] = h_new;
     if (h_new >= pc.dry) __atomic_store_n(&r.blk[3 * (i >> 8) + 0], 1u, __ATOMIC_RELAXED);
   }
-  // ... then the post-source dt bound of each wet source cell.
+  // ... then the post-source dt bound of each wet source cell. The fast order
+  // takes (u + c) times the cell's perimeter (its edge lengths summed in CSR
+  // order) where the cell's 256-cell block is wet throughout after the
+  // sources and no neighbour is dry: the same sum, distributed.
+  std::vector<uint8_t> all_wet(r.NB, 1u);
+  for (uint32_t i = 0; i < r.N; ++i)
+    if (!(r.h[i] >= pc.dry)) all_wet[i >> 8] = 0u;
   uint32_t best = 0xffffffffu;
 #pragma omp parallel for schedule(static) reduction(min : best)
   for (int64_t k = 0; k < n; ++k) {
@@ -760,16 +766,20 @@ void source(cref& r) {
     float u = F ? std::sqrt(std::fma(r.hu[i], r.hu[i], r.hv[i] * r.hv[i])) / h_new
                 : std::sqrt(r.hu[i] * r.hu[i] + r.hv[i] * r.hv[i]) / h_new;
     float c = F ? std::sqrt(pc.g) * std::sqrt(h_new) : std::sqrt(pc.g * h_new);
-    float lam = 0.0f;
+    float lam = 0.0f, perim = 0.0f;
+    bool any_dry = false;
     for (uint32_t kk = r.cptr[i]; kk < r.cptr[i + 1]; ++kk) {
       uint32_t s = r.cidx[kk];
       uint32_t e = s >> 1;
       int nb = ((s & 1u) != 0u) ? r.ecL[e] : r.ecR[e];
       bool dry_side = nb >= 0 && r.h[nb] < pc.dry;
+      any_dry = any_dry || dry_side;
       float kf = dry_side ? 2.0f : 1.0f;
       if (F) lam = std::fma(std::fma(kf, c, u), r.elen[e], lam);
       else lam += (u + kf * c) * r.elen[e];
+      perim += r.elen[e];
     }
+    if (F && all_wet[i >> 8] && !any_dry) lam = (u + c) * perim;
     if (lam > 1e-30f && r.area[i] > 1e-10f) best = std::min(best, fbits(pc.cfl * r.area[i] / lam));
   }
   r.dtbuf[1] = std::min(r.dtbuf[1], best);

@@ -201,6 +201,28 @@ def test_sparse_rain_dt_bound_is_bitwise_the_fp32_oracle(ctx, mesh, grid, order,
         D.assert_same(D.snapshot(gpu, n, nb), D.snapshot(ref, n, nb), f"block {k}")
 
 
+@pytest.mark.parametrize("order", [0, EXACT], ids=["fast", "exact"])
+def test_rain_on_a_wet_lake_is_bitwise_the_fp32_oracle(ctx, mesh, grid, order) -> None:  # noqa: ANN001
+    """Rain on every cell of a moving lake: every block is wet after the
+    sources, the case whose source dt bound the fast order takes from each
+    cell's perimeter; the clock must still follow the oracle bit for bit."""
+    gpu, ref = _both(ctx, mesh)
+    n = grid.N
+    nb = num_blocks(n, 256)
+    h, hu, hv = f32_state(*CASES["wet_lake"](grid))
+    cells = np.arange(n, dtype=np.uint32)
+    rates = np.full(n, 2e-3, dtype=np.float32)
+    rates[::7] = 0.0  # some cells without a source
+    for be in (gpu, ref):
+        D.setup(be, n, h, hu, hv, MANNING, dry_tol=DRY_TOL, src_cells=cells, src_rates=rates,
+                flags=D.FLAGS_DEFAULT | order)
+        D.start(be, 3.0, 0.5, 0.05)
+    for k in range(4):
+        gpu.steps(30, use_graph=True)
+        ref.steps(30)
+        D.assert_same(D.snapshot(gpu, n, nb), D.snapshot(ref, n, nb), f"block {k}")
+
+
 def test_active_set_on_equals_off(ctx, mesh, grid) -> None:  # noqa: ANN001
     """Skipping blocks is exact (reference test_gpu_active_set.py:77-91)."""
     gpu, _ = _both(ctx, mesh)
