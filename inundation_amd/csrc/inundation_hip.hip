@@ -211,7 +211,8 @@ __device__ __forceinline__ void stg(T* base, uint32_t idx, T v) {
 // block's CSR slots (local edge numbers).
 struct Lds {
   float4* cell;  // [kBlockCells + max_ghost][2]: the cells, then the ghosts
-  float4* edge;  // [max_local][2]
+  float4* edge;  // [max_local][2]; the predictor's: [max_local] words 0-3
+  float2* e2;    // the predictor's [max_local] words 4-5 (carve_stage<0>)
   uint16_t* s_slot;
 };
 
@@ -238,8 +239,37 @@ __device__ __forceinline__ Lds carve(float* sm, uint32_t max_local, uint32_t max
   Lds l;
   l.cell = reinterpret_cast<float4*>(sm);
   l.edge = reinterpret_cast<float4*>(sm + 8 * (kBlockCells + max_ghost));
+  l.e2 = reinterpret_cast<float2*>(l.edge) + 2;
   l.s_slot = reinterpret_cast<uint16_t*>(sm + 8 * (kBlockCells + max_ghost) + 8 * max_local);
   return l;
+}
+
+// The predictor reads only an edge's six words, so its records are two
+// arrays (16 + 8 bytes per edge instead of 32): with less LDS per block,
+// one more workgroup fits on a CU.
+__host__ __device__ inline size_t lds_words_stage0(uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
+  return 8 * (size_t)(kBlockCells + max_ghost) + 6 * (size_t)max_local + (max_slots + 1) / 2;
+}
+
+template <int ST>
+__device__ __forceinline__ Lds carve_stage(float* sm, uint32_t max_local, uint32_t max_ghost) {
+  if (ST != 0) return carve(sm, max_local, max_ghost);
+  Lds l;
+  l.cell = reinterpret_cast<float4*>(sm);
+  l.edge = reinterpret_cast<float4*>(sm + 8 * (kBlockCells + max_ghost));
+  l.e2 = reinterpret_cast<float2*>(sm + 8 * (kBlockCells + max_ghost) + 4 * max_local);
+  l.s_slot = reinterpret_cast<uint16_t*>(sm + 8 * (kBlockCells + max_ghost) + 6 * max_local);
+  return l;
+}
+
+// Words 0-3 and 4-5 of local edge j in a stage kernel's layout.
+template <int ST>
+__device__ __forceinline__ float4& ew4(const Lds& l, uint32_t j) {
+  return ST == 0 ? l.edge[j] : l.edge[2 * j];
+}
+template <int ST>
+__device__ __forceinline__ float2& ew2(const Lds& l, uint32_t j) {
+  return ST == 0 ? l.e2[j] : l.e2[4 * j];
 }
 
 // One local edge record: sides word, normal, length and the signed bed step
@@ -375,9 +405,8 @@ __device__ __forceinline__ void open_pass(const Args& a, const Lds& l, uint32_t 
     const uint2 jk = a.open->jk[t];
     const float4* o = a.open->words + 2 * ((size_t)ST * a.n_open + jk.y);
     const float4 w0 = o[0], w1 = o[1];
-    l.edge[2 * jk.x] = w0;
-    l.edge[2 * jk.x + 1].x = w1.x;
-    l.edge[2 * jk.x + 1].y = w1.y;
+    ew4<ST>(l, jk.x) = w0;
+    ew2<ST>(l, jk.x) = make_float2(w1.x, w1.y);
   }
 }
 
@@ -419,8 +448,9 @@ template <int ST, bool FM, bool LO>
 __device__ __forceinline__ void edge_eval(const Args& a, const Lds& l, const EdgeRec& r, uint32_t j) {
   float w[6];
   edge_words_lds<FM, LO>(a, l, r, w);
-  l.edge[2 * j] = make_float4(w[0], w[1], w[2], w[3]);
-  l.edge[2 * j + 1] = make_float4(w[4], w[5], __uint_as_float(r.side), r.len);
+  ew4<ST>(l, j) = make_float4(w[0], w[1], w[2], w[3]);
+  if (ST == 0) ew2<ST>(l, j) = make_float2(w[4], w[5]);
+  else l.edge[2 * j + 1] = make_float4(w[4], w[5], __uint_as_float(r.side), r.len);  // sides, length: source bound
 }
 
 // Unordered accumulation (the reference's atomic flux variant,
@@ -588,8 +618,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     for (uint32_t k = k0; !ATOM && k < k1; ++k) {
       const uint32_t s = l.s_slot[k];
       const uint32_t j = s >> 1;
-      const float4 wa = l.edge[2 * j];
-      const float2 wb = *reinterpret_cast<const float2*>(&l.edge[2 * j + 1]);
+      const float4 wa = ew4<ST>(l, j);
+      const float2 wb = ew2<ST>(l, j);
       if (s & 1u) {
         sdh -= wa.x;
         sdhu -= wa.w;
@@ -750,10 +780,10 @@ __device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t en
 }
 
 template <int ST, bool COMP, bool FM, int TILE, bool REC = false, bool ATOM = false>
-__global__ __launch_bounds__(256, TILE == 0 ? 5 : 1) void k_stage(const Args a) {
+__global__ __launch_bounds__(256, TILE == 0 ? (ST == 0 ? 6 : 5) : 1) void k_stage(const Args a) {
   extern __shared__ float sm[];
   if (a.ctl->any_dt == 0u) return;  // no-op step: every scenario is at its stop
-  const Lds l = carve(sm, a.max_local, a.max_ghost);
+  const Lds l = carve_stage<ST>(sm, a.max_local, a.max_ghost);
   // XCD-aware split: workgroup w runs on XCD w % 8, so each XCD walks one
   // contiguous (spatially compact) slice of the block list and its L2 keeps
   // the ghost cells its neighbours read.
@@ -2128,9 +2158,9 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
     max_slots = std::max(max_slots, L.max_slots);
     max_ghost = std::max(max_ghost, L.max_ghost);
   }
-  // The edge words' LDS region also holds the unordered sums of a block's
-  // cells (one float4 each).
-  max_local = std::max(max_local, kBlockCells / 2);
+  // The edge words' LDS region (16 bytes per edge in the predictor) also
+  // holds the unordered sums of a block's cells (one float4 each).
+  max_local = std::max(max_local, kBlockCells);
   cell0[n_scen] = (uint32_t)n_tot;
   block0[n_scen] = (uint32_t)nb_tot;
   uint32_t n_open = 0;
@@ -2319,8 +2349,8 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   c->own_pos.swap(own_pos);
   c->h_ldz.swap(ldz);
   c->src_nonneg = true;
-  c->lds_stage0 = lds_words(max_local, max_slots, max_ghost) * sizeof(float);
-  c->lds_stage1 = c->lds_stage0;
+  c->lds_stage0 = lds_words_stage0(max_local, max_slots, max_ghost) * sizeof(float);
+  c->lds_stage1 = lds_words(max_local, max_slots, max_ghost) * sizeof(float);
   if (c->lds_stage1 > 160 * 1024) return fail(c, INU_ERANGE, "a block's edge list does not fit in LDS");
   // Persistent grids: as many workgroups as fit at once, a multiple of the
   // XCD count, per stage kernel variant.
