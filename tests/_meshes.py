@@ -44,6 +44,16 @@ def channel_grid(
     return hilbert_reorder(geom, verbose=False)[0]
 
 
+def step_shore() -> tuple[MeshGeometry, np.ndarray]:
+    """A 32 x 32 basin of unit cells in row order, half a 1 m deep pool at
+    rest and half a bank 2 m high: 256-cell blocks 0 and 1 are wet, 2 and 3
+    dry, and the last row of block 1 borders the dry bank."""
+    verts, faces = quad_grid(32, 32, 1.0)
+    cy = verts[faces].mean(axis=1)[:, 1]
+    g = build_geometry(verts, faces, zb_from_file=np.where(cy < 16, 0.0, 2.0).astype(np.float32))
+    return g, np.where(g.zb == 0.0, 1.0, 0.0).astype(np.float32)
+
+
 def channel_end(x: float, width: float = 200.0, dx: float = 10.0) -> tuple[list[tuple[float, float]], float]:
     """The line across the channel at ``x`` and its pick tolerance (a quarter cell)."""
     return [(x, 0.0), (x, width)], 0.25 * dx

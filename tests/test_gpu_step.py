@@ -21,7 +21,7 @@ from inundation_amd.hip.solver import device_mesh_arrays
 from inundation_amd.mesh.blocks import num_blocks
 from oracle import cpu_ref, ref64
 from tests import _drive as D
-from tests._meshes import CASES, DRY_TOL, DT, MANNING, f32_state, valley_grid
+from tests._meshes import CASES, DRY_TOL, DT, MANNING, f32_state, step_shore, valley_grid
 from tests.conftest import gpu_context
 
 pytestmark = pytest.mark.gpu
@@ -221,6 +221,27 @@ def test_rain_on_a_wet_lake_is_bitwise_the_fp32_oracle(ctx, mesh, grid, order) -
         gpu.steps(30, use_graph=True)
         ref.steps(30)
         D.assert_same(D.snapshot(gpu, n, nb), D.snapshot(ref, n, nb), f"block {k}")
+
+
+@pytest.mark.parametrize("order", [0, EXACT], ids=["fast", "exact"])
+def test_rain_beside_a_dry_bank_is_bitwise_the_fp32_oracle(ctx, order) -> None:  # noqa: ANN001
+    """A block wet in every cell whose last row borders a dry block: its shore
+    cells must take 2c across the dry edge, not the all-wet perimeter form."""
+    g, h = step_shore()
+    m = device_mesh_arrays(g)
+    gpu, ref = _both(ctx, m)
+    z = np.zeros(g.N, np.float32)
+    cells = np.arange(g.N, dtype=np.uint32)
+    rates = np.full(g.N, 2e-3, dtype=np.float32)
+    nb = num_blocks(g.N, 256)
+    for be in (gpu, ref):
+        D.setup(be, g.N, h, z, z, MANNING, dry_tol=DRY_TOL, src_cells=cells, src_rates=rates,
+                flags=D.FLAGS_DEFAULT | order)
+        D.start(be, 2.0, 0.5, 0.05)
+    for k in range(3):
+        gpu.steps(10, use_graph=True)
+        ref.steps(10)
+        D.assert_same(D.snapshot(gpu, g.N, nb), D.snapshot(ref, g.N, nb), f"block {k}")
 
 
 def test_active_set_on_equals_off(ctx, mesh, grid) -> None:  # noqa: ANN001

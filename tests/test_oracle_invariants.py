@@ -27,9 +27,9 @@ from inundation_amd.hip.solver import device_mesh_arrays
 from inundation_amd.mesh.geometry import build_geometry, edge_bed_steps
 from inundation_amd.mesh.synth import quad_grid
 from inundation_amd.workloads import lake_at_rest
-from oracle import cpu_ref
+from oracle import cpu_ref, ref64
 from tests import _drive as D
-from tests._meshes import CASES, f32_state, valley_grid
+from tests._meshes import CASES, f32_state, step_shore, valley_grid
 
 EXACT = 16
 
@@ -130,3 +130,56 @@ def test_dry_cells_start_and_stay_without_momentum() -> None:
     _, s = _run(g, h, hu, hv, 0.03, 8, dry_tol=1e-3, dt_max=2.0**-10, dt_init=2.0**-10, stop=8 * 2.0**-10)
     dry = s["h"] < 1e-3
     assert not s["hu"][dry].any() and not s["hv"][dry].any()
+
+
+def _first_step_bounds(g, h, hu, hv, rates: np.ndarray, order: int) -> tuple[float, float, float]:  # noqa: ANN001
+    """The FP32 oracle's next-step CFL bound after one step with rain, and the
+    FP64 oracle's bound after the same step with and without the sources."""
+    n = g.N
+    cells = np.flatnonzero(rates).astype(np.uint32)
+    ref, _ = _run(g, h, hu, hv, 0.03, 0, flags=D.FLAGS_DEFAULT | order, src=(cells, rates[cells]),
+                  dry_tol=1e-3, dt_max=0.5, dt_init=0.05)
+    dt = int(ref.ctrl_read().dt_ticks) / 2**20
+    ref.steps(1)
+    got = float(ref.ctrl_read().last_raw)
+    m64 = ref64.Mesh64.from_device_arrays(device_mesh_arrays(g))
+    s0 = ref64.State64(*(a.astype(np.float64) for a in (h, hu, hv)))
+    man = np.full(n, 0.03)
+    _, with_src = ref64.heun_step(m64, s0, man, dt, 9.81, 1e-3, 0.8, rates.astype(np.float64))
+    _, no_src = ref64.heun_step(m64, s0, man, dt, 9.81, 1e-3, 0.8)
+    return got, with_src, no_src
+
+
+@pytest.mark.parametrize("order", [0, EXACT], ids=["fast", "exact"])
+def test_rain_on_a_wet_lake_source_bound_is_the_fp64_one(order: int) -> None:
+    """Rain on every cell of a moving lake: each block is wet after the
+    sources, where the fast order sums the bound over the cell perimeter
+    (resident.py:709-728 sums (|u| + c) len edge by edge). The source bound
+    is the binding one here, and dropping its |u| term moves it by more than
+    the tolerance, so a wrong perimeter or a lost term fails."""
+    g = valley_grid(64, 32, 2.0)
+    h, hu, hv = f32_state(*CASES["wet_lake"](g))
+    rates = np.full(g.N, 2e-3, dtype=np.float32)
+    rates[::7] = 0.0
+    got, with_src, no_src = _first_step_bounds(g, h, hu, hv, rates, order)
+    assert with_src < no_src * (1.0 - 1e-4)  # the sources set the bound
+    assert got == pytest.approx(with_src, rel=1e-5)
+    # |u| is up to 0.1 m/s against c of about 2.5 m/s: a bound without it is
+    # several 1e-3 larger.
+    i = np.flatnonzero(rates)
+    perim = np.add.reduceat(g.edge_len[g.cell_edge_idx].astype(np.float64), g.cell_edge_ptr[:-1])
+    c_only = float(np.min(0.8 * g.area[i] / (np.sqrt(9.81 * h[i].astype(np.float64)) * perim[i])))
+    assert abs(c_only - got) / got > 1e-3
+
+
+@pytest.mark.parametrize("order", [0, EXACT], ids=["fast", "exact"])
+def test_rain_beside_a_dry_bank_source_bound_is_the_fp64_one(order: int) -> None:
+    """A block wet in every cell whose cells border a dry block: the shore
+    cells take 2c across the dry edge (resident.py:709-728), which sets the
+    bound at 4/5 of the edge-free one. The perimeter form must not apply."""
+    g, h = step_shore()
+    z = np.zeros(g.N, np.float32)
+    rates = np.full(g.N, 2e-3, dtype=np.float32)
+    got, with_src, no_src = _first_step_bounds(g, h, z, z, rates, order)
+    assert with_src == pytest.approx(0.8 * no_src, rel=1e-3)
+    assert got == pytest.approx(with_src, rel=1e-5)
