@@ -1,7 +1,7 @@
 // MI355X (gfx950) shallow-water step backend: kernels, device runtime, C ABI.
 //
 // One context owns the device copy of one mesh, its state, one HIP stream and
-// the hipGraphs of recorded step blocks. A Heun step is six kernels in stream
+// the hipGraphs of recorded step blocks. A Heun step is five kernels in stream
 // order (no host round trip):
 //
 //   k_stage<0>      predictor: per active 256-cell block, every edge touching
@@ -10,10 +10,10 @@
 //   k_stage<1>      corrector: same on the predictor state, Heun average,
 //                   the block's sources, wet mark, CFL min for the next step;
 //   k_source_idle   sources of blocks the active set skips this step;
-//   k_source_fix    the exact dt bound of wetted source cells beside a dry
-//                   cell of another block;
-//   k_dilate        active = wet over each block's two-ring, plus the list of
-//                   blocks the next step visits (wave-ballot compaction);
+//   k_step_tail     in one launch, the dilation (active = wet over each
+//                   block's two-ring, plus the list of blocks the next step
+//                   visits) and the source fix (the exact dt bound of wetted
+//                   source cells beside a dry cell of another block);
 //   k_step_control  the uint64 tick clock and the next dt.
 //
 // The fused stage kernel replaces the reference's flux kernel + gather update
@@ -70,9 +70,8 @@ struct Ctl {
   uint32_t any_dt;     // 1 when some scenario has a nonzero dt this step
   uint32_t pad[2];
   // Work of the active set, for accounting: {cells, owned edges, CSR slots}
-  // of the active blocks of the next step, of the current step, and summed
-  // over the real steps taken.
-  unsigned long long act_next[3];
+  // of the active blocks of the current step, and summed over the real steps
+  // taken (the next step's is in Args::act_part).
   unsigned long long act_cur[3];
   unsigned long long act_total[3];
   unsigned long long act_steps;
@@ -96,6 +95,7 @@ __host__ __device__ __forceinline__ uint32_t bi_scen(const BlockInfo& bi) { retu
 struct Args {
   Phys p;
   uint32_t N, NB, max_local, max_slots, max_ghost, n_src, track_clamp;
+  uint32_t src_neg;  // 1 when a source rate may be negative
   // cells
   float *h, *hu, *hv, *hlo, *h1, *hu1, *hv1, *h1lo, *area, *nman, *clamp, *src;
   const uint32_t* cptr;
@@ -110,12 +110,24 @@ struct Args {
   const uint32_t* gx;
   // blocks
   uint32_t *wet, *active, *forced, *has_src, *list, *idle;
+  // 1 when a cell of the block may be below dry_tol after the last step's
+  // sources, 0 when every cell is at or above it
+  uint32_t* part_dry;
   const uint32_t *bptr, *bidx;
+  // k_dilate's {cells, owned edges, CSR slots} of the next step's active
+  // blocks, one triple per k_dilate workgroup (n_dwg of them)
+  unsigned long long* act_part;
+  uint32_t n_dwg;
   // sources
   const uint32_t* src_cell;
   const uint32_t* src_blocks;
-  const uint2* src_pair;  // (source cell, its neighbour in another block)
-  uint32_t n_src_blocks, n_src_pairs;
+  // (source cell, its neighbour in another block), grouped by the
+  // neighbour's block: group f holds [fix_ptr[f], fix_ptr[f + 1]), whose
+  // neighbours lie in block fix_blk[f]
+  const uint2* src_pair;
+  const uint32_t* fix_blk;
+  const uint32_t* fix_ptr;
+  uint32_t n_src_blocks, n_src_pairs, n_fix;
   Ctl* ctl;
   Scen* scen;
   uint32_t n_scen;
@@ -724,8 +736,17 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   }
   if (ST == 1) {
     const int n_wet = __syncthreads_count(wet ? 1 : 0);
-    if (tid == 0) a.wet[b] = n_wet != 0 ? 1u : 0u;
-    if (PERIM && a.n_src != 0u && (uint32_t)n_wet == nc) {
+    // Every cell at or above dry_tol after the sources. Sources that only
+    // add water keep a wet cell wet (the compensated sum of a non-negative
+    // increment never rounds below the pair's high word), so that is n_wet
+    // == nc; a negative rate can dry a cell and needs its own count.
+    const bool all_wet = a.src_neg ? __syncthreads_or((tid < nc && !(hi >= p.dry)) ? 1 : 0) == 0
+                                   : (uint32_t)n_wet == nc;
+    if (tid == 0) {
+      a.wet[b] = n_wet != 0 ? 1u : 0u;
+      a.part_dry[b] = all_wet ? 0u : 1u;
+    }
+    if (PERIM && a.n_src != 0u && all_wet) {
       // Every cell of the block is wet after the sources, so no neighbour in
       // it is dry and each edge term of the bound below is (u + c) len: the
       // sum is (u + c) times the perimeter. A dry neighbour in another block
@@ -879,6 +900,7 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
     // (u + c) times the perimeter, as the corrector does.
     const bool all_wet =
         __syncthreads_count((threadIdx.x >= bi_cells(bi) || hn >= p.dry) ? 1 : 0) == (int)blockDim.x;
+    if (threadIdx.x == 0) a.part_dry[b] = all_wet ? 0u : 1u;
     if (wet) {
       const float hui = a.hu[i];
       const float hvi = a.hv[i];
@@ -907,43 +929,53 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
 
 // Exact dt bound (reference resident.py:709-728) of the wetted source cells
 // with a neighbour in another block that is dry after every source of the
-// step: the corrector and k_source_idle counted that neighbour as wet. One
-// thread per (cell, neighbour) pair; the bound with the dry side is never
-// above the one taken before (every term and rounding is monotone), so the
-// minimum over both equals the exact one.
+// step: the corrector and k_source_idle counted that neighbour as wet. The
+// bound with the dry side is never above the one taken before (every term
+// and rounding is monotone), so the minimum over both equals the exact one.
+// The pairs come in groups whose neighbours share a block; a group whose
+// block is wet throughout (part_dry 0) has nothing to fix, which is most of
+// them once the flood has spread. Each wave takes kFixGroups consecutive
+// groups: one flag per lane decides whether the wave has any work, and if
+// so its lanes walk the groups' pairs. Runs inside k_step_tail.
+constexpr uint32_t kFixGroups = 16;                     // pair groups per wave
+constexpr uint32_t kFixGroupsWg = kFixGroups * 4u;      // per workgroup
+
 template <bool FM>
-__global__ __launch_bounds__(256) void k_source_fix(const Args a) {
-  if (a.ctl->any_dt == 0u) return;
+__device__ __forceinline__ void source_fix_wg(const Args& a, uint32_t wg) {
   const Phys& p = a.p;
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < a.n_src_pairs) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t g0 = wg * kFixGroupsWg + (threadIdx.x >> 6) * kFixGroups;
+  if (g0 >= a.n_fix) return;
+  const uint32_t g1 = min(g0 + kFixGroups, a.n_fix);
+  const bool work = lane < g1 - g0 && a.part_dry[a.fix_blk[g0 + lane]] != 0u;
+  if (__ballot(work ? 1 : 0) == 0ull) return;
+  const uint32_t k1 = a.fix_ptr[g1];
+  for (uint32_t k = a.fix_ptr[g0] + lane; k < k1; k += 64u) {
     const uint2 pr = a.src_pair[k];
-    if (a.h[pr.y] < p.dry) {
-      const uint32_t i = pr.x;
-      const float hn = a.h[i];
-      if (hn >= p.dry && a.src[i] != 0.0f) {  // a wetted cell whose source is on
-        const float hui = a.hu[i];
-        const float hvi = a.hv[i];
-        const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
-        const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
-        float lam = 0.0f;
-        const uint32_t s1 = a.cptr[i + 1];
-        for (uint32_t s = a.cptr[i]; s < s1; ++s) {
-          const int nb = a.snb[s];
-          const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
-          const float kf = dry_side ? 2.0f : 1.0f;
-          if (FM) lam = fmaf(fmaf(kf, c, u), a.slen[s], lam);
-          else lam += (u + kf * c) * a.slen[s];
-        }
-        const float A = a.area[i];
-        if (lam > 1e-30f && A > 1e-10f) {
-          // Rare: one atomic per such cell, into its scenario's slot. A
-          // scenario at its stop ignores the slot (k_step_control).
-          const uint32_t bb = __float_as_uint(p.cfl * A / lam);
-          uint32_t* slot = &a.scen[bi_scen(a.binfo[i >> 8])].dt_next;
-          if (bb < __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMin(slot, bb);
-        }
-      }
+    if (a.part_dry[pr.y / kBlockCells] == 0u || !(a.h[pr.y] < p.dry)) continue;
+    const uint32_t i = pr.x;
+    const float hn = a.h[i];
+    if (!(hn >= p.dry) || a.src[i] == 0.0f) continue;  // only wetted cells whose source is on
+    const float hui = a.hu[i];
+    const float hvi = a.hv[i];
+    const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
+    const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
+    float lam = 0.0f;
+    const uint32_t s1 = a.cptr[i + 1];
+    for (uint32_t s = a.cptr[i]; s < s1; ++s) {
+      const int nb = a.snb[s];
+      const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
+      const float kf = dry_side ? 2.0f : 1.0f;
+      if (FM) lam = fmaf(fmaf(kf, c, u), a.slen[s], lam);
+      else lam += (u + kf * c) * a.slen[s];
+    }
+    const float A = a.area[i];
+    if (lam > 1e-30f && A > 1e-10f) {
+      // Rare: one atomic per such pair, into its scenario's slot. A scenario
+      // at its stop ignores the slot (k_step_control).
+      const uint32_t bb = __float_as_uint(p.cfl * A / lam);
+      uint32_t* slot = &a.scen[bi_scen(a.binfo[i >> 8])].dt_next;
+      if (bb < __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMin(slot, bb);
     }
   }
 }
@@ -961,7 +993,6 @@ __global__ void k_block_mark(const Args a, int all_active) {
   if (b == 0) {
     a.ctl->cnt_next = 0u;
     a.ctl->idle_next = 0u;
-    for (int k = 0; k < 3; ++k) a.ctl->act_next[k] = 0ull;
   }
   if (b >= a.NB) return;
   if (all_active) {
@@ -995,73 +1026,136 @@ __device__ __forceinline__ void wave_append(bool take, uint32_t v, uint32_t* cnt
 
 // active = (wet | forced) over the block's two-ring (reference
 // resident.py:778-805). Active blocks are appended to `list`, inactive blocks
-// holding a source to `idle`, with one atomic per wave and list, in block
-// order within it. Blocks of a scenario that took a no-op step keep their
-// flags and are left off both lists.
-__global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int keep_flags) {
-  if (!force && a.ctl->any_dt == 0u) return;  // no-op step changed nothing
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  bool take = false, take_idle = false;
-  uint32_t w_cells = 0, w_edges = 0, w_slots = 0;
-  if (b < a.NB) {
-    const BlockInfo bi = a.binfo[b];
-    // A scenario at its stop took a no-op step: its flags stand.
-    const bool frozen = !force && a.scen[bi_scen(bi)].dt_cur == 0u;
-    uint32_t v;
-    if (keep_flags || frozen) {
-      v = a.active[b];
+// holding a source to `idle`, in block order within each workgroup's
+// kDilateBlocks blocks, with one atomic per workgroup and list (the
+// counters are single words that every workgroup of the launch hits). The
+// workgroup's {cells, owned edges, CSR slots} of the next step's active
+// blocks go to act_part without atomics; k_step_control sums them. Blocks
+// of a scenario that took a no-op step keep their flags and are left off
+// both lists. The loads of a thread's blocks are issued together, and the
+// two-ring is read eight neighbours at a time: the kernel is a chain of
+// dependent loads, not a stream.
+constexpr uint32_t kDilatePer = 2;                     // blocks per thread
+constexpr uint32_t kDilateBlocks = kDilatePer * 256u;  // blocks per workgroup
+
+__device__ __forceinline__ void dilate_wg(const Args& a, int force, int keep_flags, uint32_t wg) {
+  __shared__ uint32_t s_off[2][kDilatePer * 4];  // per (round, wave): list, idle
+  __shared__ uint32_t s_base[2];
+  __shared__ uint32_t s_work[4][3];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint32_t b0 = wg * kDilateBlocks + threadIdx.x;
+  BlockInfo bi[kDilatePer];
+  uint32_t v[kDilatePer], has[kDilatePer];
+#pragma unroll
+  for (uint32_t r = 0; r < kDilatePer; ++r) {
+    const uint32_t b = b0 + r * 256u;
+    if (b < a.NB) {
+      bi[r] = a.binfo[b];
+      v[r] = (keep_flags ? a.active[b] : (a.wet[b] | a.forced[b]));
+      has[r] = a.has_src[b];
     } else {
-      v = a.wet[b] | a.forced[b];
-      for (uint32_t k = a.bptr[b]; v == 0u && k < a.bptr[b + 1]; ++k) {
-        const uint32_t n = a.bidx[k];
-        v = a.wet[n] | a.forced[n];
-      }
-      a.active[b] = v;
-    }
-    // A frozen scenario's blocks stay off the lists until a retarget
-    // rebuilds them from the kept flags.
-    take = v != 0u && !frozen;
-    take_idle = v == 0u && !frozen && a.has_src[b] != 0u;
-    if (take) {
-      const uint32_t c0 = b << 8;
-      const uint32_t c1 = c0 + bi_cells(bi);
-      w_cells = c1 - c0;
-      w_edges = bi.n_own;
-      w_slots = a.cptr[c1] - a.cptr[c0];
+      bi[r].meta = 0u;
+      v[r] = has[r] = 0u;
     }
   }
-  // Active-set work of the next step: one atomic per workgroup and counter.
+  uint32_t w_cells = 0, w_edges = 0, w_slots = 0;
+  uint64_t m_take[kDilatePer], m_idle[kDilatePer];
+#pragma unroll
+  for (uint32_t r = 0; r < kDilatePer; ++r) {
+    const uint32_t b = b0 + r * 256u;
+    bool take = false, take_idle = false;
+    if (b < a.NB) {
+      // A scenario at its stop took a no-op step: its flags stand. (Without
+      // force the step moved some scenario; with one scenario, that one.)
+      const bool frozen = !force && a.n_scen > 1u && a.scen[bi_scen(bi[r])].dt_cur == 0u;
+      uint32_t act = v[r];
+      if (frozen && !keep_flags) act = a.active[b];
+      if (!keep_flags && !frozen) {
+        const uint32_t k1 = a.bptr[b + 1];
+        for (uint32_t k = a.bptr[b]; act == 0u && k < k1; k += 8u) {
+          uint32_t n[8];
+#pragma unroll
+          for (uint32_t q = 0; q < 8u; ++q) n[q] = (k + q < k1) ? a.bidx[k + q] : b;
+#pragma unroll
+          for (uint32_t q = 0; q < 8u; ++q) act |= a.wet[n[q]] | a.forced[n[q]];
+        }
+        a.active[b] = act;
+      }
+      // A frozen scenario's blocks stay off the lists until a retarget
+      // rebuilds them from the kept flags.
+      take = act != 0u && !frozen;
+      take_idle = act == 0u && !frozen && has[r] != 0u;
+      if (take) {
+        w_cells += bi_cells(bi[r]);
+        w_edges += bi[r].n_own;
+        w_slots += bi[r].n_slots;
+      }
+    }
+    m_take[r] = __ballot(take ? 1 : 0);
+    m_idle[r] = __ballot(take_idle ? 1 : 0);
+    if (lane == 0u) {
+      s_off[0][r * 4u + w] = (uint32_t)__popcll(m_take[r]);
+      s_off[1][r * 4u + w] = (uint32_t)__popcll(m_idle[r]);
+    }
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     w_cells += (uint32_t)__shfl_xor((int)w_cells, off, 64);
     w_edges += (uint32_t)__shfl_xor((int)w_edges, off, 64);
     w_slots += (uint32_t)__shfl_xor((int)w_slots, off, 64);
   }
-  {
-    __shared__ uint32_t part[4][3];
-    const uint32_t w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63u) == 0u) {
-      part[w][0] = w_cells;
-      part[w][1] = w_edges;
-      part[w][2] = w_slots;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-      unsigned long long tot = 0;
-      for (uint32_t k = 0; k < blockDim.x / 64u; ++k) tot += part[k][threadIdx.x];
-      if (tot) atomicAdd(&a.ctl->act_next[threadIdx.x], tot);
-    }
+  if (lane == 0u) {
+    s_work[w][0] = w_cells;
+    s_work[w][1] = w_edges;
+    s_work[w][2] = w_slots;
   }
-  wave_append(take, b, &a.ctl->cnt_next, a.list);
-  wave_append(take_idle, b, &a.ctl->idle_next, a.idle);
+  __syncthreads();
+  if (threadIdx.x < 2u) {
+    // Exclusive offsets in block order (round, then wave), one atomic.
+    uint32_t run = 0;
+    for (uint32_t j = 0; j < kDilatePer * 4u; ++j) {
+      const uint32_t cnt = s_off[threadIdx.x][j];
+      s_off[threadIdx.x][j] = run;
+      run += cnt;
+    }
+    uint32_t* ctr = threadIdx.x == 0u ? &a.ctl->cnt_next : &a.ctl->idle_next;
+    s_base[threadIdx.x] = run ? atomicAdd(ctr, run) : 0u;
+  } else if (threadIdx.x >= 64u && threadIdx.x < 67u) {
+    const uint32_t k = threadIdx.x - 64u;
+    a.act_part[3u * wg + k] = (unsigned long long)s_work[0][k] + s_work[1][k] + s_work[2][k] + s_work[3][k];
+  }
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (uint32_t r = 0; r < kDilatePer; ++r) {
+    const uint32_t b = b0 + r * 256u;
+    if ((m_take[r] >> lane) & 1ull)
+      a.list[s_base[0] + s_off[0][r * 4u + w] + (uint32_t)__popcll(m_take[r] & below)] = b;
+    if ((m_idle[r] >> lane) & 1ull)
+      a.idle[s_base[1] + s_off[1][r * 4u + w] + (uint32_t)__popcll(m_idle[r] & below)] = b;
+  }
 }
 
-// Empties the lists being built and their activity counters (before a
-// rebuild outside the step sequence).
+__global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int keep_flags) {
+  if (!force && a.ctl->any_dt == 0u) return;  // no-op step changed nothing
+  dilate_wg(a, force, keep_flags, blockIdx.x);
+}
+
+// The end of a step before k_step_control, in one launch: workgroups
+// [0, n_dil) run k_dilate's work, the rest k_source_fix's (independent of
+// each other; both read what the corrector and k_source_idle wrote).
+template <bool FM>
+__global__ __launch_bounds__(256) void k_step_tail(const Args a, uint32_t n_dil) {
+  if (a.ctl->any_dt == 0u) return;  // no-op step changed nothing
+  if (blockIdx.x < n_dil) dilate_wg(a, 0, 0, blockIdx.x);
+  else source_fix_wg<FM>(a, blockIdx.x - n_dil);
+}
+
+// Empties the lists being built (before a rebuild outside the step
+// sequence).
 __global__ void k_list_reset(const Args a) {
   a.ctl->cnt_next = 0u;
   a.ctl->idle_next = 0u;
-  for (int k = 0; k < 3; ++k) a.ctl->act_next[k] = 0ull;
 }
 
 // Prologue CFL bound of the current state, every cell (reference
@@ -1195,20 +1289,30 @@ __global__ void k_step_control(const Args a, int mode) {
   }
   const int any_real = __syncthreads_or(real ? 1 : 0);
   const int any_moving = __syncthreads_or(moving ? 1 : 0);
+  const bool swap = mode == 0 || mode == 2 || (mode == 1 && any_real && t->swap_on != 0u);
+  // The next step's active-set work, summed over k_dilate's workgroups.
+  __shared__ unsigned long long s_act[4][3];
+  if (swap) {
+    long part[3] = {0, 0, 0};
+    for (uint32_t j = threadIdx.x; j < a.n_dwg; j += blockDim.x)
+      for (int k = 0; k < 3; ++k) part[k] += (long)a.act_part[3u * j + k];
+    for (int k = 0; k < 3; ++k)
+      for (int off = 32; off > 0; off >>= 1) part[k] += __shfl_xor(part[k], off, 64);
+    if ((threadIdx.x & 63u) == 0u)
+      for (int k = 0; k < 3; ++k) s_act[threadIdx.x >> 6][k] = (unsigned long long)part[k];
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     if (mode == 1 && any_real) {
       for (int k = 0; k < 3; ++k) t->act_total[k] += t->act_cur[k];
       t->act_steps += 1ull;
     }
-    if (mode == 0 || mode == 2 || (mode == 1 && any_real && t->swap_on != 0u)) {
+    if (swap) {
       t->cnt_cur = t->cnt_next;
       t->cnt_next = 0u;
       t->idle_cur = t->idle_next;
       t->idle_next = 0u;
-      for (int k = 0; k < 3; ++k) {
-        t->act_cur[k] = t->act_next[k];
-        t->act_next[k] = 0ull;
-      }
+      for (int k = 0; k < 3; ++k) t->act_cur[k] = s_act[0][k] + s_act[1][k] + s_act[2][k] + s_act[3][k];
     }
     t->any_dt = any_moving ? 1u : 0u;
   }
@@ -1679,6 +1783,14 @@ int dalloc(inu_ctx* c, T** p, size_t n) {
   return 0;
 }
 
+// New depths from the host: every block may hold a dry cell until a step
+// has looked (k_source_fix reads part_dry).
+int depths_replaced(inu_ctx* c) {
+  HIPCHK(c, hipMemsetAsync(c->a.part_dry, 1, (size_t)c->NB * 4, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 template <typename T>
 int upload(inu_ctx* c, T* dst, const T* src, size_t n) {
   if (n == 0) return 0;
@@ -1845,16 +1957,13 @@ int launch_step(inu_ctx* c, const hipEvent_t* ev) {
   if (c->n_open)
     hipLaunchKernelGGL(k_open_tally, dim3((c->n_stretch + 63) / 64), dim3(64), 0, c->stream, a, c->stretch_ptr,
                        c->open_cell, c->tally, c->n_stretch);
-  if (c->n_src) {
-    launch_source_idle(c, a, comp, fm);
-    if (c->n_src_pairs) {
-      const uint32_t gs = (c->n_src_pairs + 255) / 256;
-      if (fm) hipLaunchKernelGGL((k_source_fix<true>), dim3(gs), dim3(256), 0, c->stream, a);
-      else hipLaunchKernelGGL((k_source_fix<false>), dim3(gs), dim3(256), 0, c->stream, a);
-    }
+  if (c->n_src) launch_source_idle(c, a, comp, fm);
+  const uint32_t n_dil = (c->flags & INU_FLAG_ACTIVE_SET) ? a.n_dwg : 0u;
+  const uint32_t n_fix = c->n_src ? (a.n_fix + kFixGroupsWg - 1) / kFixGroupsWg : 0u;
+  if (n_dil + n_fix) {
+    if (fm) hipLaunchKernelGGL((k_step_tail<true>), dim3(n_dil + n_fix), dim3(256), 0, c->stream, a, n_dil);
+    else hipLaunchKernelGGL((k_step_tail<false>), dim3(n_dil + n_fix), dim3(256), 0, c->stream, a, n_dil);
   }
-  if (c->flags & INU_FLAG_ACTIVE_SET)
-    hipLaunchKernelGGL(k_dilate, dim3((c->NB + 255) / 256), dim3(256), 0, c->stream, a, 0, 0);
   hipLaunchKernelGGL(k_step_control, dim3(1), dim3(256), 0, c->stream, a, 1);
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -2263,9 +2372,14 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   rc |= dalloc(c, &dbptr, NB + 1); rc |= dalloc(c, &dbidx, bidx.size());
   rc |= dalloc(c, &a.wet, NB); rc |= dalloc(c, &a.active, NB); rc |= dalloc(c, &a.forced, NB);
   rc |= dalloc(c, &a.has_src, NB); rc |= dalloc(c, &a.list, NB); rc |= dalloc(c, &a.idle, NB);
+  rc |= dalloc(c, &a.part_dry, NB);
+  a.n_dwg = (NB + kDilateBlocks - 1) / kDilateBlocks;
+  rc |= dalloc(c, &a.act_part, 3 * (size_t)a.n_dwg);
   uint32_t* src_cell; uint32_t* src_blocks;
   uint2* src_pair;
+  uint32_t *fix_blk, *fix_ptr;
   rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1); rc |= dalloc(c, &src_pair, 1);
+  rc |= dalloc(c, &fix_blk, 1); rc |= dalloc(c, &fix_ptr, 1);
   rc |= dalloc(c, &c->ctl, 1);
   rc |= dalloc(c, &c->scen, n_scen);
   c->n_open = n_open;
@@ -2298,6 +2412,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   a.lrec = dlrec; a.ldz = dldz;
   a.binfo = dbinfo; a.gx = dgx; a.bptr = dbptr; a.bidx = dbidx;
   a.src_cell = src_cell; a.src_blocks = src_blocks; a.src_pair = src_pair; a.n_src_pairs = 0;
+  a.fix_blk = fix_blk; a.fix_ptr = fix_ptr; a.n_fix = 0; a.src_neg = 0;
   a.ctl = c->ctl;
   a.scen = c->scen;
   a.n_scen = n_scen;
@@ -2310,6 +2425,8 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   HIPCHK(c, hipMemsetAsync(a.src, 0, NP * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.wet, 0, NB * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.active, 0, NB * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.forced, 0, NB * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.has_src, 0, NB * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.part_dry, 1, NB * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.act_part, 0, 3 * (size_t)a.n_dwg * 8, c->stream));
   HIPCHK(c, hipMemsetAsync(c->ctl, 0, sizeof(Ctl), c->stream));
   HIPCHK(c, hipMemsetAsync(c->scen, 0, sizeof(Scen) * n_scen, c->stream));
   if (n_open) {
@@ -2483,6 +2600,7 @@ int inu_set_state(inu_ctx* c, const float* h, const float* hu, const float* hv, 
   if (rc) return rc;
   if (!h || !hu || !hv) return fail(c, INU_EINVAL, "inu_set_state: null array");
   if ((rc = upload_cells(c, c->a.h, h))) return rc;
+  if ((rc = depths_replaced(c))) return rc;
   if ((rc = upload_cells(c, c->a.hu, hu))) return rc;
   if ((rc = upload_cells(c, c->a.hv, hv))) return rc;
   if (h_lo) {
@@ -2531,11 +2649,23 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   std::vector<uint2> pairs;
   for (const auto& pr : c->cross)
     if (dense[pr.first] != 0.0f) pairs.push_back(make_uint2(pr.first, pr.second));
+  // Grouped by the neighbour's block (k_source_fix takes a group per wave).
+  std::stable_sort(pairs.begin(), pairs.end(),
+                   [](const uint2& x, const uint2& y) { return x.y / kBlockCells < y.y / kBlockCells; });
+  std::vector<uint32_t> fix_blk, fix_ptr;
+  for (uint32_t k = 0; k < pairs.size(); ++k) {
+    if (k == 0 || pairs[k].y / kBlockCells != pairs[k - 1].y / kBlockCells) {
+      fix_blk.push_back(pairs[k].y / kBlockCells);
+      fix_ptr.push_back(k);
+    }
+  }
+  fix_ptr.push_back((uint32_t)pairs.size());
   std::vector<uint32_t> blocks;
   for (uint32_t b = 0; b < c->NB; ++b)
     if (has[b]) blocks.push_back(b);
   // Replace the source lists (their sizes change).
-  for (void* q : {(void*)c->a.src_cell, (void*)c->a.src_blocks, (void*)c->a.src_pair}) {
+  for (void* q : {(void*)c->a.src_cell, (void*)c->a.src_blocks, (void*)c->a.src_pair, (void*)c->a.fix_blk,
+                  (void*)c->a.fix_ptr}) {
     auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
     if (it != c->allocs.end()) {
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2546,7 +2676,10 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   c->a.src_cell = nullptr;
   c->a.src_blocks = nullptr;
   c->a.src_pair = nullptr;
-  uint32_t *dcell = nullptr, *dblk = nullptr;
+  c->a.fix_blk = nullptr;
+  c->a.fix_ptr = nullptr;
+  c->a.n_fix = 0;
+  uint32_t *dcell = nullptr, *dblk = nullptr, *dfblk = nullptr, *dfptr = nullptr;
   uint2* dpair = nullptr;
   int rc2 = dalloc(c, &dcell, cells.size());
   if (rc2) return rc2;
@@ -2554,8 +2687,17 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   if (rc2) return rc2;
   rc2 = dalloc(c, &dpair, pairs.size());
   if (rc2) return rc2;
+  rc2 = dalloc(c, &dfblk, fix_blk.size());
+  if (rc2) return rc2;
+  rc2 = dalloc(c, &dfptr, fix_ptr.size());
+  if (rc2) return rc2;
   if ((rc = upload(c, dpair, pairs.data(), pairs.size()))) return rc;
+  if ((rc = upload(c, dfblk, fix_blk.data(), fix_blk.size()))) return rc;
+  if ((rc = upload(c, dfptr, fix_ptr.data(), fix_ptr.size()))) return rc;
   c->a.src_pair = dpair;
+  c->a.fix_blk = dfblk;
+  c->a.fix_ptr = dfptr;
+  c->a.n_fix = (uint32_t)fix_blk.size();
   c->n_src_pairs = (uint32_t)pairs.size();
   c->a.n_src_pairs = c->n_src_pairs;
   if ((rc = upload(c, dcell, cells.data(), cells.size()))) return rc;
@@ -2568,6 +2710,7 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   c->src_nonneg = true;
   for (uint32_t k = 0; k < n; ++k)
     if (rate[k] < 0.0f) c->src_nonneg = false;
+  c->a.src_neg = c->src_nonneg ? 0u : 1u;
   c->n_src = (uint32_t)cells.size();
   c->n_src_blocks = (uint32_t)blocks.size();
   c->a.n_src = c->n_src;
@@ -2632,7 +2775,7 @@ int inu_prologue(inu_ctx* c) {
   const uint32_t gb = (c->NB + 255) / 256;
   const uint32_t gm = (std::max(c->NB, c->n_scen) + 255) / 256;
   hipLaunchKernelGGL(k_block_mark, dim3(gm), dim3(256), 0, c->stream, a, act ? 0 : 1);
-  hipLaunchKernelGGL(k_dilate, dim3(gb), dim3(256), 0, c->stream, a, 1, act ? 0 : 1);
+  hipLaunchKernelGGL(k_dilate, dim3(a.n_dwg), dim3(256), 0, c->stream, a, 1, act ? 0 : 1);
   const uint32_t gp = std::min<uint32_t>(c->NB, c->grid_stage);
   if (c->flags & INU_FLAG_EXACT_ORDER)
     hipLaunchKernelGGL((k_prologue_cfl<false, false>), dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
@@ -2653,7 +2796,7 @@ int inu_retarget(inu_ctx* c) {
   // stop dropped theirs); the flags themselves are kept.
   const uint32_t gb = (c->NB + 255) / 256;
   hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(1), 0, c->stream, c->a);
-  hipLaunchKernelGGL(k_dilate, dim3(gb), dim3(256), 0, c->stream, c->a, 1, 1);
+  hipLaunchKernelGGL(k_dilate, dim3(c->a.n_dwg), dim3(256), 0, c->stream, c->a, 1, 1);
   hipLaunchKernelGGL(k_step_control, dim3(1), dim3(256), 0, c->stream, c->a, 2);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2768,7 +2911,8 @@ int inu_upload(inu_ctx* c, int which, const void* in) {
     case INU_FIELD_CLAMP: dst = c->a.clamp; break;
     default: return fail(c, INU_EINVAL, "field cannot be uploaded");
   }
-  return upload_cells(c, static_cast<float*>(dst), static_cast<const float*>(in));
+  if ((rc = upload_cells(c, static_cast<float*>(dst), static_cast<const float*>(in)))) return rc;
+  return which == INU_FIELD_H ? depths_replaced(c) : 0;
 }
 
 int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out7, float g, float dry_tol) {
@@ -2943,6 +3087,12 @@ int inu_set_source_rates(inu_ctx* c, uint32_t n, const uint32_t* cell, const flo
   if (e != hipSuccess) return fail(c, INU_EHIP, std::string("set_source_rates: ") + hipGetErrorString(e));
   for (uint32_t k = 0; k < n; ++k)
     if (rate[k] < 0.0f) c->src_nonneg = false;
+  if (!c->src_nonneg && c->a.src_neg == 0u) {
+    // The stage kernels' all-wet test changes: re-record the step graphs.
+    c->a.src_neg = 1u;
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+  }
   if (c->adj.on) {
     // The recording replays these rates from the current step on.
     if (rc_nonneg_required(c)) return fail(c, INU_EINVAL, "the adjoint needs non-negative source rates");
@@ -3202,6 +3352,7 @@ static int adj_restore(inu_ctx* c, uint32_t cs) {
   const float* ck = d.ck + (size_t)cs * 4 * NP;
   const Args& a = c->a;
   HIPCHK(c, hipMemcpyAsync(a.h, ck, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.part_dry, 1, (size_t)c->NB * 4, c->stream));
   HIPCHK(c, hipMemcpyAsync(a.hu, ck + NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(a.hv, ck + 2 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(a.hlo, ck + 3 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));

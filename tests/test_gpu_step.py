@@ -244,6 +244,52 @@ def test_rain_beside_a_dry_bank_is_bitwise_the_fp32_oracle(ctx, order) -> None: 
         D.assert_same(D.snapshot(gpu, g.N, nb), D.snapshot(ref, g.N, nb), f"block {k}")
 
 
+@pytest.fixture(scope="module")
+def big():  # noqa: ANN201
+    g = valley_grid(1024, 512, 2.0)
+    return g, device_mesh_arrays(g)
+
+
+def test_lists_and_source_fix_over_two_thousand_blocks(ctx, big) -> None:  # noqa: ANN001
+    """2048 blocks: the active list spans several list-building workgroups,
+    and rain on part of a dam break leaves blocks wet, dry and mixed, so the
+    source fix skips some neighbour groups and works through others. Bitwise
+    the oracle; the work counters equal the listed blocks' cells and slots."""
+    g, m = big
+    gpu, ref = _both(ctx, m)
+    h, hu, hv = f32_state(*CASES["dam_break"](g))
+    x = g.centroid[:, 0]
+    cells = np.flatnonzero(x < 0.4 * x.max()).astype(np.uint32)
+    rates = np.full(cells.size, 2e-3, dtype=np.float32)
+    nb = num_blocks(g.N, 256)
+    for be in (gpu, ref):
+        D.setup(be, g.N, h, hu, hv, MANNING, dry_tol=DRY_TOL, src_cells=cells, src_rates=rates)
+        D.start(be, 5.0, 0.5, 0.05)
+    act = gpu.download(L.FIELD_BLOCK_ACTIVE, nb, np.uint32) != 0
+    assert 0 < act.sum() < nb
+    gpu.activity(reset=True)
+    gpu.steps(1, use_graph=False)
+    ref.steps(1)
+    w = gpu.activity()
+    cptr = m["cptr"].astype(np.int64)
+    lo = np.arange(nb) * 256
+    hi = np.minimum(lo + 256, g.N)
+    assert w["steps"] == 1
+    assert w["cells"] == int((hi - lo)[act].sum())
+    assert w["slots"] == int((cptr[hi] - cptr[lo])[act].sum())
+    for k in range(3):
+        gpu.steps(20, use_graph=True)
+        ref.steps(20)
+        D.assert_same(D.snapshot(gpu, g.N, nb), D.snapshot(ref, g.N, nb), f"block {k}")
+    # Active set off: every block listed, every cell, edge and slot counted.
+    D.setup(gpu, g.N, h, hu, hv, MANNING, dry_tol=DRY_TOL, flags=L.FLAG_DETERMINISTIC | L.FLAG_COMPENSATED)
+    D.start(gpu, 5.0, 0.5, 0.05)
+    gpu.activity(reset=True)
+    gpu.steps(4, use_graph=False)
+    w = gpu.activity()
+    assert (w["steps"], w["cells"], w["edges"], w["slots"]) == (4, 4 * g.N, 4 * g.E, 4 * int(cptr[g.N]))
+
+
 def test_active_set_on_equals_off(ctx, mesh, grid) -> None:  # noqa: ANN001
     """Skipping blocks is exact (reference test_gpu_active_set.py:77-91)."""
     gpu, _ = _both(ctx, mesh)
