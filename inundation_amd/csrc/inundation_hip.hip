@@ -122,11 +122,12 @@ struct Args {
   const uint32_t* src_cell;
   const uint32_t* src_blocks;
   // (source cell, its neighbour in another block), grouped by the
-  // neighbour's block: group f holds [fix_ptr[f], fix_ptr[f + 1]), whose
-  // neighbours lie in block fix_blk[f]
+  // neighbour's block: the neighbours of group f lie in block fix_blk[f];
+  // the pairs [kFixChunk w, kFixChunk (w + 1)) touch the groups
+  // [fix_chunk[w].x, fix_chunk[w].y)
   const uint2* src_pair;
   const uint32_t* fix_blk;
-  const uint32_t* fix_ptr;
+  const uint2* fix_chunk;
   uint32_t n_src_blocks, n_src_pairs, n_fix;
   Ctl* ctl;
   Scen* scen;
@@ -934,50 +935,83 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
 // and rounding is monotone), so the minimum over both equals the exact one.
 // The pairs come in groups whose neighbours share a block; a group whose
 // block is wet throughout (part_dry 0) has nothing to fix, which is most of
-// them once the flood has spread. Each wave takes kFixGroups consecutive
-// groups: one flag per lane decides whether the wave has any work, and if
-// so its lanes walk the groups' pairs. Runs inside k_step_tail.
-constexpr uint32_t kFixGroups = 16;                     // pair groups per wave
-constexpr uint32_t kFixGroupsWg = kFixGroups * 4u;      // per workgroup
+// them once the flood has spread. A workgroup takes kFixChunk consecutive
+// pairs: it first reads the flags of the groups they touch, and only if one
+// is set loads its pairs (kFixPer per thread, loads issued together),
+// gathers the source cells whose neighbour is dry into LDS and bounds them
+// a thread each, so a partly dry mesh keeps the parallelism of a thread per
+// pair. Runs inside k_step_tail.
+constexpr uint32_t kFixPer = 8;                    // pairs per thread
+constexpr uint32_t kFixChunk = kFixPer * 256u;     // pairs per workgroup
+
+// The exact bound of source cell i, whose neighbour in another block is dry.
+template <bool FM>
+__device__ __forceinline__ void fix_pair(const Args& a, uint32_t i) {
+  const Phys& p = a.p;
+  const float hn = a.h[i];
+  if (!(hn >= p.dry) || a.src[i] == 0.0f) return;  // only wetted cells whose source is on
+  const float hui = a.hu[i];
+  const float hvi = a.hv[i];
+  const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
+  const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
+  float lam = 0.0f;
+  const uint32_t s1 = a.cptr[i + 1];
+  for (uint32_t s = a.cptr[i]; s < s1; ++s) {
+    const int nb = a.snb[s];
+    const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
+    const float kf = dry_side ? 2.0f : 1.0f;
+    if (FM) lam = fmaf(fmaf(kf, c, u), a.slen[s], lam);
+    else lam += (u + kf * c) * a.slen[s];
+  }
+  const float A = a.area[i];
+  if (lam > 1e-30f && A > 1e-10f) {
+    // Rare: one atomic per such pair, into its scenario's slot. A scenario
+    // at its stop ignores the slot (k_step_control).
+    const uint32_t bb = __float_as_uint(p.cfl * A / lam);
+    uint32_t* slot = &a.scen[bi_scen(a.binfo[i >> 8])].dt_next;
+    if (bb < __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMin(slot, bb);
+  }
+}
 
 template <bool FM>
 __device__ __forceinline__ void source_fix_wg(const Args& a, uint32_t wg) {
-  const Phys& p = a.p;
+  __shared__ uint32_t s_n;
+  __shared__ uint32_t s_cell[kFixChunk];
+  const uint2 gr = a.fix_chunk[wg];
+  bool work = false;
+  for (uint32_t g = gr.x + threadIdx.x; g < gr.y; g += blockDim.x) work = work || a.part_dry[a.fix_blk[g]] != 0u;
+  if (threadIdx.x == 0) s_n = 0u;
+  if (!__syncthreads_or(work ? 1 : 0)) return;
+  const uint32_t k0 = wg * kFixChunk + threadIdx.x;
+  uint2 pr[kFixPer];
+#pragma unroll
+  for (uint32_t u = 0; u < kFixPer; ++u) {
+    const uint32_t k = k0 + 256u * u;
+    pr[u] = (k < a.n_src_pairs) ? a.src_pair[k] : make_uint2(0u, 0u);
+  }
+  uint32_t pd[kFixPer];
+  float hy[kFixPer];
+#pragma unroll
+  for (uint32_t u = 0; u < kFixPer; ++u) {
+    pd[u] = a.part_dry[pr[u].y / kBlockCells];
+    hy[u] = a.h[pr[u].y];
+  }
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t g0 = wg * kFixGroupsWg + (threadIdx.x >> 6) * kFixGroups;
-  if (g0 >= a.n_fix) return;
-  const uint32_t g1 = min(g0 + kFixGroups, a.n_fix);
-  const bool work = lane < g1 - g0 && a.part_dry[a.fix_blk[g0 + lane]] != 0u;
-  if (__ballot(work ? 1 : 0) == 0ull) return;
-  const uint32_t k1 = a.fix_ptr[g1];
-  for (uint32_t k = a.fix_ptr[g0] + lane; k < k1; k += 64u) {
-    const uint2 pr = a.src_pair[k];
-    if (a.part_dry[pr.y / kBlockCells] == 0u || !(a.h[pr.y] < p.dry)) continue;
-    const uint32_t i = pr.x;
-    const float hn = a.h[i];
-    if (!(hn >= p.dry) || a.src[i] == 0.0f) continue;  // only wetted cells whose source is on
-    const float hui = a.hu[i];
-    const float hvi = a.hv[i];
-    const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
-    const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
-    float lam = 0.0f;
-    const uint32_t s1 = a.cptr[i + 1];
-    for (uint32_t s = a.cptr[i]; s < s1; ++s) {
-      const int nb = a.snb[s];
-      const bool dry_side = nb >= 0 && a.h[nb] < p.dry;
-      const float kf = dry_side ? 2.0f : 1.0f;
-      if (FM) lam = fmaf(fmaf(kf, c, u), a.slen[s], lam);
-      else lam += (u + kf * c) * a.slen[s];
-    }
-    const float A = a.area[i];
-    if (lam > 1e-30f && A > 1e-10f) {
-      // Rare: one atomic per such pair, into its scenario's slot. A scenario
-      // at its stop ignores the slot (k_step_control).
-      const uint32_t bb = __float_as_uint(p.cfl * A / lam);
-      uint32_t* slot = &a.scen[bi_scen(a.binfo[i >> 8])].dt_next;
-      if (bb < __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMin(slot, bb);
+  const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (uint32_t u = 0; u < kFixPer; ++u) {
+    const bool dry_nb = k0 + 256u * u < a.n_src_pairs && pd[u] != 0u && hy[u] < a.p.dry;
+    const uint64_t m = __ballot(dry_nb ? 1 : 0);
+    if (m != 0ull) {
+      uint32_t base = 0u;
+      if (lane == 0u) base = atomicAdd(&s_n, (uint32_t)__popcll(m));
+      base = (uint32_t)__shfl((int)base, 0, 64);
+      if (dry_nb) s_cell[base + (uint32_t)__popcll(m & below)] = pr[u].x;
     }
   }
+  __syncthreads();
+  const uint32_t n = s_n;
+  for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) fix_pair<FM>(a, s_cell[q]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1959,7 +1993,7 @@ int launch_step(inu_ctx* c, const hipEvent_t* ev) {
                        c->open_cell, c->tally, c->n_stretch);
   if (c->n_src) launch_source_idle(c, a, comp, fm);
   const uint32_t n_dil = (c->flags & INU_FLAG_ACTIVE_SET) ? a.n_dwg : 0u;
-  const uint32_t n_fix = c->n_src ? (a.n_fix + kFixGroupsWg - 1) / kFixGroupsWg : 0u;
+  const uint32_t n_fix = c->n_src ? (a.n_src_pairs + kFixChunk - 1) / kFixChunk : 0u;
   if (n_dil + n_fix) {
     if (fm) hipLaunchKernelGGL((k_step_tail<true>), dim3(n_dil + n_fix), dim3(256), 0, c->stream, a, n_dil);
     else hipLaunchKernelGGL((k_step_tail<false>), dim3(n_dil + n_fix), dim3(256), 0, c->stream, a, n_dil);
@@ -2377,9 +2411,10 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   rc |= dalloc(c, &a.act_part, 3 * (size_t)a.n_dwg);
   uint32_t* src_cell; uint32_t* src_blocks;
   uint2* src_pair;
-  uint32_t *fix_blk, *fix_ptr;
+  uint32_t* fix_blk;
+  uint2* fix_chunk;
   rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1); rc |= dalloc(c, &src_pair, 1);
-  rc |= dalloc(c, &fix_blk, 1); rc |= dalloc(c, &fix_ptr, 1);
+  rc |= dalloc(c, &fix_blk, 1); rc |= dalloc(c, &fix_chunk, 1);
   rc |= dalloc(c, &c->ctl, 1);
   rc |= dalloc(c, &c->scen, n_scen);
   c->n_open = n_open;
@@ -2412,7 +2447,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   a.lrec = dlrec; a.ldz = dldz;
   a.binfo = dbinfo; a.gx = dgx; a.bptr = dbptr; a.bidx = dbidx;
   a.src_cell = src_cell; a.src_blocks = src_blocks; a.src_pair = src_pair; a.n_src_pairs = 0;
-  a.fix_blk = fix_blk; a.fix_ptr = fix_ptr; a.n_fix = 0; a.src_neg = 0;
+  a.fix_blk = fix_blk; a.fix_chunk = fix_chunk; a.n_fix = 0; a.src_neg = 0;
   a.ctl = c->ctl;
   a.scen = c->scen;
   a.n_scen = n_scen;
@@ -2459,6 +2494,12 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   if ((rc = upload(c, dgx, gx.data(), gx.size()))) return rc;
   if ((rc = upload(c, dbptr, bptr.data(), NB + 1))) return rc;
   if ((rc = upload(c, dbidx, bidx.data(), bidx.size()))) return rc;
+  // Grouped by the neighbour's block once here, so that every source list
+  // keeps the groups k_source_fix walks without sorting again.
+  std::stable_sort(cross.begin(), cross.end(), [](const std::pair<uint32_t, uint32_t>& x,
+                                                  const std::pair<uint32_t, uint32_t>& y) {
+    return x.second / kBlockCells < y.second / kBlockCells;
+  });
   c->cross.swap(cross);
   c->e_cl.swap(e_cl);
   c->e_cr.swap(e_cr);
@@ -2649,23 +2690,25 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   std::vector<uint2> pairs;
   for (const auto& pr : c->cross)
     if (dense[pr.first] != 0.0f) pairs.push_back(make_uint2(pr.first, pr.second));
-  // Grouped by the neighbour's block (k_source_fix takes a group per wave).
-  std::stable_sort(pairs.begin(), pairs.end(),
-                   [](const uint2& x, const uint2& y) { return x.y / kBlockCells < y.y / kBlockCells; });
-  std::vector<uint32_t> fix_blk, fix_ptr;
+  // The groups of pairs whose neighbours share a block (c->cross is in that
+  // order), and the groups each chunk of kFixChunk pairs touches, for
+  // k_source_fix.
+  std::vector<uint32_t> fix_blk, pair_group(pairs.size());
   for (uint32_t k = 0; k < pairs.size(); ++k) {
-    if (k == 0 || pairs[k].y / kBlockCells != pairs[k - 1].y / kBlockCells) {
-      fix_blk.push_back(pairs[k].y / kBlockCells);
-      fix_ptr.push_back(k);
-    }
+    if (k == 0 || pairs[k].y / kBlockCells != pairs[k - 1].y / kBlockCells) fix_blk.push_back(pairs[k].y / kBlockCells);
+    pair_group[k] = (uint32_t)fix_blk.size() - 1u;
   }
-  fix_ptr.push_back((uint32_t)pairs.size());
+  std::vector<uint2> fix_chunk;
+  for (size_t k = 0; k < pairs.size(); k += kFixChunk) {
+    const size_t last = std::min(pairs.size(), k + kFixChunk) - 1;
+    fix_chunk.push_back(make_uint2(pair_group[k], pair_group[last] + 1u));
+  }
   std::vector<uint32_t> blocks;
   for (uint32_t b = 0; b < c->NB; ++b)
     if (has[b]) blocks.push_back(b);
   // Replace the source lists (their sizes change).
   for (void* q : {(void*)c->a.src_cell, (void*)c->a.src_blocks, (void*)c->a.src_pair, (void*)c->a.fix_blk,
-                  (void*)c->a.fix_ptr}) {
+                  (void*)c->a.fix_chunk}) {
     auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
     if (it != c->allocs.end()) {
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2677,10 +2720,10 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   c->a.src_blocks = nullptr;
   c->a.src_pair = nullptr;
   c->a.fix_blk = nullptr;
-  c->a.fix_ptr = nullptr;
+  c->a.fix_chunk = nullptr;
   c->a.n_fix = 0;
-  uint32_t *dcell = nullptr, *dblk = nullptr, *dfblk = nullptr, *dfptr = nullptr;
-  uint2* dpair = nullptr;
+  uint32_t *dcell = nullptr, *dblk = nullptr, *dfblk = nullptr;
+  uint2 *dpair = nullptr, *dfchunk = nullptr;
   int rc2 = dalloc(c, &dcell, cells.size());
   if (rc2) return rc2;
   rc2 = dalloc(c, &dblk, blocks.size());
@@ -2689,14 +2732,14 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   if (rc2) return rc2;
   rc2 = dalloc(c, &dfblk, fix_blk.size());
   if (rc2) return rc2;
-  rc2 = dalloc(c, &dfptr, fix_ptr.size());
+  rc2 = dalloc(c, &dfchunk, fix_chunk.size());
   if (rc2) return rc2;
   if ((rc = upload(c, dpair, pairs.data(), pairs.size()))) return rc;
   if ((rc = upload(c, dfblk, fix_blk.data(), fix_blk.size()))) return rc;
-  if ((rc = upload(c, dfptr, fix_ptr.data(), fix_ptr.size()))) return rc;
+  if ((rc = upload(c, dfchunk, fix_chunk.data(), fix_chunk.size()))) return rc;
   c->a.src_pair = dpair;
   c->a.fix_blk = dfblk;
-  c->a.fix_ptr = dfptr;
+  c->a.fix_chunk = dfchunk;
   c->a.n_fix = (uint32_t)fix_blk.size();
   c->n_src_pairs = (uint32_t)pairs.size();
   c->a.n_src_pairs = c->n_src_pairs;
