@@ -244,6 +244,31 @@ def test_rain_beside_a_dry_bank_is_bitwise_the_fp32_oracle(ctx, order) -> None: 
         D.assert_same(D.snapshot(gpu, g.N, nb), D.snapshot(ref, g.N, nb), f"block {k}")
 
 
+@pytest.mark.parametrize("order", [0, EXACT], ids=["fast", "exact"])
+def test_sinks_that_dry_cells_are_bitwise_the_fp32_oracle(ctx, mesh, grid, order) -> None:  # noqa: ANN001
+    """Negative source rates drain cells of a dam-break pool below dry_tol
+    while rain falls elsewhere: the blocks' all-wet test and dry flags must
+    follow the post-source depths, not the wet mark taken before the sinks."""
+    gpu, ref = _both(ctx, mesh)
+    n = grid.N
+    nb = num_blocks(n, 256)
+    h, hu, hv = f32_state(*CASES["dam_break"](grid))
+    wet = np.flatnonzero(h > 0.0)
+    sinks = wet[:: max(1, wet.size // 24)]
+    cells = np.arange(n, dtype=np.uint32)
+    rates = np.full(n, 2e-3, dtype=np.float32)
+    rates[sinks] = -1.0
+    for be in (gpu, ref):
+        D.setup(be, n, h, hu, hv, MANNING, dry_tol=DRY_TOL, src_cells=cells, src_rates=rates,
+                flags=D.FLAGS_DEFAULT | order)
+        D.start(be, 3.0, 0.05, 0.05)
+    for k in range(3):
+        gpu.steps(20, use_graph=True)
+        ref.steps(20)
+        D.assert_same(D.snapshot(gpu, n, nb), D.snapshot(ref, n, nb), f"block {k}")
+    assert (gpu.download(L.FIELD_H, n)[sinks] < DRY_TOL).any()  # some sink cells ran dry
+
+
 @pytest.fixture(scope="module")
 def big():  # noqa: ANN201
     g = valley_grid(1024, 512, 2.0)
