@@ -619,6 +619,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // The corrector's fast order takes the source dt bound of all-wet blocks
   // from the cells' perimeters.
   constexpr bool PERIM = ST == 1 && FM && !ATOM;
+  float perim = 0.0f;  // PERIM: the cell's edge lengths summed in CSR order
   if (tid < nc) {
     float sdh = 0.0f, sdhu = 0.0f, sdhv = 0.0f, lam = 0.0f;
     if (ATOM) {
@@ -632,7 +633,14 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       const uint32_t s = l.s_slot[k];
       const uint32_t j = s >> 1;
       const float4 wa = ew4<ST>(l, j);
-      const float2 wb = ew2<ST>(l, j);
+      float2 wb;
+      if (PERIM) {
+        const float4 wr = l.edge[2 * j + 1];  // words 4 and 5, sides, length
+        wb = make_float2(wr.x, wr.y);
+        perim += wr.w;
+      } else {
+        wb = ew2<ST>(l, j);
+      }
       if (s & 1u) {
         sdh -= wa.x;
         sdhu -= wa.w;
@@ -751,11 +759,9 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       // Every cell of the block is wet after the sources, so no neighbour in
       // it is dry and each edge term of the bound below is (u + c) len: the
       // sum is (u + c) times the perimeter. A dry neighbour in another block
-      // is k_source_fix's, as below. The perimeter is the cell's edge lengths
-      // summed in CSR order.
+      // is the source fix's, as below. The perimeter was summed beside the
+      // edge words above.
       if (tid < nc && rate != 0.0f) {
-        float perim = 0.0f;
-        for (uint32_t k = k0; k < k1; ++k) perim += l.edge[2 * (l.s_slot[k] >> 1) + 1].w;
         const float u = sqrtf(fmaf(hui, hui, hvi * hvi)) / hi;
         const float c = p.sqrt_g * sqrtf(hi);
         const float lam = (u + c) * perim;
