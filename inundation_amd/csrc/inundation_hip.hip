@@ -98,6 +98,7 @@ struct Args {
   uint32_t src_neg;  // 1 when a source rate may be negative
   // cells
   float *h, *hu, *hv, *hlo, *h1, *hu1, *hv1, *h1lo, *area, *nman, *clamp, *src;
+  float* inv_area;  // 1 / area, read by the fast order instead of area
   const uint32_t* cptr;
   const uint16_t* lidx;
   const int32_t* snb;
@@ -208,6 +209,17 @@ __device__ __forceinline__ void block_atomic_min(uint32_t* dst, uint32_t bits) {
 // Element idx of a device array addressed as base + a 32-bit byte offset
 // (every array is below 4 GiB), so the address is a scalar base plus one
 // vector offset shared by the arrays read at the same index.
+// dt bound bits of a cell over its summed wave speeds lam: cfl A / lam for
+// lam > 1e-30 and A > 1e-10 (reference resident.py:509, :604, :726), else
+// none. The fast order reads the cell's inverse area aw = 1 / A and takes
+// cfl / (aw lam), guarded by aw < 1e10; the exact order reads aw = A.
+template <bool FM>
+__device__ __forceinline__ uint32_t cell_bound(const Phys& p, float aw, float lam) {
+  if (!(lam > 1e-30f)) return kNoBound;
+  if (FM) return aw < 1e10f ? __float_as_uint(p.cfl / (aw * lam)) : kNoBound;
+  return aw > 1e-10f ? __float_as_uint(p.cfl * aw / lam) : kNoBound;
+}
+
 template <typename T>
 __device__ __forceinline__ T ldg(const T* base, uint32_t idx) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (idx * (uint32_t)sizeof(T)));
@@ -536,7 +548,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (tid < bi.n_ghost) gs = load_ghost<ST, LO>(a, gid);
   const uint32_t i = c0 + tid;
-  float hi = 0.0f, hui = 0.0f, hvi = 0.0f, loi = 0.0f, A = 1.0f, nm = 0.0f;
+  // aw: the cell's area, or its inverse in the fast order
+  float hi = 0.0f, hui = 0.0f, hvi = 0.0f, loi = 0.0f, aw = 1.0f, nm = 0.0f;
   float h0 = 0.0f, hu0 = 0.0f, hv0 = 0.0f, lo0 = 0.0f, rate = 0.0f;
   uint32_t k0 = 0, k1 = 0;
   if (tid < nc) {
@@ -548,7 +561,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       k0 = ldg(a.cptr, i) - bi.s0;
       k1 = ldg(a.cptr, i + 1) - bi.s0;
     }
-    A = ldg(a.area, i);
+    aw = ldg(FM ? a.inv_area : a.area, i);
     nm = ldg(a.nman, i);
     if (ST == 1) {
       h0 = ldg(a.h, i);
@@ -652,7 +665,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       }
       if (ST == 1) lam += wb.y;
     }
-    const float dti = dt / A;
+    const float dti = FM ? dt * aw : dt / aw;
     float h_new, lo_new = 0.0f, h_raw;
     if (COMP) {
       inu::add_compensated(hi, loi, -dti * sdh, h_new, lo_new);
@@ -678,7 +691,10 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       t[5 * q] = hv_new;
       t[6 * q] = (h_raw < 0.0f) ? 1.0f : 0.0f;
     }
-    if (a.track_clamp) stg(a.clamp, i, ldg(a.clamp, i) + 0.5f * A * (h_new - h_raw));
+    if (a.track_clamp) {
+      const float A = FM ? ldg(a.area, i) : aw;
+      stg(a.clamp, i, ldg(a.clamp, i) + 0.5f * A * (h_new - h_raw));
+    }
     if (h_new < p.dry) {
       hu_new = 0.0f;
       hv_new = 0.0f;
@@ -792,7 +808,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
         if (lam > 1e-30f) lam_b = inu::gmax(lam_b, lam);
       }
     }
-    if (tid < nc && lam_b > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam_b);
+    if (tid < nc) bound = cell_bound<FM>(p, aw, lam_b);
   }
   return bound;
 }
@@ -924,11 +940,8 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
         perim += a.slen[s];
       }
       if (FM && all_wet) lam = (u + c) * perim;
-      const float A = a.area[i];
-      if (lam > 1e-30f && A > 1e-10f) {
-        const uint32_t bb = __float_as_uint(p.cfl * A / lam);
-        bound = (bb < bound) ? bb : bound;
-      }
+      const uint32_t bb = cell_bound<FM>(p, FM ? a.inv_area[i] : a.area[i], lam);
+      bound = (bb < bound) ? bb : bound;
     }
   }
   if (cur_s != ~0u) block_atomic_min(&a.scen[cur_s].dt_next, bound);
@@ -969,11 +982,10 @@ __device__ __forceinline__ void fix_pair(const Args& a, uint32_t i) {
     if (FM) lam = fmaf(fmaf(kf, c, u), a.slen[s], lam);
     else lam += (u + kf * c) * a.slen[s];
   }
-  const float A = a.area[i];
-  if (lam > 1e-30f && A > 1e-10f) {
+  const uint32_t bb = cell_bound<FM>(p, FM ? a.inv_area[i] : a.area[i], lam);
+  if (bb != kNoBound) {
     // Rare: one atomic per such pair, into its scenario's slot. A scenario
     // at its stop ignores the slot (k_step_control).
-    const uint32_t bb = __float_as_uint(p.cfl * A / lam);
     uint32_t* slot = &a.scen[bi_scen(a.binfo[i >> 8])].dt_next;
     if (bb < __atomic_load_n(slot, __ATOMIC_RELAXED)) atomicMin(slot, bb);
   }
@@ -1272,8 +1284,7 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
       const uint32_t i = c0 + tid;
       float lam = 0.0f;
       for (uint32_t k = a.cptr[i]; k < a.cptr[i + 1]; ++k) lam += l.edge[2 * (a.lidx[k] >> 1)].x;
-      const float A = a.area[i];
-      if (lam > 1e-30f && A > 1e-10f) bound = __float_as_uint(p.cfl * A / lam);
+      bound = cell_bound<FM>(p, FM ? a.inv_area[i] : a.area[i], lam);
     }
     best = (bound < best) ? bound : best;
   }
@@ -2340,6 +2351,9 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   std::vector<uint32_t> bptr(NB + 1, 0), bidx;
   bidx.reserve(ring_tot);
   std::vector<float> area(NP, 0.0f);
+  // The fast order's inverse areas: 1 / A of every real cell (the same float
+  // division as the oracle's), 0 in the padding.
+  std::vector<float> inv_area(NP, 0.0f);
   std::vector<int32_t> e_cl, e_cr;
   e_cl.reserve(e_tot);
   e_cr.reserve(e_tot);
@@ -2380,6 +2394,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
     slen.insert(slen.end(), L.slen.begin(), L.slen.end());
     for (const auto& pr : L.cross) cross.emplace_back(c0 + pr.first, c0 + pr.second);
     std::copy(m.area, m.area + L.N, area.begin() + c0);
+    for (uint32_t i = 0; i < L.N; ++i) inv_area[c0 + i] = 1.0f / m.area[i];
     s_off += L.S;
   }
 
@@ -2401,6 +2416,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   rc |= dalloc(c, &a.h, NP); rc |= dalloc(c, &a.hu, NP); rc |= dalloc(c, &a.hv, NP); rc |= dalloc(c, &a.hlo, NP);
   rc |= dalloc(c, &a.h1, NP); rc |= dalloc(c, &a.hu1, NP); rc |= dalloc(c, &a.hv1, NP); rc |= dalloc(c, &a.h1lo, NP);
   rc |= dalloc(c, &a.area, NP); rc |= dalloc(c, &a.nman, NP); rc |= dalloc(c, &a.clamp, NP); rc |= dalloc(c, &a.src, NP);
+  rc |= dalloc(c, &a.inv_area, NP);
   uint32_t* dcptr; uint16_t* dlid; int32_t* dsn; float* dsl;
   rc |= dalloc(c, &dcptr, NP + 1); rc |= dalloc(c, &dlid, lidx.size()); rc |= dalloc(c, &dsn, snb.size());
   rc |= dalloc(c, &dsl, slen.size());
@@ -2490,6 +2506,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   std::vector<float> nm(NP, 0.03f);
   if ((rc = upload(c, a.nman, nm.data(), NP))) return rc;
   if ((rc = upload(c, a.area, area.data(), NP))) return rc;
+  if ((rc = upload(c, a.inv_area, inv_area.data(), NP))) return rc;
   if ((rc = upload(c, dcptr, cptr.data(), NP + 1))) return rc;
   if ((rc = upload(c, dlid, lidx.data(), lidx.size()))) return rc;
   if ((rc = upload(c, dsn, snb.data(), snb.size()))) return rc;

@@ -505,17 +505,16 @@ __device__ __forceinline__ float rcbrt(float x) {
 }
 
 // Semi-implicit Manning factor 1 / (1 + dt g n^2 |u| h^(-4/3)) with film drag
-// applied first; (hu, hv) updated in place for a wet cell.
+// applied first; (hu, hv) updated in place for a wet cell. |u| is
+// |(hu, hv)| h^-1 with h^-1 taken as r^3 from the inverse cube root r that
+// the h^(-4/3) factor needs anyway: no division.
 __device__ __forceinline__ void friction(const Phys& p, float h, float n, float dt, float& hu, float& hv) {
   const float drag = fast::film_drag(p, h, dt);
   hu *= drag;
   hv *= drag;
-  const float ih = 1.0f / h;
-  const float u = hu * ih;
-  const float v = hv * ih;
-  const float spd = sqrtf(fmaf(u, u, v * v));
   const float r = rcbrt(h);
   const float r2 = r * r;
+  const float spd = sqrtf(fmaf(hu, hu, hv * hv)) * (r2 * r);
   const float Sf = p.g * n * n * spd * (r2 * r2);
   const float id = 1.0f / fmaf(dt, Sf, 1.0f);
   hu *= id;

@@ -427,11 +427,9 @@ inline void friction(const K& k, float h, float n, float dt, float& hu, float& h
   float drag = film(k, h, dt);
   hu *= drag;
   hv *= drag;
-  float ih = 1.0f / h;
-  float u = hu * ih, v = hv * ih;
-  float spd = std::sqrt(std::fma(u, u, v * v));
   float r = rcbrt(h);
   float r2 = r * r;
+  float spd = std::sqrt(std::fma(hu, hu, hv * hv)) * (r2 * r);  // |u| with 1/h as r^3
   float Sf = k.g * n * n * spd * (r2 * r2);
   float id = 1.0f / std::fma(dt, Sf, 1.0f);
   hu *= id;
@@ -446,6 +444,7 @@ struct cref {
   uint32_t N = 0, E = 0, NB = 0, NWG = 0;
   std::vector<int32_t> ecL, ecR;
   std::vector<float> enx, eny, elen, edz, area;
+  std::vector<float> ia;  // 1 / area, for the fast order
   std::vector<uint32_t> cptr, cidx, erange, bptr, bidx;
   // state
   std::vector<float> h, hu, hv, h1, hu1, hv1, hlo, h1lo, nman, clamp, ef;
@@ -466,6 +465,16 @@ struct cref {
 namespace {
 
 inline bool block_active(const cref& r, uint32_t b) { return r.blk[3 * b + 1] != 0u; }
+
+// dt bound bits of cell i over its summed wave speeds lam: cfl A / lam for
+// lam > 1e-30 and A > 1e-10 (resident.py:509, :604, :726), else none. The
+// fast order takes cfl / (A^-1 lam) with the cell's inverse area, guarded by
+// A^-1 < 1e10.
+inline uint32_t cell_bound(const cref& r, const Pc& pc, uint32_t i, float lam) {
+  if (!(lam > 1e-30f)) return 0xffffffffu;
+  if (r.fast) return r.ia[i] < 1e10f ? fbits(pc.cfl / (r.ia[i] * lam)) : 0xffffffffu;
+  return r.area[i] > 1e-10f ? fbits(pc.cfl * r.area[i] / lam) : 0xffffffffu;
+}
 
 // Low word of the stage input depth (compensated runs; zero otherwise).
 inline float load_lo(const cref& r, int stage, int c) {
@@ -628,7 +637,8 @@ void update(cref& r, int stage) {
     const uint32_t b = i >> 8;
     if (!block_active(r, b)) continue;
     float A = r.area[i];
-    float dti = dt / A;
+    // The fast order multiplies by the cell's inverse area.
+    float dti = r.fast ? dt * r.ia[i] : dt / A;
     float hi, hui, hvi;
     if (stage == 0) { hi = r.h[i]; hui = r.hu[i]; hvi = r.hv[i]; }
     else { hi = r.h1[i]; hui = r.hu1[i]; hvi = r.hv1[i]; }
@@ -722,7 +732,7 @@ void update(cref& r, int stage) {
       r.hu[i] = huf;
       r.hv[i] = hvf;
       if (hf >= pc.dry) __atomic_store_n(&r.blk[3 * b + 0], 1u, __ATOMIC_RELAXED);
-      if (lam > 1e-30f && A > 1e-10f) best = std::min(best, fbits(pc.cfl * A / lam));
+      best = std::min(best, cell_bound(r, pc, i, lam));
     }
   }
   if (stage == 1) r.dtbuf[1] = std::min(r.dtbuf[1], best);
@@ -780,7 +790,7 @@ void source(cref& r) {
       perim += r.elen[e];
     }
     if (F && all_wet[i >> 8] && !any_dry) lam = (u + c) * perim;
-    if (lam > 1e-30f && r.area[i] > 1e-10f) best = std::min(best, fbits(pc.cfl * r.area[i] / lam));
+    best = std::min(best, cell_bound(r, pc, i, lam));
   }
   r.dtbuf[1] = std::min(r.dtbuf[1], best);
 }
@@ -850,7 +860,7 @@ void cfl_prologue(cref& r) {
     uint32_t i = (uint32_t)ii;
     float lam = 0.0f;
     for (uint32_t k = r.cptr[i]; k < r.cptr[i + 1]; ++k) lam += r.ef[(size_t)kWords * (r.cidx[k] >> 1) + kCfl];
-    if (lam > 1e-30f && r.area[i] > 1e-10f) best = std::min(best, fbits(pc.cfl * r.area[i] / lam));
+    best = std::min(best, cell_bound(r, pc, i, lam));
   }
   r.dtbuf[0] = std::min(r.dtbuf[0], best);
 }
@@ -909,6 +919,8 @@ cref* cref_create(uint32_t N, uint32_t E, const int32_t* ecL, const int32_t* ecR
   r->elen.assign(len, len + E);
   r->edz.assign(dz2, dz2 + 2 * (size_t)E);
   r->area.assign(area, area + N);
+  r->ia.resize(N);
+  for (uint32_t i = 0; i < N; ++i) r->ia[i] = 1.0f / area[i];
   r->cptr.assign(cptr, cptr + N + 1);
   r->cidx.assign(cidx, cidx + cptr[N]);
   r->erange.assign(erange, erange + 2 * (size_t)r->NWG);
