@@ -786,7 +786,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     } else if (a.n_src != 0u) {
       // Next-step dt bound of the block's wetted source cells
       // (resident.py:709-728) from the post-source depths, with every
-      // neighbour in another block taken as wet; k_source_fix recomputes the
+      // neighbour in another block taken as wet; the source fix recomputes the
       // cells where such a neighbour is dry.
       if (tid < nc) l.cell[2 * tid].x = hi;
       __syncthreads();
@@ -873,7 +873,7 @@ __global__ __launch_bounds__(256, TILE == 0 ? (ST == 0 ? 6 : 5) : 1) void k_stag
 }
 
 // Sources of blocks the active set skipped this step (reference
-// resident.py:691-708), listed by k_dilate in `idle`: nothing else reads or
+// resident.py:691-708), listed by the dilation in `idle`: nothing else reads or
 // writes those cells now. The block's post-source depths then give the dt
 // bound of its wetted source cells (resident.py:709-728), with every
 // neighbour in another block taken as wet, as in the corrector.
@@ -1194,8 +1194,9 @@ __global__ __launch_bounds__(256) void k_dilate(const Args a, int force, int kee
 }
 
 // The end of a step before k_step_control, in one launch: workgroups
-// [0, n_dil) run k_dilate's work, the rest k_source_fix's (independent of
-// each other; both read what the corrector and k_source_idle wrote).
+// [0, n_dil) run the dilation (dilate_wg), the rest the source fix
+// (source_fix_wg). The two are independent of each other; both read what
+// the corrector and k_source_idle wrote.
 template <bool FM>
 __global__ __launch_bounds__(256) void k_step_tail(const Args a, uint32_t n_dil) {
   if (a.ctl->any_dt == 0u) return;  // no-op step changed nothing
@@ -1835,7 +1836,7 @@ int dalloc(inu_ctx* c, T** p, size_t n) {
 }
 
 // New depths from the host: every block may hold a dry cell until a step
-// has looked (k_source_fix reads part_dry).
+// has looked (the source fix reads part_dry).
 int depths_replaced(inu_ctx* c) {
   HIPCHK(c, hipMemsetAsync(c->a.part_dry, 1, (size_t)c->NB * 4, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -2518,7 +2519,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   if ((rc = upload(c, dbptr, bptr.data(), NB + 1))) return rc;
   if ((rc = upload(c, dbidx, bidx.data(), bidx.size()))) return rc;
   // Grouped by the neighbour's block once here, so that every source list
-  // keeps the groups k_source_fix walks without sorting again.
+  // keeps the groups the source fix walks without sorting again.
   std::stable_sort(cross.begin(), cross.end(), [](const std::pair<uint32_t, uint32_t>& x,
                                                   const std::pair<uint32_t, uint32_t>& y) {
     return x.second / kBlockCells < y.second / kBlockCells;
@@ -2715,7 +2716,7 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
     if (dense[pr.first] != 0.0f) pairs.push_back(make_uint2(pr.first, pr.second));
   // The groups of pairs whose neighbours share a block (c->cross is in that
   // order), and the groups each chunk of kFixChunk pairs touches, for
-  // k_source_fix.
+  // the source fix.
   std::vector<uint32_t> fix_blk, pair_group(pairs.size());
   for (uint32_t k = 0; k < pairs.size(); ++k) {
     if (k == 0 || pairs[k].y / kBlockCells != pairs[k - 1].y / kBlockCells) fix_blk.push_back(pairs[k].y / kBlockCells);
