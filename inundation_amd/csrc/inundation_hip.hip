@@ -235,7 +235,9 @@ __device__ __forceinline__ void stg(T* base, uint32_t idx, T v) {
 // record: its words {w0, w1, w2, w3}, {w4, w5, sides word, length}. Then the
 // block's CSR slots (local edge numbers).
 struct Lds {
-  float4* cell;  // [kBlockCells + max_ghost][2]: the cells, then the ghosts
+  float4* cell;  // the cells, then the ghosts: [kBlockCells + max_ghost][2] in
+                 // carve(); the stage kernels' (h, hu, hv, lo) only
+  float2* cx;    // the stage kernels' fast-order (vel_factor, sqrt h) per cell
   float4* edge;  // [max_local][2]; the predictor's: [max_local] words 0-3
   float2* e2;    // the predictor's [max_local] words 4-5 (carve_stage<0>)
   uint16_t* s_slot;
@@ -269,21 +271,38 @@ __device__ __forceinline__ Lds carve(float* sm, uint32_t max_local, uint32_t max
   return l;
 }
 
+// The stage kernels keep 24 bytes per cell: (h, hu, hv, lo) and, for the fast
+// order, (vel_factor, sqrt h) in a second array; the pressure half_g h^2 is
+// formed again where an edge needs it. Cell count rounded up to even, so the
+// edge records after them stay 16-byte aligned.
+__host__ __device__ inline uint32_t stage_cells(uint32_t max_ghost) { return (kBlockCells + max_ghost + 1u) & ~1u; }
+
 // The predictor reads only an edge's six words, so its records are two
 // arrays (16 + 8 bytes per edge instead of 32): with less LDS per block,
 // one more workgroup fits on a CU.
 __host__ __device__ inline size_t lds_words_stage0(uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
-  return 8 * (size_t)(kBlockCells + max_ghost) + 6 * (size_t)max_local + (max_slots + 1) / 2;
+  return 6 * (size_t)stage_cells(max_ghost) + 6 * (size_t)max_local + (max_slots + 1) / 2;
+}
+// The corrector keeps 32-byte edge records: words 0-5, sides word, length.
+__host__ __device__ inline size_t lds_words_stage1(uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
+  return 6 * (size_t)stage_cells(max_ghost) + 8 * (size_t)max_local + (max_slots + 1) / 2;
 }
 
 template <int ST>
 __device__ __forceinline__ Lds carve_stage(float* sm, uint32_t max_local, uint32_t max_ghost) {
-  if (ST != 0) return carve(sm, max_local, max_ghost);
+  const uint32_t kc = stage_cells(max_ghost);
   Lds l;
   l.cell = reinterpret_cast<float4*>(sm);
-  l.edge = reinterpret_cast<float4*>(sm + 8 * (kBlockCells + max_ghost));
-  l.e2 = reinterpret_cast<float2*>(sm + 8 * (kBlockCells + max_ghost) + 4 * max_local);
-  l.s_slot = reinterpret_cast<uint16_t*>(sm + 8 * (kBlockCells + max_ghost) + 6 * max_local);
+  l.cx = reinterpret_cast<float2*>(sm + 4 * kc);
+  float* e = sm + 6 * kc;
+  l.edge = reinterpret_cast<float4*>(e);
+  if (ST == 0) {
+    l.e2 = reinterpret_cast<float2*>(e + 4 * max_local);
+    l.s_slot = reinterpret_cast<uint16_t*>(e + 6 * max_local);
+  } else {
+    l.e2 = reinterpret_cast<float2*>(l.edge) + 2;
+    l.s_slot = reinterpret_cast<uint16_t*>(e + 8 * max_local);
+  }
   return l;
 }
 
@@ -343,12 +362,11 @@ __device__ __forceinline__ void rec_ghost(const Args& a, uint32_t g, float4 s) {
 }
 
 // Stage input of one cell or ghost into LDS record q, with the fast order's
-// per-depth vel_factor, square root and hydrostatic pressure.
+// per-depth vel_factor and square root.
 template <bool FM>
 __device__ __forceinline__ void put_cell(const Lds& l, const Phys& p, uint32_t q, float4 s) {
-  l.cell[2 * q] = s;
-  if (FM) l.cell[2 * q + 1] = make_float4(inu::fast::vel_factor(p, s.x), sqrtf(inu::gmax(s.x, 0.0f)),
-                                          p.half_g * s.x * s.x, 0.0f);
+  l.cell[q] = s;
+  if (FM) l.cx[q] = make_float2(inu::fast::vel_factor(p, s.x), sqrtf(inu::gmax(s.x, 0.0f)));
 }
 
 // The words of an open edge with right side (hR, huR, hvR) beyond it; `wall`
@@ -446,21 +464,23 @@ __device__ __forceinline__ void edge_words_lds(const Args& a, const Lds& l, cons
   const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
   const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
   for (int q = 0; q < 6; ++q) w[q] = 0.0f;
-  const float4 sl = l.cell[2 * ql];
+  const float4 sl = l.cell[ql];
   float4 sr;
   if (wall) {
     sr = sl;
     if (FM) inu::fast::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
     else inu::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
   } else {
-    sr = l.cell[2 * qr];
+    sr = l.cell[qr];
   }
   if (!(sl.x < a.p.dry && sr.x < a.p.dry)) {
     if (FM) {
-      const float4 xl = l.cell[2 * ql + 1];
-      const float4 xr = wall ? xl : l.cell[2 * qr + 1];
-      const inu::fast::Side L{sl.x, sl.y, sl.z, LO ? sl.w : 0.0f, xl.x, xl.y, xl.z};
-      const inu::fast::Side R{sr.x, sr.y, sr.z, LO ? sr.w : 0.0f, xr.x, xr.y, xr.z};
+      const float2 xl = l.cx[ql];
+      const float2 xr = wall ? xl : l.cx[qr];
+      const float pl = a.p.half_g * sl.x * sl.x;
+      const float pr = wall ? pl : a.p.half_g * sr.x * sr.x;
+      const inu::fast::Side L{sl.x, sl.y, sl.z, LO ? sl.w : 0.0f, xl.x, xl.y, pl};
+      const inu::fast::Side R{sr.x, sr.y, sr.z, LO ? sr.w : 0.0f, xr.x, xr.y, pr};
       inu::fast::edge_words(a.p, wall, L, R, dzL, dzR, r.nx, r.ny, r.len, w);
     } else {
       inu::edge_words(a.p, wall, sl.x, sl.y, sl.z, sr.x, sr.y, sr.z, dzL, dzR, r.nx, r.ny, r.len, w);
@@ -788,7 +808,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       // (resident.py:709-728) from the post-source depths, with every
       // neighbour in another block taken as wet; the source fix recomputes the
       // cells where such a neighbour is dry.
-      if (tid < nc) l.cell[2 * tid].x = hi;
+      if (tid < nc) l.cell[tid].x = hi;
       __syncthreads();
       if (tid < nc && rate != 0.0f && hi >= p.dry) {
         const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hi : sqrtf(hui * hui + hvi * hvi) / hi;
@@ -800,7 +820,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
           const float2 sd = reinterpret_cast<const float2*>(&l.edge[2 * j + 1])[1];  // sides, length
           const uint32_t pk = __float_as_uint(sd.x);
           const uint32_t q = (s & 1u) ? (pk >> 16) : (pk & 0xffffu);
-          const bool dry_side = q < kBlockCells && l.cell[2 * q].x < p.dry;  // walls and ghosts: wet
+          const bool dry_side = q < kBlockCells && l.cell[q].x < p.dry;  // walls and ghosts: wet
           const float kf = dry_side ? 2.0f : 1.0f;
           if (FM) lam = fmaf(fmaf(kf, c, u), sd.y, lam);
           else lam += (u + kf * c) * sd.y;
@@ -824,7 +844,7 @@ __device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t en
 }
 
 template <int ST, bool COMP, bool FM, int TILE, bool REC = false, bool ATOM = false>
-__global__ __launch_bounds__(256, TILE == 0 ? (ST == 0 ? 6 : 5) : 1) void k_stage(const Args a) {
+__global__ __launch_bounds__(256, TILE == 0 ? 6 : 1) void k_stage(const Args a) {
   extern __shared__ float sm[];
   if (a.ctl->any_dt == 0u) return;  // no-op step: every scenario is at its stop
   const Lds l = carve_stage<ST>(sm, a.max_local, a.max_ghost);
@@ -1765,7 +1785,8 @@ struct inu_ctx {
   uint32_t grid_stage = 0;        // default corrector grid (reported by inu_mesh_info)
   uint32_t grid[2][2][2][kTiles] = {};  // [stage][compensated][fast order][tile]
   int tile = 0;                         // stage kernel tile level (inu_autotune)
-  size_t lds_stage0 = 0, lds_stage1 = 0;
+  size_t lds_stage0 = 0, lds_stage1 = 0;  // the predictor's and the corrector's
+  size_t lds_full = 0;                     // the prologue's and the adjoint's (carve())
   // (cell, neighbour) for every neighbour in another block, by cell
   std::vector<std::pair<uint32_t, uint32_t>> cross;
   // Edges (union numbering), for the bed gradient and inu_set_bed: cells
@@ -2532,8 +2553,9 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   c->h_ldz.swap(ldz);
   c->src_nonneg = true;
   c->lds_stage0 = lds_words_stage0(max_local, max_slots, max_ghost) * sizeof(float);
-  c->lds_stage1 = lds_words(max_local, max_slots, max_ghost) * sizeof(float);
-  if (c->lds_stage1 > 160 * 1024) return fail(c, INU_ERANGE, "a block's edge list does not fit in LDS");
+  c->lds_stage1 = lds_words_stage1(max_local, max_slots, max_ghost) * sizeof(float);
+  c->lds_full = lds_words(max_local, max_slots, max_ghost) * sizeof(float);
+  if (c->lds_full > 160 * 1024) return fail(c, INU_ERANGE, "a block's edge list does not fit in LDS");
   // Persistent grids: as many workgroups as fit at once, a multiple of the
   // XCD count, per stage kernel variant.
   const void* variants[2][2][2][kTiles] = {
@@ -2845,11 +2867,11 @@ int inu_prologue(inu_ctx* c) {
   hipLaunchKernelGGL(k_dilate, dim3(a.n_dwg), dim3(256), 0, c->stream, a, 1, act ? 0 : 1);
   const uint32_t gp = std::min<uint32_t>(c->NB, c->grid_stage);
   if (c->flags & INU_FLAG_EXACT_ORDER)
-    hipLaunchKernelGGL((k_prologue_cfl<false, false>), dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
+    hipLaunchKernelGGL((k_prologue_cfl<false, false>), dim3(gp), dim3(256), c->lds_full, c->stream, a);
   else if (c->flags & INU_FLAG_COMPENSATED)
-    hipLaunchKernelGGL((k_prologue_cfl<true, true>), dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
+    hipLaunchKernelGGL((k_prologue_cfl<true, true>), dim3(gp), dim3(256), c->lds_full, c->stream, a);
   else
-    hipLaunchKernelGGL((k_prologue_cfl<true, false>), dim3(gp), dim3(256), c->lds_stage1, c->stream, a);
+    hipLaunchKernelGGL((k_prologue_cfl<true, false>), dim3(gp), dim3(256), c->lds_full, c->stream, a);
   hipLaunchKernelGGL(k_step_control, dim3(1), dim3(256), 0, c->stream, a, 0);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -3374,8 +3396,8 @@ static int adj_enqueue_reverse(inu_ctx* c, uint32_t m) {
   const uint32_t gl = (c->NB + 255) / 256;
   for (uint32_t j = 0; j < m; ++j) {
     hipLaunchKernelGGL(k_adj_lists, dim3(gl), dim3(256), 0, c->stream, c->a, st);
-    hipLaunchKernelGGL(k_adj_stage<1>, dim3(d.grid_adj), dim3(256), c->lds_stage1, c->stream, x);
-    hipLaunchKernelGGL(k_adj_stage<0>, dim3(d.grid_adj), dim3(256), c->lds_stage1, c->stream, x);
+    hipLaunchKernelGGL(k_adj_stage<1>, dim3(d.grid_adj), dim3(256), c->lds_full, c->stream, x);
+    hipLaunchKernelGGL(k_adj_stage<0>, dim3(d.grid_adj), dim3(256), c->lds_full, c->stream, x);
     hipLaunchKernelGGL(k_adj_end, dim3(1), dim3(256), 0, c->stream, c->a, d.cur, -1, nullptr, d.gauges, d.n_gauges,
                        d.every, d.max_samples);
   }
@@ -3556,7 +3578,7 @@ static int adj_work(inu_ctx* c) {
   rc |= dalloc(c, &d.rseries, (size_t)std::max(d.max_samples, 1u) * d.n_gauges);
   if (rc) return fail(c, INU_EHIP, "adjoint: out of device memory (the tape needs segment x 56 B per cell)");
   int occ = 0;
-  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_adj_stage<0>, 256, c->lds_stage1));
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_adj_stage<0>, 256, c->lds_full));
   d.grid_adj = std::max<uint32_t>(1u, std::min<uint32_t>(c->NB, (uint32_t)std::max(occ, 1) * c->cus));
   return 0;
 }
