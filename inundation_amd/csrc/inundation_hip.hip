@@ -1783,6 +1783,7 @@ struct inu_ctx {
   Ctl* ctl = nullptr;
   // launch shape
   uint32_t grid_stage = 0;        // default corrector grid (reported by inu_mesh_info)
+  uint32_t grid_prologue = 0;     // k_prologue_cfl's grid
   uint32_t grid[2][2][2][kTiles] = {};  // [stage][compensated][fast order][tile]
   int tile = 0;                         // stage kernel tile level (inu_autotune)
   size_t lds_stage0 = 0, lds_stage1 = 0;  // the predictor's and the corrector's
@@ -2588,6 +2589,14 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
         }
   c->tile = 0;
   c->grid_stage = c->grid[1][1][1][0];
+  {
+    // The prologue keeps the two-record cell layout (carve()); its grid is
+    // what fits with that LDS.
+    int occ = 0;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_prologue_cfl<true, true>, 256,
+                                                            c->lds_full));
+    c->grid_prologue = std::max<uint32_t>(1u, (uint32_t)std::max(occ, 1) * c->cus);
+  }
   c->a.p = c->p;
   c->has_mesh = true;
   return 0;
@@ -2865,7 +2874,7 @@ int inu_prologue(inu_ctx* c) {
   const uint32_t gm = (std::max(c->NB, c->n_scen) + 255) / 256;
   hipLaunchKernelGGL(k_block_mark, dim3(gm), dim3(256), 0, c->stream, a, act ? 0 : 1);
   hipLaunchKernelGGL(k_dilate, dim3(a.n_dwg), dim3(256), 0, c->stream, a, 1, act ? 0 : 1);
-  const uint32_t gp = std::min<uint32_t>(c->NB, c->grid_stage);
+  const uint32_t gp = std::min<uint32_t>(c->NB, c->grid_prologue);
   if (c->flags & INU_FLAG_EXACT_ORDER)
     hipLaunchKernelGGL((k_prologue_cfl<false, false>), dim3(gp), dim3(256), c->lds_full, c->stream, a);
   else if (c->flags & INU_FLAG_COMPENSATED)
