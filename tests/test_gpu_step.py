@@ -315,6 +315,43 @@ def test_lists_and_source_fix_over_two_thousand_blocks(ctx, big) -> None:  # noq
     assert (w["steps"], w["cells"], w["edges"], w["slots"]) == (4, 4 * g.N, 4 * g.E, 4 * int(cptr[g.N]))
 
 
+def test_depths_uploaded_mid_run_reset_the_dry_flags(ctx, big) -> None:  # noqa: ANN001
+    """A lake wet everywhere leaves every block's dry flag clear. Depths
+    uploaded mid-run then empty the whole mesh, and rain falls on every cell
+    of one block only. When its cells turn wet, every dry neighbour they have
+    lies in another block that no step has looked at since the upload, so
+    the dt bound (which sets the step here) is right only if the source fix
+    takes those blocks as possibly dry, which the upload told it. Bitwise
+    the oracle, clock included."""
+    g, m = big
+    gpu, ref = _both(ctx, m)
+    n = g.N
+    nb = num_blocks(n, 256)
+    h, hu, hv = f32_state(*CASES["wet_lake"](g))
+    for be in (gpu, ref):
+        D.setup(be, n, h, hu, hv, MANNING, dry_tol=DRY_TOL)
+        D.start(be, 1.0, 0.05, 0.05)
+    gpu.steps(5, use_graph=False)
+    ref.steps(5)
+    z = np.zeros(n, dtype=np.float32)
+    for which in (L.FIELD_H, L.FIELD_HU, L.FIELD_HV, L.FIELD_H_LO):
+        gpu.upload(which, z)
+    ref.set_state(z, z, z, z)
+    x, y = g.centroid[:, 0], g.centroid[:, 1]
+    b = int(np.argmin((x - 0.8 * x.max()) ** 2 + (y - 0.5 * y.max()) ** 2)) // 256
+    cells = np.arange(256 * b, min(256 * (b + 1), n), dtype=np.uint32)
+    rates = np.full(cells.size, 5e-3, dtype=np.float32)
+    for be in (gpu, ref):
+        be.set_sources(cells, rates)
+        D.start(be, 60.0, 5.0, 0.05)
+    for k in range(4):
+        gpu.steps(4, use_graph=False)
+        ref.steps(4)
+        D.assert_same(D.snapshot(gpu, n, nb), D.snapshot(ref, n, nb), f"after the upload, block {k}")
+    c = gpu.ctrl_read()
+    assert c.dt_ticks > 0.05 * 2**20  # the wetted block's bound took over from the dry-mesh dt
+
+
 def test_active_set_on_equals_off(ctx, mesh, grid) -> None:  # noqa: ANN001
     """Skipping blocks is exact (reference test_gpu_active_set.py:77-91)."""
     gpu, _ = _both(ctx, mesh)
