@@ -233,6 +233,15 @@ int inu_probe(inu_ctx* ctx, uint32_t n, const float* pairs, float* out7, float g
 int inu_probe_open(inu_ctx* ctx, uint32_t n, const float* cases6, const uint32_t* kinds, const float* values,
                    float* out4, float g, float dry_tol);
 
+/* Test probe of the integer clock: step_control (mode 0 START, 1 STEP, 2
+ * RETARGET) once on a hand-set control block and dt words, in place (dt4[0]
+ * the prologue bound bits, dt4[1] the next-step bound bits; on return this
+ * step's dt bits, the sentinel 1e30, 0, 0). The reference tests its kernel
+ * the same way: tests/unit/test_gpu_resident_clock.py:46-104 runs
+ * step_control (resident.py:813-885) on a hand-built control block. Uses
+ * device memory of its own; the context's mesh and clock are not touched. */
+int inu_probe_clock(inu_ctx* ctx, int mode, inu_ctrl* ctrl, uint32_t* dt4);
+
 /* A plain device copy of n_bytes (a positive multiple of 16), launched
  * `reps` times after one untimed launch; *ms_out = mean duration of one copy.
  * Measurement support with no counterpart in the reference: a kernel of known
@@ -240,6 +249,16 @@ int inu_probe_open(inu_ctx* ctx, uint32_t n, const float* cases6, const uint32_t
  * against, and the attainable copy bandwidth. Allocates and frees its own
  * buffers. */
 int inu_stream_copy(inu_ctx* ctx, uint64_t n_bytes, uint32_t reps, float* ms_out);
+
+/* Attainable HBM rate, for the roofline's context (measurement, no reference
+ * counterpart): kind 0 copies n_bytes (read and written), 1 reads them (plus
+ * one word per thread written), 2 writes them, 3 is hipMemcpyAsync device to
+ * device. Kinds 0-2 move `unroll` (1, 2, 4, 8) 16-byte words per thread per
+ * trip, loads before stores, with nontemporal hints, on wg_per_cu 256-thread
+ * workgroups per CU. `reps` launches after one untimed; *ms_out = mean
+ * milliseconds of one. Allocates and frees its own buffers. */
+int inu_stream_bw(inu_ctx* ctx, uint64_t n_bytes, uint32_t reps, int kind, uint32_t unroll, uint32_t wg_per_cu,
+                  float* ms_out);
 
 /* Autotuner of the stage kernels' tiles (batching extension; no
  * reference counterpart). Times `reps` launches of the predictor at each tile

@@ -1721,6 +1721,55 @@ __global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ 
     out[k] = in[k];
 }
 
+// Bandwidth probes of the attainable HBM rate (measurement only). Each thread
+// moves `U` 16-byte words per trip, all loads issued before any store, with
+// nontemporal hints (streamed data is not kept in the caches); the grid is a
+// whole number of workgroups per CU.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int U>
+__global__ __launch_bounds__(256) void k_bw_copy(const f32x4* __restrict__ in, f32x4* __restrict__ out, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256u * U;
+  for (size_t base = (size_t)blockIdx.x * 256u * U + threadIdx.x; base < n; base += stride) {
+    f32x4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + (size_t)u * 256u < n) v[u] = __builtin_nontemporal_load(in + base + (size_t)u * 256u);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + (size_t)u * 256u < n) __builtin_nontemporal_store(v[u], out + base + (size_t)u * 256u);
+  }
+}
+
+// Read only: every word summed, one word per thread written at the end.
+template <int U>
+__global__ __launch_bounds__(256) void k_bw_read(const f32x4* __restrict__ in, float* __restrict__ out, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256u * U;
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (size_t base = (size_t)blockIdx.x * 256u * U + threadIdx.x; base < n; base += stride) {
+    f32x4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      v[u] = (base + (size_t)u * 256u < n) ? __builtin_nontemporal_load(in + base + (size_t)u * 256u)
+                                            : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += v[u];
+  }
+  out[(size_t)blockIdx.x * 256u + threadIdx.x] = acc.x + acc.y + acc.z + acc.w;
+}
+
+// Write only.
+template <int U>
+__global__ __launch_bounds__(256) void k_bw_write(f32x4* __restrict__ out, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256u * U;
+  const f32x4 v = {1.0f, 2.0f, 3.0f, 4.0f};
+  for (size_t base = (size_t)blockIdx.x * 256u * U + threadIdx.x; base < n; base += stride) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + (size_t)u * 256u < n) __builtin_nontemporal_store(v, out + base + (size_t)u * 256u);
+  }
+}
+
 // Arithmetic probes: pairs (a, b) -> two-sum error, a / b, sqrt|a|, a^(4/3),
 // film drag of depth a at dt b, a^(-1/3), fast film drag (reference
 // solver.py:449-467 checks that the two-sum error term survives compilation).
@@ -2761,60 +2810,69 @@ int inu_set_sources(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* r
   std::vector<uint32_t> blocks;
   for (uint32_t b = 0; b < c->NB; ++b)
     if (has[b]) blocks.push_back(b);
-  // Replace the source lists (their sizes change).
+  // The new lists go to fresh device buffers first; the old ones stay in
+  // place (with their counts) until every allocation and upload of the new
+  // ones has succeeded, so a failure leaves the context as it was.
+  struct List {
+    const void* host;
+    size_t bytes;
+    void* dev;
+  };
+  List fresh[5] = {{cells.data(), cells.size() * 4, nullptr},
+                   {blocks.data(), blocks.size() * 4, nullptr},
+                   {pairs.data(), pairs.size() * 8, nullptr},
+                   {fix_blk.data(), fix_blk.size() * 4, nullptr},
+                   {fix_chunk.data(), fix_chunk.size() * 8, nullptr}};
+  hipError_t e = hipSuccess;
+  for (List& f : fresh) {
+    if (e == hipSuccess) e = hipMalloc(&f.dev, f.bytes ? f.bytes : 16);
+    if (e == hipSuccess && f.bytes) e = hipMemcpyAsync(f.dev, f.host, f.bytes, hipMemcpyHostToDevice, c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    for (List& f : fresh)
+      if (f.dev) (void)hipFree(f.dev);
+    return fail(c, INU_EHIP, std::string("inu_set_sources: ") + hipGetErrorString(e));
+  }
+  // Swap: free the old lists (the stream is idle after the synchronisation).
   for (void* q : {(void*)c->a.src_cell, (void*)c->a.src_blocks, (void*)c->a.src_pair, (void*)c->a.fix_blk,
                   (void*)c->a.fix_chunk}) {
     auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
     if (it != c->allocs.end()) {
-      HIPCHK(c, hipStreamSynchronize(c->stream));
       (void)hipFree(q);
       c->allocs.erase(it);
     }
   }
-  c->a.src_cell = nullptr;
-  c->a.src_blocks = nullptr;
-  c->a.src_pair = nullptr;
-  c->a.fix_blk = nullptr;
-  c->a.fix_chunk = nullptr;
-  c->a.n_fix = 0;
-  uint32_t *dcell = nullptr, *dblk = nullptr, *dfblk = nullptr;
-  uint2 *dpair = nullptr, *dfchunk = nullptr;
-  int rc2 = dalloc(c, &dcell, cells.size());
-  if (rc2) return rc2;
-  rc2 = dalloc(c, &dblk, blocks.size());
-  if (rc2) return rc2;
-  rc2 = dalloc(c, &dpair, pairs.size());
-  if (rc2) return rc2;
-  rc2 = dalloc(c, &dfblk, fix_blk.size());
-  if (rc2) return rc2;
-  rc2 = dalloc(c, &dfchunk, fix_chunk.size());
-  if (rc2) return rc2;
-  if ((rc = upload(c, dpair, pairs.data(), pairs.size()))) return rc;
-  if ((rc = upload(c, dfblk, fix_blk.data(), fix_blk.size()))) return rc;
-  if ((rc = upload(c, dfchunk, fix_chunk.data(), fix_chunk.size()))) return rc;
-  c->a.src_pair = dpair;
-  c->a.fix_blk = dfblk;
-  c->a.fix_chunk = dfchunk;
+  for (List& f : fresh) c->allocs.push_back(f.dev);
+  c->a.src_cell = static_cast<const uint32_t*>(fresh[0].dev);
+  c->a.src_blocks = static_cast<const uint32_t*>(fresh[1].dev);
+  c->a.src_pair = static_cast<const uint2*>(fresh[2].dev);
+  c->a.fix_blk = static_cast<const uint32_t*>(fresh[3].dev);
+  c->a.fix_chunk = static_cast<const uint2*>(fresh[4].dev);
   c->a.n_fix = (uint32_t)fix_blk.size();
   c->n_src_pairs = (uint32_t)pairs.size();
   c->a.n_src_pairs = c->n_src_pairs;
-  if ((rc = upload(c, dcell, cells.data(), cells.size()))) return rc;
-  if ((rc = upload(c, dblk, blocks.data(), blocks.size()))) return rc;
-  if ((rc = upload(c, c->a.src, dense.data(), dense.size()))) return rc;
-  if ((rc = upload(c, c->a.has_src, has.data(), has.size()))) return rc;
-  c->a.src_cell = dcell;
-  c->a.src_blocks = dblk;
-  c->src_cells = cells;
-  c->src_nonneg = true;
-  for (uint32_t k = 0; k < n; ++k)
-    if (rate[k] < 0.0f) c->src_nonneg = false;
-  c->a.src_neg = c->src_nonneg ? 0u : 1u;
   c->n_src = (uint32_t)cells.size();
   c->n_src_blocks = (uint32_t)blocks.size();
   c->a.n_src = c->n_src;
   c->a.n_src_blocks = c->n_src_blocks;
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   c->graphs.clear();
+  // The dense rates and per-block flags live in buffers of fixed size; if
+  // their upload fails, the context runs with no sources rather than with
+  // lists that disagree with the rates.
+  rc = upload(c, c->a.src, dense.data(), dense.size());
+  if (!rc) rc = upload(c, c->a.has_src, has.data(), has.size());
+  if (rc) {
+    c->n_src = c->n_src_blocks = c->n_src_pairs = 0;
+    c->a.n_src = c->a.n_src_blocks = c->a.n_src_pairs = c->a.n_fix = 0;
+    return rc;
+  }
+  c->src_cells = cells;
+  c->src_nonneg = true;
+  for (uint32_t k = 0; k < n; ++k)
+    if (rate[k] < 0.0f) c->src_nonneg = false;
+  c->a.src_neg = c->src_nonneg ? 0u : 1u;
   return 0;
 }
 
@@ -3038,6 +3096,45 @@ int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out7, float g, 
   return 0;
 }
 
+int inu_probe_clock(inu_ctx* c, int mode, inu_ctrl* ctrl, uint32_t* dt4) {
+  if (!c) return INU_EINVAL;
+  if (!ctrl || !dt4) return fail(c, INU_EINVAL, "inu_probe_clock: null control block or dt words");
+  if (mode < 0 || mode > 2) return fail(c, INU_EINVAL, "inu_probe_clock: mode must be 0 (start), 1 (step) or 2 (retarget)");
+  HIPCHK(c, hipSetDevice(c->device));
+  // A scenario record and control words of their own: the context's mesh,
+  // lists and clock are not touched.
+  Scen sc{};
+  sc.ctrl = *ctrl;
+  sc.dt_cur = dt4[0];
+  sc.dt_next = dt4[1];
+  Ctl ctl{};
+  void* dev = nullptr;
+  HIPCHK(c, hipMalloc(&dev, sizeof(Scen) + sizeof(Ctl)));
+  Scen* dscen = static_cast<Scen*>(dev);
+  Ctl* dctl = reinterpret_cast<Ctl*>(dscen + 1);
+  hipError_t e = hipMemcpyAsync(dscen, &sc, sizeof(Scen), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dctl, &ctl, sizeof(Ctl), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    Args a{};
+    a.scen = dscen;
+    a.ctl = dctl;
+    a.n_scen = 1;
+    a.n_dwg = 0;
+    hipLaunchKernelGGL(k_step_control, dim3(1), dim3(256), 0, c->stream, a, mode);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&sc, dscen, sizeof(Scen), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dev);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("inu_probe_clock: ") + hipGetErrorString(e));
+  *ctrl = sc.ctrl;
+  dt4[0] = sc.dt_cur;
+  dt4[1] = sc.dt_next;
+  dt4[2] = 0u;
+  dt4[3] = 0u;
+  return 0;
+}
+
 int inu_probe_open(inu_ctx* c, uint32_t n, const float* cases6, const uint32_t* kinds, const float* values,
                    float* out4, float g, float dry_tol) {
   if (!c) return INU_EINVAL;
@@ -3096,6 +3193,56 @@ int inu_stream_copy(inu_ctx* c, uint64_t n_bytes, uint32_t reps, float* ms_out) 
   for (void* q : buf)
     if (q) (void)hipFree(q);
   if (e != hipSuccess) return fail(c, INU_EHIP, std::string("stream copy: ") + hipGetErrorString(e));
+  *ms_out = ms / (float)reps;
+  return 0;
+}
+
+int inu_stream_bw(inu_ctx* c, uint64_t n_bytes, uint32_t reps, int kind, uint32_t unroll, uint32_t wg_per_cu,
+                  float* ms_out) {
+  if (!c) return INU_EINVAL;
+  if (!ms_out || reps == 0u || n_bytes == 0u || n_bytes % 16u != 0u)
+    return fail(c, INU_EINVAL, "inu_stream_bw: n_bytes must be a positive multiple of 16, reps positive");
+  if (kind < 0 || kind > 3) return fail(c, INU_EINVAL, "inu_stream_bw: kind must be 0 (copy), 1 (read), 2 (write), 3 (hipMemcpy)");
+  if (unroll != 1u && unroll != 2u && unroll != 4u && unroll != 8u)
+    return fail(c, INU_EINVAL, "inu_stream_bw: unroll must be 1, 2, 4 or 8");
+  if (wg_per_cu == 0u || wg_per_cu > 32u) return fail(c, INU_EINVAL, "inu_stream_bw: wg_per_cu in [1, 32]");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t n = n_bytes / 16u;
+  const uint32_t grid = (uint32_t)std::max<size_t>(1, std::min<size_t>((size_t)c->cus * wg_per_cu, (n + 255) / 256));
+  void* buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipError_t e = hipMalloc(&buf[0], n_bytes);
+  if (e == hipSuccess) e = hipMalloc(&buf[1], std::max<size_t>(n_bytes, (size_t)grid * 256u * 4u));
+  if (e == hipSuccess) e = hipMemsetAsync(buf[0], 0, n_bytes, c->stream);
+  if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+  if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+  const f32x4* in = static_cast<const f32x4*>(buf[0]);
+  f32x4* out = static_cast<f32x4*>(buf[1]);
+  auto launch = [&]() -> hipError_t {
+    if (kind == 3) return hipMemcpyAsync(buf[1], buf[0], n_bytes, hipMemcpyDeviceToDevice, c->stream);
+#define INU_BW_LAUNCH(UU)                                                                                  \
+  if (kind == 0) hipLaunchKernelGGL(k_bw_copy<UU>, dim3(grid), dim3(256), 0, c->stream, in, out, n);      \
+  else if (kind == 1) hipLaunchKernelGGL(k_bw_read<UU>, dim3(grid), dim3(256), 0, c->stream, in, (float*)out, n); \
+  else hipLaunchKernelGGL(k_bw_write<UU>, dim3(grid), dim3(256), 0, c->stream, out, n);
+    if (unroll == 1u) { INU_BW_LAUNCH(1) }
+    else if (unroll == 2u) { INU_BW_LAUNCH(2) }
+    else if (unroll == 4u) { INU_BW_LAUNCH(4) }
+    else { INU_BW_LAUNCH(8) }
+#undef INU_BW_LAUNCH
+    return hipGetLastError();
+  };
+  if (e == hipSuccess) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(ev[0], c->stream);
+  for (uint32_t r = 0; e == hipSuccess && r < reps; ++r) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(ev[1], c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+  for (hipEvent_t x : ev)
+    if (x) (void)hipEventDestroy(x);
+  for (void* q : buf)
+    if (q) (void)hipFree(q);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("inu_stream_bw: ") + hipGetErrorString(e));
   *ms_out = ms / (float)reps;
   return 0;
 }

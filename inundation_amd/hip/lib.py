@@ -88,6 +88,8 @@ EXPORTS = (
     "inu_probe_open",
     "inu_adj_end",
     "inu_probe_vjp",
+    "inu_probe_clock",
+    "inu_stream_bw",
 )
 
 
@@ -191,6 +193,8 @@ def load() -> C.CDLL:
         "inu_probe_open": (_I, [_P, _U32, _P, _P, _P, _P, _F, _F]),
         "inu_adj_end": (_I, [_P]),
         "inu_probe_vjp": (_I, [_P, _U32, _P, _P, _F, _F]),
+        "inu_probe_clock": (_I, [_P, _I, C.POINTER(InuCtrl), _P]),
+        "inu_stream_bw": (_I, [_P, C.c_uint64, _U32, _I, _U32, _U32, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -399,6 +403,12 @@ class Context:
         self.call("inu_stream_copy", int(n_bytes), int(reps), ptr(ms))
         return float(ms[0])
 
+    def stream_bw(self, n_bytes: int, kind: int, unroll: int = 4, wg_per_cu: int = 8, reps: int = 5) -> float:
+        """Mean milliseconds of one bandwidth probe (kind 0 copy, 1 read, 2 write, 3 hipMemcpy) of ``n_bytes``."""
+        ms = np.zeros(1, dtype=np.float32)
+        self.call("inu_stream_bw", int(n_bytes), int(reps), int(kind), int(unroll), int(wg_per_cu), ptr(ms))
+        return float(ms[0])
+
     def autotune(self, reps: int = 5) -> dict[str, Any]:
         """Pick the stage kernels' tile level by timing the predictor; returns the choice and the timings."""
         out = np.zeros(4, dtype=np.uint32)
@@ -497,6 +507,15 @@ class Context:
         out = np.empty((c.shape[0], 4), dtype=np.float32)
         self.call("inu_probe_open", c.shape[0], ptr(c), ptr(k), ptr(v), ptr(out), g, dry_tol)
         return out
+
+    def probe_clock(self, mode: int, ctrl: InuCtrl, dt_words: Sequence[int]) -> tuple[InuCtrl, NDArray[np.uint32]]:
+        """step_control (0 START, 1 STEP, 2 RETARGET) on a copy of ``ctrl`` and the dt words (prologue
+        bound bits, next-step bound bits); returns the updated copies. The context's clock is untouched."""
+        out = InuCtrl.from_buffer_copy(ctrl)
+        dt = np.zeros(4, dtype=np.uint32)
+        dt[: len(dt_words)] = np.asarray(dt_words, dtype=np.uint32)
+        self.call("inu_probe_clock", int(mode), C.byref(out), ptr(dt))
+        return out, dt
 
     def probe(self, pairs: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:
         p = np.ascontiguousarray(pairs, dtype=np.float32).reshape(-1, 2)

@@ -139,7 +139,6 @@ class SWESolverHipResident:
         self._h0 = np.asarray(h0, dtype=np.float32).ravel().copy()
         self._hu0 = np.asarray(hu0, dtype=np.float32).ravel().copy()
         self._hv0 = np.asarray(hv0, dtype=np.float32).ravel().copy()
-        self._ctx = L.Context(device)
         m = device_mesh_arrays(geom)
         stretch_ptr = mark_open_edges(geom, m, open_edges)
         self._open_stretches = [np.asarray(e, dtype=np.int64).ravel() for e in open_edges]
@@ -147,9 +146,7 @@ class SWESolverHipResident:
         self._open_cells = np.asarray(geom.edge_cellL)[
             np.concatenate([np.empty(0, dtype=np.int64), *self._open_stretches])
         ]
-        self._ctx.set_mesh(
-            m["cell_l"], m["cell_r"], m["nx"], m["ny"], m["len"], m["dz2"], m["area"], m["cptr"], m["cidx"]
-        )
+        self._ctx = self._open_context(device, m)
         if self.n_open:
             self._ctx.set_open_stretches(stretch_ptr)
         self._ctx.set_manning(np.asarray(n0, dtype=np.float32))
@@ -157,6 +154,12 @@ class SWESolverHipResident:
         self._ctx.set_state(self._h0, self._hu0, self._hv0, None)
         self._eft_checked = False
         self.steps_total = 0
+
+    def _open_context(self, device: int, m: dict[str, Any]) -> Any:
+        """A device context on ``device`` holding mesh ``m`` (the device edge order, open edges marked)."""
+        ctx = L.Context(device)
+        ctx.set_mesh(m["cell_l"], m["cell_r"], m["nx"], m["ny"], m["len"], m["dz2"], m["area"], m["cptr"], m["cidx"])
+        return ctx
 
     # -- parameters ---------------------------------------------------------
 

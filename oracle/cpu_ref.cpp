@@ -1027,6 +1027,18 @@ void cref_prologue(cref* r) {
 
 void cref_retarget(cref* r) { step_control(*r, 2); }
 
+// step_control alone on a hand-set control block and dt words (dt4[0] the
+// prologue bound, dt4[1] the next-step bound), both updated in place; the
+// same probe as the backend's inu_probe_clock.
+void cref_probe_clock(int mode, void* ctrl48, uint32_t* dt4) {
+  cref r;
+  memcpy(&r.ctrl, ctrl48, 48);
+  memcpy(r.dtbuf, dt4, 16);
+  step_control(r, mode);
+  memcpy(ctrl48, &r.ctrl, 48);
+  memcpy(dt4, r.dtbuf, 16);
+}
+
 // gpu_resident_batch.py:489-513
 void cref_steps(cref* r, uint32_t n) {
   for (uint32_t k = 0; k < n; ++k) {
@@ -1051,6 +1063,13 @@ void cref_download(const cref* r, int which, void* out) {
     uint32_t* o = static_cast<uint32_t*>(out);
     for (uint32_t b = 0; b < r->NB; ++b) o[b] = r->blk[3 * b + (which - 9)];
   }
+}
+
+// Overwrites one cell field (the backend's inu_upload): h hu hv h_lo h1 hu1
+// hv1 h1_lo clamp.
+void cref_upload(cref* r, int which, const void* in) {
+  std::vector<float>* f[] = {&r->h, &r->hu, &r->hv, &r->hlo, &r->h1, &r->hu1, &r->hv1, &r->h1lo, &r->clamp};
+  if (which >= 0 && which <= 8) memcpy(f[which]->data(), in, 4 * (size_t)r->N);
 }
 
 // Same probe as the backend's inu_probe_open: case k = (h, hu, hv, nx, ny, fh)

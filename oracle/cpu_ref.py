@@ -60,6 +60,8 @@ def load() -> C.CDLL:
         "cref_set_open_conditions": (None, [_P, _P, _P]),
         "cref_open_count": (_U, [_P]),
         "cref_open_tally": (None, [_P, _P]),
+        "cref_probe_clock": (None, [C.c_int, _P, _P]),
+        "cref_upload": (None, [_P, C.c_int, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -98,6 +100,16 @@ def probe_open(cases, kinds, values, g: float, dry_tol: float) -> NDArray[np.flo
     out = np.empty((c.shape[0], 4), dtype=np.float32)
     load().cref_probe_open(c.shape[0], _p(c), _p(k), _p(v), _p(out), g, dry_tol)
     return out
+
+
+def probe_clock(mode: int, ctrl: Any, dt_words: NDArray[np.uint32]) -> tuple[Any, NDArray[np.uint32]]:
+    """step_control (mode 0 START, 1 STEP, 2 RETARGET) on a copy of ``ctrl`` (an ``InuCtrl``) and the
+    dt words (prologue bound bits, next-step bound bits, 0, 0); returns the updated copies."""
+    out = type(ctrl).from_buffer_copy(ctrl)
+    dt = np.zeros(4, dtype=np.uint32)
+    dt[: len(dt_words)] = np.asarray(dt_words, dtype=np.uint32)
+    load().cref_probe_clock(int(mode), C.addressof(out), _p(dt))
+    return out, dt
 
 
 class CpuRef:
@@ -207,15 +219,28 @@ class CpuRef:
             self._lib.cref_open_tally(self._h, _p(out))
         return out.reshape(n, 4)
 
-    def open_volumes(self) -> NDArray[np.float64]:
+    def open_volumes(self, n_stretches: int | None = None) -> NDArray[np.float64]:
         """Per stretch (out, in) [m^3], each as hi + lo in FP64."""
         t = self.open_tally().astype(np.float64)
-        return np.stack([t[:, 0] + t[:, 1], t[:, 2] + t[:, 3]], axis=1)
+        out = np.stack([t[:, 0] + t[:, 1], t[:, 2] + t[:, 3]], axis=1)
+        assert n_stretches is None or n_stretches == out.shape[0]
+        return out
 
     def download(self, which: int, n: int, dtype: type = np.float32) -> NDArray:
         out = np.empty(n, dtype=dtype)
         self._lib.cref_download(self._h, which, _p(out))
         return out
+
+    def upload(self, which: int, values) -> None:  # noqa: ANN001
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        assert a.size == self.N and 0 <= which <= 8
+        self._lib.cref_upload(self._h, which, _p(a))
+
+    def probe(self, pairs, g: float, dry_tol: float) -> NDArray[np.float32]:  # noqa: ANN001
+        return probe(pairs, g, dry_tol)
+
+    def close(self) -> None:
+        """Nothing to release early (the library handle goes with the object)."""
 
 
 __all__ = ["CpuRef", "load", "max_threads", "probe", "probe_open", "set_threads"]

@@ -111,3 +111,40 @@ CASES = {
 
 def f32_state(h, hu, hv):  # noqa: ANN001, ANN201
     return tuple(np.asarray(a, dtype=np.float32) for a in (h, hu, hv))
+
+
+def split_sides(verts: np.ndarray, faces: np.ndarray, offsets: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Every polygon side cut at its midpoint (one shared vertex per side): twice the edges and CSR slots per cell.
+
+    Two collinear edges then join each pair of neighbours; the mesh stays
+    conforming. Dense blocks of such cells exceed what the stage kernels'
+    tile levels preload, so the loops after their first barrier run.
+    """
+    deg = np.diff(offsets)
+    cell = np.repeat(np.arange(deg.size), deg)
+    pos = np.arange(faces.size) - offsets[cell]
+    nxt = faces[offsets[cell] + (pos + 1) % deg[cell]]
+    a, b = np.minimum(faces, nxt), np.maximum(faces, nxt)
+    key, inv = np.unique(a.astype(np.int64) * (verts.shape[0] + 1) + b, return_inverse=True)
+    ka, kb = key // (verts.shape[0] + 1), key % (verts.shape[0] + 1)
+    mids = 0.5 * (verts[ka] + verts[kb])
+    mid_id = verts.shape[0] + inv
+    out = np.empty(2 * faces.size, dtype=np.int32)
+    out[0::2] = faces
+    out[1::2] = mid_id
+    return np.concatenate([verts, mids]), out, (2 * offsets).astype(np.int32)
+
+
+def dense_voronoi(n_points: int = 12_000, seed: int = 31, length: float = 600.0) -> MeshGeometry:
+    """A Voronoi mesh with split sides over fBm terrain, Hilbert-ordered (about 12 sides per cell)."""
+    from inundation_amd.mesh.synth import fbm_terrain, voronoi_mesh
+
+    rng = np.random.default_rng(seed)
+    verts, faces, offsets = voronoi_mesh(n_points, length, rng)
+    verts, faces, offsets = split_sides(np.asarray(verts, dtype=np.float64), faces, offsets)
+    deg = np.diff(offsets)
+    cell = np.repeat(np.arange(deg.size), deg)
+    cx = np.bincount(cell, verts[faces, 0], minlength=deg.size) / deg
+    cy = np.bincount(cell, verts[faces, 1], minlength=deg.size) / deg
+    zb = fbm_terrain(cx, cy, length, rng)
+    return hilbert_reorder(build_geometry(verts, faces, offsets, zb_from_file=zb), verbose=False)[0]
