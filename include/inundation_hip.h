@@ -272,6 +272,19 @@ int inu_autotune(inu_ctx* ctx, uint32_t reps, uint32_t* out4);
 /* Sets the tile level (0, 1 or 2) directly; inu_set_mesh/inu_set_batch reset it to 0. */
 int inu_set_tile(inu_ctx* ctx, uint32_t tile);
 
+/* inu_set_source_rates without device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It stages the cell ids and rates through a fixed 512 KiB device
+ * array that each GPU holds for the life of the process, so it takes at most
+ * 65536 rates per call (INU_EINVAL beyond). Calls that overlap on one GPU,
+ * from any context or thread, race on that array. Refused (INU_ESTATE) during
+ * an adjoint recording, whose rate log allocates, and (INU_EINVAL) for a
+ * negative rate while every source is non-negative (the step graphs would be
+ * recorded again); inu_set_source_rates takes both. Same results as
+ * inu_set_source_rates otherwise. */
+int inu_set_source_rates_noalloc(inu_ctx* ctx, uint32_t n, const uint32_t* cell, const float* rate);
+
 /* Open boundary stretches: open edge k belongs to stretch s when
  * stretch_ptr[s] <= k < stretch_ptr[s + 1] (n_stretches + 1 words, from 0 to
  * the open edge count; reference SWESolver's open_edges, solver.py:140-165,

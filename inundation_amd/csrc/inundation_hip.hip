@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1417,6 +1418,12 @@ __global__ void k_open_tally(const Args a, const uint32_t* stretch_ptr, const ui
   }
 }
 
+// Staging of inu_set_source_rates_noalloc: up to kScratchRates internal cell
+// ids, then as many rate words. Each GPU holds one copy for the life of the
+// process; every call on a GPU shares it.
+constexpr uint32_t kScratchRates = 1u << 16;
+static __device__ __align__(256) uint32_t scratch[2 * kScratchRates];
+
 // rates[k] into the dense source array at cells[k] (inu_set_source_rates).
 __global__ void k_scatter_rates(float* src, const uint32_t* cells, const float* rates, uint32_t n) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2092,6 +2099,24 @@ int launch_step(inu_ctx* c, const hipEvent_t* ev) {
   return 0;
 }
 
+// Device address of `scratch` on each GPU, taken once per device with that
+// device current (host code must use these, never the symbol's own name).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void* ptr = nullptr;
+  g_scratch_err[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_ptr[device] = (g_scratch_err[device] == hipSuccess) ? ptr : nullptr;
+  g_scratch_done[device] = true;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2117,6 +2142,7 @@ int inu_create(int device, inu_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
   c->p = make_phys(9.81f, 1e-4f, 0.8f);
+  scratch_setup(device);  // with the device current
   *out = c;
   return 0;
 }
@@ -3354,6 +3380,59 @@ int inu_set_source_rates(inu_ctx* c, uint32_t n, const uint32_t* cell, const flo
     }
     if ((rc = copy_sync(c, ev, c->a.src, NP * 4, hipMemcpyDeviceToDevice))) return rc;
   }
+  return 0;
+}
+
+int inu_set_source_rates_noalloc(inu_ctx* c, uint32_t n, const uint32_t* cell, const float* rate) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  if (!cell || !rate) return fail(c, INU_EINVAL, "inu_set_source_rates_noalloc: null array");
+  if (n > kScratchRates)
+    return fail(c, INU_EINVAL, "inu_set_source_rates_noalloc: at most 65536 rates per call (split the list)");
+  if (n > c->n_src) return fail(c, INU_EINVAL, "more rates than source cells");
+  if (c->adj.on)
+    return fail(c, INU_ESTATE,
+                "inu_set_source_rates_noalloc: not during an adjoint recording, whose rate log allocates "
+                "(use inu_set_source_rates)");
+  void* base = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+  if (!base) {
+    const hipError_t e = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_err[c->device] : hipErrorInvalidDevice;
+    return fail(c, INU_EHIP, std::string("inu_set_source_rates_noalloc: no scratch address on this device: ") +
+                                 hipGetErrorString(e));
+  }
+  for (uint32_t k = 0; k < n; ++k) {
+    if (cell[k] >= c->N) return fail(c, INU_EINVAL, "source cell out of range");
+    if (!std::isfinite(rate[k])) return fail(c, INU_EINVAL, "source rate must be finite");
+    // A negative rate among non-negative sources would change the stage
+    // kernels' all-wet test and need the step graphs recorded again.
+    if (rate[k] < 0.0f && c->src_nonneg)
+      return fail(c, INU_EINVAL,
+                  "inu_set_source_rates_noalloc: a negative rate among non-negative sources re-records the step "
+                  "graphs (use inu_set_source_rates)");
+    if (!std::binary_search(c->src_cells.begin(), c->src_cells.end(), internal_cell(c, cell[k])))
+      return fail(c, INU_EINVAL, "inu_set_source_rates_noalloc: a cell is not a source cell (use inu_set_sources)");
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  uint32_t* dcell = static_cast<uint32_t*>(base);
+  float* drate = reinterpret_cast<float*>(dcell + kScratchRates);
+  // Internal cell ids through a stack buffer, one chunk at a time (each copy
+  // is waited for before the buffer is refilled).
+  uint32_t chunk[1024];
+  hipError_t e = hipSuccess;
+  for (uint32_t k0 = 0; e == hipSuccess && k0 < n; k0 += 1024u) {
+    const uint32_t m = std::min(1024u, n - k0);
+    for (uint32_t k = 0; k < m; ++k) chunk[k] = internal_cell(c, cell[k0 + k]);
+    e = hipMemcpyAsync(dcell + k0, chunk, (size_t)m * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(drate, rate, (size_t)n * 4, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_scatter_rates, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->a.src, dcell, drate, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("set_source_rates_noalloc: ") + hipGetErrorString(e));
   return 0;
 }
 

@@ -90,6 +90,7 @@ EXPORTS = (
     "inu_probe_vjp",
     "inu_probe_clock",
     "inu_stream_bw",
+    "inu_set_source_rates_noalloc",
 )
 
 
@@ -195,6 +196,7 @@ def load() -> C.CDLL:
         "inu_probe_vjp": (_I, [_P, _U32, _P, _P, _F, _F]),
         "inu_probe_clock": (_I, [_P, _I, C.POINTER(InuCtrl), _P]),
         "inu_stream_bw": (_I, [_P, C.c_uint64, _U32, _I, _U32, _U32, _P]),
+        "inu_set_source_rates_noalloc": (_I, [_P, _U32, _P, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -342,6 +344,14 @@ class Context:
         if c.size != r.size:
             raise ValueError("cells and rates differ in length")
         self.call("inu_set_source_rates", c.size, ptr(c) if c.size else None, ptr(r) if r.size else None)
+
+    def set_source_rates_noalloc(self, cells: NDArray[np.uint32], rates: NDArray[np.float32]) -> None:
+        """``set_source_rates`` with no device allocation (at most 65536 rates; see the header)."""
+        c = np.ascontiguousarray(cells, dtype=np.uint32)
+        r = np.ascontiguousarray(rates, dtype=np.float32)
+        if c.size != r.size:
+            raise ValueError("cells and rates differ in length")
+        self.call("inu_set_source_rates_noalloc", c.size, ptr(c) if c.size else None, ptr(r) if r.size else None)
 
     def ctrl_write(self, ctrl: InuCtrl | Sequence[InuCtrl]) -> None:
         """One control block, or one per scenario."""
