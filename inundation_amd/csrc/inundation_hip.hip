@@ -367,7 +367,7 @@ __device__ __forceinline__ void rec_ghost(const Args& a, uint32_t g, float4 s) {
 template <bool FM>
 __device__ __forceinline__ void put_cell(const Lds& l, const Phys& p, uint32_t q, float4 s) {
   l.cell[q] = s;
-  if (FM) l.cx[q] = make_float2(inu::fast::vel_factor(p, s.x), sqrtf(inu::gmax(s.x, 0.0f)));
+  if (FM) l.cx[q] = make_float2(inu::fast::vel_factor(p, s.x), inu::sqrt_rn(inu::gmax(s.x, 0.0f)));
 }
 
 // The words of an open edge with right side (hR, huR, hvR) beyond it; `wall`
@@ -378,7 +378,7 @@ __device__ __forceinline__ void open_words(const Phys& p, float4 sl, float4 xl, 
                                            bool wall, const EdgeRec& r, float w[6], float& fh) {
   if (FM) {
     const inu::fast::Side L{sl.x, sl.y, sl.z, 0.0f, xl.x, xl.y, xl.z};
-    const inu::fast::Side R{hR, huR, hvR, 0.0f, inu::fast::vel_factor(p, hR), sqrtf(inu::gmax(hR, 0.0f)),
+    const inu::fast::Side R{hR, huR, hvR, 0.0f, inu::fast::vel_factor(p, hR), inu::sqrt_rn(inu::gmax(hR, 0.0f)),
                             p.half_g * hR * hR};
     inu::fast::edge_words(p, wall, L, R, 0.0f, 0.0f, r.nx, r.ny, r.len, w, &fh);
   } else {
@@ -422,7 +422,7 @@ __global__ void k_open_words(const Args a, const uint32_t* open_cell, const floa
   const uint32_t i = open_cell[k];
   const float h = (ST == 0) ? a.h[i] : a.h1[i];
   const float4 sl = make_float4(h, (ST == 0) ? a.hu[i] : a.hu1[i], (ST == 0) ? a.hv[i] : a.hv1[i], 0.0f);
-  const float4 xl = FM ? make_float4(inu::fast::vel_factor(p, h), sqrtf(inu::gmax(h, 0.0f)), p.half_g * h * h, 0.0f)
+  const float4 xl = FM ? make_float4(inu::fast::vel_factor(p, h), inu::sqrt_rn(inu::gmax(h, 0.0f)), p.half_g * h * h, 0.0f)
                        : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float4 g = open_geom[k];
   EdgeRec r;
@@ -727,7 +727,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       hv_new *= drag;
       const float u = hu_new / h_new;
       const float v = hv_new / h_new;
-      const float spd = sqrtf(u * u + v * v);
+      const float spd = inu::sqrt_rn(u * u + v * v);
       const float h43 = inu::pow43(h_new);
       const float Sf = p.g * nm * nm * spd / h43;
       const float den = 1.0f + dt * Sf;
@@ -799,8 +799,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       // is the source fix's, as below. The perimeter was summed beside the
       // edge words above.
       if (tid < nc && rate != 0.0f) {
-        const float u = sqrtf(fmaf(hui, hui, hvi * hvi)) / hi;
-        const float c = p.sqrt_g * sqrtf(hi);
+        const float u = inu::sqrt_rn(fmaf(hui, hui, hvi * hvi)) / hi;
+        const float c = p.sqrt_g * inu::sqrt_rn(hi);
         const float lam = (u + c) * perim;
         if (lam > 1e-30f) lam_b = inu::gmax(lam_b, lam);
       }
@@ -812,8 +812,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       if (tid < nc) l.cell[tid].x = hi;
       __syncthreads();
       if (tid < nc && rate != 0.0f && hi >= p.dry) {
-        const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hi : sqrtf(hui * hui + hvi * hvi) / hi;
-        const float c = FM ? p.sqrt_g * sqrtf(hi) : sqrtf(p.g * hi);
+        const float u = FM ? inu::sqrt_rn(fmaf(hui, hui, hvi * hvi)) / hi : inu::sqrt_rn(hui * hui + hvi * hvi) / hi;
+        const float c = FM ? p.sqrt_g * inu::sqrt_rn(hi) : inu::sqrt_rn(p.g * hi);
         float lam = 0.0f;
         for (uint32_t k = k0; k < k1; ++k) {
           const uint32_t s = l.s_slot[k];
@@ -948,8 +948,8 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
     if (wet) {
       const float hui = a.hu[i];
       const float hvi = a.hv[i];
-      const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
-      const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
+      const float u = FM ? inu::sqrt_rn(fmaf(hui, hui, hvi * hvi)) / hn : inu::sqrt_rn(hui * hui + hvi * hvi) / hn;
+      const float c = FM ? p.sqrt_g * inu::sqrt_rn(hn) : inu::sqrt_rn(p.g * hn);
       float lam = 0.0f, perim = 0.0f;
       const uint32_t s1 = a.cptr[i + 1];
       for (uint32_t s = a.cptr[i]; s < s1; ++s) {
@@ -992,8 +992,8 @@ __device__ __forceinline__ void fix_pair(const Args& a, uint32_t i) {
   if (!(hn >= p.dry) || a.src[i] == 0.0f) return;  // only wetted cells whose source is on
   const float hui = a.hu[i];
   const float hvi = a.hv[i];
-  const float u = FM ? sqrtf(fmaf(hui, hui, hvi * hvi)) / hn : sqrtf(hui * hui + hvi * hvi) / hn;
-  const float c = FM ? p.sqrt_g * sqrtf(hn) : sqrtf(p.g * hn);
+  const float u = FM ? inu::sqrt_rn(fmaf(hui, hui, hvi * hvi)) / hn : inu::sqrt_rn(hui * hui + hvi * hvi) / hn;
+  const float c = FM ? p.sqrt_g * inu::sqrt_rn(hn) : inu::sqrt_rn(p.g * hn);
   float lam = 0.0f;
   const uint32_t s1 = a.cptr[i + 1];
   for (uint32_t s = a.cptr[i]; s < s1; ++s) {
@@ -1789,7 +1789,7 @@ __global__ void k_probe(const float* in, float* out, uint32_t n, Phys p) {
   float* o = out + (size_t)kProbeOut * k;
   o[0] = e;
   o[1] = x / y;
-  o[2] = sqrtf(fabsf(x));
+  o[2] = inu::sqrt_rn(fabsf(x));
   o[3] = (x > 0.0f) ? inu::pow43(x) : 0.0f;
   o[4] = (x > 0.0f) ? inu::film_drag(p, x, y) : 0.0f;
   o[5] = (x > 0.0f) ? inu::fast::rcbrt(x) : 0.0f;

@@ -40,19 +40,34 @@ __host__ __device__ __forceinline__ float gmin(float x, float y) { return (y < x
 __host__ __device__ __forceinline__ float gmax(float x, float y) { return (x < y) ? y : x; }
 __host__ __device__ __forceinline__ float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
 
+// Square root, correctly rounded (the same bits as sqrtf). For 0 and for
+// x >= 2^-96 the library's own sequence is the hardware estimate moved one
+// ulp down or up by the signs of two fused residuals; its rescaling of tiny
+// arguments and its special-case step do nothing there, so they are skipped
+// unless the argument is below 2^-96 (or negative or NaN), which takes the
+// library path. Infinity comes out of the residual steps unchanged.
+__device__ __forceinline__ float sqrt_rn(float x) {
+  if (!(x >= 0x1p-96f) && x != 0.0f) return sqrtf(x);
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+  const float up = __uint_as_float(__float_as_uint(s) + 1u);
+  const float r = (fmaf(-dn, s, x) <= 0.0f) ? dn : s;
+  return (fmaf(-up, s, x) > 0.0f) ? up : r;
+}
+
 __device__ __forceinline__ float safe_vel(const Phys& p, float h, float hq) {
   return (h > p.dry) ? (2.0f * h * hq) / (h * h + gmax(h * h, p.e_film2)) : 0.0f;
 }
 
 __device__ __forceinline__ void hllc_speeds(const Phys& p, float hL, float unL, float hR, float unR,
                                             float& SL, float& SR) {
-  const float cL = sqrtf(p.g * gmax(hL, 0.0f));
-  const float cR = sqrtf(p.g * gmax(hR, 0.0f));
-  const float sqL = sqrtf(gmax(hL, 0.0f));
-  const float sqR = sqrtf(gmax(hR, 0.0f));
+  const float cL = sqrt_rn(p.g * gmax(hL, 0.0f));
+  const float cR = sqrt_rn(p.g * gmax(hR, 0.0f));
+  const float sqL = sqrt_rn(gmax(hL, 0.0f));
+  const float sqR = sqrt_rn(gmax(hR, 0.0f));
   const float den = sqL + sqR;
   const float u_roe = (den > 1e-12f) ? (sqL * unL + sqR * unR) / den : 0.0f;
-  const float c_roe = sqrtf(p.half_g * (hL + hR));
+  const float c_roe = sqrt_rn(p.half_g * (hL + hR));
   float sl = gmin(unL - cL, u_roe - c_roe);
   float sr = gmax(unR + cR, u_roe + c_roe);
   const float wetL = gclamp(hL / p.dry - 1.0f, 0.0f, 1.0f);
@@ -256,7 +271,7 @@ __device__ __forceinline__ void ghost_state(const Phys& p, uint32_t kind, float 
   const float vL = safe_vel(p, hL, hvL);
   const float unL = uL * nx + vL * ny;
   float ut = -uL * ny + vL * nx;
-  const float cL = sqrtf(p.g * gmax(hL, 0.0f));
+  const float cL = sqrt_rn(p.g * gmax(hL, 0.0f));
   float un = -unL;
   hG = hL;
   if (kind == kOpenFree) {
@@ -267,7 +282,7 @@ __device__ __forceinline__ void ghost_state(const Phys& p, uint32_t kind, float 
     // A level cannot reach a boundary against supercritical outflow.
     if (hL < p.dry || unL < cL) {
       hG = value;
-      un = unL + 2.0f * (cL - sqrtf(p.g * value));
+      un = unL + 2.0f * (cL - sqrt_rn(p.g * value));
     } else {
       un = unL;
     }
@@ -320,14 +335,14 @@ __device__ __forceinline__ float vel_factor(const Phys& p, float h) {
   return (2.0f * h) / (hh + gmax(hh, p.e_film2));
 }
 
-// sqL, sqR: sqrtf(gmax(hL, 0)), sqrtf(gmax(hR, 0)).
+// sqL, sqR: sqrt_rn(gmax(hL, 0)), sqrt_rn(gmax(hR, 0)).
 __device__ __forceinline__ void hllc_speeds_sq(const Phys& p, float hL, float sqL, float unL, float hR, float sqR,
                                                float unR, float& SL, float& SR) {
   const float cL = p.sqrt_g * sqL;
   const float cR = p.sqrt_g * sqR;
   const float den = sqL + sqR;
   const float u_roe = (den > 1e-12f) ? fmaf(sqL, unL, sqR * unR) / den : 0.0f;
-  const float c_roe = sqrtf(p.half_g * (hL + hR));
+  const float c_roe = sqrt_rn(p.half_g * (hL + hR));
   const float sl = gmin(unL - cL, u_roe - c_roe);
   const float sr = gmax(unR + cR, u_roe + c_roe);
   const float wetL = gclamp(fmaf(hL, p.inv_dry, -1.0f), 0.0f, 1.0f);
@@ -341,7 +356,7 @@ __device__ __forceinline__ void hllc_speeds_sq(const Phys& p, float hL, float sq
 
 __device__ __forceinline__ void hllc_speeds(const Phys& p, float hL, float unL, float hR, float unR,
                                             float& SL, float& SR) {
-  fast::hllc_speeds_sq(p, hL, sqrtf(gmax(hL, 0.0f)), unL, hR, sqrtf(gmax(hR, 0.0f)), unR, SL, SR);
+  fast::hllc_speeds_sq(p, hL, sqrt_rn(gmax(hL, 0.0f)), unL, hR, sqrt_rn(gmax(hR, 0.0f)), unR, SL, SR);
 }
 
 // HLLC on face states whose velocities are already desingularised; sqL, sqR
@@ -434,7 +449,7 @@ __device__ __forceinline__ void wall_ghost(float huL, float hvL, float nx, float
 }
 
 // Cell quantities of one side of an edge: depth, momenta, compensated low
-// word, vel_factor(h), sqrtf(gmax(h, 0)) and the pressure half_g * h * h.
+// word, vel_factor(h), sqrt_rn(gmax(h, 0)) and the pressure half_g * h * h.
 struct Side {
   float h, hu, hv, lo, vf, sq, pc;
 };
@@ -460,7 +475,7 @@ __device__ __forceinline__ void edge_words(const Phys& p, bool wall, const Side&
   const bool stepR = dzR != 0.0f;
   const float hs = stepL ? f.hLs : f.hRs;
   const float vfs = vel_factor(p, hs);
-  const float sqs = sqrtf(gmax(hs, 0.0f));
+  const float sqs = sqrt_rn(gmax(hs, 0.0f));
   const float rL = stepL ? vfs : L.vf;
   const float rR = stepR ? vfs : R.vf;
   const float sqL = stepL ? sqs : L.sq;
@@ -514,7 +529,7 @@ __device__ __forceinline__ void friction(const Phys& p, float h, float n, float 
   hv *= drag;
   const float r = rcbrt(h);
   const float r2 = r * r;
-  const float spd = sqrtf(fmaf(hu, hu, hv * hv)) * (r2 * r);
+  const float spd = sqrt_rn(fmaf(hu, hu, hv * hv)) * (r2 * r);
   const float Sf = p.g * n * n * spd * (r2 * r2);
   const float id = 1.0f / fmaf(dt, Sf, 1.0f);
   hu *= id;
