@@ -80,10 +80,11 @@ def main() -> int:
     ap.add_argument("--no-unordered", action="store_true", help="skip the unordered-accumulation comparison run")
     ap.add_argument(
         "--workload",
-        choices=("rainfall", "batched", "adjoint"),
+        choices=("rainfall", "batched", "adjoint", "inflow"),
         default="rainfall",
         help="rainfall: the plain 16 M-cell step (default); batched: scenarios on five Voronoi meshes, batched "
-        "against sequential; adjoint: Manning/bed gradients of a gauge misfit on a 250 k-cell Voronoi flood",
+        "against sequential; adjoint: Manning/bed gradients of a gauge misfit on a 250 k-cell Voronoi flood; "
+        "inflow: the 1 M-cell Voronoi inflow flood through the workflow, 60 one-minute hydrograph phases",
     )
     ap.add_argument("--cells", type=int, default=250_000, help="adjoint: Voronoi points")
     ap.add_argument("--fd-cells", type=int, default=20, help="adjoint: cells checked by central differences")
@@ -112,6 +113,8 @@ def main() -> int:
         return bench_batched(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "inflow":
+        return bench_inflow(args)
 
     from inundation_amd.hip import lib as L
     from inundation_amd.hip.solver import device_mesh_arrays
@@ -616,6 +619,116 @@ def run_adjoint(cells: int, steps: int, fd_cells: int, cpu_seconds: float | None
         "cpu_baseline": cpu,
     }
     return out
+
+
+def bench_inflow(args) -> int:  # noqa: ANN001
+    """The 1 M-cell Voronoi inflow flood, 1 h simulated, through ``SWEWorkflow`` on the hip_resident backend.
+
+    60 one-minute phases carry the triangular hydrograph (peak 20 m^3/s at
+    30 min) into one source cell; outputs every 60 s. The value is the
+    reference's convention: steps x cells over the run loop's wall clock
+    (``WorkflowResult.wall_seconds``, snapshot downloads included). The CPU
+    baseline is the FP32 oracle continuing from the GPU's final state for a
+    bounded number of steps, checked bit for bit against the GPU on them.
+    """
+    from inundation_amd.hip import lib as L
+    from inundation_amd.hip.solver import device_mesh_arrays
+    from inundation_amd.workflow import SWEWorkflow, WorkflowConfig
+    from inundation_amd.workloads import inflow
+
+    t0 = time.perf_counter()
+    wl, phases = inflow()
+    geom = wl.geom
+    log(f"mesh {geom.N} cells, {geom.E} edges built in {time.perf_counter() - t0:.1f}s")
+    cfg = WorkflowConfig(mesh_source="synthetic", manning_n=0.035, output_interval_s=60.0, dt_max=wl.dt_max,
+                         dt_init=wl.dt_init, cfl_interval=10, dry_tol=wl.dry_tol, cfl=wl.cfl, progress=False)
+    runs = []
+    for r in range(2):  # a warm-up run, then the timed one
+        wf = SWEWorkflow(cfg)
+        wf.prepare(geometry=geom)
+        res = wf.run(phases)
+        runs.append(res)
+        log(f"run {r}: {res.steps_total} steps in {res.wall_seconds:.2f}s, {len(res.snapshots)} snapshots, "
+            f"volume {res.volume_final_m3:.1f} of {res.volume_injected_m3:.1f} m^3 injected")
+        if r == 0:
+            wf.solver.close()
+    res = runs[-1]
+    if [np.asarray(a).view(np.uint32).tobytes() for a in runs[0].snapshots] != [
+            np.asarray(a).view(np.uint32).tobytes() for a in res.snapshots]:
+        raise RuntimeError("the two runs differ")
+    value = res.steps_total * geom.N / res.wall_seconds
+    ctx = wf.solver.context
+    wet = int((res.h_final >= wl.dry_tol).sum())
+    cpu = None
+    if not args.no_cpu:
+        from oracle.cpu_ref import CpuRef, set_threads
+
+        m = device_mesh_arrays(geom)
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:  # pragma: no cover
+            cores = os.cpu_count() or 1
+        cores = max(1, min(cores, 16))
+        set_threads(cores)
+        ref = CpuRef(m)
+        state = tuple(ctx.download(f, geom.N) for f in (L.FIELD_H, L.FIELD_HU, L.FIELD_HV, L.FIELD_H_LO))
+        ctrl = ctx.ctrl_read()
+        ref.set_params(9.81, wl.dry_tol, wl.cfl, L.FLAG_DETERMINISTIC | L.FLAG_COMPENSATED | L.FLAG_ACTIVE_SET)
+        ref.set_manning(wl.manning)
+        ref.set_sources(np.empty(0, np.uint32), np.empty(0, np.float32))
+        ctx.set_sources(np.empty(0, np.uint32), np.empty(0, np.float32))
+        for be in (ref, ctx):
+            be.set_state(*state)
+            c = L.InuCtrl.from_buffer_copy(ctrl)
+            c.steps = 0
+            c.stop_ticks = int(c.now_ticks) + 3600 * 2**20
+            be.ctrl_write(c)
+            be.prologue()
+        t1 = time.perf_counter()
+        ref.steps(1)
+        one = time.perf_counter() - t1
+        k = max(1, min(2000, int(args.cpu_seconds / max(one, 1e-6)) - 1))
+        t1 = time.perf_counter()
+        ref.steps(k)
+        dt = time.perf_counter() - t1
+        ctx.steps(k + 1, use_graph=False)
+        same = all(np.array_equal(ctx.download(f, geom.N).view(np.uint32), ref.download(f, geom.N).view(np.uint32))
+                   for f in (L.FIELD_H, L.FIELD_HU, L.FIELD_HV, L.FIELD_H_LO))
+        log(f"cpu oracle: {k} steps in {dt:.2f}s on {cores} threads; bitwise equal to GPU: {same}")
+        if not same:
+            raise RuntimeError("GPU and FP32 oracle differ on the inflow mesh")
+        cpu = {"value": k * geom.N / dt, "unit": "cell-steps/s", "cores": cores, "kind": "port",
+               "sample": f"{k} Heun steps of the same {geom.N}-cell mesh from the GPU state after the hour (FP32 "
+               f"C++/OpenMP oracle, active set on; bitwise equal to the GPU on those steps)"}
+    wf.solver.close()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": int(res.steps_total),
+        "warmup": 1,
+        "ms_per_step": res.wall_seconds / res.steps_total * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": wl.name + ", 60 one-minute phases, outputs every 60 s, through SWEWorkflow",
+            "cells": geom.N,
+            "edges": geom.E,
+            "simulated_s": 3600.0,
+            "wall_s": res.wall_seconds,
+            "snapshots": len(res.snapshots),
+            "wet_cells_at_end": wet,
+            "volume_final_m3": res.volume_final_m3,
+            "volume_injected_m3": res.volume_injected_m3,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
 
 
 def bench_batched(args) -> int:  # noqa: ANN001
