@@ -1,14 +1,6 @@
 """Mesh geometry, Hilbert ordering, active-set block topology and synthetic meshes."""
 
-from .blocks import (
-    block_adjacency,
-    block_two_ring,
-    dilate_blocks,
-    edge_order_by_block,
-    edge_workgroup_block_ranges,
-    num_blocks,
-    remap_cell_edges,
-)
+from .blocks import EdgeGrouping, edge_order_by_block, group_edges_by_block, num_blocks, remap_cell_edges
 from .geometry import (
     MeshGeometry,
     build_geometry,
@@ -19,14 +11,12 @@ from .geometry import (
 )
 
 __all__ = [
+    "EdgeGrouping",
     "MeshGeometry",
-    "block_adjacency",
-    "block_two_ring",
     "build_geometry",
-    "dilate_blocks",
     "edge_bed_steps",
     "edge_order_by_block",
-    "edge_workgroup_block_ranges",
+    "group_edges_by_block",
     "hilbert_distances",
     "hilbert_permutation",
     "hilbert_reorder",

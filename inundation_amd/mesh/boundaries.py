@@ -1,9 +1,19 @@
-"""Open boundary stretches: which boundary edges a line picks, and how an inflow spreads.
+"""Open boundary stretches: which boundary edges a line picks, and how an inflow spreads over them.
 
-Reference ``inundation/mesh/boundaries.py``: a stretch is a polyline in world
-coordinates and picks the boundary edges whose midpoints lie within a
-tolerance of it (``:25-58``); an inflow's discharge is spread over a
-stretch's edges by Manning conveyance at a common water level (``:61-98``).
+The interface follows the reference's ``inundation/mesh/boundaries.py``: a
+stretch is a polyline in world coordinates that picks the boundary edges
+whose midpoints lie within a tolerance of it, and an inflow's discharge is
+spread over the picked edges in proportion to their Manning conveyance at one
+common water level. The implementation here is its own:
+
+- the picking computes every midpoint-to-segment distance of the polyline
+  in one array (edges x segments) and keeps an edge when the nearest segment
+  is within the tolerance;
+- the level is found exactly on the piecewise structure of the conveyance:
+  the sorted bed elevations bracket it between two consecutive beds (where
+  the set of wetted edges is fixed), and Newton's method on that interval,
+  started from its upper end where the function is convex, converges
+  monotonically to it.
 """
 
 from __future__ import annotations
@@ -19,32 +29,40 @@ if TYPE_CHECKING:
 
     from .geometry import MeshGeometry
 
-# Halvings of the level bracket: 80 take a 1 m bracket below float64 spacing.
-LEVEL_BISECTIONS = 80
+# Manning's conveyance grows as depth^(5/3).
+_CONVEYANCE_POWER = 5.0 / 3.0
+_NEWTON_STEPS = 60
 
 
 def boundary_edge_midpoints(geom: MeshGeometry) -> tuple[NDArray[np.int64], NDArray[np.float64]]:
-    """Boundary (``edge_cellR == -1``) edge indices and their midpoints relative to ``geom.origin``."""
-    edges = np.flatnonzero(np.asarray(geom.edge_cellR) == -1)
-    cell = np.asarray(geom.edge_cellL)[edges].astype(np.int64)
-    side = np.asarray(geom.edge_sideL)[edges].astype(np.int64)
-    v0 = np.asarray(geom.cell_vertex_ptr)[cell].astype(np.int64)
-    deg = np.asarray(geom.cell_vertex_ptr)[cell + 1].astype(np.int64) - v0
-    verts = np.asarray(geom.cell_vertices, dtype=np.float64)
-    return edges, 0.5 * (verts[v0 + side] + verts[v0 + (side + 1) % deg])
+    """Indices of the boundary edges (no right cell) and their midpoints, relative to ``geom.origin``.
+
+    A boundary edge is side ``edge_sideL`` of its left cell's polygon: it runs
+    from that polygon's vertex ``side`` to the next one (cyclically).
+    """
+    cell_r = np.asarray(geom.edge_cellR)
+    edges = np.nonzero(cell_r == -1)[0].astype(np.int64)
+    ptr = np.asarray(geom.cell_vertex_ptr, dtype=np.int64)
+    cells = np.asarray(geom.edge_cellL, dtype=np.int64)[edges]
+    start = ptr[cells]
+    n_sides = ptr[cells + 1] - start
+    side = np.asarray(geom.edge_sideL, dtype=np.int64)[edges]
+    corners = np.asarray(geom.cell_vertices, dtype=np.float64)
+    tail = corners[start + side]
+    head = corners[start + np.where(side + 1 == n_sides, 0, side + 1)]
+    return edges, (tail + head) / 2.0
 
 
 def distance_to_line(points: NDArray[np.float64], line: NDArray[np.float64]) -> NDArray[np.float64]:
-    """Euclidean distance from each point to the nearest segment of a polyline."""
-    p = np.asarray(points, dtype=np.float64).reshape(-1, 2)
+    """Distance from each point to the nearest point of a polyline (all segments at once)."""
+    p = np.asarray(points, dtype=np.float64).reshape(-1, 1, 2)
     q = np.asarray(line, dtype=np.float64).reshape(-1, 2)
-    out = np.full(p.shape[0], np.inf)
-    for a, b in zip(q[:-1], q[1:], strict=True):
-        d = b - a
-        t = np.clip(((p - a) @ d) / max(float(d @ d), 1e-30), 0.0, 1.0)
-        foot = a + t[:, None] * d
-        out = np.minimum(out, np.hypot(p[:, 0] - foot[:, 0], p[:, 1] - foot[:, 1]))
-    return out
+    a, b = q[:-1][None, :, :], q[1:][None, :, :]  # (1, segments, 2)
+    ab = b - a
+    span = np.maximum(np.einsum("psk,psk->ps", ab, ab), 1e-30)
+    along = np.clip(np.einsum("psk,psk->ps", p - a, ab) / span, 0.0, 1.0)
+    gap = p - (a + along[..., None] * ab)
+    return np.sqrt(np.einsum("psk,psk->ps", gap, gap)).min(axis=1)
 
 
 def select_boundary_edges(
@@ -53,9 +71,42 @@ def select_boundary_edges(
     """Boundary edges whose midpoints lie within ``tolerance_m`` of a world-coordinate polyline."""
     if len(line) < 2:
         raise ValueError(f"a stretch needs at least 2 points, got {len(line)}")
-    local = np.asarray(line, dtype=np.float64) - np.asarray(geom.origin, dtype=np.float64)
     edges, mid = boundary_edge_midpoints(geom)
-    return edges[distance_to_line(mid, local) <= tolerance_m]
+    local = np.asarray(line, dtype=np.float64) - np.asarray(geom.origin, dtype=np.float64)
+    near = distance_to_line(mid, local) <= tolerance_m
+    return edges[near]
+
+
+def _conveyance_level(rate: NDArray[np.float64], bed: NDArray[np.float64], discharge: float) -> float:
+    """The level L with sum(rate * max(L - bed, 0)^(5/3)) = discharge (discharge > 0)."""
+    order = np.argsort(bed, kind="stable")
+    b, r = bed[order], rate[order]
+
+    def carried(level: float, k: int) -> tuple[float, float]:
+        d = level - b[:k]
+        return float((r[:k] * d**_CONVEYANCE_POWER).sum()), float((_CONVEYANCE_POWER * r[:k] * d ** (2.0 / 3.0)).sum())
+
+    # Smallest k such that the level lies in [b[k-1], b[k]): the first bed
+    # at which the k lower edges already carry the discharge.
+    k = b.size
+    for j in range(1, b.size):
+        if carried(float(b[j]), j)[0] >= discharge:
+            k = j
+            break
+    lo = float(b[k - 1])
+    # Upper start: on [b[k-1], inf) with the k lowest edges wet, the sum is
+    # increasing and convex, so Newton from above decreases to the root.
+    level = float(b[k]) if k < b.size else lo + 1.0
+    while carried(level, k)[0] < discharge:
+        level = lo + 2.0 * (level - lo)
+    for _ in range(_NEWTON_STEPS):
+        f, df = carried(level, k)
+        step = (f - discharge) / df if df > 0.0 else 0.0
+        nxt = max(level - step, lo)
+        if nxt >= level:
+            break
+        level = nxt
+    return level
 
 
 def conveyance_split(
@@ -67,37 +118,25 @@ def conveyance_split(
 ) -> NDArray[np.float32]:
     """Discharge per unit width on each edge of a section that carries ``discharge_m3s``.
 
-    The section fills to the level at which its Manning conveyance
-    ``sum(len * sqrt(S) / n * max(level - bed, 0)^(5/3))`` carries the
-    discharge (bisection on the level), so water enters along the low part of
-    the section. Each edge takes its share of that conveyance; the result is
-    float32, for the kernels, and the edge with the largest share absorbs the
-    cast's rounding so that ``sum(q * len)`` is the discharge.
+    Each edge carries ``len * sqrt(S) / n * max(L - bed, 0)^(5/3)`` at the
+    level L where the section carries the whole discharge, so water enters
+    along the low part of the section. The result is float32 for the
+    kernels; the longest wetted edge takes the float32 rounding, so that
+    ``sum(q * len)`` is the discharge.
     """
-    if discharge_m3s <= 0.0:
-        return np.zeros(np.asarray(lengths).size, dtype=np.float32)
     length = np.asarray(lengths, dtype=np.float64)
+    out = np.zeros(length.size, dtype=np.float32)
+    if discharge_m3s <= 0.0 or length.size == 0:
+        return out
     bed = np.asarray(beds, dtype=np.float64)
     rate = length * np.sqrt(energy_slope) / np.asarray(manning, dtype=np.float64)
-
-    def carried(level: float) -> NDArray[np.float64]:
-        return rate * np.maximum(level - bed, 0.0) ** (5.0 / 3.0)
-
-    lo = float(bed.min())
-    hi = lo + 1.0
-    while carried(hi).sum() < discharge_m3s:
-        hi = lo + 2.0 * (hi - lo)
-    for _ in range(LEVEL_BISECTIONS):
-        mid = 0.5 * (lo + hi)
-        if carried(mid).sum() < discharge_m3s:
-            lo = mid
-        else:
-            hi = mid
-    share = carried(hi)
-    q = (discharge_m3s * share / share.sum() / length).astype(np.float32)
-    k = int(np.argmax(share))
-    q[k] += np.float32((discharge_m3s - float((q.astype(np.float64) * length).sum())) / length[k])
-    return q
+    level = _conveyance_level(rate, bed, float(discharge_m3s))
+    share = rate * np.maximum(level - bed, 0.0) ** _CONVEYANCE_POWER
+    out[:] = (float(discharge_m3s) / share.sum()) * share / length
+    wet = np.nonzero(share > 0.0)[0]
+    k = int(wet[np.argmax(length[wet])])
+    out[k] += np.float32((float(discharge_m3s) - float((out.astype(np.float64) * length).sum())) / length[k])
+    return out
 
 
 __all__ = ["boundary_edge_midpoints", "conveyance_split", "distance_to_line", "select_boundary_edges"]

@@ -397,15 +397,22 @@ def hilbert_permutation(centroid: NDArray[np.float32], hilbert_p: int = 10) -> N
     return np.argsort(hilbert_distances(ix, iy, hilbert_p), kind="stable").astype(np.int32)
 
 
-def _permuted_rows(
+def _rows_in_new_order(
     ptr: NDArray[np.int32], perm: NDArray[np.int32]
 ) -> tuple[NDArray[np.int32], NDArray[np.int64]]:
-    """Row pointers of a CSR after permuting its rows, and each new slot's old slot."""
-    deg = np.diff(ptr)[perm]
-    new_ptr = np.zeros(ptr.size, dtype=np.int32)
-    np.cumsum(deg, out=new_ptr[1:])
-    shift = ptr[perm].astype(np.int64) - new_ptr[:-1]
-    return new_ptr, np.repeat(shift, deg) + np.arange(int(new_ptr[-1]))
+    """A CSR whose rows are taken in the order ``perm``: its row pointers and the old slot of each new slot.
+
+    Every old slot is labelled with the new position of its row; a stable
+    sort on that label lists the slots row by row in the new order, each
+    row's slots in their old order.
+    """
+    count = np.diff(ptr).astype(np.int64)
+    new_pos = np.empty(count.size, dtype=np.int64)
+    new_pos[np.asarray(perm, dtype=np.int64)] = np.arange(count.size, dtype=np.int64)
+    label = np.repeat(new_pos, count)
+    old_slot = np.argsort(label, kind="stable").astype(np.int64)
+    new_ptr = np.concatenate([[0], np.cumsum(count[perm])]).astype(np.int32)
+    return new_ptr, old_slot
 
 
 def hilbert_reorder(
@@ -425,7 +432,7 @@ def hilbert_reorder(
     inv[perm] = np.arange(n, dtype=np.int32)
 
     centroid = geom.centroid[perm]
-    vptr, vslots = _permuted_rows(geom.cell_vertex_ptr, perm)
+    vptr, vslots = _rows_in_new_order(geom.cell_vertex_ptr, perm)
     cverts = geom.cell_vertices[vslots]
     del vslots
 
@@ -434,7 +441,7 @@ def hilbert_reorder(
     inner = geom.edge_cellR != -1
     cr[inner] = inv[geom.edge_cellR[inner]]
 
-    eptr, eslots = _permuted_rows(geom.cell_edge_ptr, perm)
+    eptr, eslots = _rows_in_new_order(geom.cell_edge_ptr, perm)
     cidx = geom.cell_edge_idx[eslots]
     cside = geom.cell_edge_side[eslots]
     old_nbr = geom.cell_nbr_idx[eslots]
@@ -445,7 +452,7 @@ def hilbert_reorder(
     step = centroid[cr[m]] - centroid[cl[m]]
     dots = step[:, 0] * geom.edge_nx[m] + step[:, 1] * geom.edge_ny[m]
     if not (dots >= -2.0 * np.spacing(centroid.max())).all():
-        raise ValueError("Normal orientation broken after Hilbert reordering!")
+        raise ValueError("hilbert_reorder: an edge normal no longer points from its left cell to its right cell")
 
     key = np.where(cr >= 0, np.minimum(cl, cr), cl)
     eperm = np.argsort(key, kind="stable").astype(np.int32)

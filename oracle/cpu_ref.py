@@ -15,12 +15,9 @@ from typing import Any
 import numpy as np
 from numpy.typing import NDArray
 
-from inundation_amd.mesh.blocks import (
-    block_adjacency,
-    block_two_ring,
-    edge_workgroup_block_ranges,
-)
 from inundation_amd.tuning import ACTIVE_BLOCK_CELLS
+
+from .blocks_ref import block_adjacency, block_two_ring, edge_workgroup_block_ranges
 
 LIB_PATH = Path(__file__).resolve().parent / "libcpu_ref.so"
 _lib: C.CDLL | None = None

