@@ -120,61 +120,116 @@ def lattice_tri_quad(
     return verts, faces, offsets.astype(np.int32)
 
 
+def _mirrored(p: NDArray[np.float64], length: float, band: float) -> NDArray[np.float64]:
+    """The points followed by their mirror images across each side of the box, for those within ``band`` of it."""
+    parts = [p]
+    for axis, side in ((0, 0.0), (0, length), (1, 0.0), (1, length)):
+        near = p[np.abs(p[:, axis] - side) < band]
+        m = near.copy()
+        m[:, axis] = 2.0 * side - m[:, axis]
+        parts.append(m)
+    return np.concatenate(parts)
+
+
+def _regions(pts: NDArray[np.float64], n_own: int) -> tuple[NDArray[np.float64], NDArray[np.int64], NDArray[np.int64]]:
+    """Voronoi vertices of ``pts`` and the polygons (flat vertex ids, offsets) of the first ``n_own`` points."""
+    from scipy.spatial import Voronoi
+
+    vor = Voronoi(pts)
+    regions = [vor.regions[r] for r in vor.point_region[:n_own].tolist()]
+    deg = np.fromiter((len(r) for r in regions), dtype=np.int64, count=len(regions))
+    offsets = np.zeros(deg.size + 1, dtype=np.int64)
+    np.cumsum(deg, out=offsets[1:])
+    flat = np.fromiter((v for r in regions for v in r), dtype=np.int64, count=int(offsets[-1]))
+    if (flat < 0).any():
+        raise RuntimeError("an unbounded Voronoi region inside the box")
+    return vor.vertices, flat, offsets
+
+
+def _centroids(pts: NDArray[np.float64], n_own: int) -> NDArray[np.float64]:
+    """Area centroids of the Voronoi cells of the first ``n_own`` points of ``pts``."""
+    verts, flat, offsets = _regions(pts, n_own)
+    nxt = np.arange(1, flat.size + 1)
+    nxt[offsets[1:] - 1] = offsets[:-1]  # each polygon's last vertex wraps to its first
+    xs, ys = verts[flat, 0], verts[flat, 1]
+    xn, yn = xs[nxt], ys[nxt]
+    cr = xs * yn - xn * ys
+    a6 = 3.0 * np.add.reduceat(cr, offsets[:-1])
+    cx = np.add.reduceat((xs + xn) * cr, offsets[:-1]) / a6
+    cy = np.add.reduceat((ys + yn) * cr, offsets[:-1]) / a6
+    return np.column_stack([cx, cy])
+
+
+def _tile_centroids(args: tuple) -> NDArray[np.float64]:  # noqa: ANN401
+    own, halo = args
+    return _centroids(np.concatenate([own, halo]), own.shape[0])
+
+
+def _lloyd_step(pts: NDArray[np.float64], length: float, band: float, tiles: int, processes: int) -> NDArray[np.float64]:
+    """One Lloyd step: every point to its cell's centroid.
+
+    With ``tiles`` > 1 the box is cut into tiles x tiles squares; each tile's
+    cells come from the Voronoi diagram of its own points plus every point
+    (mirrors included) within a halo of 8 mean spacings around it, which
+    holds all the neighbours a cell of uniform points has. The tiles run in
+    ``processes`` worker processes (spawned, so a parent that already uses a
+    GPU is safe).
+    """
+    everything = _mirrored(pts, length, band)
+    if tiles <= 1:
+        return _centroids(everything, pts.shape[0])
+    halo_w = 2.0 * band
+    edges = np.linspace(0.0, length, tiles + 1)
+    ti = np.minimum((pts[:, 0] / length * tiles).astype(np.int64), tiles - 1)
+    tj = np.minimum((pts[:, 1] / length * tiles).astype(np.int64), tiles - 1)
+    jobs, owners = [], []
+    for i in range(tiles):
+        for j in range(tiles):
+            mine = np.flatnonzero((ti == i) & (tj == j))
+            x0, x1, y0, y1 = edges[i] - halo_w, edges[i + 1] + halo_w, edges[j] - halo_w, edges[j + 1] + halo_w
+            inside = (everything[:, 0] >= x0) & (everything[:, 0] <= x1) & (everything[:, 1] >= y0) & (
+                everything[:, 1] <= y1)
+            inside[mine] = False
+            jobs.append((pts[mine], everything[inside]))
+            owners.append(mine)
+    out = np.empty_like(pts)
+    if processes > 1:
+        import multiprocessing as mp
+
+        with mp.get_context("spawn").Pool(min(processes, len(jobs))) as pool:
+            results = pool.map(_tile_centroids, jobs)
+    else:
+        results = [_tile_centroids(job) for job in jobs]
+    for mine, c in zip(owners, results):
+        out[mine] = c
+    return out
+
+
 def voronoi_mesh(
-    n_points: int, length: float, rng: np.random.Generator, *, lloyd: int = 2
+    n_points: int, length: float, rng: np.random.Generator, *, lloyd: int = 2, processes: int = 1
 ) -> tuple[NDArray[np.float64], NDArray[np.int32], NDArray[np.int32]]:
     """Lloyd-relaxed Voronoi cells of seeded uniform points in ``[0, length]^2``.
 
     Points within a few mean spacings of a side are mirrored across it, so
     every cell is closed and the boundary cells end exactly on the square (the
-    bisector of a point and its mirror is the side). Returns ``(verts,
-    faces_flat, face_offsets)`` with the welded Voronoi vertices.
+    bisector of a point and its mirror is the side). The Lloyd steps of more
+    than 400 000 points run on 4 x 4 tiles (the cells are those of the whole
+    box; their centroids can differ from a whole-box step in the last bits),
+    in ``processes`` worker processes; the result does not depend on
+    ``processes``. Returns ``(verts, faces_flat, face_offsets)`` with the
+    welded Voronoi vertices of the final diagram, which is always computed
+    over the whole box.
     """
-    from scipy.spatial import Voronoi
-
     pts = rng.uniform(0.0, length, size=(n_points, 2))
     band = 4.0 * length / np.sqrt(max(n_points, 1))
-
-    def cells(p: NDArray[np.float64]) -> tuple[Voronoi, NDArray[np.int64]]:
-        parts = [p]
-        for axis, side in ((0, 0.0), (0, length), (1, 0.0), (1, length)):
-            near = p[np.abs(p[:, axis] - side) < band]
-            m = near.copy()
-            m[:, axis] = 2.0 * side - m[:, axis]
-            parts.append(m)
-        vor = Voronoi(np.concatenate(parts))
-        return vor, np.asarray(vor.point_region[: p.shape[0]], dtype=np.int64)
-
-    def flat_regions(vor: Voronoi, reg: NDArray[np.int64]) -> tuple[NDArray[np.int64], NDArray[np.int64]]:
-        regions = [vor.regions[r] for r in reg.tolist()]
-        deg = np.fromiter((len(r) for r in regions), dtype=np.int64, count=len(regions))
-        offsets = np.zeros(deg.size + 1, dtype=np.int64)
-        np.cumsum(deg, out=offsets[1:])
-        flat = np.fromiter((v for r in regions for v in r), dtype=np.int64, count=int(offsets[-1]))
-        return flat, offsets
-
+    tiles = 4 if n_points > 400_000 else 1
     for _ in range(lloyd):
-        vor, reg = cells(pts)
-        flat, offsets = flat_regions(vor, reg)
-        if (flat < 0).any():
-            raise RuntimeError("an unbounded Voronoi region inside the box")
-        nxt = np.arange(1, flat.size + 1)
-        nxt[offsets[1:] - 1] = offsets[:-1]  # each polygon's last vertex wraps to its first
-        xs, ys = vor.vertices[flat, 0], vor.vertices[flat, 1]
-        xn, yn = xs[nxt], ys[nxt]
-        cr = xs * yn - xn * ys
-        a6 = 3.0 * np.add.reduceat(cr, offsets[:-1])
-        cx = np.add.reduceat((xs + xn) * cr, offsets[:-1]) / a6
-        cy = np.add.reduceat((ys + yn) * cr, offsets[:-1]) / a6
-        pts = np.clip(np.column_stack([cx, cy]), 1e-9 * length, length * (1.0 - 1e-9))
-    vor, reg = cells(pts)
-    flat, offsets = flat_regions(vor, reg)
-    if (flat < 0).any():
-        raise RuntimeError("an unbounded Voronoi region inside the box")
+        pts = np.clip(_lloyd_step(pts, length, band, tiles, processes), 1e-9 * length, length * (1.0 - 1e-9))
+    verts_all, flat, offsets = _regions(_mirrored(pts, length, band), n_points)
     used = np.unique(flat)
-    remap = np.full(vor.vertices.shape[0], -1, dtype=np.int64)
+    remap = np.full(verts_all.shape[0], -1, dtype=np.int64)
     remap[used] = np.arange(used.size)
-    verts = np.clip(vor.vertices[used], 0.0, length)
+    verts = np.clip(verts_all[used], 0.0, length)
     return verts, remap[flat].astype(np.int32), offsets.astype(np.int32)
 
 

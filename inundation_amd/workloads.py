@@ -110,10 +110,12 @@ def rainfall(scale: int = 1, seed: int = 1003, rain_mm_h: float = 50.0) -> Workl
     )
 
 
-def inflow(n_points: int = 1_000_000, seed: int = 1002, length: float = 2000.0) -> tuple[Workload, list[SimulationPhase]]:
-    """Voronoi inflow flood and its 60 one-minute hydrograph phases."""
+def inflow(
+    n_points: int = 1_000_000, seed: int = 1002, length: float = 2000.0, processes: int = 8
+) -> tuple[Workload, list[SimulationPhase]]:
+    """Voronoi inflow flood and its 60 one-minute hydrograph phases (``processes``: mesh-building workers)."""
     rng = np.random.default_rng(seed)
-    verts, faces, offsets = voronoi_mesh(n_points, length, rng)
+    verts, faces, offsets = voronoi_mesh(n_points, length, rng, processes=processes)
     cx, cy = _vertex_means(verts, faces, offsets)
     zb = fbm_terrain(cx, cy, length, rng)
     geom, _ = hilbert_reorder(build_geometry(verts, faces, offsets, zb_from_file=zb), verbose=False)
@@ -134,9 +136,9 @@ def inflow(n_points: int = 1_000_000, seed: int = 1002, length: float = 2000.0) 
     return wl, phases
 
 
-def _voronoi_terrain(n_points: int, seed: int, length: float) -> MeshGeometry:
+def _voronoi_terrain(n_points: int, seed: int, length: float, processes: int = 1) -> MeshGeometry:
     rng = np.random.default_rng(seed)
-    verts, faces, offsets = voronoi_mesh(n_points, length, rng)
+    verts, faces, offsets = voronoi_mesh(n_points, length, rng, processes=processes)
     cx, cy = _vertex_means(verts, faces, offsets)
     zb = fbm_terrain(cx, cy, length, rng)
     return hilbert_reorder(build_geometry(verts, faces, offsets, zb_from_file=zb), verbose=False)[0]
@@ -181,24 +183,31 @@ def batched(
 
     Scenario k runs on mesh k mod len(sizes), with a seeded inflow peak
     U[5, 50] m^3/s into the channel near the upstream edge and uniform rain
-    U[0, 80] mm/h. The meshes are built in parallel worker processes (start
-    them before the GPU is used).
+    U[0, 80] mm/h. With ``processes`` > 1 the meshes are built in spawned
+    worker processes, safe in a process that already uses a GPU.
     """
     import multiprocessing as mp
 
     args = [(n, seed * 10 + i, length) for i, n in enumerate(sizes)]
+    geoms: list[MeshGeometry | None] = [None] * len(sizes)
     if processes > 1 and len(sizes) > 1:
-        with mp.get_context("fork").Pool(min(processes, len(sizes))) as pool:
-            pending = [pool.apply_async(_voronoi_terrain, x) for x in args]
-            geoms = []
-            for x, p in zip(args, pending):
-                geoms.append(p.get())
+        # The largest mesh is built here with its Lloyd tiles spread over
+        # workers, the others side by side in a pool (spawned workers: safe
+        # in a process that already uses a GPU).
+        big = int(np.argmax(sizes))
+        rest = [k for k in range(len(sizes)) if k != big]
+        with mp.get_context("spawn").Pool(min(processes, len(rest))) as pool:
+            pending = {k: pool.apply_async(_voronoi_terrain, args[k]) for k in rest}
+            geoms[big] = _voronoi_terrain(*args[big], processes=max(2, processes))
+            if progress:
+                progress(f"mesh of {sizes[big]} points built")
+            for k in rest:
+                geoms[k] = pending[k].get()
                 if progress:
-                    progress(f"mesh of {x[0]} points built")
+                    progress(f"mesh of {sizes[k]} points built")
     else:
-        geoms = []
-        for x in args:
-            geoms.append(_voronoi_terrain(*x))
+        for k, x in enumerate(args):
+            geoms[k] = _voronoi_terrain(*x)
             if progress:
                 progress(f"mesh of {x[0]} points built")
     x0 = 0.05 * length

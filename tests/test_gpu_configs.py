@@ -12,6 +12,9 @@
   tolerance.
 - C2, the 1 M-cell Voronoi inflow flood: the first three one-minute
   hydrograph phases bitwise the FP32 oracle, phase by phase.
+- C4, 64 scenarios over five Voronoi meshes of 50 k to 2 M cells: ten of
+  them (two per mesh) batched through two phases at the autotuned tile
+  level, each bit for bit the same scenario run alone and the FP32 oracle.
 - C5, the 250 k-cell adjoint calibration mesh: the GPU gradient of a gauge
   misfit over the first 10 steps against the FP64 adjoint, and the replay of
   the recording bit for bit.
@@ -288,5 +291,93 @@ def test_c5_calibration_gradient_matches_fp64_and_replays_bitwise(say) -> None: 
         say(f"C5: gradient of {k_steps} recorded steps vs the FP64 adjoint: n {en:.2e}, z {ez:.2e} (relative norm); "
             f"replay bitwise")
         assert en < 1e-4 and ez < 1e-4
+    finally:
+        ctx.close()
+
+
+# -- C4 ---------------------------------------------------------------------------
+
+
+def _drive_phases(be, parts, n_phases: int, multi: bool) -> None:  # noqa: ANN001
+    """One-minute phases on a context or oracle holding the scenarios ``parts`` back to back.
+
+    ``parts`` is [(scenario, geometry)]; each phase sets every scenario's
+    sources for its mid-time, moves each stop one minute on and steps in
+    blocks of 50 until every scenario is there.
+    """
+    base = np.cumsum([0] + [g.N for _, g in parts])
+    for j in range(n_phases):
+        t_mid = (j + 0.5) * 60.0
+        pairs = [s.source_pairs(g, t_mid) for s, g in parts]
+        be.set_sources(np.concatenate([c + np.uint32(base[k]) for k, (c, _) in enumerate(pairs)]).astype(np.uint32),
+                       np.concatenate([r for _, r in pairs]).astype(np.float32))
+        cs = be.ctrls_read() if multi else [be.ctrl_read()]
+        for c in cs:
+            c.stop_ticks = (j + 1) * 60 * 2**20
+        be.ctrl_write(cs if multi else cs[0])
+        be.retarget() if j else be.prologue()
+        while any(int(c.now_ticks) < int(c.stop_ticks) for c in (be.ctrls_read() if multi else [be.ctrl_read()])):
+            be.steps(50, use_graph=True)
+
+
+def test_c4_batched_scenarios_on_the_five_meshes_are_each_alone_and_the_oracle(say) -> None:  # noqa: ANN001
+    from inundation_amd.workloads import batched
+
+    say("C4: building the five Voronoi meshes (50 k to 2 M cells)")
+    geoms, scen = batched(n_scenarios=64, progress=say, processes=8)
+    picked = list(range(10))  # two scenarios on each mesh
+    parts = [(scen[k], geoms[scen[k].mesh]) for k in picked]
+    dev = [device_mesh_arrays(g) for g in geoms]
+    n_phases = 2
+    zero_ctrl = _ctrl(0, 0, 1.0, 0.1)
+    ctx = gpu_context()
+    try:
+        ctx.set_batch(dev, [s.mesh for s, _ in parts])
+        n_all = sum(g.N for _, g in parts)
+        D.setup(ctx, n_all, *(np.zeros(n_all, np.float32) for _ in range(3)), 0.035, dry_tol=1e-4, cfl=0.8)
+        ctx.ctrl_write([_ctrl(0, 0, 1.0, 0.1) for _ in parts])
+        # Autotune on a flooding state, then the batched run from the start at that tile level.
+        pairs = [s.source_pairs(g, 30.0) for s, g in parts]
+        base = np.cumsum([0] + [g.N for _, g in parts])
+        ctx.set_sources(np.concatenate([c + np.uint32(base[k]) for k, (c, _) in enumerate(pairs)]).astype(np.uint32),
+                        np.concatenate([r for _, r in pairs]).astype(np.float32))
+        cs = ctx.ctrls_read()
+        for c in cs:
+            c.stop_ticks = 10**6 * 2**20
+        ctx.ctrl_write(cs)
+        ctx.prologue()
+        ctx.steps(100, use_graph=False)
+        tile = ctx.autotune(3)["tile"]
+        z = np.zeros(n_all, np.float32)
+        ctx.set_state(z, z, z, None)
+        ctx.ctrl_write([_ctrl(0, 0, 1.0, 0.1) for _ in parts])
+        _drive_phases(ctx, parts, n_phases, multi=True)
+        say(f"C4: {len(parts)} scenarios batched over {n_phases} phases at tile level {tile}: "
+            f"{[int(c.steps) for c in ctx.ctrls_read()]} steps")
+        fields = (L.FIELD_H, L.FIELD_HU, L.FIELD_HV, L.FIELD_H_LO)
+        got = {f: ctx.download(f, n_all) for f in fields}
+        ctrls = ctx.ctrls_read()
+        for k, (s, g) in enumerate(parts):
+            alone = gpu_context()
+            try:
+                _set_mesh(alone, dev[s.mesh])
+                ref = cpu_ref.CpuRef(dev[s.mesh])
+                for be in (alone, ref):
+                    D.setup(be, g.N, *(np.zeros(g.N, np.float32) for _ in range(3)), 0.035, dry_tol=1e-4, cfl=0.8)
+                    be.ctrl_write(zero_ctrl)
+                alone.set_tile(tile)
+                _drive_phases(alone, [(s, g)], n_phases, multi=False)
+                _drive_phases(ref, [(s, g)], n_phases, multi=False)
+                for f in fields:
+                    mine = got[f][base[k]:base[k + 1]]
+                    for other, who in ((alone, "alone"), (ref, "oracle")):
+                        o = other.download(f, g.N)
+                        d = np.flatnonzero(mine.view(np.uint32) != o.view(np.uint32))
+                        assert d.size == 0, f"scenario {picked[k]} field {f}: {d.size} words differ from {who}"
+                for other, who in ((alone, "alone"), (ref, "oracle")):
+                    assert bytes(other.ctrl_read()) == bytes(ctrls[k]), f"scenario {picked[k]}: clock differs from {who}"
+            finally:
+                alone.close()
+            say(f"C4 scenario {picked[k]} ({g.N} cells, {int(ctrls[k].steps)} steps): batched = alone = oracle")
     finally:
         ctx.close()
