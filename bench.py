@@ -3,8 +3,10 @@
 
 Workload: the 16 003 008-cell rainfall flood (mixed triangle/quad lattice over
 8 km of fBm terrain, 50 mm/h uniform rain from a dry start, per-cell Manning
-n, FP32 with compensated depth, active set on, 2000 Heun steps). A "step" is
-one Heun step of the whole mesh.
+n, FP32 with compensated depth, active set on). A "step" is one Heun step of
+the whole mesh. After 20 warm-up steps the flood restarts from dry, runs 100
+untimed lead steps through the wetting transient, then three consecutive
+timed windows of 2000 steps; ``value`` comes from the median window.
 
 - ``value``: cell-steps/s over the timed steps, all cells counted every step
   (the reference's convention, flood_propagation.py:645-646), with the state
@@ -12,7 +14,8 @@ one Heun step of the whole mesh.
 - ``roofline``: the corrector stage kernel (the larger of the two stage
   kernels), algorithmic bytes of the active blocks (28 B per edge, 4 B per
   CSR slot, 60 B per cell with the compensated low words) over its
-  HIP-event-timed duration, against 8 TB/s;
+  HIP-event-timed duration, against 8 TB/s, with the HBM traffic measured
+  by rocprofv3 counters and the attainable copy rates of the same box;
 - ``cpu_baseline``: the FP32 oracle (C++/OpenMP) on the same mesh and state
   for a bounded number of steps, checked bit for bit against the GPU on the
   same steps.
