@@ -326,35 +326,51 @@ def main() -> int:
 
 
 def _phase_loop(ctx, geoms, scen, n_phases: int, block: int, first: int = 0, verbose: bool = False) -> None:  # noqa: ANN001
-    """Run every scenario of ctx through the one-minute phases: sources, retarget, steps to the stop."""
+    """Run every scenario of ctx through its one-minute phases, each scenario on its own schedule.
+
+    A scenario that reaches the end of a phase moves on at once: its inflow
+    rate for the next minute, its stop one minute on and a retarget. The
+    others keep stepping; a retarget leaves a scenario whose stop did not
+    move bit for bit where it was (the same next dt from its last bound,
+    the same lists). Alone, this is the plain phase loop.
+    """
     sel = scen[first:first + ctx.n_scenarios]
     base = np.cumsum([0] + [geoms[s.mesh].N for s in sel])
-    for j in range(n_phases):
-        t_mid = (j + 0.5) * 60.0
-        pairs = [s.source_pairs(geoms[s.mesh], t_mid) for s in sel]
-        if j == 0:
-            cells = np.concatenate([c + np.uint32(base[k]) for k, (c, _) in enumerate(pairs)]).astype(np.uint32)
-            ctx.set_sources(cells, np.concatenate([r for _, r in pairs]).astype(np.float32))
-        else:
-            # Only the inflow cells' rates change between phases.
-            at = [int(np.searchsorted(pairs[k][0], s.inflow_cell)) for k, s in enumerate(sel)]
-            cells = np.array([base[k] + s.inflow_cell for k, s in enumerate(sel)], np.uint32)
-            rates = np.array([pairs[k][1][at[k]] for k in range(len(sel))], np.float32)
-            ctx.set_source_rates(cells, rates)
+    pairs = [s.source_pairs(geoms[s.mesh], 30.0) for s in sel]
+    cells = np.concatenate([c + np.uint32(base[k]) for k, (c, _) in enumerate(pairs)]).astype(np.uint32)
+    ctx.set_sources(cells, np.concatenate([r for _, r in pairs]).astype(np.float32))
+    cs = ctx.ctrls_read()
+    for c in cs:
+        c.stop_ticks = int(c.stop_ticks) + 60 * 2**20
+    ctx.ctrl_write(cs)
+    ctx.prologue()
+    phase = [0] * len(sel)
+    reported = 0
+    while True:
+        ctx.steps(block, use_graph=True)
         cs = ctx.ctrls_read()
-        for c in cs:
-            c.stop_ticks = int(c.stop_ticks) + 60 * 2**20
-        ctx.ctrl_write(cs)
-        if j == 0:
-            ctx.prologue()
-        else:
-            ctx.retarget()
-        while True:
-            ctx.steps(block, use_graph=True)
-            if all(int(c.now_ticks) >= int(c.stop_ticks) for c in ctx.ctrls_read()):
+        at_stop = [int(c.now_ticks) >= int(c.stop_ticks) for c in cs]
+        move = [k for k in range(len(sel)) if at_stop[k] and phase[k] + 1 < n_phases]
+        if not move:
+            if all(at_stop):
                 break
-        if verbose:
-            log(f"phase {j + 1}/{n_phases} done")
+            continue
+        # Only the inflow cells' rates change between phases.
+        rates = []
+        for k in move:
+            phase[k] += 1
+            c_k, r_k = sel[k].source_pairs(geoms[sel[k].mesh], (phase[k] + 0.5) * 60.0)
+            rates.append(r_k[int(np.searchsorted(c_k, sel[k].inflow_cell))])
+            cs[k].stop_ticks = int(cs[k].stop_ticks) + 60 * 2**20
+        ctx.set_source_rates(np.array([base[k] + sel[k].inflow_cell for k in move], np.uint32),
+                             np.array(rates, np.float32))
+        ctx.ctrl_write(cs)
+        ctx.retarget()
+        if verbose and min(phase) > reported:
+            reported = min(phase)
+            log(f"every scenario in phase {reported + 1}/{n_phases} or later")
+    if verbose:
+        log(f"all {len(sel)} scenarios through {n_phases} phases")
 
 
 def _start_ctrls(n: int):  # noqa: ANN202
@@ -628,6 +644,39 @@ def run_adjoint(cells: int, steps: int, fd_cells: int, cpu_seconds: float | None
     log(f"plain forward {plain * 1e3:.1f} ms, recorded forward {fwd * 1e3:.1f} ms, backward {bwd * 1e3:.1f} ms: "
         f"gradient = {ratio:.2f}x forward; loss {loss:.4e}; wet cells at the end {int((h_end > 1e-4).sum())}")
 
+    # The same gradient under binomial checkpointing: 5 checkpoint slots
+    # instead of one per segment; the extra replays cost time, not bits.
+    slots = 5
+    run.slots = slots
+    t_bfwd, t_bbwd = [], []
+    try:
+        for _ in range(2):
+            t0 = time.perf_counter()
+            run.forward(n0, z)
+            t1 = time.perf_counter()
+            gn_b, gz_b = run.backward(seeds)
+            t2 = time.perf_counter()
+            t_bfwd.append(t1 - t0)
+            t_bbwd.append(t2 - t1)
+        replayed = ctx.adj_replayed()
+    finally:
+        run.slots = 0
+    n_seg = -(-cal.steps // cal.seg)
+    binomial = {
+        "slots": slots,
+        "segments": n_seg,
+        "checkpoint_bytes": slots * 16 * geom.N,
+        "uniform_checkpoint_bytes": (n_seg + 1) * 16 * geom.N,
+        "recorded_forward_s": min(t_bfwd),
+        "backward_s": min(t_bbwd),
+        "cost_over_forward": (min(t_bfwd) + min(t_bbwd)) / plain,
+        "steps_advanced_without_tape": replayed["advanced"],
+        "steps_replayed_with_tape": replayed["taped"],
+        "gradient_bitwise_uniform": bool(np.array_equal(gn_b.view(np.uint32), gn.view(np.uint32))
+                                         and np.array_equal(gz_b.view(np.uint32), gz.view(np.uint32))),
+    }
+    log(f"binomial checkpointing: {binomial}")
+
     # Central differences of replays (the recorded dt sequence and source events).
     def loss_of(nn, zz) -> float:  # noqa: ANN001
         s_ = run.replay(nn, zz).astype(np.float64)
@@ -713,6 +762,7 @@ def run_adjoint(cells: int, steps: int, fd_cells: int, cpu_seconds: float | None
             "cost_over_forward": ratio,
             "loss": loss,
             "replay_bitwise": bool(replay_same),
+            "binomial": binomial,
             "active_block_fraction_at_end": active_frac,
             "fd_eps": [eps, eps2],
             "fd": fd,
@@ -865,7 +915,7 @@ def adjoint_leg(args) -> dict:  # noqa: ANN001
     return {"workload": r["config"]["workload"], "value": r["value"], "unit": r["unit"],
             "cost_over_forward": a["cost_over_forward"], "plain_forward_s": a["plain_forward_s"],
             "recorded_forward_s": a["recorded_forward_s"], "backward_s": a["backward_s"], "fd": a["fd"],
-            "replay_bitwise": a["replay_bitwise"]}
+            "replay_bitwise": a["replay_bitwise"], "binomial": a["binomial"]}
 
 
 def adjoint_cost_wet(scale: int = 2, steps: int = 300, seg: int = 50) -> dict:

@@ -353,6 +353,23 @@ int inu_adj_stats(inu_ctx* ctx, unsigned long long* out3);
  * compound at every wetting front. 0 holds none; negative restores the
  * default, 100 x dry_tol. Also used by inu_probe_vjp. */
 int inu_adj_set_thin(inu_ctx* ctx, float depth);
+/* Checkpointing of the next recording (call before inu_adj_begin). slots = 0
+ * (the default): a checkpoint every `seg` steps, written by the recording,
+ * and each segment replayed once by the backward pass. slots in [1, 64]:
+ * binomial checkpointing (Griewank's revolve) over the recording's segments
+ * with `slots` checkpoints of 16 B per cell: the recording keeps only its
+ * start, and the backward pass follows the optimal offline schedule,
+ * advancing segments without tape to store checkpoints and replaying each
+ * segment with tape once, just before reversing it. Device memory for the
+ * checkpoints: slots x 16 B per cell instead of (steps / seg + 1) x 16 B;
+ * the gradient is bitwise the same. At most 2048 segments. */
+int inu_adj_set_slots(inu_ctx* ctx, uint32_t slots);
+/* Steps the last inu_adj_backward replayed: out2 = (without tape, with tape). */
+int inu_adj_replayed(inu_ctx* ctx, unsigned long long* out2);
+/* Host only: the fewest segments the binomial schedule advances without tape
+ * to reverse n_segments segments with `slots` checkpoints (slot 0 holding the
+ * start); the backward pass also replays each segment once with tape. */
+int inu_adj_schedule_cost(uint32_t n_segments, uint32_t slots, unsigned long long* out);
 int inu_adj_end(inu_ctx* ctx);
 
 /* Edge-word VJP probe: per row in17 = {hL, huL, hvL, hR, huR, hvR, dzL, dzR,

@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -1857,6 +1858,12 @@ struct inu_ctx {
   struct {
     bool on = false;
     uint32_t max_steps = 0, seg = 1, every = 1, n_gauges = 0, max_samples = 0, n_ck = 0;
+    // 0: a checkpoint every segment, written by the recording; else the
+    // number of checkpoint slots of the binomial schedule (slot 0 holds the
+    // recording's start, the others are filled by the backward pass).
+    uint32_t slots = 0;
+    // steps the last backward replayed without tape, and with tape
+    unsigned long long replayed[2] = {0ull, 0ull};
     uint32_t* log = nullptr;
     uint8_t* codes = nullptr;  // per step and block: 1 listed, 2 idle source block
     uint32_t* gauges = nullptr;
@@ -1875,6 +1882,8 @@ struct inu_ctx {
   // Reverse pass: face depth below which a side's velocity and celerity are
   // held; negative: kAdjThinDry x dry_tol.
   float adj_thin = -1.0f;
+  // Checkpoint slots of the next recording's binomial schedule (0: uniform).
+  uint32_t adj_slots = 0;
   // Open boundaries: open edge k's inside cell (internal id), the stretches
   // (open edges [ptr[s], ptr[s + 1]) form stretch s) and their volume tally.
   uint32_t n_open = 0, n_stretch = 0;
@@ -1965,8 +1974,20 @@ void free_all(inu_ctx* c) {
   c->a = Args{};
   c->ctl = nullptr;
   c->scen = nullptr;
-  c->adj.on = false;
-  c->adj.events.clear();
+  // The recording's buffers went with the rest: forget their addresses, which
+  // the next allocations may reuse (releasing the recording later must not
+  // free those).
+  auto& d = c->adj;
+  d.on = false;
+  d.events.clear();
+  d.log = nullptr;
+  d.codes = nullptr;
+  d.gauges = nullptr;
+  d.series = nullptr;
+  d.ck = nullptr;
+  d.cur = nullptr;
+  d.tape = d.lam = d.l1 = d.gn = d.dseed = d.rseries = nullptr;
+  d.gdz = nullptr;
   c->n_scen = 0;
   c->has_mesh = false;
 }
@@ -3669,20 +3690,140 @@ static int adj_replay_range(inu_ctx* c, uint32_t ka, uint32_t kb, uint32_t k0, b
   return 0;
 }
 
-// Restores checkpoint cs (the state of step cs * seg) and the rates in effect then.
-static int adj_restore(inu_ctx* c, uint32_t cs) {
-  const auto& d = c->adj;
+// Restores the state of step k from the checkpoint at ck, and the rates in
+// effect then.
+static int adj_restore_from(inu_ctx* c, const float* ck, uint32_t k) {
   const size_t NP = (size_t)c->NB * kBlockCells;
-  const float* ck = d.ck + (size_t)cs * 4 * NP;
   const Args& a = c->a;
   HIPCHK(c, hipMemcpyAsync(a.h, ck, NP * 4, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemsetAsync(a.part_dry, 1, (size_t)c->NB * 4, c->stream));
   HIPCHK(c, hipMemcpyAsync(a.hu, ck + NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(a.hv, ck + 2 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(a.hlo, ck + 3 * NP, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  const float* rates = adj_rates_at(c, cs * d.seg);
+  const float* rates = adj_rates_at(c, k);
   if (rates) HIPCHK(c, hipMemcpyAsync(a.src, rates, NP * 4, hipMemcpyDeviceToDevice, c->stream));
   return 0;
+}
+
+// Restores checkpoint cs of the uniform scheme (the state of step cs * seg).
+static int adj_restore(inu_ctx* c, uint32_t cs) {
+  const size_t NP = (size_t)c->NB * kBlockCells;
+  return adj_restore_from(c, c->adj.ck + (size_t)cs * 4 * NP, cs * c->adj.seg);
+}
+
+// Writes the state in place to the checkpoint at ck.
+static int adj_store_to(inu_ctx* c, float* ck) {
+  const size_t NP = (size_t)c->NB * kBlockCells;
+  const Args& a = c->a;
+  HIPCHK(c, hipMemcpyAsync(ck, a.h, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ck + NP, a.hu, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ck + 2 * NP, a.hv, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ck + 3 * NP, a.hlo, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+// Largest segment count the binomial schedule takes (its table is built in
+// O(n^2 s) host operations).
+constexpr uint32_t kMaxScheduleSegments = 2048u, kMaxSlots = 64u;
+
+// The optimal offline checkpoint schedule for reversing n segments with the
+// start state held and `free` more checkpoint slots (Griewank's binomial
+// checkpointing, built by dynamic programming over the recursion it
+// solves). cost[l * (free + 1) + f]: the fewest segments advanced without
+// tape to reverse l segments with f free slots; split[...]: how many
+// segments to advance before storing the next checkpoint. Reversing one
+// segment costs no advance (its replay with tape is counted apart, once per
+// segment); with no free slot, segment i of l is reached by advancing i.
+struct Schedule {
+  uint32_t free = 0;
+  std::vector<unsigned long long> cost;
+  std::vector<uint32_t> split;
+  unsigned long long at(uint32_t l, uint32_t f) const { return cost[(size_t)l * (free + 1) + f]; }
+  uint32_t cut(uint32_t l, uint32_t f) const { return split[(size_t)l * (free + 1) + f]; }
+};
+
+static Schedule revolve_schedule(uint32_t n, uint32_t free) {
+  Schedule t;
+  t.free = free;
+  const size_t w = (size_t)free + 1;
+  t.cost.assign((size_t)(n + 1) * w, 0ull);
+  t.split.assign((size_t)(n + 1) * w, 0u);
+  for (uint32_t l = 2; l <= n; ++l) {
+    t.cost[l * w] = (unsigned long long)l * (l - 1) / 2;
+    for (uint32_t f = 1; f <= free; ++f) {
+      unsigned long long best = ~0ull;
+      uint32_t arg = 1;
+      for (uint32_t j = 1; j < l; ++j) {
+        const unsigned long long v = j + t.cost[(l - j) * w + f - 1] + t.cost[j * w + f];
+        if (v < best) {
+          best = v;
+          arg = j;
+        }
+      }
+      t.cost[l * w + f] = best;
+      t.split[l * w + f] = arg;
+    }
+  }
+  return t;
+}
+
+// Replays segment i (the state of its first step in place) writing the tape,
+// then reverses it.
+static int adj_turn(inu_ctx* c, uint32_t i, uint32_t K) {
+  auto& d = c->adj;
+  const uint32_t k0 = i * d.seg, k1 = std::min(K, k0 + d.seg);
+  int rc = adj_replay_range(c, k0, k1, k0, true, nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_adj_cursor, dim3(1), dim3(1), 0, c->stream, c->a, d.cur, k1 - 1, k0);
+  rc = run_graph(c, kKeyReverse, k1 - k0, [&](uint32_t m) { return adj_enqueue_reverse(c, m); });
+  d.replayed[1] += k1 - k0;
+  return rc;
+}
+
+// The backward pass by the binomial schedule: K recorded steps in segments of
+// seg steps, checkpoint slot 0 holding the start.
+static int adj_backward_binomial(inu_ctx* c, uint32_t K) {
+  auto& d = c->adj;
+  const size_t NP = (size_t)c->NB * kBlockCells;
+  const uint32_t n = (K + d.seg - 1) / d.seg;
+  const Schedule sch = revolve_schedule(n, std::min(d.n_ck - 1, n - 1));
+  std::vector<uint8_t> busy(d.n_ck, 0);
+  busy[0] = 1;
+  auto slot = [&](uint32_t t) { return d.ck + (size_t)t * 4 * NP; };
+  // The state of segment q, from the state of segment a in slot t, one
+  // segment per replay (so the replay graphs are those of one segment).
+  auto advance = [&](uint32_t t, uint32_t a, uint32_t q) -> int {
+    int rc = adj_restore_from(c, slot(t), a * d.seg);
+    for (uint32_t p = a; p < q && !rc; ++p) {
+      rc = adj_replay_range(c, p * d.seg, (p + 1) * d.seg, p * d.seg, false, nullptr);
+      d.replayed[0] += d.seg;
+      // A range applies the events strictly inside it: leave the rates in
+      // effect at the next segment's first step (an event there included),
+      // as a restore at that step would.
+      const float* rates = adj_rates_at(c, (p + 1) * d.seg);
+      if (!rc && rates) HIPCHK(c, hipMemcpyAsync(c->a.src, rates, NP * 4, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return rc;
+  };
+  // Reverses segments [a, b), the state of segment a in slot t, f slots free.
+  std::function<int(uint32_t, uint32_t, uint32_t, uint32_t)> rev = [&](uint32_t a, uint32_t b, uint32_t t,
+                                                                      uint32_t f) -> int {
+    int rc = 0;
+    if (b - a == 1 || f == 0) {
+      for (uint32_t i = b; i-- > a && !rc;)
+        if (!(rc = advance(t, a, i))) rc = adj_turn(c, i, K);
+      return rc;
+    }
+    const uint32_t m = a + sch.cut(b - a, f);
+    uint32_t u = 1;
+    while (busy[u]) ++u;
+    if ((rc = advance(t, a, m)) || (rc = adj_store_to(c, slot(u)))) return rc;
+    busy[u] = 1;
+    rc = rev(m, b, u, f - 1);
+    busy[u] = 0;
+    return rc ? rc : rev(a, m, t, f);
+  };
+  return rev(0, n, 0, sch.free);
 }
 
 }  // namespace
@@ -3709,6 +3850,9 @@ int inu_adj_begin(inu_ctx* c, uint32_t max_steps, uint32_t seg, uint32_t n_gauge
   std::vector<Scen> sc;
   if ((rc = scen_read(c, sc))) return rc;
   if (sc[0].ctrl.steps != 0ull) return fail(c, INU_ESTATE, "start the recording at step 0 (after inu_prologue)");
+  const uint32_t n_segments = (max_steps + std::min(seg, max_steps) - 1) / std::min(seg, max_steps);
+  if (c->adj_slots && n_segments > kMaxScheduleSegments)
+    return fail(c, INU_EINVAL, "inu_adj_begin: the binomial schedule takes at most 2048 segments (a longer seg)");
   adj_release(c);
   auto& d = c->adj;
   d.max_steps = max_steps;
@@ -3716,7 +3860,8 @@ int inu_adj_begin(inu_ctx* c, uint32_t max_steps, uint32_t seg, uint32_t n_gauge
   d.every = every;
   d.n_gauges = n_gauges;
   d.max_samples = max_steps / every;
-  d.n_ck = (max_steps + d.seg - 1) / d.seg + 1;
+  d.slots = c->adj_slots;
+  d.n_ck = d.slots ? std::min(d.slots, n_segments) : n_segments + 1;
   const size_t NP = (size_t)c->NB * kBlockCells;
   std::vector<uint32_t> gi(n_gauges);
   for (uint32_t k = 0; k < n_gauges; ++k) gi[k] = internal_cell(c, gauges[k]);
@@ -3739,8 +3884,8 @@ int inu_adj_begin(inu_ctx* c, uint32_t max_steps, uint32_t seg, uint32_t n_gauge
   HIPCHK(c, hipMemsetAsync(d.series, 0, (size_t)std::max(d.max_samples, 1u) * n_gauges * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(d.cur, 0, sizeof(AdjCur), c->stream));
   HIPCHK(c, hipMemcpyAsync(ev, c->a.src, NP * 4, hipMemcpyDeviceToDevice, c->stream));
-  const AdjRec r{d.log, d.codes, d.series, d.gauges, d.ck, d.max_steps, d.n_gauges, d.every, d.max_samples, d.seg,
-                 d.n_ck};
+  const AdjRec r{d.log,         d.codes, d.series,      d.gauges, d.ck, d.max_steps, d.n_gauges, d.every,
+                 d.max_samples, d.seg,   d.slots ? 1u : d.n_ck};
   hipLaunchKernelGGL(k_adj_record, dim3(std::min<uint32_t>(c->NB, 1024u)), dim3(256), 0, c->stream, c->a, r);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -3755,8 +3900,8 @@ int inu_adj_steps(inu_ctx* c, uint32_t n) {
   if (!d.on) return fail(c, INU_ESTATE, "no adjoint recording (inu_adj_begin)");
   if (n == 0) return 0;
   if (n > kKeyLenMask) return fail(c, INU_EINVAL, "inu_adj_steps: too many steps at once");
-  const AdjRec r{d.log, d.codes, d.series, d.gauges, d.ck, d.max_steps, d.n_gauges, d.every, d.max_samples, d.seg,
-                 d.n_ck};
+  const AdjRec r{d.log,         d.codes, d.series,      d.gauges, d.ck, d.max_steps, d.n_gauges, d.every,
+                 d.max_samples, d.seg,   d.slots ? 1u : d.n_ck};
   rc = run_graph(c, kKeyRecord, n, [&](uint32_t m) {
     for (uint32_t k = 0; k < m; ++k) {
       int r2 = launch_step(c, nullptr);
@@ -3834,14 +3979,13 @@ int inu_adj_backward(inu_ctx* c, const float* seeds, float* grad_n, float* grad_
   HIPCHK(c, hipMemsetAsync(d.cur, 0, sizeof(AdjCur), c->stream));
   HIPCHK(c, hipMemcpyAsync(d.dseed, seeds, (size_t)d.max_samples * d.n_gauges * 4, hipMemcpyHostToDevice,
                            c->stream));
-  const uint32_t n_seg = (K + d.seg - 1) / d.seg;
-  for (uint32_t cs = n_seg; cs-- > 0;) {
-    const uint32_t k0 = cs * d.seg, k1 = std::min(K, k0 + d.seg);
-    // Replay the segment from its checkpoint, writing the tape, then reverse it.
-    if ((rc = adj_restore(c, cs)) || (rc = adj_replay_range(c, k0, k1, k0, true, nullptr))) return rc;
-    hipLaunchKernelGGL(k_adj_cursor, dim3(1), dim3(1), 0, c->stream, c->a, d.cur, k1 - 1, k0);
-    rc = run_graph(c, kKeyReverse, k1 - k0, [&](uint32_t m) { return adj_enqueue_reverse(c, m); });
-    if (rc) return rc;
+  d.replayed[0] = d.replayed[1] = 0ull;
+  if (d.slots) {
+    if ((rc = adj_backward_binomial(c, K))) return rc;
+  } else {
+    // Each segment from its checkpoint: replayed writing the tape, then reversed.
+    for (uint32_t cs = (K + d.seg - 1) / d.seg; cs-- > 0;)
+      if ((rc = adj_restore(c, cs)) || (rc = adj_turn(c, cs, K))) return rc;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   AdjCur cur;
@@ -3903,6 +4047,30 @@ int inu_adj_stats(inu_ctx* c, unsigned long long* out3) {
   if (!c) return INU_EINVAL;
   if (!out3) return fail(c, INU_EINVAL, "null output");
   for (int q = 0; q < 3; ++q) out3[q] = c->adj.stats[q];
+  return 0;
+}
+
+int inu_adj_set_slots(inu_ctx* c, uint32_t slots) {
+  if (!c) return INU_EINVAL;
+  if (c->adj.on) return fail(c, INU_ESTATE, "inu_adj_set_slots: set before inu_adj_begin (a recording is open)");
+  if (slots > kMaxSlots) return fail(c, INU_EINVAL, "inu_adj_set_slots: at most 64 checkpoint slots");
+  c->adj_slots = slots;
+  return 0;
+}
+
+int inu_adj_replayed(inu_ctx* c, unsigned long long* out2) {
+  if (!c) return INU_EINVAL;
+  if (!out2) return fail(c, INU_EINVAL, "null output");
+  out2[0] = c->adj.replayed[0];
+  out2[1] = c->adj.replayed[1];
+  return 0;
+}
+
+int inu_adj_schedule_cost(uint32_t n_segments, uint32_t slots, unsigned long long* out) {
+  if (!out || n_segments == 0 || slots == 0 || slots > kMaxSlots || n_segments > kMaxScheduleSegments)
+    return INU_EINVAL;
+  *out = revolve_schedule(n_segments, std::min(slots - 1, n_segments - 1)).at(n_segments,
+                                                                             std::min(slots - 1, n_segments - 1));
   return 0;
 }
 

@@ -3,9 +3,10 @@
 ``gauge_depths(n, z, run)`` is a ``torch.autograd.Function``: its forward runs
 the HIP solver from ``run``'s initial state (CFL-driven steps, recording each
 step's dt, gauge depths every ``every`` steps and checkpoints every ``seg``
-steps); its backward runs the hand-written HIP adjoint (``inu_adj_backward``),
-which replays each segment from its checkpoint with the recorded dt sequence
-and source rates and runs it in reverse, following the forward's wet/dry and
+steps, or only its start under binomial checkpointing, ``slots`` > 0); its
+backward runs the hand-written HIP adjoint (``inu_adj_backward``), which
+replays each segment from a checkpoint with the recorded dt sequence and
+source rates and runs it in reverse, following the forward's wet/dry and
 HLLC branches. The reference has no gradient path; the forward is the same
 resident Heun step as ``hip_resident`` (reference
 ``inundation/vulkan/gpu_resident_batch.py``).
@@ -34,6 +35,9 @@ class GaugeRun:
     every: int
     n_steps: int
     seg: int = 100
+    # checkpoints: 0 keeps one every seg steps; s >= 1 uses the binomial
+    # schedule with s checkpoint slots (less memory, more replayed steps)
+    slots: int = 0
     g: float = 9.81
     dry_tol: float = 1e-4
     cfl: float = 0.8
@@ -105,6 +109,7 @@ class GaugeRun:
     def forward(self, n: NDArray[np.float32], z: NDArray[np.float32]) -> NDArray[np.float32]:
         """The recorded run: gauge depths (samples x gauges); the adjoint's tape is kept on the device."""
         cells = self._start(n, z)
+        self.ctx.adj_set_slots(self.slots)
         self.ctx.adj_begin(self.n_steps, self.seg, self.gauges, self.every)
         self._drive(cells, self.ctx.adj_steps)
         return self.ctx.adj_series()

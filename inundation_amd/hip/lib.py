@@ -82,6 +82,9 @@ EXPORTS = (
     "inu_adj_dt_log",
     "inu_adj_stats",
     "inu_adj_set_thin",
+    "inu_adj_set_slots",
+    "inu_adj_replayed",
+    "inu_adj_schedule_cost",
     "inu_set_open_stretches",
     "inu_set_open_conditions",
     "inu_open_volumes",
@@ -188,6 +191,9 @@ def load() -> C.CDLL:
         "inu_adj_dt_log": (_I, [_P, _P]),
         "inu_adj_stats": (_I, [_P, _P]),
         "inu_adj_set_thin": (_I, [_P, _F]),
+        "inu_adj_set_slots": (_I, [_P, _U32]),
+        "inu_adj_replayed": (_I, [_P, _P]),
+        "inu_adj_schedule_cost": (_I, [_U32, _U32, _P]),
         "inu_set_open_stretches": (_I, [_P, _U32, _P]),
         "inu_set_open_conditions": (_I, [_P, _U32, _P, _P]),
         "inu_open_volumes": (_I, [_P, _U32, _P]),
@@ -204,6 +210,15 @@ def load() -> C.CDLL:
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def schedule_cost(n_segments: int, slots: int) -> int:
+    """Segments the binomial checkpoint schedule advances without tape (host only; see inu_adj_schedule_cost)."""
+    out = C.c_ulonglong(0)
+    rc = load().inu_adj_schedule_cost(int(n_segments), int(slots), C.byref(out))
+    if rc != OK:
+        raise ValueError(f"schedule_cost: segments in [1, 2048] and slots in [1, 64] (code {rc})")
+    return int(out.value)
 
 
 def ptr(a: NDArray) -> int:
@@ -476,6 +491,16 @@ class Context:
     def adj_set_thin(self, depth: float) -> None:
         """Face depth below which the reverse pass holds a side's velocity and celerity."""
         self.call("inu_adj_set_thin", float(depth))
+
+    def adj_set_slots(self, slots: int) -> None:
+        """Checkpoint slots of the next recording: 0 uniform (every seg steps), else binomial with that many."""
+        self.call("inu_adj_set_slots", int(slots))
+
+    def adj_replayed(self) -> dict[str, int]:
+        """Steps the last backward pass replayed without tape and with tape."""
+        out = np.zeros(2, dtype=np.uint64)
+        self.call("inu_adj_replayed", ptr(out))
+        return {"advanced": int(out[0]), "taped": int(out[1])}
 
     def adj_end(self) -> None:
         self.call("inu_adj_end")

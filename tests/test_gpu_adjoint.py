@@ -210,3 +210,56 @@ def test_adjoint_matches_fp64_central_differences_all_wet(ctx) -> None:  # noqa:
             worst = max(worst, abs(grad[i] - fd) / abs(fd))
     print(f"all-wet valley, worst GPU adjoint vs FP64 central difference: {worst:.2e}")
     assert worst < 1e-3, worst
+
+
+@pytest.mark.parametrize("slots", [1, 2, 3, 8])
+def test_binomial_checkpointing_gives_the_uniform_gradient_bitwise(ctx, slots: int) -> None:  # noqa: ANN001
+    """Binomial checkpointing replays the same steps from checkpoints it stores itself: the same bits.
+
+    Wet/dry valley, 60 steps in segments of 8 (the last one partial), source
+    rates changing at step 16 (a segment boundary) and step 23 (inside a
+    segment). The backward replays every step once with tape and advances the
+    schedule's count of segments without it.
+    """
+    g, m, z, n, state0, gauges, run, src = _run_case(ctx, wet=False, n_steps=60, seg=8)
+    rates = run.phases[0][1]
+    run.phases = [(0, rates), (16, 2.0 * rates), (23, 0.5 * rates)]
+    run.graph_steps = 8
+    series = run.forward(n, z)
+    seeds = (2.0 * (series - 0.9 * series)).astype(np.float32)
+    gn_u, gz_u = run.backward(seeds)
+    assert ctx.adj_replayed() == {"advanced": 0, "taped": 60}
+    run.slots = slots
+    try:
+        series_b = run.forward(n, z)
+        assert np.array_equal(series_b.view(np.uint32), series.view(np.uint32))
+        gn_b, gz_b = run.backward(seeds)
+    finally:
+        run.slots = 0
+    assert np.array_equal(gn_b.view(np.uint32), gn_u.view(np.uint32))
+    assert np.array_equal(gz_b.view(np.uint32), gz_u.view(np.uint32))
+    assert np.abs(gn_u).max() > 0.0 and np.abs(gz_u).max() > 0.0
+    # 8 segments; every segment advanced without tape is a whole one.
+    assert ctx.adj_replayed() == {"advanced": 8 * L.schedule_cost(8, slots), "taped": 60}
+
+
+def test_a_new_mesh_after_a_recording_keeps_its_buffers(ctx) -> None:  # noqa: ANN001
+    """Setting a mesh frees the recording with everything else; ending the recording later frees nothing live.
+
+    The new mesh's allocations may reuse the old recording's addresses. Three
+    rounds of record, differentiate, set the mesh again and end the
+    recording; each following run must give the first run's gauge series
+    bit for bit.
+    """
+    first = None
+    for _ in range(3):
+        g, m, z, n, state0, gauges, run, src = _run_case(ctx, wet=True, n_steps=20, seg=8)
+        series = run.forward(n, z)
+        run.backward(np.ones_like(series))
+        if first is None:
+            first = series
+        assert np.array_equal(series.view(np.uint32), first.view(np.uint32))
+        _set = device_mesh_arrays(g)
+        ctx.set_mesh(_set["cell_l"], _set["cell_r"], _set["nx"], _set["ny"], _set["len"], _set["dz2"],
+                     _set["area"], _set["cptr"], _set["cidx"])
+        ctx.adj_end()

@@ -301,23 +301,45 @@ def test_c5_calibration_gradient_matches_fp64_and_replays_bitwise(say) -> None: 
 def _drive_phases(be, parts, n_phases: int, multi: bool) -> None:  # noqa: ANN001
     """One-minute phases on a context or oracle holding the scenarios ``parts`` back to back.
 
-    ``parts`` is [(scenario, geometry)]; each phase sets every scenario's
-    sources for its mid-time, moves each stop one minute on and steps in
-    blocks of 50 until every scenario is there.
+    ``parts`` is [(scenario, geometry)]; steps go in blocks of 50. A
+    scenario that reaches the end of its phase moves on at once, as the
+    bench's batched run does: every scenario's sources for its own phase's
+    mid-time, its stop one minute on, and a retarget, which leaves the
+    scenarios still inside a phase where they were. Alone, this is the
+    plain phase loop.
     """
     base = np.cumsum([0] + [g.N for _, g in parts])
-    for j in range(n_phases):
-        t_mid = (j + 0.5) * 60.0
-        pairs = [s.source_pairs(g, t_mid) for s, g in parts]
+    phase = [0] * len(parts)
+
+    def set_sources() -> None:
+        pairs = [s.source_pairs(g, (phase[k] + 0.5) * 60.0) for k, (s, g) in enumerate(parts)]
         be.set_sources(np.concatenate([c + np.uint32(base[k]) for k, (c, _) in enumerate(pairs)]).astype(np.uint32),
                        np.concatenate([r for _, r in pairs]).astype(np.float32))
-        cs = be.ctrls_read() if multi else [be.ctrl_read()]
-        for c in cs:
-            c.stop_ticks = (j + 1) * 60 * 2**20
+
+    def ctrls() -> list:  # noqa: ANN202
+        return be.ctrls_read() if multi else [be.ctrl_read()]
+
+    set_sources()
+    cs = ctrls()
+    for c in cs:
+        c.stop_ticks = 60 * 2**20
+    be.ctrl_write(cs if multi else cs[0])
+    be.prologue()
+    while True:
+        be.steps(50, use_graph=True)
+        cs = ctrls()
+        at_stop = [int(c.now_ticks) >= int(c.stop_ticks) for c in cs]
+        move = [k for k in range(len(parts)) if at_stop[k] and phase[k] + 1 < n_phases]
+        if not move:
+            if all(at_stop):
+                return
+            continue
+        for k in move:
+            phase[k] += 1
+            cs[k].stop_ticks = (phase[k] + 1) * 60 * 2**20
+        set_sources()
         be.ctrl_write(cs if multi else cs[0])
-        be.retarget() if j else be.prologue()
-        while any(int(c.now_ticks) < int(c.stop_ticks) for c in (be.ctrls_read() if multi else [be.ctrl_read()])):
-            be.steps(50, use_graph=True)
+        be.retarget()
 
 
 def test_c4_batched_scenarios_on_the_five_meshes_are_each_alone_and_the_oracle(say) -> None:  # noqa: ANN001
