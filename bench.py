@@ -96,6 +96,8 @@ def main() -> int:
     )
     ap.add_argument("--cells", type=int, default=250_000, help="adjoint: Voronoi points")
     ap.add_argument("--fd-cells", type=int, default=20, help="adjoint: cells checked by central differences")
+    ap.add_argument("--adj-thin", type=float, default=None,
+                    help="adjoint: face depth (m) below which the reverse pass holds velocities (default 100 dry_tol)")
     ap.add_argument("--scenarios", type=int, default=64, help="batched: number of scenarios")
     ap.add_argument("--t-end", type=float, default=3600.0, help="batched: simulated seconds per scenario")
     ap.add_argument("--sizes", type=str, default="50000,200000,500000,1000000,2000000",
@@ -431,11 +433,12 @@ def run_batched(sizes: tuple[int, ...], n_scenarios: int, t_end: float, processe
     """Scenarios on five Voronoi meshes: batched in one context against one by one (the plain solver).
 
     Both runs take each scenario from a dry start through one-minute phases
-    (the inflow hydrograph changes per phase, the rain is constant), replaying
-    50-step hipGraphs until every scenario reaches the phase's end. The wall
-    clock covers the phase loops only; cell-steps count each scenario's cells
-    times its real steps. Each batched scenario is bit-identical to it alone,
-    so both runs take the same steps.
+    (the inflow hydrograph changes per phase, the rain is constant),
+    replaying 50-step hipGraphs; a scenario moves to its next phase as soon
+    as it reaches the end of one (``_phase_loop``). The wall clock covers the
+    phase loops only; cell-steps count each scenario's cells times its real
+    steps. Each batched scenario is bit-identical to it alone, so both runs
+    take the same steps.
 
     The bound on the speed-up: the algorithmic bytes of all the visited
     blocks' steps (the device's activity counters), moved at ``fill_Bps``,
@@ -580,7 +583,7 @@ def run_batched(sizes: tuple[int, ...], n_scenarios: int, t_end: float, processe
     return out
 
 
-def run_adjoint(cells: int, steps: int, fd_cells: int, cpu_seconds: float | None) -> dict:
+def run_adjoint(cells: int, steps: int, fd_cells: int, cpu_seconds: float | None, thin: float | None = None) -> dict:
     """Gauge-misfit gradients w.r.t. Manning n and bed z: the adjoint against the plain forward run.
 
     A truth run with the smooth 'true' n field picks 32 gauges among the cells
@@ -609,6 +612,8 @@ def run_adjoint(cells: int, steps: int, fd_cells: int, cpu_seconds: float | None
     run = GaugeRun(ctx=ctx, state0=(zero, zero, zero), gauges=np.array([cal.inflow_cell], np.uint32),
                    every=cal.every, n_steps=cal.steps, seg=cal.seg, dry_tol=1e-4, cfl=0.8, dt_max=1.0, dt_init=0.1,
                    source_cells=cal.source_cells(), rates_at=cal.rates_at, graph_steps=50)
+    if thin is not None:
+        ctx.adj_set_thin(thin)
     z = geom.zb.astype(np.float32)
     # Truth run: gauges among the cells it wets, then their observed depths.
     run.forward_plain(cal.n_true, z)
@@ -766,7 +771,7 @@ def run_adjoint(cells: int, steps: int, fd_cells: int, cpu_seconds: float | None
             "active_block_fraction_at_end": active_frac,
             "fd_eps": [eps, eps2],
             "fd": fd,
-            "thin_layer_depth_m": 100.0 * run.dry_tol,
+            "thin_layer_depth_m": 100.0 * run.dry_tol if thin is None else thin,
             "reverse_stats": stats,
             "wet_mesh_cost": wet,
         },
@@ -892,7 +897,8 @@ def bench_batched(args) -> int:  # noqa: ANN001
 
 
 def bench_adjoint(args) -> int:  # noqa: ANN001
-    print(json.dumps(run_adjoint(args.cells, args.steps, args.fd_cells, None if args.no_cpu else args.cpu_seconds)))
+    print(json.dumps(run_adjoint(args.cells, args.steps, args.fd_cells, None if args.no_cpu else args.cpu_seconds,
+                                 args.adj_thin)))
     return 0
 
 
