@@ -1076,10 +1076,46 @@ def cpu_baseline(ctx, wl, m, info, budget_s: float) -> dict:  # noqa: ANN001
         "value": k * n / dt,
         "unit": "cell-steps/s",
         "cores": cores,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": f"{k} Heun steps of the same {n}-cell mesh from the GPU state after the timed run "
         f"(FP32 C++/OpenMP oracle; bitwise equal to the GPU on those steps)",
+        "fp64_oracle_c1": fp64_c1_rate(),
     }
+
+
+def cpu_model() -> str:
+    """The host CPU's model name."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def fp64_c1_rate(budget_s: float = 5.0) -> dict:
+    """The FP64 NumPy oracle on the 4096-cell lake at rest, one process, for about budget_s seconds."""
+    from inundation_amd.hip.solver import device_mesh_arrays
+    from inundation_amd.workloads import lake_at_rest
+    from oracle import ref64
+
+    wl = lake_at_rest()
+    m64 = ref64.Mesh64.from_device_arrays(device_mesh_arrays(wl.geom))
+    n = wl.geom.N
+    s = ref64.State64(wl.h0.astype(np.float64), np.zeros(n), np.zeros(n))
+    nm = wl.manning.astype(np.float64)
+    k = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        s, _ = ref64.heun_step(m64, s, nm, 0.05, 9.81, wl.dry_tol)
+        k += 1
+    dt = time.perf_counter() - t0
+    log(f"fp64 oracle on the lake at rest: {k} steps in {dt:.2f}s, one process")
+    return {"value": k * n / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{k} FP64 NumPy Heun steps (dt 0.05 s) of the {n}-cell lake at rest, one process"}
 
 
 if __name__ == "__main__":
