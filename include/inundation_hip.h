@@ -219,10 +219,18 @@ int inu_upload(inu_ctx* ctx, int which, const void* in);
 
 /* Arithmetic probe on n (a, b) pairs: out7 per pair = {two-sum error of a + b,
  * a / b, sqrt|a|, a^(4/3), film drag of depth a over dt b, a^(-1/3), film drag
- * in the fast arithmetic}. Generalises the EFT probe (reference
+ * in the fast arithmetic (0 outside its domain a >= dry_tol, b >= 0)}. Generalises the EFT probe (reference
  * EFT_PROBE_GLSL, resident.py:889-907, and _check_error_free_arithmetic,
  * solver.py:449-467). */
 int inu_probe(inu_ctx* ctx, uint32_t n, const float* pairs, float* out7, float g, float dry_tol);
+
+/* Test probe of the fast arithmetic's reciprocal: every float bit pattern u
+ * in [lo, hi) and its negative, compared bitwise with the IEEE division 1 / x
+ * (NaNs equal); *mismatches gets the count that differ and *first the smallest
+ * such pattern (0xffffffff if none). hi <= 0x80000000. No reference
+ * counterpart: it pins the claim that the reciprocal is correctly rounded on
+ * [2^-126, 2^126] (lo = 0x00800000, hi = 0x7e800001). */
+int inu_probe_rcp(inu_ctx* ctx, uint32_t lo, uint32_t hi, uint64_t* mismatches, uint32_t* first);
 
 /* Test probe of the open boundary formulas (the reference tests them the same
  * way: tests/unit/test_gpu_open_ghost.py:46-68 runs open_ghost and

@@ -231,15 +231,15 @@ __device__ __forceinline__ void stg(T* base, uint32_t idx, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (idx * (uint32_t)sizeof(T))) = v;
 }
 
-// LDS of one block. Each cell and ghost cell has one 32-byte record: stage
-// input {h, hu, hv, lo} and, for the fast order, {vel_factor(h),
-// sqrt(max(h, 0)), g h^2 / 2, unused}. Each local edge has one 32-byte
-// record: its words {w0, w1, w2, w3}, {w4, w5, sides word, length}. Then the
-// block's CSR slots (local edge numbers).
+// LDS of one block. Each cell and ghost cell has a record of its stage input
+// (h, hu, hv, lo) in the exact order, or (h, u, v, lo) with the
+// desingularised velocity and, beside it, (phi(h), sqrt h) in the fast order.
+// Each local edge has one 32-byte record: its words {w0, w1, w2, w3}, {w4, w5,
+// sides word, length}. Then the block's CSR slots (local edge numbers).
 struct Lds {
   float4* cell;  // the cells, then the ghosts: [kBlockCells + max_ghost][2] in
-                 // carve(); the stage kernels' (h, hu, hv, lo) only
-  float2* cx;    // the stage kernels' fast-order (vel_factor, sqrt h) per cell
+                 // carve() (prologue); the stage kernels' first record only
+  float2* cx;    // the stage kernels' fast-order (phi(h), sqrt h) per cell
   float4* edge;  // [max_local][2]; the predictor's: [max_local] words 0-3
   float2* e2;    // the predictor's [max_local] words 4-5 (carve_stage<0>)
   uint16_t* s_slot;
@@ -273,8 +273,8 @@ __device__ __forceinline__ Lds carve(float* sm, uint32_t max_local, uint32_t max
   return l;
 }
 
-// The stage kernels keep 24 bytes per cell: (h, hu, hv, lo) and, for the fast
-// order, (vel_factor, sqrt h) in a second array; the pressure half_g h^2 is
+// The stage kernels keep 24 bytes per cell: (h, u, v, lo) and, for the fast
+// order, (phi(h), sqrt h) in a second array; the pressure half_g h^2 is
 // formed again where an edge needs it. Cell count rounded up to even, so the
 // edge records after them stay 16-byte aligned.
 __host__ __device__ inline uint32_t stage_cells(uint32_t max_ghost) { return (kBlockCells + max_ghost + 1u) & ~1u; }
@@ -363,48 +363,59 @@ __device__ __forceinline__ void rec_ghost(const Args& a, uint32_t g, float4 s) {
   t[2 * (size_t)a.N] = s.z;
 }
 
-// Stage input of one cell or ghost into LDS record q, with the fast order's
-// per-depth vel_factor and square root.
+// Stage input s = (h, hu, hv, lo) of one cell or ghost into LDS record q. The
+// exact order keeps it as is; the fast order keeps (h, u, v, lo) with the
+// desingularised velocity, and (phi(h), sqrt h) beside it.
 template <bool FM>
 __device__ __forceinline__ void put_cell(const Lds& l, const Phys& p, uint32_t q, float4 s) {
-  l.cell[q] = s;
-  if (FM) l.cx[q] = make_float2(inu::fast::vel_factor(p, s.x), inu::sqrt_rn(inu::gmax(s.x, 0.0f)));
+  if (FM) {
+    const inu::fast::Cell c = inu::fast::cell(p, s.x, s.y, s.z, s.w);
+    l.cell[q] = make_float4(c.h, c.u, c.v, c.lo);
+    l.cx[q] = make_float2(c.phi, c.sq);
+  } else {
+    l.cell[q] = s;
+  }
 }
 
-// The words of an open edge with right side (hR, huR, hvR) beyond it; `wall`
-// passes no mass; fh receives the HLLC mass flux. The face depths are the
-// cell depths (no bed step).
+// The fast-order side of LDS record q.
+__device__ __forceinline__ inu::fast::Cell get_cell(const Lds& l, uint32_t q) {
+  const float4 s = l.cell[q];
+  const float2 x = l.cx[q];
+  return inu::fast::Cell{s.x, s.y, s.z, s.w, x.x, x.y};
+}
+
+// The words of an open edge from its inside cell sl = (h, hu, hv) with the
+// state (hR, huR, hvR) beyond it; `wall` passes no mass; fh receives the HLLC
+// mass flux. The face depths are the cell depths (no bed step).
 template <bool FM>
-__device__ __forceinline__ void open_words(const Phys& p, float4 sl, float4 xl, float hR, float huR, float hvR,
-                                           bool wall, const EdgeRec& r, float w[6], float& fh) {
+__device__ __forceinline__ void open_words(const Phys& p, float4 sl, float hR, float huR, float hvR, bool wall,
+                                           const EdgeRec& r, float w[6], float& fh) {
   if (FM) {
-    const inu::fast::Side L{sl.x, sl.y, sl.z, 0.0f, xl.x, xl.y, xl.z};
-    const inu::fast::Side R{hR, huR, hvR, 0.0f, inu::fast::vel_factor(p, hR), inu::sqrt_rn(inu::gmax(hR, 0.0f)),
-                            p.half_g * hR * hR};
-    inu::fast::edge_words(p, wall, L, R, 0.0f, 0.0f, r.nx, r.ny, r.len, w, &fh);
+    const inu::fast::Cell L = inu::fast::cell(p, sl.x, sl.y, sl.z, 0.0f);
+    const inu::fast::Cell R = inu::fast::cell(p, hR, huR, hvR, 0.0f);
+    inu::fast::edge_words(p, wall, L, R, false, 0.0f, r.nx, r.ny, r.len, w, &fh);
   } else {
     inu::edge_words(p, wall, sl.x, sl.y, sl.z, hR, huR, hvR, 0.0f, 0.0f, r.nx, r.ny, r.len, w, &fh);
   }
 }
 
 // Words of an open edge (reference resident.py:130-142 and flux.py:271-296)
-// from its inside cell's stage input (sl, and xl = its vel_factor, square root
-// and pressure in the fast order): the ghost state of its condition; a dry
+// from its inside cell's stage input sl = (h, hu, hv): the ghost state of its
+// condition; a dry
 // edge passes nothing unless an inflow keeps it live; an outflow-only edge
 // whose flux would draw water in is redone against the wall ghost; otherwise
 // the mass word is the condition's boundary flux.
 template <bool FM>
-__device__ void open_edge_words(const Phys& p, uint32_t kind, float value, float4 sl, float4 xl, const EdgeRec& r,
-                                float w[6]) {
+__device__ void open_edge_words(const Phys& p, uint32_t kind, float value, float4 sl, const EdgeRec& r, float w[6]) {
   float hG, huG, hvG;
   inu::ghost_state(p, kind, value, sl.x, sl.y, sl.z, r.nx, r.ny, hG, huG, hvG);
   for (int q = 0; q < 6; ++q) w[q] = 0.0f;
   if (!(sl.x < p.dry && hG < p.dry) || inu::open_forced(kind, value)) {
     float fh;
-    open_words<FM>(p, sl, xl, hG, huG, hvG, false, r, w, fh);
+    open_words<FM>(p, sl, hG, huG, hvG, false, r, w, fh);
     if (inu::open_outflow_only(kind) && fh < 0.0f) {
       inu::ghost_state(p, inu::kOpenWall, 0.0f, sl.x, sl.y, sl.z, r.nx, r.ny, hG, huG, hvG);
-      open_words<FM>(p, sl, xl, hG, huG, hvG, true, r, w, fh);
+      open_words<FM>(p, sl, hG, huG, hvG, true, r, w, fh);
     } else {
       w[0] = inu::boundary_mass(kind, value, fh) * r.len;
     }
@@ -423,8 +434,6 @@ __global__ void k_open_words(const Args a, const uint32_t* open_cell, const floa
   const uint32_t i = open_cell[k];
   const float h = (ST == 0) ? a.h[i] : a.h1[i];
   const float4 sl = make_float4(h, (ST == 0) ? a.hu[i] : a.hu1[i], (ST == 0) ? a.hv[i] : a.hv1[i], 0.0f);
-  const float4 xl = FM ? make_float4(inu::fast::vel_factor(p, h), inu::sqrt_rn(inu::gmax(h, 0.0f)), p.half_g * h * h, 0.0f)
-                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const float4 g = open_geom[k];
   EdgeRec r;
   r.side = 0u;
@@ -433,7 +442,7 @@ __global__ void k_open_words(const Args a, const uint32_t* open_cell, const floa
   r.len = g.z;
   r.d = 0.0f;
   float w[6];
-  open_edge_words<FM>(p, a.open->kind[k], a.open->value[k], sl, xl, r, w);
+  open_edge_words<FM>(p, a.open->kind[k], a.open->value[k], sl, r, w);
   float4* o = a.open->words + 2 * ((size_t)ST * a.n_open + k);
   o[0] = make_float4(w[0], w[1], w[2], w[3]);
   o[1] = make_float4(w[4], w[5], 0.0f, 0.0f);
@@ -463,30 +472,41 @@ __device__ __forceinline__ void edge_words_lds(const Args& a, const Lds& l, cons
   const uint32_t ql = r.side >> 16;
   const uint32_t qr = r.side & 0xffffu;
   const bool wall = qr == kSideWall;
+  for (int q = 0; q < 6; ++q) w[q] = 0.0f;
+  if (FM) {
+    // A wall's right side is the left cell's mirror.
+    const inu::fast::Cell L = get_cell(l, ql);
+    const inu::fast::Cell R = wall ? L : get_cell(l, qr);
+    if (!(L.h < a.p.dry && R.h < a.p.dry)) {
+      const inu::fast::Cell Lo{L.h, L.u, L.v, LO ? L.lo : 0.0f, L.phi, L.sq};
+      const inu::fast::Cell Ro{R.h, R.u, R.v, LO ? R.lo : 0.0f, R.phi, R.sq};
+      inu::fast::edge_words(a.p, wall, Lo, Ro, wall, r.d, r.nx, r.ny, r.len, w);
+    }
+    return;
+  }
   const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
   const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
-  for (int q = 0; q < 6; ++q) w[q] = 0.0f;
   const float4 sl = l.cell[ql];
   float4 sr;
   if (wall) {
     sr = sl;
-    if (FM) inu::fast::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
-    else inu::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
+    inu::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
   } else {
     sr = l.cell[qr];
   }
-  if (!(sl.x < a.p.dry && sr.x < a.p.dry)) {
-    if (FM) {
-      const float2 xl = l.cx[ql];
-      const float2 xr = wall ? xl : l.cx[qr];
-      const float pl = a.p.half_g * sl.x * sl.x;
-      const float pr = wall ? pl : a.p.half_g * sr.x * sr.x;
-      const inu::fast::Side L{sl.x, sl.y, sl.z, LO ? sl.w : 0.0f, xl.x, xl.y, pl};
-      const inu::fast::Side R{sr.x, sr.y, sr.z, LO ? sr.w : 0.0f, xr.x, xr.y, pr};
-      inu::fast::edge_words(a.p, wall, L, R, dzL, dzR, r.nx, r.ny, r.len, w);
-    } else {
-      inu::edge_words(a.p, wall, sl.x, sl.y, sl.z, sr.x, sr.y, sr.z, dzL, dzR, r.nx, r.ny, r.len, w);
-    }
+  if (!(sl.x < a.p.dry && sr.x < a.p.dry))
+    inu::edge_words(a.p, wall, sl.x, sl.y, sl.z, sr.x, sr.y, sr.z, dzL, dzR, r.nx, r.ny, r.len, w);
+}
+
+// Speed |u| and celerity c of a wetted source cell of depth h for its dt bound
+// (reference resident.py:709-728).
+template <bool FM>
+__device__ __forceinline__ void src_uc(const Phys& p, float h, float hu, float hv, float& u, float& c) {
+  if (FM) {
+    inu::fast::src_speeds(p, h, hu, hv, u, c);
+  } else {
+    u = inu::sqrt_rn(hu * hu + hv * hv) / h;
+    c = inu::sqrt_rn(p.g * h);
   }
 }
 
@@ -800,8 +820,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       // is the source fix's, as below. The perimeter was summed beside the
       // edge words above.
       if (tid < nc && rate != 0.0f) {
-        const float u = inu::sqrt_rn(fmaf(hui, hui, hvi * hvi)) / hi;
-        const float c = p.sqrt_g * inu::sqrt_rn(hi);
+        float u, c;
+        inu::fast::src_speeds(p, hi, hui, hvi, u, c);
         const float lam = (u + c) * perim;
         if (lam > 1e-30f) lam_b = inu::gmax(lam_b, lam);
       }
@@ -813,8 +833,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       if (tid < nc) l.cell[tid].x = hi;
       __syncthreads();
       if (tid < nc && rate != 0.0f && hi >= p.dry) {
-        const float u = FM ? inu::sqrt_rn(fmaf(hui, hui, hvi * hvi)) / hi : inu::sqrt_rn(hui * hui + hvi * hvi) / hi;
-        const float c = FM ? p.sqrt_g * inu::sqrt_rn(hi) : inu::sqrt_rn(p.g * hi);
+        float u, c;
+        src_uc<FM>(p, hi, hui, hvi, u, c);
         float lam = 0.0f;
         for (uint32_t k = k0; k < k1; ++k) {
           const uint32_t s = l.s_slot[k];
@@ -949,8 +969,8 @@ __global__ __launch_bounds__(256) void k_source_idle(const Args a) {
     if (wet) {
       const float hui = a.hu[i];
       const float hvi = a.hv[i];
-      const float u = FM ? inu::sqrt_rn(fmaf(hui, hui, hvi * hvi)) / hn : inu::sqrt_rn(hui * hui + hvi * hvi) / hn;
-      const float c = FM ? p.sqrt_g * inu::sqrt_rn(hn) : inu::sqrt_rn(p.g * hn);
+      float u, c;
+      src_uc<FM>(p, hn, hui, hvi, u, c);
       float lam = 0.0f, perim = 0.0f;
       const uint32_t s1 = a.cptr[i + 1];
       for (uint32_t s = a.cptr[i]; s < s1; ++s) {
@@ -993,8 +1013,8 @@ __device__ __forceinline__ void fix_pair(const Args& a, uint32_t i) {
   if (!(hn >= p.dry) || a.src[i] == 0.0f) return;  // only wetted cells whose source is on
   const float hui = a.hu[i];
   const float hvi = a.hv[i];
-  const float u = FM ? inu::sqrt_rn(fmaf(hui, hui, hvi * hvi)) / hn : inu::sqrt_rn(hui * hui + hvi * hvi) / hn;
-  const float c = FM ? p.sqrt_g * inu::sqrt_rn(hn) : inu::sqrt_rn(p.g * hn);
+  float u, c;
+  src_uc<FM>(p, hn, hui, hvi, u, c);
   float lam = 0.0f;
   const uint32_t s1 = a.cptr[i + 1];
   for (uint32_t s = a.cptr[i]; s < s1; ++s) {
@@ -1263,22 +1283,21 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
       const EdgeRec r = load_edge(a, bi, j);
       const uint32_t ql = r.side >> 16;
       const uint32_t qr = r.side & 0xffffu;
+      const bool wall = qr == kSideWall;
       const float4 sl = l.cell[2 * ql];
-      float4 sr;
-      const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
-      const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
-      if (qr == kSideWall) {
-        sr = sl;
-        if (FM) inu::fast::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
-        else inu::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
-      } else {
-        sr = l.cell[2 * qr];
-      }
+      float4 sr = wall ? sl : l.cell[2 * qr];
       float contrib = 0.0f;
       if (sl.x >= p.dry || sr.x >= p.dry) {
-        contrib = (FM ? inu::fast::wave_bound(p, sl.x, sl.y, sl.z, dzL, sr.x, sr.y, sr.z, dzR, r.nx, r.ny, sl.w, sr.w)
-                      : inu::wave_bound(p, sl.x, sl.y, sl.z, dzL, sr.x, sr.y, sr.z, dzR, r.nx, r.ny)) *
-                  r.len;
+        if (FM) {
+          const inu::fast::Cell L = inu::fast::cell(p, sl.x, sl.y, sl.z, sl.w);
+          const inu::fast::Cell R = wall ? L : inu::fast::cell(p, sr.x, sr.y, sr.z, sr.w);
+          contrib = inu::fast::wave_bound(p, L, R, wall, r.d, r.nx, r.ny) * r.len;
+        } else {
+          const float dzL = (r.d > 0.0f) ? r.d : 0.0f;
+          const float dzR = (r.d < 0.0f) ? -r.d : 0.0f;
+          if (wall) inu::wall_ghost(sl.y, sl.z, r.nx, r.ny, sr.y, sr.z);
+          contrib = inu::wave_bound(p, sl.x, sl.y, sl.z, dzL, sr.x, sr.y, sr.z, dzR, r.nx, r.ny) * r.len;
+        }
       }
       l.edge[2 * j].x = contrib;
     }
@@ -1294,7 +1313,8 @@ __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
         inu::ghost_state(p, a.open->kind[jk.y], a.open->value[jk.y], sl.x, sl.y, sl.z, r.nx, r.ny, sr.x, sr.y, sr.z);
         float contrib = 0.0f;
         if (sl.x >= p.dry || sr.x >= p.dry) {
-          contrib = (FM ? inu::fast::wave_bound(p, sl.x, sl.y, sl.z, 0.0f, sr.x, sr.y, sr.z, 0.0f, r.nx, r.ny, sl.w, sl.w)
+          contrib = (FM ? inu::fast::wave_bound(p, inu::fast::cell(p, sl.x, sl.y, sl.z, sl.w),
+                                                inu::fast::cell(p, sr.x, sr.y, sr.z, sl.w), false, 0.0f, r.nx, r.ny)
                         : inu::wave_bound(p, sl.x, sl.y, sl.z, 0.0f, sr.x, sr.y, sr.z, 0.0f, r.nx, r.ny)) *
                     r.len;
         }
@@ -1794,7 +1814,27 @@ __global__ void k_probe(const float* in, float* out, uint32_t n, Phys p) {
   o[3] = (x > 0.0f) ? inu::pow43(x) : 0.0f;
   o[4] = (x > 0.0f) ? inu::film_drag(p, x, y) : 0.0f;
   o[5] = (x > 0.0f) ? inu::fast::rcbrt(x) : 0.0f;
-  o[6] = (x > 0.0f) ? inu::fast::film_drag(p, x, y) : 0.0f;
+  o[6] = (x >= p.dry && y >= 0.0f) ? inu::fast::film_drag(p, x, y) : 0.0f;  // its domain
+}
+
+// Reciprocal probe: every float bit pattern u in [lo, hi), and its negative,
+// compared bitwise between the fast order's rcp and the IEEE division 1 / x
+// (two NaNs count as equal). counts[0] gets the number that differ, first[0]
+// the smallest such pattern.
+__global__ void k_probe_rcp(uint32_t lo, uint32_t hi, unsigned long long* counts, uint32_t* first) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < (uint64_t)(hi - lo);
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint32_t sign = 0; sign < 2; ++sign) {
+      const uint32_t u = (lo + (uint32_t)k) | (sign << 31);
+      const float x = __uint_as_float(u);
+      const float f = inu::fast::rcp(x);
+      const float q = 1.0f / x;
+      if (__float_as_uint(f) != __float_as_uint(q) && !(f != f && q != q)) {
+        atomicAdd(counts, 1ull);
+        atomicMin(first, u);
+      }
+    }
+  }
 }
 
 // Open boundary probe: cases (h, hu, hv, nx, ny, fh) with a condition and
@@ -2771,6 +2811,10 @@ int inu_set_params(inu_ctx* c, float g, float dry_tol, float cfl, uint32_t flags
     return fail(c, INU_EINVAL, "g, dry_tol and cfl must be finite and positive");
   if (!(flags & INU_FLAG_DETERMINISTIC) && c->n_open)
     return fail(c, INU_EINVAL, "open boundaries need deterministic accumulation");
+  // The fast order's reciprocals take thin-film divisors h^2 + (4 dry_tol)^2,
+  // exact while they lie in [2^-126, 2^126].
+  if (!(flags & INU_FLAG_EXACT_ORDER) && !(dry_tol >= 1e-18f && dry_tol <= 1e18f))
+    return fail(c, INU_EINVAL, "dry_tol must lie in [1e-18, 1e18] (the exact order takes any positive value)");
   if ((flags ^ c->flags) & ~INU_FLAG_TRACK_CLAMP) {
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
     c->graphs.clear();
@@ -3140,6 +3184,38 @@ int inu_probe(inu_ctx* c, uint32_t n, const float* pairs, float* out7, float g, 
   (void)hipFree(din);
   (void)hipFree(dout);
   if (e != hipSuccess) return fail(c, INU_EHIP, std::string("probe: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int inu_probe_rcp(inu_ctx* c, uint32_t lo, uint32_t hi, uint64_t* mismatches, uint32_t* first) {
+  if (!c) return INU_EINVAL;
+  if (!mismatches || !first) return fail(c, INU_EINVAL, "inu_probe_rcp: null output");
+  if (lo > hi || hi > 0x80000000u) return fail(c, INU_EINVAL, "inu_probe_rcp: need lo <= hi <= 0x80000000");
+  HIPCHK(c, hipSetDevice(c->device));
+  unsigned long long* dcnt = nullptr;
+  uint32_t* dfirst = nullptr;
+  HIPCHK(c, hipMalloc(&dcnt, sizeof(unsigned long long)));
+  hipError_t e = hipMalloc(&dfirst, sizeof(uint32_t));
+  if (e != hipSuccess) {
+    (void)hipFree(dcnt);
+    return fail(c, INU_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  const unsigned long long zero = 0ull;
+  const uint32_t none = 0xffffffffu;
+  e = hipMemcpyAsync(dcnt, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dfirst, &none, sizeof(none), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && hi > lo) {
+    hipLaunchKernelGGL(k_probe_rcp, dim3(8192), dim3(256), 0, c->stream, lo, hi, dcnt, dfirst);
+    e = hipGetLastError();
+  }
+  unsigned long long cnt = 0ull;
+  if (e == hipSuccess) e = hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(first, dfirst, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dcnt);
+  (void)hipFree(dfirst);
+  if (e != hipSuccess) return fail(c, INU_EHIP, std::string("probe_rcp: ") + hipGetErrorString(e));
+  *mismatches = cnt;
   return 0;
 }
 

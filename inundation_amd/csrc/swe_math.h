@@ -312,66 +312,159 @@ __device__ __forceinline__ float boundary_mass(uint32_t kind, float value, float
 }
 
 // ---------------------------------------------------------------------------
-// Fast arithmetic: the same scheme with shared reciprocals and explicit FMAs.
+// Fast arithmetic: the same scheme, algebraically unchanged, in fewer
+// instructions.
 //
-// Each function is the exact-order one above, algebraically unchanged, with
-//  - one division per side for the desingularised velocities (2h / (h^2 +
-//    max(h^2, e^2)) times each momentum instead of one division each);
-//  - c = sqrt(g) * sqrt(h) instead of sqrt(g * h);
-//  - (h / dry_tol) as h * (1 / dry_tol);
-//  - one star state (the side the face lies on) formed branch-free;
-//  - a * b + c as fmaf(a, b, c) where the exact order adds a product;
-//  - h^(-4/3) for Manning from a division-free inverse cube root.
+//  - Per cell, once: the desingularised velocity (u, v) = (hu, hv) vf(h) and
+//    the face factor phi(h) = h vf(h), where vf(h) = 2h / (h^2 + max(h^2,
+//    e^2)) is 1/h for h >= e_film, so phi is exactly 1 there. A side's face
+//    velocity is its cell velocity, turned into the edge frame, times
+//    phi(face depth): the reference's safe_vel(h*, h* u) (flux.py:63-66,
+//    146-169). Only a face that steps up has its own phi, and only a thin
+//    one (below e_film) needs a division for it.
+//  - A wall's mirror state is the inside velocity with its normal part
+//    negated (resident.py:126-129), taken in the edge frame directly.
+//  - Divisions are products with a correctly rounded reciprocal, rcp(x)
+//    (below), whose divisors stay in [2^-126, 2^126].
+//  - min, max and the [0, 1] clamp are the hardware's (minNum / maxNum with
+//    -0 below +0; the clamp takes NaN and -0 to +0).
+//  - c = sqrt(g) * sqrt(h) instead of sqrt(g * h); (h / dry_tol) as h *
+//    (1 / dry_tol); one star state (the side the face lies on) formed
+//    branch-free; a * b + c as fmaf(a, b, c) where the exact order adds a
+//    product; h^(-4/3) for Manning from a division-free inverse cube root.
 // Rounding differs from the exact order by a few ulp; results depend only on
-// the inputs (IEEE operations and fmaf, no contraction), and the FP32 oracle
-// restates the same sequence.
+// the inputs (IEEE operations, fmaf and the reciprocal, no contraction), and
+// the FP32 oracle restates the same sequence.
 // ---------------------------------------------------------------------------
 namespace fast {
 
-// safe_vel(h, q) == q * vel_factor(h).
-__device__ __forceinline__ float vel_factor(const Phys& p, float h) {
-  if (!(h > p.dry)) return 0.0f;
-  const float hh = h * h;
-  return (2.0f * h) / (hh + gmax(hh, p.e_film2));
+// The largest divisor rcp takes exactly.
+constexpr float kRcpMax = 0x1p126f;
+
+// 1/x, correctly rounded, for 2^-126 <= |x| <= 2^126: the hardware estimate
+// and one Newton step with a fused residual. Equal to the IEEE division 1/x
+// for every float in that range (checked exhaustively, inu_probe_rcp), and
+// not outside it (zero, subnormals, |x| > 2^126, infinities).
+__device__ __forceinline__ float rcp(float x) {
+  const float y = __builtin_amdgcn_rcpf(x);
+  return fmaf(fmaf(-x, y, 1.0f), y, y);
 }
 
-// sqL, sqR: sqrt_rn(gmax(hL, 0)), sqrt_rn(gmax(hR, 0)).
-__device__ __forceinline__ void hllc_speeds_sq(const Phys& p, float hL, float sqL, float unL, float hR, float sqR,
-                                               float unR, float& SL, float& SR) {
-  const float cL = p.sqrt_g * sqL;
-  const float cR = p.sqrt_g * sqR;
-  const float den = sqL + sqR;
-  const float u_roe = (den > 1e-12f) ? fmaf(sqL, unL, sqR * unR) / den : 0.0f;
-  const float c_roe = sqrt_rn(p.half_g * (hL + hR));
-  const float sl = gmin(unL - cL, u_roe - c_roe);
-  const float sr = gmax(unR + cR, u_roe + c_roe);
-  const float wetL = gclamp(fmaf(hL, p.inv_dry, -1.0f), 0.0f, 1.0f);
-  const float wetR = gclamp(fmaf(hR, p.inv_dry, -1.0f), 0.0f, 1.0f);
+// x clamped to [-2^126, 2^126] (NaN to -2^126): a divisor already bounded
+// away from zero brought inside rcp's range.
+__device__ __forceinline__ float rcp_range(float x) { return __builtin_amdgcn_fmed3f(x, -kRcpMax, kRcpMax); }
+
+// The hardware's min and max (a NaN operand yields the other; -0 < +0) and
+// [0, 1] clamp (NaN and -0 give +0).
+__device__ __forceinline__ float vmin(float x, float y) { return __builtin_fminf(x, y); }
+__device__ __forceinline__ float vmax(float x, float y) { return __builtin_fmaxf(x, y); }
+__device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
+
+// Face factor of a thin depth h < e_film: 2h^2 / (h^2 + e^2) above dry_tol,
+// else 0. (h^2 + e^2 lies in [e^2, 2e^2], inside rcp's range for the dry_tol
+// inu_set_params accepts.)
+__device__ __forceinline__ float thin_phi(const Phys& p, float h) {
+  const float hh = h * h;
+  return (h > p.dry) ? (2.0f * hh) * rcp(hh + p.e_film2) : 0.0f;
+}
+
+// vf(h) and phi(h) = h vf(h):
+//   h >= e_film:      vf = 1 / min(h, 2^126),   phi = 1;
+//   dry < h < e_film: vf = 2h / (h^2 + e^2),    phi = 2h^2 / (h^2 + e^2);
+//   otherwise (and NaN): both 0.
+__device__ __forceinline__ void factors(const Phys& p, float h, float& vf, float& phi) {
+  const bool thick = h >= p.e_film;
+  const float hh = h * h;
+  const float r = rcp(thick ? vmin(h, kRcpMax) : hh + p.e_film2);
+  vf = thick ? r : (2.0f * h) * r;
+  phi = thick ? 1.0f : (2.0f * hh) * r;
+  if (!(h > p.dry)) {
+    vf = 0.0f;
+    phi = 0.0f;
+  }
+}
+
+// One side of an edge as the edge arithmetic reads it: depth, desingularised
+// velocity, compensated low word (zero without compensation), phi(h) and
+// sqrt(max(h, 0)).
+struct Cell {
+  float h, u, v, lo, phi, sq;
+};
+
+__device__ __forceinline__ Cell cell(const Phys& p, float h, float hu, float hv, float lo) {
+  float vf, phi;
+  factors(p, h, vf, phi);
+  return Cell{h, hu * vf, hv * vf, lo, phi, sqrt_rn(vmax(h, 0.0f))};
+}
+
+// Face states of an edge with signed bed step d (d > 0: the left face steps
+// up by d; d < 0: the right one by -d; at most one side steps). The stepped
+// face depth is the other side's depth plus the surface difference from the
+// full compensated depths (h + lo), so level water gives both sides the same
+// bits: t = ((hL - hR) - d) + (loL - loR), hL* = max(hR + t, 0), hR* =
+// max(hL - t, 0). `mirror`: R is L's wall mirror.
+struct Face {
+  float hL, hR, sqL, sqR, unL, utL, unR, utR;
+};
+
+__device__ __forceinline__ Face face(const Phys& p, const Cell& L, const Cell& R, bool mirror, float d, float nx,
+                                     float ny) {
+  Face f;
+  const float t = ((L.h - R.h) - d) + (L.lo - R.lo);
+  const bool stepL = d > 0.0f;
+  const bool stepR = d < 0.0f;
+  f.hL = stepL ? vmax(R.h + t, 0.0f) : L.h;
+  f.hR = stepR ? vmax(L.h - t, 0.0f) : R.h;
+  const float hs = stepL ? f.hL : f.hR;
+  const float sqs = sqrt_rn(hs);
+  float phs = 1.0f;
+  if (!(hs >= p.e_film)) phs = thin_phi(p, hs);
+  f.sqL = stepL ? sqs : L.sq;
+  f.sqR = stepR ? sqs : R.sq;
+  const float phL = stepL ? phs : L.phi;
+  const float phR = stepR ? phs : R.phi;
+  const float nL = fmaf(L.u, nx, L.v * ny);
+  const float tL = fmaf(L.v, nx, -(L.u * ny));
+  const float nR = mirror ? -nL : fmaf(R.u, nx, R.v * ny);
+  const float tR = mirror ? tL : fmaf(R.v, nx, -(R.u * ny));
+  f.unL = nL * phL;
+  f.utL = tL * phL;
+  f.unR = nR * phR;
+  f.utR = tR * phR;
+  return f;
+}
+
+// HLLC wave speeds (reference flux.py:75-95): Einfeldt/Roe bounds blended
+// with Toro's dry-bed speeds over one dry_tol of face depth.
+__device__ __forceinline__ void speeds(const Phys& p, const Face& f, float& SL, float& SR) {
+  const float cL = p.sqrt_g * f.sqL;
+  const float cR = p.sqrt_g * f.sqR;
+  const float den = f.sqL + f.sqR;
+  const float u_roe = (den > 1e-12f) ? fmaf(f.sqL, f.unL, f.sqR * f.unR) * rcp(vmin(den, kRcpMax)) : 0.0f;
+  const float c_roe = sqrt_rn(p.half_g * (f.hL + f.hR));
+  const float sl = vmin(f.unL - cL, u_roe - c_roe);
+  const float sr = vmax(f.unR + cR, u_roe + c_roe);
+  const float wetL = clamp01(fmaf(f.hL, p.inv_dry, -1.0f));
+  const float wetR = clamp01(fmaf(f.hR, p.inv_dry, -1.0f));
   const float dryLeft = (1.0f - wetL) * wetR;
   const float dryRight = (1.0f - wetR) * wetL;
-  const float roe = 1.0f - dryLeft - dryRight;
-  SL = fmaf(roe, sl, fmaf(dryLeft, unR - 2.0f * cR, dryRight * (unL - cL)));
-  SR = fmaf(roe, sr, fmaf(dryLeft, unR + cR, dryRight * (unL + 2.0f * cL)));
+  const float roe = (1.0f - dryLeft) - dryRight;
+  SL = fmaf(roe, sl, fmaf(dryLeft, f.unR - 2.0f * cR, dryRight * (f.unL - cL)));
+  SR = fmaf(roe, sr, fmaf(dryLeft, f.unR + cR, dryRight * (f.unL + 2.0f * cL)));
 }
 
-__device__ __forceinline__ void hllc_speeds(const Phys& p, float hL, float unL, float hR, float unR,
-                                            float& SL, float& SR) {
-  fast::hllc_speeds_sq(p, hL, sqrt_rn(gmax(hL, 0.0f)), unL, hR, sqrt_rn(gmax(hR, 0.0f)), unR, SL, SR);
-}
-
-// HLLC on face states whose velocities are already desingularised; sqL, sqR
-// as for hllc_speeds_sq.
-__device__ __forceinline__ void hllc_vel(const Phys& p, float hL, float sqL, float unL, float utL, float hR,
-                                         float sqR, float unR, float utR, float& Fh, float& Fhn, float& Fht,
-                                         float& Smax) {
+// HLLC flux in the edge frame (reference flux.py:98-140).
+__device__ __forceinline__ void hllc(const Phys& p, const Face& f, float& Fh, float& Fhn, float& Fht,
+                                     float& Smax) {
   float SL, SR;
-  fast::hllc_speeds_sq(p, hL, sqL, unL, hR, sqR, unR, SL, SR);
-  Smax = gmax(fabsf(SL), fabsf(SR));
+  speeds(p, f, SL, SR);
+  Smax = vmax(fabsf(SL), fabsf(SR));
+  const float hL = f.hL, hR = f.hR, unL = f.unL, unR = f.unR, utL = f.utL, utR = f.utR;
   // Toro's contact speed, numerator regrouped as SL SR (hL - hR) +
   // (SL hR unR - SR hL unL): exactly zero for still water.
   const float num_sm = fmaf(SL * SR, hL - hR, fmaf(SL * hR, unR, -(SR * hL * unL)));
   const float den_sm = fmaf(hR, unR - SR, -(hL * (unL - SL)));
-  const float SM = (fabsf(den_sm) > 1e-12f) ? num_sm / den_sm : 0.0f;
+  const float SM = (fabsf(den_sm) > 1e-12f) ? num_sm * rcp(rcp_range(den_sm)) : 0.0f;
   const float FL_h = hL * unL;
   const float FL_hn = fmaf(FL_h, unL, p.half_g * hL * hL);
   const float FL_ht = FL_h * utL;
@@ -387,12 +480,12 @@ __device__ __forceinline__ void hllc_vel(const Phys& p, float hL, float sqL, flo
   const float FS_hn = left ? FL_hn : FR_hn;
   const float FS_ht = left ? FL_ht : FR_ht;
   // Star depth minus side depth, hS (SM - unS) / (S - SM): zero when the
-  // contact does not move relative to the side.
+  // contact does not move relative to the side. One reciprocal for both
+  // cases: hS (SM - unS) / dS, or hS (S - unS) / 1e-14 - hS when the star
+  // region is degenerate.
   const float dS = S - SM;
-  // One division for both cases: hS (SM - unS) / dS, or hS (S - unS) / 1e-14
-  // - hS when the star region is degenerate.
   const bool dS_ok = fabsf(dS) > 1e-14f;
-  const float ks_q = (hS * (dS_ok ? SM - unS : S - unS)) / (dS_ok ? dS : 1e-14f);
+  const float ks_q = (hS * (dS_ok ? SM - unS : S - unS)) * rcp(dS_ok ? rcp_range(dS) : 1e-14f);
   const float ks_h = dS_ok ? ks_q : ks_q - hS;
   const float kS = hS + ks_h;
   if (SL >= 0.0f) {
@@ -406,89 +499,23 @@ __device__ __forceinline__ void hllc_vel(const Phys& p, float hL, float sqL, flo
   }
 }
 
-struct Face {
-  float hLs, hRs;          // reconstructed depths
-  float qnL, qtL, qnR, qtR;  // face-frame momenta
-};
-
-// loL, loR: low words of compensated depths (zero without compensation).
-__device__ __forceinline__ Face face_states(const Phys& p, float hL, float huL, float hvL, float dzL,
-                                            float hR, float huR, float hvR, float dzR, float nx,
-                                            float ny, float loL, float loR) {
-  Face f;
-  // At most one side steps up to the face. Its face depth is the other side's
-  // depth plus the surface difference, formed from the full compensated
-  // depths (h + lo), so level water gives both sides the same bits.
-  f.hLs = (dzL == 0.0f) ? hL : gmax(0.0f, hR + (((hL - hR) - dzL) + (loL - loR)));
-  f.hRs = (dzR == 0.0f) ? hR : gmax(0.0f, hL + (((hR - hL) - dzR) + (loR - loL)));
-  const float rL = vel_factor(p, hL) * f.hLs;
-  const float rR = vel_factor(p, hR) * f.hRs;
-  const float huLs = huL * rL, hvLs = hvL * rL;
-  const float huRs = huR * rR, hvRs = hvR * rR;
-  f.qnL = fmaf(huLs, nx, hvLs * ny);
-  f.qtL = fmaf(hvLs, nx, -(huLs * ny));
-  f.qnR = fmaf(huRs, nx, hvRs * ny);
-  f.qtR = fmaf(hvRs, nx, -(huRs * ny));
-  return f;
-}
-
-__device__ __forceinline__ float wave_bound(const Phys& p, float hL, float huL, float hvL, float dzL,
-                                            float hR, float huR, float hvR, float dzR, float nx,
-                                            float ny, float loL, float loR) {
-  const Face f = fast::face_states(p, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx, ny, loL, loR);
-  float SL, SR;
-  fast::hllc_speeds(p, f.hLs, f.qnL * vel_factor(p, f.hLs), f.hRs, f.qnR * vel_factor(p, f.hRs), SL, SR);
-  return gmax(fabsf(SL), fabsf(SR));
-}
-
-__device__ __forceinline__ void wall_ghost(float huL, float hvL, float nx, float ny, float& huR,
-                                           float& hvR) {
-  const float q2 = 2.0f * fmaf(huL, nx, hvL * ny);
-  huR = fmaf(-q2, nx, huL);
-  hvR = fmaf(-q2, ny, hvL);
-}
-
-// Cell quantities of one side of an edge: depth, momenta, compensated low
-// word, vel_factor(h), sqrt_rn(gmax(h, 0)) and the pressure half_g * h * h.
-struct Side {
-  float h, hu, hv, lo, vf, sq, pc;
-};
-
-// The edge's six words. A side whose face does not step up keeps its cell
-// depth as face depth, so its vel_factor and square root are the cell's; at
-// most one side steps, and only its face depth needs new ones.
-__device__ __forceinline__ void edge_words(const Phys& p, bool wall, const Side& L, const Side& R, float dzL,
-                                           float dzR, float nx, float ny, float len, float w[6],
+// The edge's six words (reference resident.py:165-184 with flux.py:98-169,
+// 271-272): mass flux, left momentum (x, y), right momentum (x, y), all times
+// the edge length, and the CFL term Smax * len. `wall` passes no mass; fh,
+// if given, receives the HLLC mass flux before that.
+__device__ __forceinline__ void edge_words(const Phys& p, bool wall, const Cell& L, const Cell& R, bool mirror,
+                                           float d, float nx, float ny, float len, float w[6],
                                            float* fh = nullptr) {
-  Face f;
-  f.hLs = (dzL == 0.0f) ? L.h : gmax(0.0f, R.h + (((L.h - R.h) - dzL) + (L.lo - R.lo)));
-  f.hRs = (dzR == 0.0f) ? R.h : gmax(0.0f, L.h + (((R.h - L.h) - dzR) + (R.lo - L.lo)));
-  const float rL0 = L.vf * f.hLs;
-  const float rR0 = R.vf * f.hRs;
-  const float huLs = L.hu * rL0, hvLs = L.hv * rL0;
-  const float huRs = R.hu * rR0, hvRs = R.hv * rR0;
-  f.qnL = fmaf(huLs, nx, hvLs * ny);
-  f.qtL = fmaf(hvLs, nx, -(huLs * ny));
-  f.qnR = fmaf(huRs, nx, hvRs * ny);
-  f.qtR = fmaf(hvRs, nx, -(huRs * ny));
-  const bool stepL = dzL != 0.0f;
-  const bool stepR = dzR != 0.0f;
-  const float hs = stepL ? f.hLs : f.hRs;
-  const float vfs = vel_factor(p, hs);
-  const float sqs = sqrt_rn(gmax(hs, 0.0f));
-  const float rL = stepL ? vfs : L.vf;
-  const float rR = stepR ? vfs : R.vf;
-  const float sqL = stepL ? sqs : L.sq;
-  const float sqR = stepR ? sqs : R.sq;
+  const Face f = face(p, L, R, mirror, d, nx, ny);
   float Fh, Fhn, Fht, Smax;
-  hllc_vel(p, f.hLs, sqL, f.qnL * rL, f.qtL * rL, f.hRs, sqR, f.qnR * rR, f.qtR * rR, Fh, Fhn, Fht, Smax);
+  hllc(p, f, Fh, Fhn, Fht, Smax);
   if (fh) *fh = Fh;
   if (wall) Fh = 0.0f;
   // Normal momentum flux plus the hydrostatic correction of each side,
   // grouped as (Fhn - P(face depth)) + P(cell depth): still water gives
   // exactly the cell's own pressure on every edge.
-  const float nL = (Fhn - p.half_g * f.hLs * f.hLs) + L.pc;
-  const float nR = (Fhn - p.half_g * f.hRs * f.hRs) + R.pc;
+  const float nL = (Fhn - p.half_g * f.hL * f.hL) + p.half_g * L.h * L.h;
+  const float nR = (Fhn - p.half_g * f.hR * f.hR) + p.half_g * R.h * R.h;
   const float tx = Fht * nx, ty = Fht * ny;
   w[0] = Fh * len;
   w[1] = fmaf(nL, nx, -ty) * len;
@@ -498,11 +525,23 @@ __device__ __forceinline__ void edge_words(const Phys& p, bool wall, const Side&
   w[5] = Smax * len;
 }
 
+// The edge's CFL wave speed, max(|SL|, |SR|), from the same face states and
+// speeds as its words (reference flux.py:172-182, resident.py:528-610).
+__device__ __forceinline__ float wave_bound(const Phys& p, const Cell& L, const Cell& R, bool mirror, float d,
+                                            float nx, float ny) {
+  const Face f = face(p, L, R, mirror, d, nx, ny);
+  float SL, SR;
+  speeds(p, f, SL, SR);
+  return vmax(fabsf(SL), fabsf(SR));
+}
+
+// Film drag 1 / (1 + dt (e^2 - h^2) / (2h^2) sqrt(g/e)) below e_film
+// (reference flux.py:46-51), for dry_tol <= h and dt >= 0.
 __device__ __forceinline__ float film_drag(const Phys& p, float h, float dt) {
   if (h >= p.e_film) return 1.0f;
   const float hh = h * h;
-  const float rate = (p.e_film2 - hh) / (2.0f * hh) * p.sqrt_g_e;
-  return 1.0f / fmaf(dt, rate, 1.0f);
+  const float rate = ((p.e_film2 - hh) * rcp(2.0f * hh)) * p.sqrt_g_e;
+  return rcp(vmin(fmaf(dt, rate, 1.0f), kRcpMax));
 }
 
 // x^(-1/3) for positive normal x: exact exponent split, cubic start on
@@ -531,9 +570,16 @@ __device__ __forceinline__ void friction(const Phys& p, float h, float n, float 
   const float r2 = r * r;
   const float spd = sqrt_rn(fmaf(hu, hu, hv * hv)) * (r2 * r);
   const float Sf = p.g * n * n * spd * (r2 * r2);
-  const float id = 1.0f / fmaf(dt, Sf, 1.0f);
+  const float id = rcp(vmin(fmaf(dt, Sf, 1.0f), kRcpMax));
   hu *= id;
   hv *= id;
+}
+
+// Speed |u| and celerity c of a wetted source cell (h >= dry_tol) for its dt
+// bound (reference resident.py:709-728).
+__device__ __forceinline__ void src_speeds(const Phys& p, float h, float hu, float hv, float& u, float& c) {
+  u = sqrt_rn(fmaf(hu, hu, hv * hv)) * rcp(vmin(h, kRcpMax));
+  c = p.sqrt_g * sqrt_rn(h);
 }
 
 }  // namespace fast

@@ -92,6 +92,7 @@ EXPORTS = (
     "inu_adj_end",
     "inu_probe_vjp",
     "inu_probe_clock",
+    "inu_probe_rcp",
     "inu_stream_bw",
     "inu_set_source_rates_noalloc",
 )
@@ -201,6 +202,7 @@ def load() -> C.CDLL:
         "inu_adj_end": (_I, [_P]),
         "inu_probe_vjp": (_I, [_P, _U32, _P, _P, _F, _F]),
         "inu_probe_clock": (_I, [_P, _I, C.POINTER(InuCtrl), _P]),
+        "inu_probe_rcp": (_I, [_P, _U32, _U32, _P, _P]),
         "inu_stream_bw": (_I, [_P, C.c_uint64, _U32, _I, _U32, _U32, _P]),
         "inu_set_source_rates_noalloc": (_I, [_P, _U32, _P, _P]),
     }
@@ -551,6 +553,14 @@ class Context:
         dt[: len(dt_words)] = np.asarray(dt_words, dtype=np.uint32)
         self.call("inu_probe_clock", int(mode), C.byref(out), ptr(dt))
         return out, dt
+
+    def probe_rcp(self, lo: int, hi: int) -> tuple[int, int]:
+        """Bit patterns in [lo, hi) and their negatives where the fast reciprocal differs from the IEEE
+        division: (count, smallest such pattern or 0xffffffff)."""
+        cnt = np.zeros(1, dtype=np.uint64)
+        first = np.zeros(1, dtype=np.uint32)
+        self.call("inu_probe_rcp", int(lo), int(hi), ptr(cnt), ptr(first))
+        return int(cnt[0]), int(first[0])
 
     def probe(self, pairs: NDArray[np.float32], g: float, dry_tol: float) -> NDArray[np.float32]:
         p = np.ascontiguousarray(pairs, dtype=np.float32).reshape(-1, 2)

@@ -294,10 +294,20 @@ inline float boundary_mass_flux(uint32_t kind, float value, float fh) {
 }
 
 // ---- the backend's fast arithmetic, restated -------------------------------
-// Same scheme, algebraically unchanged: one division per side for the
-// desingularised velocities, c = sqrt(g) sqrt(h), h * (1 / dry_tol), one star
-// state chosen before it is formed, explicit fused multiply-adds, and
-// h^(-4/3) from a division-free inverse cube root.
+// Same scheme, algebraically unchanged (inundation_amd/csrc/swe_math.h,
+// namespace fast):
+//  - per cell the desingularised velocity (u, v) = (hu, hv) vf(h) and the
+//    face factor phi(h) = h vf(h), with vf(h) = 1/h for h >= e_film (so phi
+//    is exactly 1 there) and 2h / (h^2 + e^2) below; a side's face velocity
+//    is its cell velocity in the edge frame times phi(face depth);
+//  - a wall's mirror taken in the edge frame (normal velocity negated);
+//  - divisions as products with the IEEE reciprocal 1/x of a divisor kept in
+//    [2^-126, 2^126] (the backend forms it from the hardware estimate and one
+//    Newton step, equal to 1/x over that range);
+//  - min, max and the [0, 1] clamp with the hardware's semantics;
+//  - c = sqrt(g) sqrt(h), h * (1 / dry_tol), one star state chosen before it
+//    is formed, explicit fused multiply-adds, h^(-4/3) from a division-free
+//    inverse cube root.
 namespace fastm {
 
 struct K {
@@ -318,58 +328,116 @@ inline K consts(const Pc& pc) {
   return k;
 }
 
-inline float vel_factor(const K& k, float h) {
-  if (!(h > k.dry)) return 0.0f;
-  float hh = h * h;
-  return (2.0f * h) / (hh + gmax(hh, k.e2));
+constexpr float kRcpMax = 0x1p126f;
+
+inline float rcp(float x) { return 1.0f / x; }
+
+// The hardware's v_min_f32 / v_max_f32: a NaN operand yields the other, and
+// -0 orders below +0.
+inline float vmin(float x, float y) {
+  if (x != x) return y;
+  if (y != y) return x;
+  if (x == 0.0f && y == 0.0f) return std::signbit(x) ? x : y;
+  return (y < x) ? y : x;
+}
+inline float vmax(float x, float y) {
+  if (x != x) return y;
+  if (y != y) return x;
+  if (x == 0.0f && y == 0.0f) return std::signbit(x) ? y : x;
+  return (x < y) ? y : x;
+}
+// The [0, 1] clamp (v_med3_f32 / the clamp modifier): NaN and -0 give +0.
+inline float clamp01(float x) { return (x > 0.0f) ? ((x < 1.0f) ? x : 1.0f) : 0.0f; }
+// v_med3_f32(x, -2^126, 2^126): NaN gives -2^126.
+inline float rcp_range(float x) {
+  if (x != x) return -kRcpMax;
+  return (x < -kRcpMax) ? -kRcpMax : ((x > kRcpMax) ? kRcpMax : x);
 }
 
-inline void speeds(const K& k, float hL, float unL, float hR, float unR, float& SL, float& SR) {
-  float sqL = std::sqrt(gmax(hL, 0.0f));
-  float sqR = std::sqrt(gmax(hR, 0.0f));
-  float cL = k.sqrt_g * sqL;
-  float cR = k.sqrt_g * sqR;
-  float den = sqL + sqR;
-  float u_roe = (den > 1e-12f) ? std::fma(sqL, unL, sqR * unR) / den : 0.0f;
-  float c_roe = std::sqrt(k.half_g * (hL + hR));
-  float sl = gmin(unL - cL, u_roe - c_roe);
-  float sr = gmax(unR + cR, u_roe + c_roe);
-  float wetL = gclamp(std::fma(hL, k.inv_dry, -1.0f), 0.0f, 1.0f);
-  float wetR = gclamp(std::fma(hR, k.inv_dry, -1.0f), 0.0f, 1.0f);
+inline float thin_phi(const K& k, float h) {
+  float hh = h * h;
+  return (h > k.dry) ? (2.0f * hh) * rcp(hh + k.e2) : 0.0f;
+}
+
+inline void factors(const K& k, float h, float& vf, float& phi) {
+  if (!(h > k.dry)) {
+    vf = 0.0f;
+    phi = 0.0f;
+  } else if (h >= k.e) {
+    vf = rcp(vmin(h, kRcpMax));
+    phi = 1.0f;
+  } else {
+    float hh = h * h;
+    float r = rcp(hh + k.e2);
+    vf = (2.0f * h) * r;
+    phi = (2.0f * hh) * r;
+  }
+}
+
+struct Cell {
+  float h, u, v, lo, phi, sq;
+};
+
+inline Cell cell(const K& k, float h, float hu, float hv, float lo) {
+  float vf, phi;
+  factors(k, h, vf, phi);
+  return Cell{h, hu * vf, hv * vf, lo, phi, std::sqrt(vmax(h, 0.0f))};
+}
+
+struct Face {
+  float hL, hR, sqL, sqR, unL, utL, unR, utR;
+};
+
+// d > 0: the left face steps up by d; d < 0: the right one by -d.
+inline Face face(const K& k, const Cell& L, const Cell& R, bool mirror, float d, float nx, float ny) {
+  Face f;
+  float t = ((L.h - R.h) - d) + (L.lo - R.lo);
+  bool stepL = d > 0.0f, stepR = d < 0.0f;
+  f.hL = stepL ? vmax(R.h + t, 0.0f) : L.h;
+  f.hR = stepR ? vmax(L.h - t, 0.0f) : R.h;
+  float hs = stepL ? f.hL : f.hR;
+  float sqs = std::sqrt(hs);
+  float phs = (hs >= k.e) ? 1.0f : thin_phi(k, hs);
+  f.sqL = stepL ? sqs : L.sq;
+  f.sqR = stepR ? sqs : R.sq;
+  float phL = stepL ? phs : L.phi, phR = stepR ? phs : R.phi;
+  float nL = std::fma(L.u, nx, L.v * ny);
+  float tL = std::fma(L.v, nx, -(L.u * ny));
+  float nR = mirror ? -nL : std::fma(R.u, nx, R.v * ny);
+  float tR = mirror ? tL : std::fma(R.v, nx, -(R.u * ny));
+  f.unL = nL * phL;
+  f.utL = tL * phL;
+  f.unR = nR * phR;
+  f.utR = tR * phR;
+  return f;
+}
+
+inline void speeds(const K& k, const Face& f, float& SL, float& SR) {
+  float cL = k.sqrt_g * f.sqL, cR = k.sqrt_g * f.sqR;
+  float den = f.sqL + f.sqR;
+  float u_roe = (den > 1e-12f) ? std::fma(f.sqL, f.unL, f.sqR * f.unR) * rcp(vmin(den, kRcpMax)) : 0.0f;
+  float c_roe = std::sqrt(k.half_g * (f.hL + f.hR));
+  float sl = vmin(f.unL - cL, u_roe - c_roe);
+  float sr = vmax(f.unR + cR, u_roe + c_roe);
+  float wetL = clamp01(std::fma(f.hL, k.inv_dry, -1.0f));
+  float wetR = clamp01(std::fma(f.hR, k.inv_dry, -1.0f));
   float dryLeft = (1.0f - wetL) * wetR;
   float dryRight = (1.0f - wetR) * wetL;
-  float roe = 1.0f - dryLeft - dryRight;
-  SL = std::fma(roe, sl, std::fma(dryLeft, unR - 2.0f * cR, dryRight * (unL - cL)));
-  SR = std::fma(roe, sr, std::fma(dryLeft, unR + cR, dryRight * (unL + 2.0f * cL)));
+  float roe = (1.0f - dryLeft) - dryRight;
+  SL = std::fma(roe, sl, std::fma(dryLeft, f.unR - 2.0f * cR, dryRight * (f.unL - cL)));
+  SR = std::fma(roe, sr, std::fma(dryLeft, f.unR + cR, dryRight * (f.unL + 2.0f * cL)));
 }
 
-inline void faces(const K& k, float hL, float huL, float hvL, float dzL, float hR, float huR, float hvR,
-                  float dzR, float nx, float ny, float loL, float loR, float& hLs, float& qnL, float& qtL,
-                  float& hRs, float& qnR, float& qtR) {
-  // At most one side steps up to the face. Its face depth is the other side's
-  // depth plus the surface difference, formed from the full compensated
-  // depths (h + lo), so level water gives both sides the same bits.
-  hLs = (dzL == 0.0f) ? hL : gmax(0.0f, hR + (((hL - hR) - dzL) + (loL - loR)));
-  hRs = (dzR == 0.0f) ? hR : gmax(0.0f, hL + (((hR - hL) - dzR) + (loR - loL)));
-  float rL = vel_factor(k, hL) * hLs;
-  float rR = vel_factor(k, hR) * hRs;
-  float huLs = huL * rL, hvLs = hvL * rL, huRs = huR * rR, hvRs = hvR * rR;
-  qnL = std::fma(huLs, nx, hvLs * ny);
-  qtL = std::fma(hvLs, nx, -(huLs * ny));
-  qnR = std::fma(huRs, nx, hvRs * ny);
-  qtR = std::fma(hvRs, nx, -(huRs * ny));
-}
-
-inline void hllc(const K& k, float hL, float unL, float utL, float hR, float unR, float utR, float& Fh,
-                 float& Fhn, float& Fht, float& Smax) {
+inline void hllc(const K& k, const Face& f, float& Fh, float& Fhn, float& Fht, float& Smax) {
   float SL, SR;
-  speeds(k, hL, unL, hR, unR, SL, SR);
-  Smax = gmax(std::fabs(SL), std::fabs(SR));
+  speeds(k, f, SL, SR);
+  Smax = vmax(std::fabs(SL), std::fabs(SR));
+  const float hL = f.hL, hR = f.hR, unL = f.unL, unR = f.unR, utL = f.utL, utR = f.utR;
   // Toro's contact speed with the numerator regrouped as
   // SL SR (hL - hR) + (SL hR unR - SR hL unL): exactly zero for still water.
   float num_sm = std::fma(SL * SR, hL - hR, std::fma(SL * hR, unR, -(SR * hL * unL)));
   float den_sm = std::fma(hR, unR - SR, -(hL * (unL - SL)));
-  float SM = (std::fabs(den_sm) > 1e-12f) ? num_sm / den_sm : 0.0f;
+  float SM = (std::fabs(den_sm) > 1e-12f) ? num_sm * rcp(rcp_range(den_sm)) : 0.0f;
   float FL_h = hL * unL, FL_hn = std::fma(FL_h, unL, k.half_g * hL * hL), FL_ht = FL_h * utL;
   float FR_h = hR * unR, FR_hn = std::fma(FR_h, unR, k.half_g * hR * hR), FR_ht = FR_h * utR;
   bool left = SM >= 0.0f;
@@ -378,7 +446,9 @@ inline void hllc(const K& k, float hL, float unL, float utL, float hR, float unR
   // Star depth minus the side depth, hS (SM - unS) / (S - SM): zero when the
   // contact does not move relative to the side.
   float dS = S - SM;
-  float ksh = (std::fabs(dS) > 1e-14f) ? hS * (SM - unS) / dS : hS * (S - unS) / 1e-14f - hS;
+  bool dS_ok = std::fabs(dS) > 1e-14f;
+  float ks_q = (hS * (dS_ok ? SM - unS : S - unS)) * rcp(dS_ok ? rcp_range(dS) : 1e-14f);
+  float ksh = dS_ok ? ks_q : ks_q - hS;
   float kS = hS + ksh;
   if (SL >= 0.0f) { Fh = FL_h; Fhn = FL_hn; Fht = FL_ht; }
   else if (!left && !(SR >= 0.0f)) { Fh = FR_h; Fhn = FR_hn; Fht = FR_ht; }
@@ -389,26 +459,41 @@ inline void hllc(const K& k, float hL, float unL, float utL, float hR, float unR
   }
 }
 
-inline void wall(float huL, float hvL, float nx, float ny, float& huR, float& hvR) {
-  float q2 = 2.0f * std::fma(huL, nx, hvL * ny);
-  huR = std::fma(-q2, nx, huL);
-  hvR = std::fma(-q2, ny, hvL);
+// The six words of an edge; `wall` passes no mass, `mirror`: R is L's wall
+// mirror.
+inline void words(const K& k, bool wall, const Cell& L, const Cell& R, bool mirror, float d, float nx, float ny,
+                  float len, float* o, float& fh) {
+  Face f = face(k, L, R, mirror, d, nx, ny);
+  float Fh, Fhn, Fht, Smax;
+  hllc(k, f, Fh, Fhn, Fht, Smax);
+  fh = Fh;
+  if (wall) Fh = 0.0f;
+  // Normal momentum flux plus the hydrostatic correction, grouped as
+  // (Fhn - P(face depth)) + P(cell depth): still water gives exactly the
+  // cell's own pressure on every edge.
+  float nL = (Fhn - k.half_g * f.hL * f.hL) + k.half_g * L.h * L.h;
+  float nR = (Fhn - k.half_g * f.hR * f.hR) + k.half_g * R.h * R.h;
+  float tx = Fht * nx, ty = Fht * ny;
+  o[0] = Fh * len;
+  o[1] = std::fma(nL, nx, -ty) * len;
+  o[2] = std::fma(nL, ny, tx) * len;
+  o[3] = std::fma(nR, nx, -ty) * len;
+  o[4] = std::fma(nR, ny, tx) * len;
+  o[5] = Smax * len;
 }
 
-inline float wave_bound(const K& k, float hL, float huL, float hvL, float dzL, float hR, float huR,
-                        float hvR, float dzR, float nx, float ny, float loL, float loR) {
-  float hLs, qnL, qtL, hRs, qnR, qtR;
-  faces(k, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx, ny, loL, loR, hLs, qnL, qtL, hRs, qnR, qtR);
+inline float wave_bound(const K& k, const Cell& L, const Cell& R, bool mirror, float d, float nx, float ny) {
+  Face f = face(k, L, R, mirror, d, nx, ny);
   float SL, SR;
-  speeds(k, hLs, qnL * vel_factor(k, hLs), hRs, qnR * vel_factor(k, hRs), SL, SR);
-  return gmax(std::fabs(SL), std::fabs(SR));
+  speeds(k, f, SL, SR);
+  return vmax(std::fabs(SL), std::fabs(SR));
 }
 
 inline float film(const K& k, float h, float dt) {
   if (h >= k.e) return 1.0f;
   float hh = h * h;
-  float rate = (k.e2 - hh) / (2.0f * hh) * k.sqrt_g_e;
-  return 1.0f / std::fma(dt, rate, 1.0f);
+  float rate = ((k.e2 - hh) * rcp(2.0f * hh)) * k.sqrt_g_e;
+  return rcp(vmin(std::fma(dt, rate, 1.0f), kRcpMax));
 }
 
 inline float rcbrt(float x) {
@@ -431,9 +516,15 @@ inline void friction(const K& k, float h, float n, float dt, float& hu, float& h
   float r2 = r * r;
   float spd = std::sqrt(std::fma(hu, hu, hv * hv)) * (r2 * r);  // |u| with 1/h as r^3
   float Sf = k.g * n * n * spd * (r2 * r2);
-  float id = 1.0f / std::fma(dt, Sf, 1.0f);
+  float id = rcp(vmin(std::fma(dt, Sf, 1.0f), kRcpMax));
   hu *= id;
   hv *= id;
+}
+
+// |u| and c of a wetted source cell (h >= dry_tol) for its dt bound.
+inline void src_speeds(const K& k, float h, float hu, float hv, float& u, float& c) {
+  u = std::sqrt(std::fma(hu, hu, hv * hv)) * rcp(vmin(h, kRcpMax));
+  c = k.sqrt_g * std::sqrt(h);
 }
 
 }  // namespace fastm
@@ -492,31 +583,18 @@ inline void load_state(const cref& r, int stage, int c, float& h_, float& hu_, f
 
 // The six words of one edge (resident.py:146-184): mass, left momentum x, y,
 // right momentum x, y, CFL term; fh receives the HLLC mass flux, and a wall
-// passes no mass.
-void edge_words(const cref& r, const fastm::K& k, bool wall, float hL, float huL, float hvL, float loL, float hR,
-                float huR, float hvR, float loR, float dzL, float dzR, float nx_, float ny_, float len, float* o,
-                float& fh) {
+// passes no mass. `mirror`: the right side is the left cell's wall mirror
+// (the exact order gets its mirrored momenta in huR, hvR; the fast order
+// takes the mirror in the edge frame).
+void edge_words(const cref& r, const fastm::K& k, bool wall, bool mirror, float hL, float huL, float hvL, float loL,
+                float hR, float huR, float hvR, float loR, float dzL, float dzR, float nx_, float ny_, float len,
+                float* o, float& fh) {
   const Pc& pc = r.pc;
   if (r.fast) {
-    float hLs, qnL, qtL, hRs, qnR, qtR;
-    fastm::faces(k, hL, huL, hvL, dzL, hR, huR, hvR, dzR, nx_, ny_, loL, loR, hLs, qnL, qtL, hRs, qnR, qtR);
-    float rL = fastm::vel_factor(k, hLs), rR = fastm::vel_factor(k, hRs);
-    float Fh, Fhn, Fht, Smax;
-    fastm::hllc(k, hLs, qnL * rL, qtL * rL, hRs, qnR * rR, qtR * rR, Fh, Fhn, Fht, Smax);
-    fh = Fh;
-    if (wall) Fh = 0.0f;
-    // Normal momentum flux plus the hydrostatic correction, grouped as
-    // (Fhn - P(face depth)) + P(cell depth): still water gives exactly the
-    // cell's own pressure on every edge.
-    float nL = (Fhn - k.half_g * hLs * hLs) + k.half_g * hL * hL;
-    float nR = (Fhn - k.half_g * hRs * hRs) + k.half_g * hR * hR;
-    float tx = Fht * nx_, ty = Fht * ny_;
-    o[0] = Fh * len;
-    o[1] = std::fma(nL, nx_, -ty) * len;
-    o[2] = std::fma(nL, ny_, tx) * len;
-    o[3] = std::fma(nR, nx_, -ty) * len;
-    o[4] = std::fma(nR, ny_, tx) * len;
-    o[5] = Smax * len;
+    const fastm::Cell L = fastm::cell(k, hL, huL, hvL, loL);
+    const fastm::Cell R = mirror ? L : fastm::cell(k, hR, huR, hvR, loR);
+    const float d = (dzL > 0.0f) ? dzL : -dzR;  // the backend's signed step
+    fastm::words(k, wall, L, R, mirror, d, nx_, ny_, len, o, fh);
     return;
   }
   float hLs, hqnL, hqtL, hRs, hqnR, hqtR;
@@ -559,12 +637,8 @@ void flux(cref& r, int stage) {
       const uint32_t ok = open ? (uint32_t)(-2 - cR) : 0u;
       if (cR == -1) {
         hR = hL;
-        if (r.fast) {
-          fastm::wall(huL, hvL, nx_, ny_, huR, hvR);
-        } else {
-          huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
-          hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
-        }
+        huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
+        hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
       } else if (open) {
         ghost_state(pc, r.okind[ok], r.oval[ok], hL, huL, hvL, nx_, ny_, hR, huR, hvR);
       } else {
@@ -579,14 +653,14 @@ void flux(cref& r, int stage) {
       }
       float dzL = r.edz[2 * eid], dzR = r.edz[2 * eid + 1];
       float w[kWords], fh;
-      edge_words(r, kk, cR == -1, hL, huL, hvL, loL, hR, huR, hvR, loR, dzL, dzR, nx_, ny_, len, w, fh);
+      edge_words(r, kk, cR == -1, cR == -1, hL, huL, hvL, loL, hR, huR, hvR, loR, dzL, dzR, nx_, ny_, len, w, fh);
       if (open) {
         const uint32_t kind = r.okind[ok];
         if (outflow_only(kind) && fh < 0.0f) {
           // Drawing water in through an outflow-only edge: the whole flux is
           // redone against the wall ghost and passes no mass.
           ghost_state(pc, kOpenWall, 0.0f, hL, huL, hvL, nx_, ny_, hR, huR, hvR);
-          edge_words(r, kk, true, hL, huL, hvL, loL, hR, huR, hvR, loR, dzL, dzR, nx_, ny_, len, w, fh);
+          edge_words(r, kk, true, false, hL, huL, hvL, loL, hR, huR, hvR, loR, dzL, dzR, nx_, ny_, len, w, fh);
         } else {
           w[0] = boundary_mass_flux(kind, r.oval[ok], fh) * len;
         }
@@ -773,9 +847,13 @@ void source(cref& r) {
     float h_new = r.h[i];
     if (!(h_new >= pc.dry)) continue;
     const bool F = r.fast;
-    float u = F ? std::sqrt(std::fma(r.hu[i], r.hu[i], r.hv[i] * r.hv[i])) / h_new
-                : std::sqrt(r.hu[i] * r.hu[i] + r.hv[i] * r.hv[i]) / h_new;
-    float c = F ? std::sqrt(pc.g) * std::sqrt(h_new) : std::sqrt(pc.g * h_new);
+    float u, c;
+    if (F) {
+      fastm::src_speeds(fastm::consts(pc), h_new, r.hu[i], r.hv[i], u, c);
+    } else {
+      u = std::sqrt(r.hu[i] * r.hu[i] + r.hv[i] * r.hv[i]) / h_new;
+      c = std::sqrt(pc.g * h_new);
+    }
     float lam = 0.0f, perim = 0.0f;
     bool any_dry = false;
     for (uint32_t kk = r.cptr[i]; kk < r.cptr[i + 1]; ++kk) {
@@ -833,12 +911,8 @@ void cfl_prologue(cref& r) {
     float hR, huR, hvR;
     if (cR == -1) {
       hR = hL;
-      if (r.fast) {
-        fastm::wall(huL, hvL, nx_, ny_, huR, hvR);
-      } else {
-        huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
-        hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
-      }
+      huR = huL - 2.0f * (huL * nx_ + hvL * ny_) * nx_;
+      hvR = hvL - 2.0f * (huL * nx_ + hvL * ny_) * ny_;
     } else if (cR < -1) {
       const uint32_t k = (uint32_t)(-2 - cR);
       ghost_state(pc, r.okind[k], r.oval[k], hL, huL, hvL, nx_, ny_, hR, huR, hvR);
@@ -846,12 +920,18 @@ void cfl_prologue(cref& r) {
       hR = r.h[cR]; huR = r.hu[cR]; hvR = r.hv[cR];
     }
     float contrib = 0.0f;
-    if (hL >= pc.dry || hR >= pc.dry)
-      contrib = (r.fast ? fastm::wave_bound(kk, hL, huL, hvL, r.edz[2 * eid], hR, huR, hvR,
-                                            r.edz[2 * eid + 1], nx_, ny_, load_lo(r, 0, cL),
-                                            cR >= 0 ? load_lo(r, 0, cR) : load_lo(r, 0, cL))
-                        : wave_bound(pc, hL, huL, hvL, r.edz[2 * eid], hR, huR, hvR, r.edz[2 * eid + 1], nx_, ny_)) *
-                r.elen[eid];
+    if (hL >= pc.dry || hR >= pc.dry) {
+      if (r.fast) {
+        const float loL = load_lo(r, 0, cL);
+        const fastm::Cell L = fastm::cell(kk, hL, huL, hvL, loL);
+        const fastm::Cell R = (cR == -1) ? L : fastm::cell(kk, hR, huR, hvR, cR >= 0 ? load_lo(r, 0, cR) : loL);
+        const float dzL = r.edz[2 * eid], dzR = r.edz[2 * eid + 1];
+        contrib = fastm::wave_bound(kk, L, R, cR == -1, (dzL > 0.0f) ? dzL : -dzR, nx_, ny_) * r.elen[eid];
+      } else {
+        contrib = wave_bound(pc, hL, huL, hvL, r.edz[2 * eid], hR, huR, hvR, r.edz[2 * eid + 1], nx_, ny_) *
+                  r.elen[eid];
+      }
+    }
     r.ef[(size_t)kWords * eid + kCfl] = contrib;
   }
   uint32_t best = 0xffffffffu;
@@ -1100,7 +1180,7 @@ void cref_probe(uint32_t n, const float* pairs, float* out7, float g, float dry)
     o[3] = (x > 0.0f) ? pow43(x) : 0.0f;
     o[4] = (x > 0.0f) ? film_drag(pc, x, y) : 0.0f;
     o[5] = (x > 0.0f) ? fastm::rcbrt(x) : 0.0f;
-    o[6] = (x > 0.0f) ? fastm::film(k, x, y) : 0.0f;
+    o[6] = (x >= dry && y >= 0.0f) ? fastm::film(k, x, y) : 0.0f;  // its domain
   }
 }
 

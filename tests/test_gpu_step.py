@@ -8,7 +8,8 @@
   2e-5 momentum after 8 steps of dt = 2^-10; 5e-6 / 6e-7 after 64 steps on a
   bed 300 m above the datum;
 - the arithmetic probe bit for bit against the CPU (two-sum error term, IEEE
-  division and sqrt, the friction depth power, film drag).
+  division and sqrt, the friction depth power, film drag), and the fast
+  order's reciprocal against the IEEE division over its whole range.
 """
 
 from __future__ import annotations
@@ -79,6 +80,19 @@ def test_probe_is_bitwise_the_cpu(ctx) -> None:  # noqa: ANN001
     # The two-sum error term survives: the reference's EFT probe (solver.py:453-461).
     e = ctx.probe(np.array([[1.0, 1e-9]], np.float32), 9.81, 1e-3)[0, 0]
     assert e == np.float32(1e-9)
+
+
+def test_fast_reciprocal_is_the_ieee_division_on_its_range(ctx) -> None:  # noqa: ANN001
+    """The fast order's rcp(x) (hardware estimate and one fused Newton step)
+    is bitwise 1 / x for every float with 2^-126 <= |x| <= 2^126, both
+    signs, about 4.2e9 cases; below that range (subnormals) and above it it
+    is not, which is why every divisor it takes is kept inside."""
+    n, first = ctx.probe_rcp(0x00800000, 0x7E800001)
+    assert n == 0, f"{n} differ, first 0x{first:08x}"
+    n_sub, first_sub = ctx.probe_rcp(0x00000001, 0x00800000)
+    assert n_sub > 0 and first_sub == 0x00000001
+    n_big, first_big = ctx.probe_rcp(0x7E800001, 0x7F800000)
+    assert n_big > 0 and first_big == 0x7E800001
 
 
 EXACT = L.FLAG_EXACT_ORDER
