@@ -97,10 +97,12 @@ __host__ __device__ __forceinline__ uint32_t bi_scen(const BlockInfo& bi) { retu
 struct Args {
   Phys p;
   uint32_t N, NB, max_local, max_slots, max_ghost, n_src, track_clamp;
+  uint32_t n_lrec, n_lidx;  // entries of lrec / ldz and of lidx
   uint32_t src_neg;  // 1 when a source rate may be negative
   // cells
   float *h, *hu, *hv, *hlo, *h1, *hu1, *hv1, *h1lo, *area, *nman, *clamp, *src;
   float* inv_area;  // 1 / area, read by the fast order instead of area
+  const float* perim;  // each cell's edge lengths summed in CSR order
   const uint32_t* cptr;
   const uint16_t* lidx;
   const int32_t* snb;
@@ -226,6 +228,18 @@ template <typename T>
 __device__ __forceinline__ T ldg(const T* base, uint32_t idx) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (idx * (uint32_t)sizeof(T)));
 }
+// A load through the constant address space, which the compiler issues as a
+// scalar load when the index is uniform: no vector registers and no wait on
+// the vector memory counter. Only for data the running kernel never writes
+// (the scalar cache is invalidated at every kernel launch).
+__device__ __forceinline__ uint32_t ldc(const uint32_t* base, uint32_t idx) {
+  typedef const __attribute__((address_space(4))) uint32_t* cptr_t;
+  return ((cptr_t)base)[idx];
+}
+__device__ __forceinline__ BlockInfo ldc(const BlockInfo* base, uint32_t idx) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(base) + 8 * (size_t)idx;
+  return BlockInfo{ldc(w, 0), ldc(w, 1), ldc(w, 2), ldc(w, 3), ldc(w, 4), ldc(w, 5), ldc(w, 6), ldc(w, 7)};
+}
 template <typename T>
 __device__ __forceinline__ void stg(T* base, uint32_t idx, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (idx * (uint32_t)sizeof(T))) = v;
@@ -240,8 +254,9 @@ struct Lds {
   float4* cell;  // the cells, then the ghosts: [kBlockCells + max_ghost][2] in
                  // carve() (prologue); the stage kernels' first record only
   float2* cx;    // the stage kernels' fast-order (phi(h), sqrt h) per cell
-  float4* edge;  // [max_local][2]; the predictor's: [max_local] words 0-3
-  float2* e2;    // the predictor's [max_local] words 4-5 (carve_stage<0>)
+  float4* edge;  // [max_local][2] (prologue, adjoint); the corrector's side records
+  float2* e2;    // carve(): words 4-5 beside edge
+  float* ps;     // the predictor's side records, 3 words per CSR slot value
   uint16_t* s_slot;
 };
 
@@ -279,13 +294,15 @@ __device__ __forceinline__ Lds carve(float* sm, uint32_t max_local, uint32_t max
 // edge records after them stay 16-byte aligned.
 __host__ __device__ inline uint32_t stage_cells(uint32_t max_ghost) { return (kBlockCells + max_ghost + 1u) & ~1u; }
 
-// The predictor reads only an edge's six words, so its records are two
-// arrays (16 + 8 bytes per edge instead of 32): with less LDS per block,
-// one more workgroup fits on a CU.
+// The stage kernels keep an edge's words as one record per side, indexed by
+// the CSR slot value s = (j << 1) | right of local edge j, so each slot of
+// the gather reads its own contribution: mass, momentum x, momentum y as
+// (w0, w1, w2) on the left and (-w0, -w3, -w4) on the right. The predictor
+// keeps these 3 words per side (24 bytes per edge); the corrector adds the
+// wave speed term w5 to each (two 16-byte records, 32 bytes per edge).
 __host__ __device__ inline size_t lds_words_stage0(uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
   return 6 * (size_t)stage_cells(max_ghost) + 6 * (size_t)max_local + (max_slots + 1) / 2;
 }
-// The corrector keeps 32-byte edge records: words 0-5, sides word, length.
 __host__ __device__ inline size_t lds_words_stage1(uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
   return 6 * (size_t)stage_cells(max_ghost) + 8 * (size_t)max_local + (max_slots + 1) / 2;
 }
@@ -298,24 +315,24 @@ __device__ __forceinline__ Lds carve_stage(float* sm, uint32_t max_local, uint32
   l.cx = reinterpret_cast<float2*>(sm + 4 * kc);
   float* e = sm + 6 * kc;
   l.edge = reinterpret_cast<float4*>(e);
-  if (ST == 0) {
-    l.e2 = reinterpret_cast<float2*>(e + 4 * max_local);
-    l.s_slot = reinterpret_cast<uint16_t*>(e + 6 * max_local);
-  } else {
-    l.e2 = reinterpret_cast<float2*>(l.edge) + 2;
-    l.s_slot = reinterpret_cast<uint16_t*>(e + 8 * max_local);
-  }
+  l.e2 = nullptr;
+  l.ps = e;
+  l.s_slot = reinterpret_cast<uint16_t*>(e + (ST == 0 ? 6 : 8) * max_local);
   return l;
 }
 
-// Words 0-3 and 4-5 of local edge j in a stage kernel's layout.
+// The side records of local edge j from its six words w.
 template <int ST>
-__device__ __forceinline__ float4& ew4(const Lds& l, uint32_t j) {
-  return ST == 0 ? l.edge[j] : l.edge[2 * j];
-}
-template <int ST>
-__device__ __forceinline__ float2& ew2(const Lds& l, uint32_t j) {
-  return ST == 0 ? l.e2[j] : l.e2[4 * j];
+__device__ __forceinline__ void put_sides(const Lds& l, uint32_t j, const float w[6]) {
+  if (ST == 0) {
+    float2* q = reinterpret_cast<float2*>(l.ps + 6 * j);
+    q[0] = make_float2(w[0], w[1]);
+    q[1] = make_float2(w[2], -w[0]);
+    q[2] = make_float2(-w[3], -w[4]);
+  } else {
+    l.edge[2 * j] = make_float4(w[0], w[1], w[2], w[5]);
+    l.edge[2 * j + 1] = make_float4(-w[0], -w[3], -w[4], w[5]);
+  }
 }
 
 // One local edge record: sides word, normal, length and the signed bed step
@@ -325,29 +342,56 @@ struct EdgeRec {
   float nx, ny, len, d;
 };
 
-__device__ __forceinline__ EdgeRec load_edge(const Args& a, const BlockInfo& bi, uint32_t j) {
-  const uint4 q = ldg(a.lrec, bi.e0 + j);
+// The record at position q of the local edge streams.
+__device__ __forceinline__ EdgeRec load_edge_at(const Args& a, uint32_t q) {
+  const uint4 v = ldg(a.lrec, q);
   EdgeRec r;
-  r.side = q.x;
-  r.nx = __uint_as_float(q.y);
-  r.ny = __uint_as_float(q.z);
-  r.len = __uint_as_float(q.w);
-  r.d = ldg(a.ldz, bi.e0 + j);
+  r.side = v.x;
+  r.nx = __uint_as_float(v.y);
+  r.ny = __uint_as_float(v.z);
+  r.len = __uint_as_float(v.w);
+  r.d = ldg(a.ldz, q);
   return r;
 }
 
+__device__ __forceinline__ EdgeRec load_edge(const Args& a, const BlockInfo& bi, uint32_t j) {
+  return load_edge_at(a, bi.e0 + j);
+}
+
+__device__ __forceinline__ uint32_t umin(uint32_t x, uint32_t y) { return x < y ? x : y; }
+
 // Stage input of ghost cell g (a cell of another block). The corrector reads
 // the predictor state only where that cell's block was active this step
-// (reference resident.py:209-226); it loads that state first and the start
-// state only for a skipped block.
+// (reference resident.py:209-226); it loads the block's flag and both states
+// at once and keeps one, so no load waits for another.
 template <int ST, bool LO>
 __device__ __forceinline__ float4 load_ghost(const Args& a, uint32_t g) {
+  const float4 s0 = make_float4(ldg(a.h, g), ldg(a.hu, g), ldg(a.hv, g), LO ? ldg(a.hlo, g) : 0.0f);
+  if (ST == 0) return s0;
+  const uint32_t act = ldg(a.active, g >> 8);
+  const float4 s1 = make_float4(ldg(a.h1, g), ldg(a.hu1, g), ldg(a.hv1, g), LO ? ldg(a.h1lo, g) : 0.0f);
+  return (act != 0u) ? s1 : s0;
+}
+
+// Loads of a ghost's stage input, issued together; ghost_pick keeps one after
+// the loads of the block are all in flight.
+struct GhostLoads {
+  float4 s0, s1;
+  uint32_t act;
+};
+template <int ST, bool LO>
+__device__ __forceinline__ GhostLoads ghost_loads(const Args& a, uint32_t g) {
+  GhostLoads q;
+  q.s0 = make_float4(ldg(a.h, g), ldg(a.hu, g), ldg(a.hv, g), LO ? ldg(a.hlo, g) : 0.0f);
   if (ST == 1) {
-    const uint32_t act = ldg(a.active, g >> 8);
-    const float4 s = make_float4(ldg(a.h1, g), ldg(a.hu1, g), ldg(a.hv1, g), LO ? ldg(a.h1lo, g) : 0.0f);
-    if (act != 0u) return s;
+    q.act = ldg(a.active, g >> 8);
+    q.s1 = make_float4(ldg(a.h1, g), ldg(a.hu1, g), ldg(a.hv1, g), LO ? ldg(a.h1lo, g) : 0.0f);
   }
-  return make_float4(ldg(a.h, g), ldg(a.hu, g), ldg(a.hv, g), LO ? ldg(a.hlo, g) : 0.0f);
+  return q;
+}
+template <int ST>
+__device__ __forceinline__ float4 ghost_pick(const GhostLoads& q) {
+  return (ST == 1 && q.act != 0u) ? q.s1 : q.s0;
 }
 
 // Adjoint tape: a ghost cell whose block is not listed this step keeps its
@@ -459,8 +503,8 @@ __device__ __forceinline__ void open_pass(const Args& a, const Lds& l, uint32_t 
     const uint2 jk = a.open->jk[t];
     const float4* o = a.open->words + 2 * ((size_t)ST * a.n_open + jk.y);
     const float4 w0 = o[0], w1 = o[1];
-    ew4<ST>(l, jk.x) = w0;
-    ew2<ST>(l, jk.x) = make_float2(w1.x, w1.y);
+    const float w[6] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y};
+    put_sides<ST>(l, jk.x, w);
   }
 }
 
@@ -515,9 +559,7 @@ template <int ST, bool FM, bool LO>
 __device__ __forceinline__ void edge_eval(const Args& a, const Lds& l, const EdgeRec& r, uint32_t j) {
   float w[6];
   edge_words_lds<FM, LO>(a, l, r, w);
-  ew4<ST>(l, j) = make_float4(w[0], w[1], w[2], w[3]);
-  if (ST == 0) ew2<ST>(l, j) = make_float2(w[4], w[5]);
-  else l.edge[2 * j + 1] = make_float4(w[4], w[5], __uint_as_float(r.side), r.len);  // sides, length: source bound
+  put_sides<ST>(l, j, w);
 }
 
 // Unordered accumulation (the reference's atomic flux variant,
@@ -579,43 +621,56 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   const uint32_t n_loc = bi.n_loc;
 
   // ---- (A) independent loads -------------------------------------------
+  // One batch of loads, none waiting for another: a group of loads is
+  // skipped only by waves with no lane in its range (a branch uniform over
+  // the wave), and lanes past the block's edges, slots or ghosts read a
+  // valid neighbouring entry whose value is not used.
+  const uint32_t wave0 = __builtin_amdgcn_readfirstlane(tid & ~63u);  // the wave's first thread
+  // The ghost this thread loads: taken first, since its id came from the
+  // last block's loads (so no load of this batch waits behind it).
+  uint32_t gsrc = (tid < bi.n_ghost) ? gid : c0;
+  asm volatile("" : "+v"(gsrc));
+  // Walk ahead (scalar loads): the next block's BlockInfo and the list entry
+  // after it.
+  if (nx.t < end) nx.bi = ldc(a.binfo, nx.b);
+  if (nx.t + per < end) nx.b2 = ldc(a.list, nx.t + per);
   EdgeRec er[kEdgesAhead];
 #pragma unroll
   for (uint32_t m = 0; m < kEdgesAhead; ++m)
-    if (tid + m * kBlockCells < n_loc) er[m] = load_edge(a, bi, tid + m * kBlockCells);
+    if (wave0 + m * kBlockCells < n_loc) er[m] = load_edge_at(a, umin(bi.e0 + tid + m * kBlockCells, a.n_lrec - 1u));
   uint32_t sl[kSlotsAhead];
 #pragma unroll
   for (uint32_t m = 0; m < kSlotsAhead; ++m)
-    if (!ATOM && tid + m * kBlockCells < bi.n_slots) sl[m] = ldg(a.lidx, bi.s0 + tid + m * kBlockCells);
-  float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (tid < bi.n_ghost) gs = load_ghost<ST, LO>(a, gid);
-  const uint32_t i = c0 + tid;
+    if (!ATOM && wave0 + m * kBlockCells < bi.n_slots) sl[m] = ldg(a.lidx, umin(bi.s0 + tid + m * kBlockCells, a.n_lidx - 1u));
+  GhostLoads gq{};
+  if (wave0 < bi.n_ghost) gq = ghost_loads<ST, LO>(a, gsrc);
+  const uint32_t i = c0 + tid;  // the arrays are padded to whole blocks
   // aw: the cell's area, or its inverse in the fast order
-  float hi = 0.0f, hui = 0.0f, hvi = 0.0f, loi = 0.0f, aw = 1.0f, nm = 0.0f;
+  const float hi0 = ldg((ST == 0) ? a.h : a.h1, i);
+  const float hui0 = ldg((ST == 0) ? a.hu : a.hu1, i);
+  const float hvi0 = ldg((ST == 0) ? a.hv : a.hv1, i);
+  const float loi0 = COMP ? ldg((ST == 0) ? a.hlo : a.h1lo, i) : 0.0f;
+  const uint32_t k0 = ATOM ? 0u : ldg(a.cptr, i) - bi.s0;
+  const uint32_t k1 = ATOM ? 0u : ldg(a.cptr, i + 1) - bi.s0;
+  const float aw = ldg(FM ? a.inv_area : a.area, i);
+  const float nm = ldg(a.nman, i);
   float h0 = 0.0f, hu0 = 0.0f, hv0 = 0.0f, lo0 = 0.0f, rate = 0.0f;
-  uint32_t k0 = 0, k1 = 0;
-  if (tid < nc) {
-    hi = ldg((ST == 0) ? a.h : a.h1, i);
-    hui = ldg((ST == 0) ? a.hu : a.hu1, i);
-    hvi = ldg((ST == 0) ? a.hv : a.hv1, i);
-    if (COMP) loi = ldg((ST == 0) ? a.hlo : a.h1lo, i);
-    if (!ATOM) {
-      k0 = ldg(a.cptr, i) - bi.s0;
-      k1 = ldg(a.cptr, i + 1) - bi.s0;
-    }
-    aw = ldg(FM ? a.inv_area : a.area, i);
-    nm = ldg(a.nman, i);
-    if (ST == 1) {
-      h0 = ldg(a.h, i);
-      hu0 = ldg(a.hu, i);
-      hv0 = ldg(a.hv, i);
-      if (COMP) lo0 = ldg(a.hlo, i);
-      if (a.n_src != 0u) rate = ldg(a.src, i);
-    }
+  if (ST == 1) {
+    h0 = ldg(a.h, i);
+    hu0 = ldg(a.hu, i);
+    hv0 = ldg(a.hv, i);
+    if (COMP) lo0 = ldg(a.hlo, i);
+    if (a.n_src != 0u) rate = ldg(a.src, i);
   }
-  // Walk ahead: the next block's BlockInfo and the list entry after it.
-  if (nx.t < end) nx.bi = a.binfo[nx.b];
-  if (nx.t + per < end) nx.b2 = a.list[nx.t + per];
+  // Walk ahead, last in the batch so nothing below waits for it: this
+  // thread's ghost cell id of the next block, and its scenario's dt.
+  if (nx.t < end) {
+    if (tid < nx.bi.n_ghost) nx.gid = ldg(a.gx, nx.bi.g0 + tid);
+    nx.dtw = ldc(&a.scen[bi_scen(nx.bi)].dt_cur, 0u);
+  }
+  float hi = hi0, hui = hui0, hvi = hvi0;
+  const float loi = loi0;
+  const float4 gs = ghost_pick<ST>(gq);
   __syncthreads();  // the previous block's LDS reads are done
   if (tid < nc) put_cell<FM>(l, p, tid, make_float4(hi, hui, hvi, loi));
   if (tid < bi.n_ghost) {
@@ -652,11 +707,6 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     if (ATOM) edge_accumulate<ST, FM, LO>(a, l, load_edge(a, bi, j));
     else edge_eval<ST, FM, LO>(a, l, load_edge(a, bi, j), j);
   }
-  // Walk ahead: this thread's ghost cell id of the next block, its dt.
-  if (nx.t < end) {
-    if (tid < nx.bi.n_ghost) nx.gid = ldg(a.gx, nx.bi.g0 + tid);
-    nx.dtw = a.scen[bi_scen(nx.bi)].dt_cur;
-  }
   __syncthreads();
   if (a.n_open != 0u) {
     open_pass<ST>(a, l, b);
@@ -674,7 +724,9 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // The corrector's fast order takes the source dt bound of all-wet blocks
   // from the cells' perimeters.
   constexpr bool PERIM = ST == 1 && FM && !ATOM;
-  float perim = 0.0f;  // PERIM: the cell's edge lengths summed in CSR order
+  // The cell's perimeter, for the all-wet source bound at the end: loaded
+  // here, it arrives while the cell is updated.
+  const float perim = (PERIM && a.n_src != 0u) ? ldg(a.perim, i) : 0.0f;
   if (tid < nc) {
     float sdh = 0.0f, sdhu = 0.0f, sdhv = 0.0f, lam = 0.0f;
     if (ATOM) {
@@ -684,28 +736,22 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       sdhv = acc[2 * kBlockCells + tid];
       lam = acc[3 * kBlockCells + tid];
     }
+    // Each slot adds its side's record: a right side's words are stored
+    // negated, and x + (-y) is x - y bit for bit.
     for (uint32_t k = k0; !ATOM && k < k1; ++k) {
       const uint32_t s = l.s_slot[k];
-      const uint32_t j = s >> 1;
-      const float4 wa = ew4<ST>(l, j);
-      float2 wb;
-      if (PERIM) {
-        const float4 wr = l.edge[2 * j + 1];  // words 4 and 5, sides, length
-        wb = make_float2(wr.x, wr.y);
-        perim += wr.w;
+      if (ST == 1) {
+        const float4 w = l.edge[s];
+        sdh += w.x;
+        sdhu += w.y;
+        sdhv += w.z;
+        lam += w.w;
       } else {
-        wb = ew2<ST>(l, j);
+        const float* q = l.ps + 3 * s;
+        sdh += q[0];
+        sdhu += q[1];
+        sdhv += q[2];
       }
-      if (s & 1u) {
-        sdh -= wa.x;
-        sdhu -= wa.w;
-        sdhv -= wb.x;
-      } else {
-        sdh += wa.x;
-        sdhu += wa.y;
-        sdhv += wa.z;
-      }
-      if (ST == 1) lam += wb.y;
     }
     const float dti = FM ? dt * aw : dt / aw;
     float h_new, lo_new = 0.0f, h_raw;
@@ -817,8 +863,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       // Every cell of the block is wet after the sources, so no neighbour in
       // it is dry and each edge term of the bound below is (u + c) len: the
       // sum is (u + c) times the perimeter. A dry neighbour in another block
-      // is the source fix's, as below. The perimeter was summed beside the
-      // edge words above.
+      // is the source fix's, as below.
       if (tid < nc && rate != 0.0f) {
         float u, c;
         inu::fast::src_speeds(p, hi, hui, hvi, u, c);
@@ -838,14 +883,13 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
         float lam = 0.0f;
         for (uint32_t k = k0; k < k1; ++k) {
           const uint32_t s = l.s_slot[k];
-          const uint32_t j = s >> 1;
-          const float2 sd = reinterpret_cast<const float2*>(&l.edge[2 * j + 1])[1];  // sides, length
-          const uint32_t pk = __float_as_uint(sd.x);
-          const uint32_t q = (s & 1u) ? (pk >> 16) : (pk & 0xffffu);
+          const uint4 er = ldg(a.lrec, bi.e0 + (s >> 1));  // sides word, length (cached: read in phase A)
+          const uint32_t q = (s & 1u) ? (er.x >> 16) : (er.x & 0xffffu);
           const bool dry_side = q < kBlockCells && l.cell[q].x < p.dry;  // walls and ghosts: wet
           const float kf = dry_side ? 2.0f : 1.0f;
-          if (FM) lam = fmaf(fmaf(kf, c, u), sd.y, lam);
-          else lam += (u + kf * c) * sd.y;
+          const float len = __uint_as_float(er.w);
+          if (FM) lam = fmaf(fmaf(kf, c, u), len, lam);
+          else lam += (u + kf * c) * len;
         }
         if (lam > 1e-30f) lam_b = inu::gmax(lam_b, lam);
       }
@@ -859,14 +903,20 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
 // once (the start of a walk, and blocks skipped without processing).
 __device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t end, uint32_t per) {
   if (nx.t >= end) return;
-  nx.bi = a.binfo[nx.b];
+  nx.bi = ldc(a.binfo, nx.b);
   nx.gid = (threadIdx.x < nx.bi.n_ghost) ? a.gx[nx.bi.g0 + threadIdx.x] : 0u;
-  nx.dtw = a.scen[bi_scen(nx.bi)].dt_cur;
-  if (nx.t + per < end) nx.b2 = a.list[nx.t + per];
+  nx.dtw = ldc(&a.scen[bi_scen(nx.bi)].dt_cur, 0u);
+  if (nx.t + per < end) nx.b2 = ldc(a.list, nx.t + per);
 }
 
+// Workgroups per CU the tile-0 stage kernels are compiled for: the predictor
+// fits six in its registers; the corrector, which also holds the start state
+// and sources of its cells across the edge phase, five without spilling.
+template <int ST>
+constexpr int stage_wgs() { return ST == 0 ? 6 : 5; }
+
 template <int ST, bool COMP, bool FM, int TILE, bool REC = false, bool ATOM = false>
-__global__ __launch_bounds__(256, TILE == 0 ? 6 : 1) void k_stage(const Args a) {
+__global__ __launch_bounds__(256, TILE == 0 ? stage_wgs<ST>() : 1) void k_stage(const Args a) {
   extern __shared__ float sm[];
   if (a.ctl->any_dt == 0u) return;  // no-op step: every scenario is at its stop
   const Lds l = carve_stage<ST>(sm, a.max_local, a.max_ghost);
@@ -2085,7 +2135,9 @@ void launch_stage_t(inu_ctx* c, const Args& a, bool comp, bool fm) {
 // Unordered accumulation (INU_FLAG_DETERMINISTIC off), tile level 0.
 template <int ST>
 void launch_stage_atom(inu_ctx* c, const Args& a, bool comp, bool fm) {
-  const size_t lds = ST == 0 ? c->lds_stage0 : c->lds_stage1;
+  // The four per-cell sums (4 x 256 floats) take the edge records' place.
+  const size_t lds = std::max(ST == 0 ? c->lds_stage0 : c->lds_stage1,
+                              (6 * (size_t)stage_cells(a.max_ghost) + 4 * (size_t)kBlockCells) * sizeof(float));
   const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0][0]), b(256);
   if (comp && fm) hipLaunchKernelGGL((k_stage<ST, true, true, 0, false, true>), g, b, lds, c->stream, a);
   else if (comp) hipLaunchKernelGGL((k_stage<ST, true, false, 0, false, true>), g, b, lds, c->stream, a);
@@ -2244,6 +2296,7 @@ struct Layout {
   std::vector<uint16_t> lidx;
   std::vector<int32_t> snb;
   std::vector<float> slen;
+  std::vector<float> perim;  // per cell: its slots' edge lengths summed in CSR order
   std::vector<std::pair<uint32_t, uint32_t>> cross;
   std::vector<uint32_t> bptr, bidx;
   std::vector<uint32_t> stream_edge;  // local edge stream position -> edge
@@ -2383,6 +2436,7 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
   snb.assign(S, 0);
   std::vector<float>& slen = L.slen;
   slen.assign(S, 0.0f);
+  L.perim.assign(N, 0.0f);
   std::vector<std::pair<uint32_t, uint32_t>>& cross = L.cross;
   cross.clear();
   for (uint32_t i = 0; i < N; ++i) {
@@ -2407,6 +2461,7 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
       lidx[k] = (uint16_t)((j << 1) | right);
       snb[k] = right ? edge_cell_l[e] : edge_cell_r[e];
       slen[k] = edge_len[e];
+      L.perim[i] += edge_len[e];
       if (snb[k] >= 0 && (uint32_t)snb[k] / kBlockCells != b) cross.emplace_back(i, (uint32_t)snb[k]);
     }
   }
@@ -2510,6 +2565,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   std::vector<uint32_t> bptr(NB + 1, 0), bidx;
   bidx.reserve(ring_tot);
   std::vector<float> area(NP, 0.0f);
+  std::vector<float> perim(NP, 0.0f);
   // The fast order's inverse areas: 1 / A of every real cell (the same float
   // division as the oracle's), 0 in the padding.
   std::vector<float> inv_area(NP, 0.0f);
@@ -2553,6 +2609,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
     slen.insert(slen.end(), L.slen.begin(), L.slen.end());
     for (const auto& pr : L.cross) cross.emplace_back(c0 + pr.first, c0 + pr.second);
     std::copy(m.area, m.area + L.N, area.begin() + c0);
+    std::copy(L.perim.begin(), L.perim.end(), perim.begin() + c0);
     for (uint32_t i = 0; i < L.N; ++i) inv_area[c0 + i] = 1.0f / m.area[i];
     s_off += L.S;
   }
@@ -2576,6 +2633,8 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   rc |= dalloc(c, &a.h1, NP); rc |= dalloc(c, &a.hu1, NP); rc |= dalloc(c, &a.hv1, NP); rc |= dalloc(c, &a.h1lo, NP);
   rc |= dalloc(c, &a.area, NP); rc |= dalloc(c, &a.nman, NP); rc |= dalloc(c, &a.clamp, NP); rc |= dalloc(c, &a.src, NP);
   rc |= dalloc(c, &a.inv_area, NP);
+  float* dperim;
+  rc |= dalloc(c, &dperim, NP);
   uint32_t* dcptr; uint16_t* dlid; int32_t* dsn; float* dsl;
   rc |= dalloc(c, &dcptr, NP + 1); rc |= dalloc(c, &dlid, lidx.size()); rc |= dalloc(c, &dsn, snb.size());
   rc |= dalloc(c, &dsl, slen.size());
@@ -2624,7 +2683,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
     for (uint32_t cl : lay[scen_mesh[0]].open_cell) c->open_cell_h.push_back(cl);  // one scenario: no offset
   }
   if (rc) return rc;
-  a.cptr = dcptr; a.lidx = dlid; a.snb = dsn; a.slen = dsl;
+  a.cptr = dcptr; a.lidx = dlid; a.snb = dsn; a.slen = dsl; a.perim = dperim;
   a.lrec = dlrec; a.ldz = dldz;
   a.binfo = dbinfo; a.gx = dgx; a.bptr = dbptr; a.bidx = dbidx;
   a.src_cell = src_cell; a.src_blocks = src_blocks; a.src_pair = src_pair; a.n_src_pairs = 0;
@@ -2633,6 +2692,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   a.scen = c->scen;
   a.n_scen = n_scen;
   a.N = (uint32_t)NP; a.NB = NB; a.max_local = max_local; a.max_slots = max_slots; a.max_ghost = max_ghost;
+  a.n_lrec = (uint32_t)std::max<size_t>(lrec.size(), 1); a.n_lidx = (uint32_t)std::max<size_t>(lidx.size(), 1);
   HIPCHK(c, hipMemsetAsync(a.h, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hu, 0, NP * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.hv, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hlo, 0, NP * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.h1, 0, NP * 4, c->stream)); HIPCHK(c, hipMemsetAsync(a.hu1, 0, NP * 4, c->stream));
@@ -2666,6 +2726,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   if ((rc = upload(c, a.nman, nm.data(), NP))) return rc;
   if ((rc = upload(c, a.area, area.data(), NP))) return rc;
   if ((rc = upload(c, a.inv_area, inv_area.data(), NP))) return rc;
+  if ((rc = upload(c, dperim, perim.data(), NP))) return rc;
   if ((rc = upload(c, dcptr, cptr.data(), NP + 1))) return rc;
   if ((rc = upload(c, dlid, lidx.data(), lidx.size()))) return rc;
   if ((rc = upload(c, dsn, snb.data(), snb.size()))) return rc;
@@ -2719,6 +2780,10 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
           int occ = 0;
           HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, variants[st][cp][fm][tl], 256,
                                                                   st ? c->lds_stage1 : c->lds_stage0));
+          // Tile 0: no more workgroups per CU than the kernel is compiled
+          // for (a seventh predictor workgroup fits in its registers and
+          // LDS but adds memory waits without adding throughput).
+          if (tl == 0) occ = std::min(occ, st ? stage_wgs<1>() : stage_wgs<0>());
           uint32_t g = (uint32_t)(std::max(occ, 1) * c->cus);
           g = std::max<uint32_t>(kXcds, std::min<uint32_t>(g, ((NB + kXcds - 1) / kXcds) * kXcds));
           c->grid[st][cp][fm][tl] = (g / kXcds) * kXcds;
