@@ -102,7 +102,6 @@ struct Args {
   // cells
   float *h, *hu, *hv, *hlo, *h1, *hu1, *hv1, *h1lo, *area, *nman, *clamp, *src;
   float* inv_area;  // 1 / area, read by the fast order instead of area
-  const float* perim;  // each cell's edge lengths summed in CSR order
   const uint32_t* cptr;
   const uint16_t* lidx;
   const int32_t* snb;
@@ -257,6 +256,7 @@ struct Lds {
   float4* edge;  // [max_local][2] (prologue, adjoint); the corrector's side records
   float2* e2;    // carve(): words 4-5 beside edge
   float* ps;     // the predictor's side records, 3 words per CSR slot value
+  float* elen;   // the corrector's edge lengths, for cell perimeters
   uint16_t* s_slot;
 };
 
@@ -299,12 +299,14 @@ __host__ __device__ inline uint32_t stage_cells(uint32_t max_ghost) { return (kB
 // the gather reads its own contribution: mass, momentum x, momentum y as
 // (w0, w1, w2) on the left and (-w0, -w3, -w4) on the right. The predictor
 // keeps these 3 words per side (24 bytes per edge); the corrector adds the
-// wave speed term w5 to each (two 16-byte records, 32 bytes per edge).
+// wave speed term w5 to each (two 16-byte records, 32 bytes per edge) and
+// keeps each edge's length beside them, from which the all-wet source bound
+// sums a cell's perimeter.
 __host__ __device__ inline size_t lds_words_stage0(uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
   return 6 * (size_t)stage_cells(max_ghost) + 6 * (size_t)max_local + (max_slots + 1) / 2;
 }
 __host__ __device__ inline size_t lds_words_stage1(uint32_t max_local, uint32_t max_slots, uint32_t max_ghost) {
-  return 6 * (size_t)stage_cells(max_ghost) + 8 * (size_t)max_local + (max_slots + 1) / 2;
+  return 6 * (size_t)stage_cells(max_ghost) + 9 * (size_t)max_local + (max_slots + 1) / 2;
 }
 
 template <int ST>
@@ -317,7 +319,8 @@ __device__ __forceinline__ Lds carve_stage(float* sm, uint32_t max_local, uint32
   l.edge = reinterpret_cast<float4*>(e);
   l.e2 = nullptr;
   l.ps = e;
-  l.s_slot = reinterpret_cast<uint16_t*>(e + (ST == 0 ? 6 : 8) * max_local);
+  l.elen = e + 8 * max_local;  // the corrector's only
+  l.s_slot = reinterpret_cast<uint16_t*>(e + (ST == 0 ? 6 : 9) * max_local);
   return l;
 }
 
@@ -554,12 +557,49 @@ __device__ __forceinline__ void src_uc(const Phys& p, float h, float hu, float h
   }
 }
 
-// Words of local edge j into LDS, for the fixed-order gather.
+// The side records of local edge j: left lw and right rw (mass, momentum x,
+// momentum y, wave speed term), the right side's first three negated.
+template <int ST>
+__device__ __forceinline__ void put_sides4(const Lds& l, uint32_t j, float4 lw, float4 rw) {
+  if (ST == 0) {
+    float2* q = reinterpret_cast<float2*>(l.ps + 6 * j);
+    q[0] = make_float2(lw.x, lw.y);
+    q[1] = make_float2(lw.z, rw.x);
+    q[2] = make_float2(rw.y, rw.z);
+  } else {
+    l.edge[2 * j] = lw;
+    l.edge[2 * j + 1] = rw;
+  }
+}
+
+// Words of local edge j into LDS, for the fixed-order gather. In the fast
+// order a wall reads its left cell twice (its right side is that cell's
+// mirror), and an edge dry on both sides stores zero words (the negated
+// right words as -0, as negating zeros would give).
 template <int ST, bool FM, bool LO>
 __device__ __forceinline__ void edge_eval(const Args& a, const Lds& l, const EdgeRec& r, uint32_t j) {
-  float w[6];
-  edge_words_lds<FM, LO>(a, l, r, w);
-  put_sides<ST>(l, j, w);
+  if (ST == 1) l.elen[j] = r.len;
+  if (!FM) {
+    float w[6];
+    edge_words_lds<FM, LO>(a, l, r, w);
+    put_sides<ST>(l, j, w);
+    return;
+  }
+  const uint32_t ql = r.side >> 16;
+  const bool wall = (r.side & 0xffffu) == kSideWall;
+  float4 sl = l.cell[ql], sr = l.cell[wall ? ql : (r.side & 0xffffu)];
+  float2 xl = l.cx[ql], xr = l.cx[wall ? ql : (r.side & 0xffffu)];
+  // Both records of each side in whole reads, before the dry test.
+  asm volatile("" : "+v"(sl.w), "+v"(sr.w), "+v"(xl.x), "+v"(xl.y), "+v"(xr.x), "+v"(xr.y));
+  const inu::fast::Cell L{sl.x, sl.y, sl.z, sl.w, xl.x, xl.y};
+  const inu::fast::Cell R{sr.x, sr.y, sr.z, sr.w, xr.x, xr.y};
+  if (L.h < a.p.dry && R.h < a.p.dry) {
+    put_sides4<ST>(l, j, make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(-0.0f, -0.0f, -0.0f, 0.0f));
+    return;
+  }
+  float4 lw, rw;
+  inu::fast::edge_sides(a.p, wall, L, R, wall, r.d, r.nx, r.ny, r.len, lw, rw);
+  put_sides4<ST>(l, j, lw, rw);
 }
 
 // Unordered accumulation (the reference's atomic flux variant,
@@ -724,9 +764,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // The corrector's fast order takes the source dt bound of all-wet blocks
   // from the cells' perimeters.
   constexpr bool PERIM = ST == 1 && FM && !ATOM;
-  // The cell's perimeter, for the all-wet source bound at the end: loaded
-  // here, it arrives while the cell is updated.
-  const float perim = (PERIM && a.n_src != 0u) ? ldg(a.perim, i) : 0.0f;
+  float perim = 0.0f;  // PERIM with sources: the cell's edge lengths summed in CSR order
   if (tid < nc) {
     float sdh = 0.0f, sdhu = 0.0f, sdhv = 0.0f, lam = 0.0f;
     if (ATOM) {
@@ -738,7 +776,18 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     }
     // Each slot adds its side's record: a right side's words are stored
     // negated, and x + (-y) is x - y bit for bit.
-    for (uint32_t k = k0; !ATOM && k < k1; ++k) {
+    if (PERIM && a.n_src != 0u) {
+      for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t s = l.s_slot[k];
+        const float4 w = l.edge[s];
+        sdh += w.x;
+        sdhu += w.y;
+        sdhv += w.z;
+        lam += w.w;
+        perim += l.elen[s >> 1];
+      }
+    }
+    for (uint32_t k = k0; !ATOM && !(PERIM && a.n_src != 0u) && k < k1; ++k) {
       const uint32_t s = l.s_slot[k];
       if (ST == 1) {
         const float4 w = l.edge[s];
@@ -2296,7 +2345,6 @@ struct Layout {
   std::vector<uint16_t> lidx;
   std::vector<int32_t> snb;
   std::vector<float> slen;
-  std::vector<float> perim;  // per cell: its slots' edge lengths summed in CSR order
   std::vector<std::pair<uint32_t, uint32_t>> cross;
   std::vector<uint32_t> bptr, bidx;
   std::vector<uint32_t> stream_edge;  // local edge stream position -> edge
@@ -2436,7 +2484,6 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
   snb.assign(S, 0);
   std::vector<float>& slen = L.slen;
   slen.assign(S, 0.0f);
-  L.perim.assign(N, 0.0f);
   std::vector<std::pair<uint32_t, uint32_t>>& cross = L.cross;
   cross.clear();
   for (uint32_t i = 0; i < N; ++i) {
@@ -2461,7 +2508,6 @@ int build_layout(inu_ctx* c, const MeshIn& m, Layout& L) {
       lidx[k] = (uint16_t)((j << 1) | right);
       snb[k] = right ? edge_cell_l[e] : edge_cell_r[e];
       slen[k] = edge_len[e];
-      L.perim[i] += edge_len[e];
       if (snb[k] >= 0 && (uint32_t)snb[k] / kBlockCells != b) cross.emplace_back(i, (uint32_t)snb[k]);
     }
   }
@@ -2565,7 +2611,6 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   std::vector<uint32_t> bptr(NB + 1, 0), bidx;
   bidx.reserve(ring_tot);
   std::vector<float> area(NP, 0.0f);
-  std::vector<float> perim(NP, 0.0f);
   // The fast order's inverse areas: 1 / A of every real cell (the same float
   // division as the oracle's), 0 in the padding.
   std::vector<float> inv_area(NP, 0.0f);
@@ -2609,7 +2654,6 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
     slen.insert(slen.end(), L.slen.begin(), L.slen.end());
     for (const auto& pr : L.cross) cross.emplace_back(c0 + pr.first, c0 + pr.second);
     std::copy(m.area, m.area + L.N, area.begin() + c0);
-    std::copy(L.perim.begin(), L.perim.end(), perim.begin() + c0);
     for (uint32_t i = 0; i < L.N; ++i) inv_area[c0 + i] = 1.0f / m.area[i];
     s_off += L.S;
   }
@@ -2633,8 +2677,6 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   rc |= dalloc(c, &a.h1, NP); rc |= dalloc(c, &a.hu1, NP); rc |= dalloc(c, &a.hv1, NP); rc |= dalloc(c, &a.h1lo, NP);
   rc |= dalloc(c, &a.area, NP); rc |= dalloc(c, &a.nman, NP); rc |= dalloc(c, &a.clamp, NP); rc |= dalloc(c, &a.src, NP);
   rc |= dalloc(c, &a.inv_area, NP);
-  float* dperim;
-  rc |= dalloc(c, &dperim, NP);
   uint32_t* dcptr; uint16_t* dlid; int32_t* dsn; float* dsl;
   rc |= dalloc(c, &dcptr, NP + 1); rc |= dalloc(c, &dlid, lidx.size()); rc |= dalloc(c, &dsn, snb.size());
   rc |= dalloc(c, &dsl, slen.size());
@@ -2683,7 +2725,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
     for (uint32_t cl : lay[scen_mesh[0]].open_cell) c->open_cell_h.push_back(cl);  // one scenario: no offset
   }
   if (rc) return rc;
-  a.cptr = dcptr; a.lidx = dlid; a.snb = dsn; a.slen = dsl; a.perim = dperim;
+  a.cptr = dcptr; a.lidx = dlid; a.snb = dsn; a.slen = dsl;
   a.lrec = dlrec; a.ldz = dldz;
   a.binfo = dbinfo; a.gx = dgx; a.bptr = dbptr; a.bidx = dbidx;
   a.src_cell = src_cell; a.src_blocks = src_blocks; a.src_pair = src_pair; a.n_src_pairs = 0;
@@ -2726,7 +2768,6 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   if ((rc = upload(c, a.nman, nm.data(), NP))) return rc;
   if ((rc = upload(c, a.area, area.data(), NP))) return rc;
   if ((rc = upload(c, a.inv_area, inv_area.data(), NP))) return rc;
-  if ((rc = upload(c, dperim, perim.data(), NP))) return rc;
   if ((rc = upload(c, dcptr, cptr.data(), NP + 1))) return rc;
   if ((rc = upload(c, dlid, lidx.data(), lidx.size()))) return rc;
   if ((rc = upload(c, dsn, snb.data(), snb.size()))) return rc;

@@ -471,7 +471,10 @@ __device__ __forceinline__ void hllc(const Phys& p, const Face& f, float& Fh, fl
   const float FR_h = hR * unR;
   const float FR_hn = fmaf(FR_h, unR, p.half_g * hR * hR);
   const float FR_ht = FR_h * utR;
-  const bool left = SM >= 0.0f;
+  // The side whose state and flux the face takes: the left one when SL >= 0
+  // (the face is left of the fan) or SM >= 0 (left of the contact), else
+  // the right one. With SL < 0 this is the star state's side.
+  const bool left = SM >= 0.0f || SL >= 0.0f;
   const float S = left ? SL : SR;
   const float hS = left ? hL : hR;
   const float unS = left ? unL : unR;
@@ -488,15 +491,12 @@ __device__ __forceinline__ void hllc(const Phys& p, const Face& f, float& Fh, fl
   const float ks_q = (hS * (dS_ok ? SM - unS : S - unS)) * rcp(dS_ok ? rcp_range(dS) : 1e-14f);
   const float ks_h = dS_ok ? ks_q : ks_q - hS;
   const float kS = hS + ks_h;
-  if (SL >= 0.0f) {
-    Fh = FL_h; Fhn = FL_hn; Fht = FL_ht;
-  } else if (!left && !(SR >= 0.0f)) {
-    Fh = FR_h; Fhn = FR_hn; Fht = FR_ht;
-  } else {
-    Fh = fmaf(S, ks_h, FS_h);
-    Fhn = fmaf(S, fmaf(kS, SM, -(hS * unS)), FS_hn);
-    Fht = fmaf(S, ks_h * utS, FS_ht);
-  }
+  // Toro's four regions: F_L for SL >= 0, F_R for SR < 0 <= ... (right of
+  // the fan), else the star flux of the chosen side.
+  const bool outside = SL >= 0.0f || (!left && !(SR >= 0.0f));
+  Fh = outside ? FS_h : fmaf(S, ks_h, FS_h);
+  Fhn = outside ? FS_hn : fmaf(S, fmaf(kS, SM, -(hS * unS)), FS_hn);
+  Fht = outside ? FS_ht : fmaf(S, ks_h * utS, FS_ht);
 }
 
 // The edge's six words (reference resident.py:165-184 with flux.py:98-169,
@@ -523,6 +523,23 @@ __device__ __forceinline__ void edge_words(const Phys& p, bool wall, const Cell&
   w[3] = fmaf(nR, nx, -ty) * len;
   w[4] = fmaf(nR, ny, tx) * len;
   w[5] = Smax * len;
+}
+
+// The edge's words as the stage kernels keep them, one record per side:
+// left (w0, w1, w2, w5) and right (-w0, -w3, -w4, w5), the negated words
+// formed directly (a product with -len, the same bits as negating w).
+__device__ __forceinline__ void edge_sides(const Phys& p, bool wall, const Cell& L, const Cell& R, bool mirror,
+                                           float d, float nx, float ny, float len, float4& lw, float4& rw) {
+  const Face f = face(p, L, R, mirror, d, nx, ny);
+  float Fh, Fhn, Fht, Smax;
+  hllc(p, f, Fh, Fhn, Fht, Smax);
+  if (wall) Fh = 0.0f;
+  const float nL = (Fhn - p.half_g * f.hL * f.hL) + p.half_g * L.h * L.h;
+  const float nR = (Fhn - p.half_g * f.hR * f.hR) + p.half_g * R.h * R.h;
+  const float tx = Fht * nx, ty = Fht * ny;
+  const float w5 = Smax * len;
+  lw = make_float4(Fh * len, fmaf(nL, nx, -ty) * len, fmaf(nL, ny, tx) * len, w5);
+  rw = make_float4(Fh * -len, fmaf(nR, nx, -ty) * -len, fmaf(nR, ny, tx) * -len, w5);
 }
 
 // The edge's CFL wave speed, max(|SL|, |SR|), from the same face states and
