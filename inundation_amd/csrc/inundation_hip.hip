@@ -376,25 +376,14 @@ __device__ __forceinline__ float4 load_ghost(const Args& a, uint32_t g) {
   return (act != 0u) ? s1 : s0;
 }
 
-// Loads of a ghost's stage input, issued together; ghost_pick keeps one after
-// the loads of the block are all in flight.
-struct GhostLoads {
-  float4 s0, s1;
-  uint32_t act;
-};
-template <int ST, bool LO>
-__device__ __forceinline__ GhostLoads ghost_loads(const Args& a, uint32_t g) {
-  GhostLoads q;
-  q.s0 = make_float4(ldg(a.h, g), ldg(a.hu, g), ldg(a.hv, g), LO ? ldg(a.hlo, g) : 0.0f);
-  if (ST == 1) {
-    q.act = ldg(a.active, g >> 8);
-    q.s1 = make_float4(ldg(a.h1, g), ldg(a.hu1, g), ldg(a.hv1, g), LO ? ldg(a.h1lo, g) : 0.0f);
-  }
-  return q;
+// A cell's step-start (h, hu, hv, lo) and predictor-stage state.
+template <bool LO>
+__device__ __forceinline__ float4 state0(const Args& a, uint32_t g) {
+  return make_float4(ldg(a.h, g), ldg(a.hu, g), ldg(a.hv, g), LO ? ldg(a.hlo, g) : 0.0f);
 }
-template <int ST>
-__device__ __forceinline__ float4 ghost_pick(const GhostLoads& q) {
-  return (ST == 1 && q.act != 0u) ? q.s1 : q.s0;
+template <bool LO>
+__device__ __forceinline__ float4 state1(const Args& a, uint32_t g) {
+  return make_float4(ldg(a.h1, g), ldg(a.hu1, g), ldg(a.hv1, g), LO ? ldg(a.h1lo, g) : 0.0f);
 }
 
 // Adjoint tape: a ghost cell whose block is not listed this step keeps its
@@ -670,6 +659,11 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // last block's loads (so no load of this batch waits behind it).
   uint32_t gsrc = (tid < bi.n_ghost) ? gid : c0;
   asm volatile("" : "+v"(gsrc));
+  // The corrector reads a ghost's predictor state only where its block was
+  // active this step (reference resident.py:209-226): the flag is the
+  // batch's first load, so the rare start-state load it calls for, issued
+  // last, waits on it alone.
+  const uint32_t gact = (ST == 1) ? ldg(a.active, gsrc >> 8) : 1u;
   // Walk ahead (scalar loads): the next block's BlockInfo and the list entry
   // after it.
   if (nx.t < end) nx.bi = ldc(a.binfo, nx.b);
@@ -682,8 +676,10 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
 #pragma unroll
   for (uint32_t m = 0; m < kSlotsAhead; ++m)
     if (!ATOM && wave0 + m * kBlockCells < bi.n_slots) sl[m] = ldg(a.lidx, umin(bi.s0 + tid + m * kBlockCells, a.n_lidx - 1u));
-  GhostLoads gq{};
-  if (wave0 < bi.n_ghost) gq = ghost_loads<ST, LO>(a, gsrc);
+  // The ghost's stage input: the predictor state in the corrector (its
+  // block's flag, loaded first, picks the start state instead below).
+  float4 gs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (wave0 < bi.n_ghost) gs = (ST == 1) ? state1<LO>(a, gsrc) : state0<LO>(a, gsrc);
   const uint32_t i = c0 + tid;  // the arrays are padded to whole blocks
   // aw: the cell's area, or its inverse in the fast order
   const float hi0 = ldg((ST == 0) ? a.h : a.h1, i);
@@ -708,9 +704,9 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     if (tid < nx.bi.n_ghost) nx.gid = ldg(a.gx, nx.bi.g0 + tid);
     nx.dtw = ldc(&a.scen[bi_scen(nx.bi)].dt_cur, 0u);
   }
+  if (ST == 1 && tid < bi.n_ghost && gact == 0u) gs = state0<LO>(a, gsrc);
   float hi = hi0, hui = hui0, hvi = hvi0;
   const float loi = loi0;
-  const float4 gs = ghost_pick<ST>(gq);
   __syncthreads();  // the previous block's LDS reads are done
   if (tid < nc) put_cell<FM>(l, p, tid, make_float4(hi, hui, hvi, loi));
   if (tid < bi.n_ghost) {
