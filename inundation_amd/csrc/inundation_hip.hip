@@ -103,6 +103,9 @@ struct Args {
   float *h, *hu, *hv, *hlo, *h1, *hu1, *hv1, *h1lo, *area, *nman, *clamp, *src;
   float* inv_area;  // 1 / area, read by the fast order instead of area
   const uint32_t* cptr;
+  // per cell: the end of its CSR slots within its block's slots (the stage
+  // kernels' 16-bit copy of cptr[i + 1] - the block's first slot)
+  const uint16_t* lend;
   const uint16_t* lidx;
   const int32_t* snb;
   const float* slen;
@@ -686,8 +689,10 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   const float hui0 = ldg((ST == 0) ? a.hu : a.hu1, i);
   const float hvi0 = ldg((ST == 0) ? a.hv : a.hv1, i);
   const float loi0 = COMP ? ldg((ST == 0) ? a.hlo : a.h1lo, i) : 0.0f;
-  const uint32_t k0 = ATOM ? 0u : ldg(a.cptr, i) - bi.s0;
-  const uint32_t k1 = ATOM ? 0u : ldg(a.cptr, i + 1) - bi.s0;
+  // The cell's slots [k0, k1) within the block's: 16-bit ends, the first
+  // cell's start is 0.
+  const uint32_t k1 = ATOM ? 0u : (uint32_t)ldg(a.lend, i);
+  const uint32_t k0 = (ATOM || tid == 0) ? 0u : (uint32_t)ldg(a.lend, i - 1u);
   const float aw = ldg(FM ? a.inv_area : a.area, i);
   const float nm = ldg(a.nman, i);
   float h0 = 0.0f, hu0 = 0.0f, hv0 = 0.0f, lo0 = 0.0f, rate = 0.0f;
@@ -2765,6 +2770,14 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   if ((rc = upload(c, a.area, area.data(), NP))) return rc;
   if ((rc = upload(c, a.inv_area, inv_area.data(), NP))) return rc;
   if ((rc = upload(c, dcptr, cptr.data(), NP + 1))) return rc;
+  {
+    std::vector<uint16_t> lend(NP, 0);
+    for (size_t i = 0; i < NP; ++i) lend[i] = (uint16_t)(cptr[i + 1] - binfo[i / kBlockCells].s0);
+    uint16_t* dlend;
+    if ((rc = dalloc(c, &dlend, NP))) return rc;
+    if ((rc = upload(c, dlend, lend.data(), NP))) return rc;
+    a.lend = dlend;
+  }
   if ((rc = upload(c, dlid, lidx.data(), lidx.size()))) return rc;
   if ((rc = upload(c, dsn, snb.data(), snb.size()))) return rc;
   if ((rc = upload(c, dsl, slen.data(), slen.size()))) return rc;
