@@ -212,13 +212,19 @@ __device__ __forceinline__ void add_compensated(float hi, float lo, float inc, f
   }
 }
 
+// floor(e / 3) for an exponent e in [-149, 128], branch-free: e + 387 is
+// positive and below 2^17 / 3, where (x * 43691) >> 17 is floor(x / 3).
+__host__ __device__ __forceinline__ int floor_div3(int e) {
+  return (int)(((uint32_t)(e + 387) * 43691u) >> 17) - 129;
+}
+
 // x^(1/3) for positive normal x, from exact exponent splitting and two Halley
 // steps on the mantissa in [1, 8). Only IEEE operations, so the result is the
 // same on every conforming device; within 2 ulp of the true cube root.
 __device__ __forceinline__ float cbrt_ieee(float x) {
   const uint32_t u = __float_as_uint(x);
   const int e = (int)((u >> 23) & 0xffu) - 127;
-  const int k = (e >= 0) ? e / 3 : -((2 - e) / 3);
+  const int k = floor_div3(e);
   const int r = e - 3 * k;  // 0, 1 or 2
   const float m = __uint_as_float((u & 0x7fffffu) | ((uint32_t)(127 + r) << 23));
   float y = 0.80169147f + m * (0.24778609f - m * 0.0127262f);  // cbrt on [1, 8), 3.7 %
@@ -566,7 +572,7 @@ __device__ __forceinline__ float film_drag(const Phys& p, float h, float dt) {
 __device__ __forceinline__ float rcbrt(float x) {
   const uint32_t u = __float_as_uint(x);
   const int e = (int)((u >> 23) & 0xffu) - 127;
-  const int k = (e >= 0) ? e / 3 : -((2 - e) / 3);
+  const int k = floor_div3(e);
   const int r = e - 3 * k;
   const float m = __uint_as_float((u & 0x7fffffu) | ((uint32_t)(127 + r) << 23));
   float y = fmaf(fmaf(fmaf(-0.00191212f, m, 0.03603631f), m, -0.25165805f), m, 1.17793f);
