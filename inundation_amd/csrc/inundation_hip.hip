@@ -47,6 +47,7 @@ namespace {
 
 constexpr uint32_t kBlockCells = 256;  // ACTIVE_BLOCK_CELLS; one workgroup per block
 constexpr uint32_t kXcds = 8;          // MI355X: 8 accelerator complex dies
+constexpr uint32_t kWalkStride = 32;    // words between walk counters (one 128-byte line each)
 constexpr float kTicksPerSecond = 1048576.0f;
 constexpr float kDtSentinel = 1e30f;
 constexpr uint32_t kNoBound = 0x7f800000u;  // +inf bits: larger than any bound
@@ -138,6 +139,10 @@ struct Args {
   uint32_t n_src_blocks, n_src_pairs, n_fix;
   Ctl* ctl;
   Scen* scen;
+  // The stage kernels' walk counters: per stage, kXcds slice counters and a
+  // count of finished workgroups, kWalkStride words apart; zero between
+  // launches.
+  uint32_t* walk;
   uint32_t n_scen;
   // open boundary edges (device record, read only where they are handled)
   const struct OpenDev* open;
@@ -631,7 +636,17 @@ struct Ahead {
   BlockInfo bi;  // its BlockInfo
   uint32_t gid;  // its ghost cell id of this thread
   uint32_t dtw;  // its scenario's dt bits
+  uint32_t t2;   // list position of the block after it (>= end: none)
   uint32_t b2;   // block after it
+};
+
+// A workgroup's share of the walk: the list positions [beg, end) of its
+// slice, the slice's counter of positions taken, and an LDS word that passes
+// a taken position from the thread that took it to the workgroup.
+struct Walk {
+  uint32_t beg, end;
+  uint32_t* ctr;
+  uint32_t* s_pos;
 };
 
 // One active block of one Heun stage (reference resident.py:425-512, gather +
@@ -642,7 +657,7 @@ struct Ahead {
 // LDS; (C) each cell sums its edges in CSR order from LDS and updates.
 template <int ST, bool COMP, bool FM, int TILE, bool REC, bool ATOM>
 __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, uint32_t gid, float dt,
-                                const Lds& l, Ahead& nx, uint32_t end, uint32_t per) {
+                                const Lds& l, Ahead& nx, const Walk& wk) {
   constexpr bool LO = COMP && FM;  // the fast order's face depths read the low words
   constexpr uint32_t kEdgesAhead = edges_ahead(TILE);
   constexpr uint32_t kSlotsAhead = slots_ahead(TILE);
@@ -667,10 +682,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // batch's first load, so the rare start-state load it calls for, issued
   // last, waits on it alone.
   const uint32_t gact = (ST == 1) ? ldg(a.active, gsrc >> 8) : 1u;
-  // Walk ahead (scalar loads): the next block's BlockInfo and the list entry
-  // after it.
-  if (nx.t < end) nx.bi = ldc(a.binfo, nx.b);
-  if (nx.t + per < end) nx.b2 = ldc(a.list, nx.t + per);
+  // Walk ahead (scalar loads): the next block's BlockInfo.
+  if (nx.t < wk.end) nx.bi = ldc(a.binfo, nx.b);
   EdgeRec er[kEdgesAhead];
 #pragma unroll
   for (uint32_t m = 0; m < kEdgesAhead; ++m)
@@ -705,11 +718,17 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   }
   // Walk ahead, last in the batch so nothing below waits for it: this
   // thread's ghost cell id of the next block, and its scenario's dt.
-  if (nx.t < end) {
+  if (nx.t < wk.end) {
     if (tid < nx.bi.n_ghost) nx.gid = ldg(a.gx, nx.bi.g0 + tid);
     nx.dtw = ldc(&a.scen[bi_scen(nx.bi)].dt_cur, 0u);
   }
   if (ST == 1 && tid < bi.n_ghost && gact == 0u) gs = state0<LO>(a, gsrc);
+  // The list position of the block after the next one, taken from the
+  // slice's counter after every load above (nothing before phase C waits
+  // for it). An increment rather than an add: the compiler's wave-wide
+  // rewrite of a uniform add would wait for its return at once.
+  uint32_t taken = 0u;
+  if (tid == 0u) taken = atomicInc(wk.ctr, 0xffffffffu);
   float hi = hi0, hui = hui0, hvi = hvi0;
   const float loi = loi0;
   __syncthreads();  // the previous block's LDS reads are done
@@ -748,6 +767,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     if (ATOM) edge_accumulate<ST, FM, LO>(a, l, load_edge(a, bi, j));
     else edge_eval<ST, FM, LO>(a, l, load_edge(a, bi, j), j);
   }
+  if (tid == 0u) *wk.s_pos = taken;
   __syncthreads();
   if (a.n_open != 0u) {
     open_pass<ST>(a, l, b);
@@ -755,6 +775,10 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   }
 
   // ---- (C) cells -----------------------------------------------------------
+  // The list entry at the position taken in phase A: a vector load, so the
+  // LDS reads below do not wait on it.
+  nx.t2 = __builtin_amdgcn_readfirstlane(wk.beg + *wk.s_pos);
+  if (nx.t2 < wk.end) nx.b2 = ldg(a.list, nx.t2);
   bool wet = false;
   uint32_t bound = kNoBound;
   // The corrector's dt bound is cfl A / lam over the cell's wave speeds and,
@@ -950,13 +974,26 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
 }
 
 // Loads of the walk-ahead state for the block at list position nx.t, all at
-// once (the start of a walk, and blocks skipped without processing).
-__device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t end, uint32_t per) {
+// once (the start of a walk, and blocks skipped without processing): its
+// BlockInfo, this thread's ghost cell id, its scenario's dt and the list entry
+// at nx.t2.
+__device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t end) {
   if (nx.t >= end) return;
   nx.bi = ldc(a.binfo, nx.b);
   nx.gid = (threadIdx.x < nx.bi.n_ghost) ? a.gx[nx.bi.g0 + threadIdx.x] : 0u;
   nx.dtw = ldc(&a.scen[bi_scen(nx.bi)].dt_cur, 0u);
-  if (nx.t + per < end) nx.b2 = ldc(a.list, nx.t + per);
+  if (nx.t2 < end) nx.b2 = ldc(a.list, nx.t2);
+}
+
+// Takes `n` consecutive list positions of the slice for the workgroup (one
+// atomic) and returns the first to every thread. Ends with the workgroup's
+// threads having read the word it passes through.
+__device__ __forceinline__ uint32_t walk_take(const Walk& wk, uint32_t n) {
+  if (threadIdx.x == 0u) *wk.s_pos = atomicAdd(wk.ctr, n);
+  __syncthreads();
+  const uint32_t t = __builtin_amdgcn_readfirstlane(wk.beg + *wk.s_pos);
+  __syncthreads();
+  return t;
 }
 
 // Workgroups per CU the tile-0 stage kernels are compiled for: the predictor
@@ -967,35 +1004,46 @@ constexpr int stage_wgs() { return ST == 0 ? 6 : 5; }
 
 template <int ST, bool COMP, bool FM, int TILE, bool REC = false, bool ATOM = false>
 __global__ __launch_bounds__(256, TILE == 0 ? stage_wgs<ST>() : 1) void k_stage(const Args a) {
-  extern __shared__ float sm[];
+  // 16-byte aligned after the static LDS words: the records are read as float4.
+  extern __shared__ __align__(16) float sm[];
   if (a.ctl->any_dt == 0u) return;  // no-op step: every scenario is at its stop
   const Lds l = carve_stage<ST>(sm, a.max_local, a.max_ghost);
-  // XCD-aware split: workgroup w runs on XCD w % 8, so each XCD walks one
-  // contiguous (spatially compact) slice of the block list and its L2 keeps
-  // the ghost cells its neighbours read.
+  __shared__ uint32_t s_pos;
+  // XCD-aware split: workgroup w runs on XCD (w + r) % 8 for a rotation r of
+  // the launch, so the workgroups of slice w % 8 share an XCD and walk one
+  // contiguous (spatially compact) part of the block list, whose ghost cells
+  // their L2 then keeps. Within the slice the blocks are taken in list order
+  // from a counter, one at a time, so a workgroup that runs faster takes
+  // more of them and all of the slice's workgroups end within about a block
+  // of each other.
   const uint32_t cnt = a.ctl->cnt_cur;
   const uint32_t xcd = blockIdx.x % kXcds;
-  const uint32_t per = gridDim.x / kXcds;
-  const uint32_t end = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
+  uint32_t* const walk = a.walk + (uint32_t)ST * (kXcds + 1u) * kWalkStride;
+  Walk wk;
+  wk.beg = (uint32_t)(((uint64_t)cnt * xcd) / kXcds);
+  wk.end = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
+  wk.ctr = walk + xcd * kWalkStride;
+  wk.s_pos = &s_pos;
   Ahead nx;
-  nx.t = (uint32_t)(((uint64_t)cnt * xcd) / kXcds) + blockIdx.x / kXcds;
+  nx.t = walk_take(wk, 2u);
+  nx.t2 = nx.t + 1u;
   nx.b = 0u;
   nx.b2 = 0u;
   nx.gid = 0u;
   nx.dtw = 0u;
-  if (nx.t < end) nx.b = a.list[nx.t];
-  walk_ahead(a, nx, end, per);
+  if (nx.t < wk.end) nx.b = ldc(a.list, nx.t);
+  walk_ahead(a, nx, wk.end);
   // This thread's smallest next-step bound, for scenario cur_s; flushed when
   // the walk moves to another scenario's blocks.
   uint32_t bound = kNoBound;
-  uint32_t cur_s = (nx.t < end) ? bi_scen(nx.bi) : 0u;
-  while (nx.t < end) {
+  uint32_t cur_s = (nx.t < wk.end) ? bi_scen(nx.bi) : 0u;
+  while (nx.t < wk.end) {
     const uint32_t b = nx.b;
     const BlockInfo bi = nx.bi;
     const uint32_t gid = nx.gid;
     const uint32_t dtw = nx.dtw;
-    nx.t += per;
-    nx.b = nx.b2;
+    nx.t = nx.t2;
+    nx.b = __builtin_amdgcn_readfirstlane(nx.b2);
     if (ST == 1 && bi_scen(bi) != cur_s) {
       // Rare (a scenario boundary in this walk): flush per thread.
       if (bound != kNoBound && bound < __atomic_load_n(&a.scen[cur_s].dt_next, __ATOMIC_RELAXED))
@@ -1004,14 +1052,18 @@ __global__ __launch_bounds__(256, TILE == 0 ? stage_wgs<ST>() : 1) void k_stage(
       cur_s = bi_scen(bi);
     }
     if (dtw == 0u) {  // this scenario is at its stop
-      walk_ahead(a, nx, end, per);
+      nx.t2 = walk_take(wk, 1u);
+      walk_ahead(a, nx, wk.end);
       continue;
     }
-    const uint32_t bb =
-        stage_block<ST, COMP, FM, TILE, REC, ATOM>(a, b, bi, gid, __uint_as_float(dtw), l, nx, end, per);
+    const uint32_t bb = stage_block<ST, COMP, FM, TILE, REC, ATOM>(a, b, bi, gid, __uint_as_float(dtw), l, nx, wk);
     bound = (bb < bound) ? bb : bound;
   }
   if (ST == 1) block_atomic_min(&a.scen[cur_s].dt_next, bound);
+  // Every position this workgroup took has been read: the last workgroup to
+  // get here clears the counters for the next launch of this stage.
+  if (threadIdx.x == 0u && atomicAdd(walk + kXcds * kWalkStride, 1u) == gridDim.x - 1u)
+    for (uint32_t k = 0; k <= kXcds; ++k) atomicExch(walk + k * kWalkStride, 0u);
 }
 
 // Sources of blocks the active set skipped this step (reference
@@ -1358,7 +1410,7 @@ __global__ void k_list_reset(const Args a) {
 // atomicMin into dt slot 0.
 template <bool FM, bool LO>
 __global__ __launch_bounds__(256) void k_prologue_cfl(const Args a) {
-  extern __shared__ float sm[];
+  extern __shared__ __align__(16) float sm[];
   const Lds l = carve(sm, a.max_local, a.max_ghost);
   const Phys& p = a.p;
   const uint32_t tid = threadIdx.x;
@@ -1596,7 +1648,7 @@ __device__ __forceinline__ float3 heun_half(const float* tp, const float* lam, s
 // boundary additions, which are counted.
 template <int ST>
 __global__ __launch_bounds__(256) void k_adj_stage(const AdjArgs x) {
-  extern __shared__ float sm[];
+  extern __shared__ __align__(16) float sm[];
   const Args& a = x.a;
   const Lds l = carve(sm, a.max_local, a.max_ghost);
   const Phys& p = a.p;
@@ -2699,6 +2751,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1); rc |= dalloc(c, &src_pair, 1);
   rc |= dalloc(c, &fix_blk, 1); rc |= dalloc(c, &fix_chunk, 1);
   rc |= dalloc(c, &c->ctl, 1);
+  rc |= dalloc(c, &a.walk, 2 * (kXcds + 1) * kWalkStride);
   rc |= dalloc(c, &c->scen, n_scen);
   c->n_open = n_open;
   c->n_stretch = n_open ? 1u : 0u;
@@ -2747,6 +2800,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   HIPCHK(c, hipMemsetAsync(a.part_dry, 1, NB * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.act_part, 0, 3 * (size_t)a.n_dwg * 8, c->stream));
   HIPCHK(c, hipMemsetAsync(c->ctl, 0, sizeof(Ctl), c->stream));
+  HIPCHK(c, hipMemsetAsync(a.walk, 0, 2 * (kXcds + 1) * kWalkStride * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(c->scen, 0, sizeof(Scen) * n_scen, c->stream));
   if (n_open) {
     // Every open edge starts closed (a wall) and the mesh is one stretch.
