@@ -139,9 +139,8 @@ struct Args {
   uint32_t n_src_blocks, n_src_pairs, n_fix;
   Ctl* ctl;
   Scen* scen;
-  // The stage kernels' walk counters: per stage, kXcds slice counters and a
-  // count of finished workgroups, kWalkStride words apart; zero between
-  // launches.
+  // The stage kernels' walk counters: per stage, kXcds slice counters,
+  // kWalkStride words apart; zero between launches.
   uint32_t* walk;
   uint32_t n_scen;
   // open boundary edges (device record, read only where they are handled)
@@ -640,14 +639,30 @@ struct Ahead {
   uint32_t b2;   // block after it
 };
 
-// A workgroup's share of the walk: the list positions [beg, end) of its
-// slice, the slice's counter of positions taken, and an LDS word that passes
-// a taken position from the thread that took it to the workgroup.
+// A workgroup's share of the walk. Its slice's list positions end at `end`;
+// those from `base` on are handed out by the slice's counter `ctr` (those
+// before it, two per workgroup, are the workgroups' first blocks). A
+// workgroup takes positions until one take comes back empty (`dry`), and
+// never takes again, so the counter ends at exactly `last` + 1 (the handed
+// out positions plus one empty take per workgroup that took any): the take
+// that returns `last` is the slice's final one and clears the counter for
+// the next launch. `s_pos` is an LDS word that passes a taken position from
+// the thread that took it to the workgroup.
+constexpr uint32_t kNoPos = 0xffffffffu;  // no list position
+
 struct Walk {
-  uint32_t beg, end;
+  uint32_t base, end, last;
+  bool dry;
   uint32_t* ctr;
   uint32_t* s_pos;
 };
+
+// Position of a take that returned counter value r (kNoPos for an empty
+// take or none); thread 0 clears the counter after the slice's final take.
+__device__ __forceinline__ uint32_t walk_pos(const Walk& wk, uint32_t r) {
+  if (threadIdx.x == 0u && r == wk.last) atomicExch(wk.ctr, 0u);
+  return (r < wk.end - wk.base && wk.base < wk.end) ? wk.base + r : kNoPos;
+}
 
 // One active block of one Heun stage (reference resident.py:425-512, gather +
 // compensated variants), with the corrector also applying this block's
@@ -727,8 +742,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // slice's counter after every load above (nothing before phase C waits
   // for it). An increment rather than an add: the compiler's wave-wide
   // rewrite of a uniform add would wait for its return at once.
-  uint32_t taken = 0u;
-  if (tid == 0u) taken = atomicInc(wk.ctr, 0xffffffffu);
+  uint32_t taken = kNoPos;
+  if (tid == 0u && !wk.dry) taken = atomicInc(wk.ctr, 0xffffffffu);
   float hi = hi0, hui = hui0, hvi = hvi0;
   const float loi = loi0;
   __syncthreads();  // the previous block's LDS reads are done
@@ -777,8 +792,8 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // ---- (C) cells -----------------------------------------------------------
   // The list entry at the position taken in phase A: a vector load, so the
   // LDS reads below do not wait on it.
-  nx.t2 = __builtin_amdgcn_readfirstlane(wk.beg + *wk.s_pos);
-  if (nx.t2 < wk.end) nx.b2 = ldg(a.list, nx.t2);
+  nx.t2 = walk_pos(wk, __builtin_amdgcn_readfirstlane(*wk.s_pos));
+  if (nx.t2 != kNoPos) nx.b2 = ldg(a.list, nx.t2);
   bool wet = false;
   uint32_t bound = kNoBound;
   // The corrector's dt bound is cfl A / lam over the cell's wave speeds and,
@@ -985,14 +1000,17 @@ __device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t en
   if (nx.t2 < end) nx.b2 = ldc(a.list, nx.t2);
 }
 
-// Takes `n` consecutive list positions of the slice for the workgroup (one
-// atomic) and returns the first to every thread. Ends with the workgroup's
-// threads having read the word it passes through.
-__device__ __forceinline__ uint32_t walk_take(const Walk& wk, uint32_t n) {
-  if (threadIdx.x == 0u) *wk.s_pos = atomicAdd(wk.ctr, n);
+// Takes the next list position of the slice for the workgroup outside a
+// block (a skipped block's turn): kNoPos once the slice is empty. Ends with
+// the workgroup's threads having read the word it passes through.
+__device__ __forceinline__ uint32_t walk_take(Walk& wk) {
+  if (wk.dry) return kNoPos;
+  __syncthreads();  // every thread has read the word's last value
+  if (threadIdx.x == 0u) *wk.s_pos = atomicInc(wk.ctr, 0xffffffffu);
   __syncthreads();
-  const uint32_t t = __builtin_amdgcn_readfirstlane(wk.beg + *wk.s_pos);
+  const uint32_t t = walk_pos(wk, __builtin_amdgcn_readfirstlane(*wk.s_pos));
   __syncthreads();
+  wk.dry = t == kNoPos;
   return t;
 }
 
@@ -1012,21 +1030,28 @@ __global__ __launch_bounds__(256, TILE == 0 ? stage_wgs<ST>() : 1) void k_stage(
   // XCD-aware split: workgroup w runs on XCD (w + r) % 8 for a rotation r of
   // the launch, so the workgroups of slice w % 8 share an XCD and walk one
   // contiguous (spatially compact) part of the block list, whose ghost cells
-  // their L2 then keeps. Within the slice the blocks are taken in list order
-  // from a counter, one at a time, so a workgroup that runs faster takes
-  // more of them and all of the slice's workgroups end within about a block
-  // of each other.
+  // their L2 then keeps. A workgroup's first two blocks are fixed (list
+  // positions j and j + per of the slice for the slice's j-th workgroup);
+  // after those the blocks are taken in list order from a counter, one at a
+  // time, so a workgroup that runs faster takes more of them and all of the
+  // slice's workgroups end within about a block of each other.
   const uint32_t cnt = a.ctl->cnt_cur;
   const uint32_t xcd = blockIdx.x % kXcds;
-  uint32_t* const walk = a.walk + (uint32_t)ST * (kXcds + 1u) * kWalkStride;
+  const uint32_t per = gridDim.x / kXcds;
+  const uint32_t beg = (uint32_t)(((uint64_t)cnt * xcd) / kXcds);
+  const uint32_t end = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
   Walk wk;
-  wk.beg = (uint32_t)(((uint64_t)cnt * xcd) / kXcds);
-  wk.end = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
-  wk.ctr = walk + xcd * kWalkStride;
+  wk.base = beg + 2u * per;
+  wk.end = end;
+  // Workgroups of the slice that take from its counter: those with a first block.
+  const uint32_t takers = umin(per, end - beg);
+  wk.last = (end > wk.base ? end - wk.base : 0u) + takers - 1u;
+  wk.dry = false;
+  wk.ctr = a.walk + ((uint32_t)ST * kXcds + xcd) * kWalkStride;
   wk.s_pos = &s_pos;
   Ahead nx;
-  nx.t = walk_take(wk, 2u);
-  nx.t2 = nx.t + 1u;
+  nx.t = beg + blockIdx.x / kXcds;
+  nx.t2 = nx.t + per;
   nx.b = 0u;
   nx.b2 = 0u;
   nx.gid = 0u;
@@ -1052,18 +1077,15 @@ __global__ __launch_bounds__(256, TILE == 0 ? stage_wgs<ST>() : 1) void k_stage(
       cur_s = bi_scen(bi);
     }
     if (dtw == 0u) {  // this scenario is at its stop
-      nx.t2 = walk_take(wk, 1u);
+      nx.t2 = walk_take(wk);
       walk_ahead(a, nx, wk.end);
       continue;
     }
     const uint32_t bb = stage_block<ST, COMP, FM, TILE, REC, ATOM>(a, b, bi, gid, __uint_as_float(dtw), l, nx, wk);
     bound = (bb < bound) ? bb : bound;
+    wk.dry = nx.t2 == kNoPos;  // the block's take came back empty (or there was none)
   }
   if (ST == 1) block_atomic_min(&a.scen[cur_s].dt_next, bound);
-  // Every position this workgroup took has been read: the last workgroup to
-  // get here clears the counters for the next launch of this stage.
-  if (threadIdx.x == 0u && atomicAdd(walk + kXcds * kWalkStride, 1u) == gridDim.x - 1u)
-    for (uint32_t k = 0; k <= kXcds; ++k) atomicExch(walk + k * kWalkStride, 0u);
 }
 
 // Sources of blocks the active set skipped this step (reference
@@ -2751,7 +2773,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1); rc |= dalloc(c, &src_pair, 1);
   rc |= dalloc(c, &fix_blk, 1); rc |= dalloc(c, &fix_chunk, 1);
   rc |= dalloc(c, &c->ctl, 1);
-  rc |= dalloc(c, &a.walk, 2 * (kXcds + 1) * kWalkStride);
+  rc |= dalloc(c, &a.walk, 2 * kXcds * kWalkStride);
   rc |= dalloc(c, &c->scen, n_scen);
   c->n_open = n_open;
   c->n_stretch = n_open ? 1u : 0u;
@@ -2800,7 +2822,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   HIPCHK(c, hipMemsetAsync(a.part_dry, 1, NB * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.act_part, 0, 3 * (size_t)a.n_dwg * 8, c->stream));
   HIPCHK(c, hipMemsetAsync(c->ctl, 0, sizeof(Ctl), c->stream));
-  HIPCHK(c, hipMemsetAsync(a.walk, 0, 2 * (kXcds + 1) * kWalkStride * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.walk, 0, 2 * kXcds * kWalkStride * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(c->scen, 0, sizeof(Scen) * n_scen, c->stream));
   if (n_open) {
     // Every open edge starts closed (a wall) and the mesh is one stretch.
