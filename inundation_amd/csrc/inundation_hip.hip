@@ -639,29 +639,37 @@ struct Ahead {
   uint32_t b2;   // block after it
 };
 
-// A workgroup's share of the walk. Its slice's list positions end at `end`;
-// those from `base` on are handed out by the slice's counter `ctr` (those
-// before it, two per workgroup, are the workgroups' first blocks). A
-// workgroup takes positions until one take comes back empty (`dry`), and
-// never takes again, so the counter ends at exactly `last` + 1 (the handed
-// out positions plus one empty take per workgroup that took any): the take
-// that returns `last` is the slice's final one and clears the counter for
-// the next launch. `s_pos` is an LDS word that passes a taken position from
-// the thread that took it to the workgroup.
 constexpr uint32_t kNoPos = 0xffffffffu;  // no list position
 
+// A workgroup's share of the walk (three words, kept in scalar registers).
+// Its slice's list positions from `base` on, `len` of them, are handed out
+// by the slice's counter (those before, two per workgroup, are the
+// workgroups' first blocks; a position past the slice is kNoPos). A
+// workgroup takes positions until one take comes back empty, and never
+// takes again (`last` becomes kNoPos), so the counter ends at exactly the
+// initial `last` + 1: the handed out positions plus one empty take per
+// workgroup that took any. The take that returns that `last` is the
+// slice's final one and clears the counter for the next launch.
 struct Walk {
-  uint32_t base, end, last;
-  bool dry;
-  uint32_t* ctr;
-  uint32_t* s_pos;
+  uint32_t base, len, last;
 };
 
-// Position of a take that returned counter value r (kNoPos for an empty
-// take or none); thread 0 clears the counter after the slice's final take.
-__device__ __forceinline__ uint32_t walk_pos(const Walk& wk, uint32_t r) {
-  if (threadIdx.x == 0u && r == wk.last) atomicExch(wk.ctr, 0u);
-  return (r < wk.end - wk.base && wk.base < wk.end) ? wk.base + r : kNoPos;
+// The counter of this workgroup's slice in stage ST's launch.
+template <int ST>
+__device__ __forceinline__ uint32_t* walk_ctr(const Args& a) {
+  return a.walk + ((uint32_t)ST * kXcds + blockIdx.x % kXcds) * kWalkStride;
+}
+
+// The stage kernels' LDS word that passes a taken position from the thread
+// that took it to the workgroup.
+__shared__ uint32_t s_walk_pos;
+
+// Position of a take that returned counter value r (kNoPos: an empty take,
+// or none); thread 0 clears the counter after the slice's final take.
+template <int ST>
+__device__ __forceinline__ uint32_t walk_pos(const Args& a, const Walk& wk, uint32_t r) {
+  if (threadIdx.x == 0u && r != kNoPos && r == wk.last) atomicExch(walk_ctr<ST>(a), 0u);
+  return (r < wk.len) ? wk.base + r : kNoPos;
 }
 
 // One active block of one Heun stage (reference resident.py:425-512, gather +
@@ -670,7 +678,7 @@ __device__ __forceinline__ uint32_t walk_pos(const Walk& wk, uint32_t r) {
 // issued at once (cells, ghost cells, CSR slots, the thread's first edge
 // records); (B) each edge touching the block is evaluated once into LDS from
 // LDS; (C) each cell sums its edges in CSR order from LDS and updates.
-template <int ST, bool COMP, bool FM, int TILE, bool REC, bool ATOM>
+template <int ST, bool COMP, bool FM, int TILE, bool REC, bool ATOM, bool XTRA>
 __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, uint32_t gid, float dt,
                                 const Lds& l, Ahead& nx, const Walk& wk) {
   constexpr bool LO = COMP && FM;  // the fast order's face depths read the low words
@@ -698,7 +706,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // last, waits on it alone.
   const uint32_t gact = (ST == 1) ? ldg(a.active, gsrc >> 8) : 1u;
   // Walk ahead (scalar loads): the next block's BlockInfo.
-  if (nx.t < wk.end) nx.bi = ldc(a.binfo, nx.b);
+  if (nx.t != kNoPos) nx.bi = ldc(a.binfo, nx.b);
   EdgeRec er[kEdgesAhead];
 #pragma unroll
   for (uint32_t m = 0; m < kEdgesAhead; ++m)
@@ -733,7 +741,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   }
   // Walk ahead, last in the batch so nothing below waits for it: this
   // thread's ghost cell id of the next block, and its scenario's dt.
-  if (nx.t < wk.end) {
+  if (nx.t != kNoPos) {
     if (tid < nx.bi.n_ghost) nx.gid = ldg(a.gx, nx.bi.g0 + tid);
     nx.dtw = ldc(&a.scen[bi_scen(nx.bi)].dt_cur, 0u);
   }
@@ -743,7 +751,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // for it). An increment rather than an add: the compiler's wave-wide
   // rewrite of a uniform add would wait for its return at once.
   uint32_t taken = kNoPos;
-  if (tid == 0u && !wk.dry) taken = atomicInc(wk.ctr, 0xffffffffu);
+  if (tid == 0u && wk.last != kNoPos) taken = atomicInc(walk_ctr<ST>(a), 0xffffffffu);
   float hi = hi0, hui = hui0, hvi = hvi0;
   const float loi = loi0;
   __syncthreads();  // the previous block's LDS reads are done
@@ -782,9 +790,9 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
     if (ATOM) edge_accumulate<ST, FM, LO>(a, l, load_edge(a, bi, j));
     else edge_eval<ST, FM, LO>(a, l, load_edge(a, bi, j), j);
   }
-  if (tid == 0u) *wk.s_pos = taken;
+  if (tid == 0u) s_walk_pos = taken;
   __syncthreads();
-  if (a.n_open != 0u) {
+  if (XTRA && a.n_open != 0u) {
     open_pass<ST>(a, l, b);
     __syncthreads();
   }
@@ -792,7 +800,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
   // ---- (C) cells -----------------------------------------------------------
   // The list entry at the position taken in phase A: a vector load, so the
   // LDS reads below do not wait on it.
-  nx.t2 = walk_pos(wk, __builtin_amdgcn_readfirstlane(*wk.s_pos));
+  nx.t2 = walk_pos<ST>(a, wk, __builtin_amdgcn_readfirstlane(s_walk_pos));
   if (nx.t2 != kNoPos) nx.b2 = ldg(a.list, nx.t2);
   bool wet = false;
   uint32_t bound = kNoBound;
@@ -868,7 +876,7 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
       t[5 * q] = hv_new;
       t[6 * q] = (h_raw < 0.0f) ? 1.0f : 0.0f;
     }
-    if (a.track_clamp) {
+    if (XTRA && a.track_clamp) {
       const float A = FM ? ldg(a.area, i) : aw;
       stg(a.clamp, i, ldg(a.clamp, i) + 0.5f * A * (h_new - h_raw));
     }
@@ -992,25 +1000,24 @@ __device__ uint32_t stage_block(const Args& a, uint32_t b, const BlockInfo& bi, 
 // once (the start of a walk, and blocks skipped without processing): its
 // BlockInfo, this thread's ghost cell id, its scenario's dt and the list entry
 // at nx.t2.
-__device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx, uint32_t end) {
-  if (nx.t >= end) return;
+__device__ __forceinline__ void walk_ahead(const Args& a, Ahead& nx) {
+  if (nx.t == kNoPos) return;
   nx.bi = ldc(a.binfo, nx.b);
   nx.gid = (threadIdx.x < nx.bi.n_ghost) ? a.gx[nx.bi.g0 + threadIdx.x] : 0u;
   nx.dtw = ldc(&a.scen[bi_scen(nx.bi)].dt_cur, 0u);
-  if (nx.t2 < end) nx.b2 = ldc(a.list, nx.t2);
+  if (nx.t2 != kNoPos) nx.b2 = ldc(a.list, nx.t2);
 }
 
 // Takes the next list position of the slice for the workgroup outside a
-// block (a skipped block's turn): kNoPos once the slice is empty. Ends with
-// the workgroup's threads having read the word it passes through.
-__device__ __forceinline__ uint32_t walk_take(Walk& wk) {
-  if (wk.dry) return kNoPos;
+// block (a skipped block's turn): kNoPos once the slice is empty.
+template <int ST>
+__device__ __forceinline__ uint32_t walk_take(const Args& a, Walk& wk) {
+  if (wk.last == kNoPos) return kNoPos;
   __syncthreads();  // every thread has read the word's last value
-  if (threadIdx.x == 0u) *wk.s_pos = atomicInc(wk.ctr, 0xffffffffu);
+  if (threadIdx.x == 0u) s_walk_pos = atomicInc(walk_ctr<ST>(a), 0xffffffffu);
   __syncthreads();
-  const uint32_t t = walk_pos(wk, __builtin_amdgcn_readfirstlane(*wk.s_pos));
-  __syncthreads();
-  wk.dry = t == kNoPos;
+  const uint32_t t = walk_pos<ST>(a, wk, __builtin_amdgcn_readfirstlane(s_walk_pos));
+  if (t == kNoPos) wk.last = kNoPos;
   return t;
 }
 
@@ -1020,13 +1027,15 @@ __device__ __forceinline__ uint32_t walk_take(Walk& wk) {
 template <int ST>
 constexpr int stage_wgs() { return ST == 0 ? 6 : 5; }
 
-template <int ST, bool COMP, bool FM, int TILE, bool REC = false, bool ATOM = false>
+// XTRA: the variant with the open-boundary pass and the clamp tally (each
+// also switched at run time); the one without holds fewer values in scalar
+// registers across the walk.
+template <int ST, bool COMP, bool FM, int TILE, bool REC = false, bool ATOM = false, bool XTRA = true>
 __global__ __launch_bounds__(256, TILE == 0 ? stage_wgs<ST>() : 1) void k_stage(const Args a) {
   // 16-byte aligned after the static LDS words: the records are read as float4.
   extern __shared__ __align__(16) float sm[];
   if (a.ctl->any_dt == 0u) return;  // no-op step: every scenario is at its stop
   const Lds l = carve_stage<ST>(sm, a.max_local, a.max_ghost);
-  __shared__ uint32_t s_pos;
   // XCD-aware split: workgroup w runs on XCD (w + r) % 8 for a rotation r of
   // the launch, so the workgroups of slice w % 8 share an XCD and walk one
   // contiguous (spatially compact) part of the block list, whose ghost cells
@@ -1042,27 +1051,25 @@ __global__ __launch_bounds__(256, TILE == 0 ? stage_wgs<ST>() : 1) void k_stage(
   const uint32_t end = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
   Walk wk;
   wk.base = beg + 2u * per;
-  wk.end = end;
-  // Workgroups of the slice that take from its counter: those with a first block.
-  const uint32_t takers = umin(per, end - beg);
-  wk.last = (end > wk.base ? end - wk.base : 0u) + takers - 1u;
-  wk.dry = false;
-  wk.ctr = a.walk + ((uint32_t)ST * kXcds + xcd) * kWalkStride;
-  wk.s_pos = &s_pos;
+  wk.len = end > wk.base ? end - wk.base : 0u;
+  // Workgroups of the slice that take from its counter: those with a first
+  // block. (None when the slice is empty; `last` is then never reached.)
+  wk.last = wk.len + umin(per, end - beg) - 1u;
   Ahead nx;
-  nx.t = beg + blockIdx.x / kXcds;
-  nx.t2 = nx.t + per;
+  const uint32_t t0 = beg + blockIdx.x / kXcds;
+  nx.t = t0 < end ? t0 : kNoPos;
+  nx.t2 = t0 + per < end ? t0 + per : kNoPos;
   nx.b = 0u;
   nx.b2 = 0u;
   nx.gid = 0u;
   nx.dtw = 0u;
-  if (nx.t < wk.end) nx.b = ldc(a.list, nx.t);
-  walk_ahead(a, nx, wk.end);
+  if (nx.t != kNoPos) nx.b = ldc(a.list, nx.t);
+  walk_ahead(a, nx);
   // This thread's smallest next-step bound, for scenario cur_s; flushed when
   // the walk moves to another scenario's blocks.
   uint32_t bound = kNoBound;
-  uint32_t cur_s = (nx.t < wk.end) ? bi_scen(nx.bi) : 0u;
-  while (nx.t < wk.end) {
+  uint32_t cur_s = (nx.t != kNoPos) ? bi_scen(nx.bi) : 0u;
+  while (nx.t != kNoPos) {
     const uint32_t b = nx.b;
     const BlockInfo bi = nx.bi;
     const uint32_t gid = nx.gid;
@@ -1077,13 +1084,14 @@ __global__ __launch_bounds__(256, TILE == 0 ? stage_wgs<ST>() : 1) void k_stage(
       cur_s = bi_scen(bi);
     }
     if (dtw == 0u) {  // this scenario is at its stop
-      nx.t2 = walk_take(wk);
-      walk_ahead(a, nx, wk.end);
+      nx.t2 = walk_take<ST>(a, wk);
+      walk_ahead(a, nx);
       continue;
     }
-    const uint32_t bb = stage_block<ST, COMP, FM, TILE, REC, ATOM>(a, b, bi, gid, __uint_as_float(dtw), l, nx, wk);
+    const uint32_t bb =
+        stage_block<ST, COMP, FM, TILE, REC, ATOM, XTRA>(a, b, bi, gid, __uint_as_float(dtw), l, nx, wk);
     bound = (bb < bound) ? bb : bound;
-    wk.dry = nx.t2 == kNoPos;  // the block's take came back empty (or there was none)
+    if (nx.t2 == kNoPos) wk.last = kNoPos;  // the block's take came back empty (or there was none)
   }
   if (ST == 1) block_atomic_min(&a.scen[cur_s].dt_next, bound);
 }
@@ -2246,14 +2254,28 @@ int scen_read(inu_ctx* c, std::vector<Scen>& out) {
 // Kernel launches of one step, in stream order. With `ev` set, events are
 // recorded before the predictor, between the two stages and after the
 // corrector (no host wait).
+// Launches a stage kernel variant: the run-time flags pick the compensated,
+// fast-order and XTRA instantiations.
+template <int ST, bool CP, bool F, int TILE, bool REC, bool ATOM>
+void launch_k_stage(inu_ctx* c, const Args& a, dim3 g, size_t lds) {
+  if (a.n_open != 0u || a.track_clamp != 0u)
+    hipLaunchKernelGGL((k_stage<ST, CP, F, TILE, REC, ATOM, true>), g, dim3(256), lds, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_stage<ST, CP, F, TILE, REC, ATOM, false>), g, dim3(256), lds, c->stream, a);
+}
+template <int ST, int TILE, bool REC, bool ATOM>
+void launch_k_stage(inu_ctx* c, const Args& a, bool comp, bool fm, dim3 g, size_t lds) {
+  if (comp && fm) launch_k_stage<ST, true, true, TILE, REC, ATOM>(c, a, g, lds);
+  else if (comp) launch_k_stage<ST, true, false, TILE, REC, ATOM>(c, a, g, lds);
+  else if (fm) launch_k_stage<ST, false, true, TILE, REC, ATOM>(c, a, g, lds);
+  else launch_k_stage<ST, false, false, TILE, REC, ATOM>(c, a, g, lds);
+}
+
 template <int ST, int TILE>
 void launch_stage_t(inu_ctx* c, const Args& a, bool comp, bool fm) {
   const size_t lds = ST == 0 ? c->lds_stage0 : c->lds_stage1;
-  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0][TILE]), b(256);
-  if (comp && fm) hipLaunchKernelGGL((k_stage<ST, true, true, TILE>), g, b, lds, c->stream, a);
-  else if (comp) hipLaunchKernelGGL((k_stage<ST, true, false, TILE>), g, b, lds, c->stream, a);
-  else if (fm) hipLaunchKernelGGL((k_stage<ST, false, true, TILE>), g, b, lds, c->stream, a);
-  else hipLaunchKernelGGL((k_stage<ST, false, false, TILE>), g, b, lds, c->stream, a);
+  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0][TILE]);
+  launch_k_stage<ST, TILE, false, false>(c, a, comp, fm, g, lds);
 }
 
 // Unordered accumulation (INU_FLAG_DETERMINISTIC off), tile level 0.
@@ -2262,11 +2284,8 @@ void launch_stage_atom(inu_ctx* c, const Args& a, bool comp, bool fm) {
   // The four per-cell sums (4 x 256 floats) take the edge records' place.
   const size_t lds = std::max(ST == 0 ? c->lds_stage0 : c->lds_stage1,
                               (6 * (size_t)stage_cells(a.max_ghost) + 4 * (size_t)kBlockCells) * sizeof(float));
-  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0][0]), b(256);
-  if (comp && fm) hipLaunchKernelGGL((k_stage<ST, true, true, 0, false, true>), g, b, lds, c->stream, a);
-  else if (comp) hipLaunchKernelGGL((k_stage<ST, true, false, 0, false, true>), g, b, lds, c->stream, a);
-  else if (fm) hipLaunchKernelGGL((k_stage<ST, false, true, 0, false, true>), g, b, lds, c->stream, a);
-  else hipLaunchKernelGGL((k_stage<ST, false, false, 0, false, true>), g, b, lds, c->stream, a);
+  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0][0]);
+  launch_k_stage<ST, 0, false, true>(c, a, comp, fm, g, lds);
 }
 
 template <int ST>
@@ -2295,11 +2314,8 @@ void launch_source_idle(inu_ctx* c, const Args& a, bool comp, bool fm) {
 template <int ST>
 void launch_stage_rec(inu_ctx* c, const Args& a, bool comp, bool fm) {
   const size_t lds = ST == 0 ? c->lds_stage0 : c->lds_stage1;
-  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0][0]), b(256);
-  if (comp && fm) hipLaunchKernelGGL((k_stage<ST, true, true, 0, true>), g, b, lds, c->stream, a);
-  else if (comp) hipLaunchKernelGGL((k_stage<ST, true, false, 0, true>), g, b, lds, c->stream, a);
-  else if (fm) hipLaunchKernelGGL((k_stage<ST, false, true, 0, true>), g, b, lds, c->stream, a);
-  else hipLaunchKernelGGL((k_stage<ST, false, false, 0, true>), g, b, lds, c->stream, a);
+  const dim3 g(c->grid[ST][comp ? 1 : 0][fm ? 1 : 0][0]);
+  launch_k_stage<ST, 0, true, false>(c, a, comp, fm, g, lds);
 }
 
 // Open edge words of stage ST (k_open_words), before that stage's kernel.
