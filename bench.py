@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # 2.0000, WRITE x 1.0000): mean over the 900 corrector launches of three
 # 300-step windows, steps 101..1000 of this bench's flood.
 CORRECTOR_TRAFFIC: dict | None = {
-    "bytes_per_launch": (717002.6 * (1048576.0 / 524296.5) + 252738.4) * 1024.0,
+    "bytes_per_launch": (711926.1 * (1048576.0 / 524296.5) + 252582.9) * 1024.0,
     "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB plain copy; steps 101..1000, all blocks "
     "active",
 }
