@@ -1047,6 +1047,10 @@ __global__ __launch_bounds__(256, TILE == 0 ? stage_wgs<ST>() : 1) void k_stage(
   const uint32_t cnt = a.ctl->cnt_cur;
   const uint32_t xcd = blockIdx.x % kXcds;
   const uint32_t per = gridDim.x / kXcds;
+  // The launchers size the grid in whole multiples of kXcds; a workgroup
+  // past the last whole multiple would repeat a slice's first blocks and
+  // upset the counter's final value.
+  if (blockIdx.x >= per * kXcds) return;
   const uint32_t beg = (uint32_t)(((uint64_t)cnt * xcd) / kXcds);
   const uint32_t end = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
   Walk wk;
