@@ -6,7 +6,9 @@ Workload: the 16 003 008-cell rainfall flood (mixed triangle/quad lattice over
 n, FP32 with compensated depth, active set on). A "step" is one Heun step of
 the whole mesh. After 20 warm-up steps the flood restarts from dry, runs 100
 untimed lead steps through the wetting transient, then three consecutive
-timed windows of 2000 steps; ``value`` comes from the median window.
+timed windows of 2000 steps; ``value`` comes from the median window. A
+fourth window of the same flood times each stage kernel launch with HIP
+events.
 
 - ``value``: cell-steps/s over the timed steps, all cells counted every step
   (the reference's convention, flood_propagation.py:645-646), with the state
@@ -14,8 +16,9 @@ timed windows of 2000 steps; ``value`` comes from the median window.
 - ``roofline``: the corrector stage kernel (the larger of the two stage
   kernels), algorithmic bytes of the active blocks (28 B per edge, 4 B per
   CSR slot, 60 B per cell with the compensated low words) over its
-  HIP-event-timed duration, against 8 TB/s, with the HBM traffic measured
-  by rocprofv3 counters and the attainable copy rates of the same box;
+  HIP-event-timed duration (fourth window), against 8 TB/s, with the HBM
+  traffic measured by rocprofv3 counters and the attainable copy rates of
+  the same box;
 - ``cpu_baseline``: the FP32 oracle (C++/OpenMP) on the same mesh and state
   for a bounded number of steps, checked bit for bit against the GPU on the
   same steps.
@@ -44,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 # 2.0000, WRITE x 1.0000): mean over the 900 corrector launches of three
 # 300-step windows, steps 101..1000 of this bench's flood.
 CORRECTOR_TRAFFIC: dict | None = {
-    "bytes_per_launch": (711926.1 * (1048576.0 / 524296.5) + 252582.9) * 1024.0,
+    "bytes_per_launch": (711391.2 * (1048576.0 / 524296.5) + 252588.2) * 1024.0,
     "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB plain copy; steps 101..1000, all blocks "
     "active",
 }
@@ -165,23 +168,36 @@ def main() -> int:
     # The reference's protocol: the median of three timed runs after a
     # warm-up. The runs are three consecutive windows of the same flood
     # (steps lead+1 .. lead+3K), with no state upload between them, so the
-    # GPU stays busy from the lead into the timed steps.
+    # GPU stays busy from the lead into the timed steps. A window's steps are
+    # launched one by one with no event between them (as fast as replaying
+    # the library's step graphs, and traceable by rocprofv3, whose kernel
+    # trace aborts on graphs replayed back to back), synchronised on both
+    # sides.
     runs = []
     for r in range(3):
         ctx.activity(reset=True)
         t0 = time.perf_counter()
-        ms_pred, ms_corr, ms_span = ctx.steps_timed(args.steps)
+        ctx.steps(args.steps, use_graph=False)
         wall = time.perf_counter() - t0
         act = ctx.activity()
         if act["steps"] != args.steps:
             raise RuntimeError(f"{args.steps - act['steps']} of the timed steps were no-ops")
-        runs.append({"wall": wall, "pred": ms_pred, "corr": ms_corr, "span": ms_span, "act": act,
-                     "first_step": args.lead + r * args.steps + 1})
-        log(f"run {r}: wall {wall:.3f}s ({wall / args.steps * 1e3:.4f} ms/step), events: predictor "
-            f"{ms_pred / args.steps:.4f} ms, corrector {ms_corr / args.steps:.4f} ms")
+        runs.append({"wall": wall, "act": act, "first_step": args.lead + r * args.steps + 1})
+        log(f"run {r}: wall {wall:.3f}s ({wall / args.steps * 1e3:.4f} ms/step)")
     order = sorted(range(3), key=lambda k: runs[k]["wall"])
     med = runs[order[1]]
-    wall, ms_pred, ms_corr, act = med["wall"], med["pred"], med["corr"], med["act"]
+    wall, act = med["wall"], med["act"]
+    # Each stage kernel's launch time: HIP events on the solver stream around
+    # every predictor and corrector launch, over a fourth window of the same
+    # flood (steps lead+3K+1 .. lead+4K). An event record between two launches
+    # adds a few microseconds to the step, so these windows are not the ones
+    # `value` is taken over.
+    ctx.activity(reset=True)
+    ms_pred, ms_corr, _ = ctx.steps_timed(args.steps)
+    act_ev = ctx.activity()
+    if act_ev["steps"] != args.steps:
+        raise RuntimeError(f"{args.steps - act_ev['steps']} of the event-timed steps were no-ops")
+    log(f"event window: predictor {ms_pred / args.steps:.4f} ms, corrector {ms_corr / args.steps:.4f} ms per launch")
     ctrl = ctx.ctrl_read()
     n, e, s = geom.N, geom.E, int(m["cptr"][-1])
     value = args.steps * n / wall
@@ -190,10 +206,10 @@ def main() -> int:
     # stage input + 12 B/cell output; the corrector also reads the start state
     # (12 B/cell); compensated low words 8 B/cell (predictor) and 12 B/cell
     # (corrector). Counted over active blocks only; uniform rain adds nothing.
-    corr_bytes = 28 * act["edges"] + 4 * act["slots"] + (36 + 12 + 12) * act["cells"]
+    corr_bytes = 28 * act_ev["edges"] + 4 * act_ev["slots"] + (36 + 12 + 12) * act_ev["cells"]
     heun_bytes = step_bytes(act)
     corr_gbs = corr_bytes / (ms_corr * 1e-3) / 1e9
-    step_gbs = heun_bytes / ((ms_pred + ms_corr) * 1e-3) / 1e9
+    step_gbs = step_bytes(act_ev) / ((ms_pred + ms_corr) * 1e-3) / 1e9
     active_frac = act["cells"] / (args.steps * n)
     log(f"median run: {wall / args.steps * 1e3:.4f} ms/step; active {active_frac:.3f}; corrector {corr_gbs:.0f} GB/s; "
         f"t_sim {ctrl.now_ticks / 2**20:.1f}s")
@@ -285,8 +301,9 @@ def main() -> int:
             "dt_max_s": wl.dt_max,
             "cfl": wl.cfl,
             "window": f"three windows of {args.steps} steps, steps {args.lead + 1}..{args.lead + 3 * args.steps} "
-            f"of the run from dry (t = {lead_now / 2**20:.1f}..{ctrl.now_ticks / 2**20:.1f} s); the median one "
-            f"starts at step {med['first_step']}",
+            f"of the run from dry, launched step by step; the median one starts at step "
+            f"{med['first_step']}; then {args.steps} steps with HIP events around each stage kernel (to t = "
+            f"{ctrl.now_ticks / 2**20:.1f} s, from t = {lead_now / 2**20:.1f} s after the lead)",
             "simulated_s": ctrl.now_ticks / 2**20,
             "active_cell_fraction": active_frac,
             "parallelism": "1 GPU",
@@ -303,6 +320,8 @@ def main() -> int:
             else traffic["bytes_per_launch"] / (corr_bytes / args.steps),
             "traffic_source": None if traffic is None else traffic["source"],
             "ms_per_launch": ms_corr / args.steps,
+            "ms_per_launch_is": f"HIP events on the solver stream around each corrector launch, steps "
+            f"{args.lead + 3 * args.steps + 1}..{args.lead + 4 * args.steps}",
             "algorithmic_bytes_per_launch": corr_bytes / args.steps,
             "frac_of_best_copy": corr_gbs / bw["best_copy_GBs"],
             "hbm_probes_GBs": bw,

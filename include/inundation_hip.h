@@ -201,9 +201,9 @@ int inu_activity(inu_ctx* ctx, uint64_t* out4, int reset);
  * (gpu_resident_batch.py:593-599). */
 int inu_retarget(inu_ctx* ctx);
 
-/* Run n recorded steps and wait; use_graph replays a cached hipGraph of n
- * steps (recorded on first use). Replaces steps_seq.eval() of
- * _record_steps (gpu_resident_batch.py:601-607, 717). */
+/* Run n recorded steps and wait; use_graph replays cached hipGraphs of at
+ * most 100 steps each (recorded on first use), back to back. Replaces
+ * steps_seq.eval() of _record_steps (gpu_resident_batch.py:601-607, 717). */
 int inu_steps(inu_ctx* ctx, uint32_t n, int use_graph);
 
 /* n steps launched one by one with HIP events around the two stage kernels:

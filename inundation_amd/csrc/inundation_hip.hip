@@ -48,6 +48,7 @@ namespace {
 constexpr uint32_t kBlockCells = 256;  // ACTIVE_BLOCK_CELLS; one workgroup per block
 constexpr uint32_t kXcds = 8;          // MI355X: 8 accelerator complex dies
 constexpr uint32_t kWalkStride = 32;    // words between walk counters (one 128-byte line each)
+constexpr uint32_t kGraphSteps = 100;   // steps per captured graph of inu_steps
 constexpr float kTicksPerSecond = 1048576.0f;
 constexpr float kDtSentinel = 1e30f;
 constexpr uint32_t kNoBound = 0x7f800000u;  // +inf bits: larger than any bound
@@ -3271,26 +3272,33 @@ int inu_steps(inu_ctx* c, uint32_t n, int use_graph) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
   }
-  auto it = c->graphs.find(n);
-  if (it == c->graphs.end()) {
-    hipGraph_t g = nullptr;
-    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    for (uint32_t k = 0; k < n; ++k) {
-      if ((rc = launch_step(c, nullptr))) {
-        hipGraph_t dummy = nullptr;
-        (void)hipStreamEndCapture(c->stream, &dummy);
-        if (dummy) (void)hipGraphDestroy(dummy);
-        return rc;
+  // Runs of more than kGraphSteps steps replay the kGraphSteps-step graph
+  // (and one of the remainder), launched back to back with one wait at the
+  // end: graphs stay small whatever n is.
+  for (uint32_t left = n; left != 0u;) {
+    const uint32_t m = std::min(left, kGraphSteps);
+    auto it = c->graphs.find(m);
+    if (it == c->graphs.end()) {
+      hipGraph_t g = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (uint32_t k = 0; k < m; ++k) {
+        if ((rc = launch_step(c, nullptr))) {
+          hipGraph_t dummy = nullptr;
+          (void)hipStreamEndCapture(c->stream, &dummy);
+          if (dummy) (void)hipGraphDestroy(dummy);
+          return rc;
+        }
       }
+      HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+      hipGraphExec_t ex = nullptr;
+      hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(c, INU_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      it = c->graphs.emplace(m, ex).first;
     }
-    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
-    hipGraphExec_t ex = nullptr;
-    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(c, INU_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    it = c->graphs.emplace(n, ex).first;
+    HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+    left -= m;
   }
-  HIPCHK(c, hipGraphLaunch(it->second, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
