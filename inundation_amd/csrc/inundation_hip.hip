@@ -140,8 +140,9 @@ struct Args {
   uint32_t n_src_blocks, n_src_pairs, n_fix;
   Ctl* ctl;
   Scen* scen;
-  // The stage kernels' walk counters: per stage, kXcds slice counters,
-  // kWalkStride words apart; zero between launches.
+  // The walk counters of the stage kernels and of the adjoint's reverse
+  // stages: per kernel, kXcds slice counters, kWalkStride words apart; zero
+  // between launches.
   uint32_t* walk;
   uint32_t n_scen;
   // open boundary edges (device record, read only where they are handled)
@@ -1695,8 +1696,26 @@ __global__ __launch_bounds__(256) void k_adj_stage(const AdjArgs x) {
   const float dt = __uint_as_float(x.log[k]);
   const float* lam_out = (ST == 0) ? x.l1 : nullptr;
   uint32_t n_cross = 0;
+  // The stage kernels' walk (k_stage): a slice of the list per XCD, each
+  // workgroup's first block fixed, then one block at a time from the
+  // slice's self-clearing counter (taken while the block is reversed).
+  __shared__ uint32_t s_adj_pos;
   const uint32_t cnt = a.ctl->cnt_cur;
-  for (uint32_t t = blockIdx.x; t < cnt; t += gridDim.x) {
+  const uint32_t per = gridDim.x / kXcds;
+  if (blockIdx.x >= per * kXcds) return;  // the host sizes the grid in whole multiples
+  const uint32_t xcd = blockIdx.x % kXcds;
+  const uint32_t beg = (uint32_t)(((uint64_t)cnt * xcd) / kXcds);
+  const uint32_t end = (uint32_t)(((uint64_t)cnt * (xcd + 1)) / kXcds);
+  uint32_t* const ctr = a.walk + ((2u + (uint32_t)ST) * kXcds + xcd) * kWalkStride;
+  Walk wk;
+  wk.base = beg + per;
+  wk.len = end > wk.base ? end - wk.base : 0u;
+  wk.last = wk.len + umin(per, end - beg) - 1u;
+  uint32_t t = beg + blockIdx.x / kXcds;
+  if (t >= end) t = kNoPos;
+  while (t != kNoPos) {
+    uint32_t taken = kNoPos;
+    if (tid == 0u && wk.last != kNoPos) taken = atomicInc(ctr, 0xffffffffu);
     const uint32_t b = a.list[t];
     const BlockInfo bi = a.binfo[b];
     const uint32_t c0 = b << 8, nc = bi_cells(bi);
@@ -1798,6 +1817,14 @@ __global__ __launch_bounds__(256) void k_adj_stage(const AdjArgs x) {
         x.l1[2 * N + i] = shv;
       }
     }
+    // The next block: the position taken above (its slice's final take
+    // clears the counter).
+    if (tid == 0u) s_adj_pos = taken;
+    __syncthreads();
+    const uint32_t r = __builtin_amdgcn_readfirstlane(s_adj_pos);
+    if (tid == 0u && r != kNoPos && r == wk.last) atomicExch(ctr, 0u);
+    t = r < wk.len ? wk.base + r : kNoPos;
+    if (t == kNoPos) wk.last = kNoPos;
   }
   if (n_cross) atomicAdd(&x.cur->stats[2], (unsigned long long)n_cross);
 }
@@ -2794,7 +2821,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   rc |= dalloc(c, &src_cell, 1); rc |= dalloc(c, &src_blocks, 1); rc |= dalloc(c, &src_pair, 1);
   rc |= dalloc(c, &fix_blk, 1); rc |= dalloc(c, &fix_chunk, 1);
   rc |= dalloc(c, &c->ctl, 1);
-  rc |= dalloc(c, &a.walk, 2 * kXcds * kWalkStride);
+  rc |= dalloc(c, &a.walk, 4 * kXcds * kWalkStride);
   rc |= dalloc(c, &c->scen, n_scen);
   c->n_open = n_open;
   c->n_stretch = n_open ? 1u : 0u;
@@ -2843,7 +2870,7 @@ int assemble(inu_ctx* c, const std::vector<MeshIn>& ins, std::vector<Layout>& la
   HIPCHK(c, hipMemsetAsync(a.part_dry, 1, NB * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(a.act_part, 0, 3 * (size_t)a.n_dwg * 8, c->stream));
   HIPCHK(c, hipMemsetAsync(c->ctl, 0, sizeof(Ctl), c->stream));
-  HIPCHK(c, hipMemsetAsync(a.walk, 0, 2 * kXcds * kWalkStride * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(a.walk, 0, 4 * kXcds * kWalkStride * 4, c->stream));
   HIPCHK(c, hipMemsetAsync(c->scen, 0, sizeof(Scen) * n_scen, c->stream));
   if (n_open) {
     // Every open edge starts closed (a wall) and the mesh is one stretch.
@@ -4254,7 +4281,8 @@ static int adj_work(inu_ctx* c) {
   if (rc) return fail(c, INU_EHIP, "adjoint: out of device memory (the tape needs segment x 56 B per cell)");
   int occ = 0;
   HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_adj_stage<0>, 256, c->lds_full));
-  d.grid_adj = std::max<uint32_t>(1u, std::min<uint32_t>(c->NB, (uint32_t)std::max(occ, 1) * c->cus));
+  // A whole multiple of the XCD count (the walk's slices), at least one per XCD.
+  d.grid_adj = std::max<uint32_t>(kXcds, std::min<uint32_t>(c->NB, (uint32_t)std::max(occ, 1) * c->cus) / kXcds * kXcds);
   return 0;
 }
 
